@@ -1,0 +1,228 @@
+// capi.cpp -- extern "C" entry points declared in include/femh.h.
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <tuple>
+
+#include "../../include/femh.h"
+#include "solver.hpp"
+
+struct femh_ctx {
+  std::unique_ptr<femh::Context> c;
+  femh_opts opts{};
+  std::string err;
+  int last_status = FEMH_OK;
+  // per-grid sub-contexts used by the batch entry point
+  std::map<std::tuple<int, int, int>, std::unique_ptr<femh::Context>> sub;
+};
+
+namespace {
+
+template <typename F>
+int guarded(femh_ctx *ctx, F &&f) {
+  if (!ctx) return FEMH_BAD_ARG;
+  try {
+    f();
+    return FEMH_OK;
+  } catch (const femh::Error &e) {
+    ctx->err = e.what();
+    ctx->last_status = e.status;
+    return e.status;
+  } catch (const std::bad_alloc &) {
+    ctx->err = "host allocation failed";
+    ctx->last_status = FEMH_OOM;
+    return FEMH_OOM;
+  } catch (const std::exception &e) {
+    ctx->err = e.what();
+    ctx->last_status = FEMH_DEVICE_ERROR;
+    return FEMH_DEVICE_ERROR;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int femh_abi_version(void) { return FEMH_ABI_VERSION; }
+
+void femh_default_opts(femh_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->nx = o->ny = o->nz = 16;
+  o->E0 = 1.0;
+  o->Emin = 0.0;
+  o->nu = 0.3;
+  o->penal = 3.0;
+  o->interp = FEMH_INTERP_SIMP;
+  o->precond = FEMH_PRECOND_MG;
+  o->levels = 0;
+  o->nu1 = 2;
+  o->nu2 = 2;
+  o->cheby_ratio = 8.0;
+  o->coarse_iters = 64;
+  o->tol = 1e-10;
+  o->max_iter = 5000;
+  o->precision = FEMH_FP64;
+  o->ch_tol = 0.0;
+  o->check_every = 1;
+  o->device = 0;
+}
+
+int femh_create(const femh_opts *o, femh_ctx **out) {
+  if (!o || !out) return FEMH_BAD_ARG;
+  *out = nullptr;
+  femh_ctx *ctx = new (std::nothrow) femh_ctx();
+  if (!ctx) return FEMH_OOM;
+  ctx->opts = *o;
+  int st = guarded(ctx, [&] { ctx->c.reset(new femh::Context(*o)); });
+  if (st != FEMH_OK) {
+    // keep the context so the caller can read the message; it cannot solve
+    *out = ctx;
+    return st;
+  }
+  *out = ctx;
+  return FEMH_OK;
+}
+
+void femh_destroy(femh_ctx *ctx) { delete ctx; }
+
+static int need_ctx(femh_ctx *ctx) {
+  if (!ctx) return FEMH_BAD_ARG;
+  if (!ctx->c) {
+    ctx->last_status = FEMH_BAD_ARG;
+    if (ctx->err.empty()) ctx->err = "context was not created successfully";
+    return FEMH_BAD_ARG;
+  }
+  return FEMH_OK;
+}
+
+int femh_homogenize(femh_ctx *ctx, const double *rho, double *C, femh_stats *stats) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!rho || !C) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] {
+    ctx->c->set_modulus_from_rho(rho, false);
+    ctx->c->homogenize(C, stats);
+  });
+}
+
+int femh_homogenize_modulus(femh_ctx *ctx, const double *E, double *C, femh_stats *stats) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!E || !C) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] {
+    int saved = ctx->c->o.interp;
+    ctx->c->o.interp = FEMH_INTERP_MODULUS;
+    try {
+      ctx->c->set_modulus(E);
+      ctx->c->homogenize(C, stats);
+    } catch (...) {
+      ctx->c->o.interp = saved;
+      throw;
+    }
+    ctx->c->o.interp = saved;
+  });
+}
+
+int femh_homogenize_device(femh_ctx *ctx, const double *d_rho, double *d_C, void *stream,
+                           femh_stats *stats) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!d_rho || !d_C) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] {
+    FEMH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    ctx->c->set_modulus_from_rho(d_rho, true);
+    double C[36];
+    ctx->c->homogenize(C, stats);
+    FEMH_HIP(hipMemcpyAsync(d_C, C, sizeof(C), hipMemcpyHostToDevice, ctx->c->stream));
+    FEMH_HIP(hipStreamSynchronize(ctx->c->stream));
+  });
+}
+
+int femh_gradient(femh_ctx *ctx, const double *dJdC, double *dJdrho) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!dJdC || !dJdrho) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] { ctx->c->gradient(dJdC, dJdrho, false); });
+}
+
+int femh_energy_terms(femh_ctx *ctx, double *W) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!W) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] {
+    double zero[36] = {0};
+    ctx->c->gradient(zero, W, true);
+  });
+}
+
+int femh_get_fields(femh_ctx *ctx, double *u) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!u) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] { ctx->c->get_fields(u); });
+}
+
+int femh_homogenize_batch(femh_ctx *ctx, int n, const int *dims, const double *const *E, double *C,
+                          femh_stats *stats) {
+  if (!ctx) return FEMH_BAD_ARG;
+  if (n < 0 || (n > 0 && (!dims || !E || !C))) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] {
+    int maxit = 0;
+    double ms = 0;
+    for (int s = 0; s < n; s++) {
+      auto key = std::make_tuple(dims[3 * s], dims[3 * s + 1], dims[3 * s + 2]);
+      auto it = ctx->sub.find(key);
+      if (it == ctx->sub.end()) {
+        femh_opts o = ctx->opts;
+        o.nx = dims[3 * s];
+        o.ny = dims[3 * s + 1];
+        o.nz = dims[3 * s + 2];
+        o.interp = FEMH_INTERP_MODULUS;
+        it = ctx->sub.emplace(key, std::unique_ptr<femh::Context>(new femh::Context(o))).first;
+      }
+      femh_stats st{};
+      it->second->set_modulus(E[s]);
+      it->second->homogenize(C + 36 * (size_t)s, &st);
+      for (int k = 0; k < 6; k++) maxit = st.iters[k] > maxit ? st.iters[k] : maxit;
+      ms += st.ms_total;
+    }
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->iters[0] = maxit;
+      stats->ms_total = ms;
+    }
+  });
+}
+
+int femh_prepare(femh_ctx *ctx, const double *E) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!E) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] {
+    ctx->c->set_modulus(E);
+    ctx->c->build_hierarchy();
+  });
+}
+
+int femh_apply_level(femh_ctx *ctx, int level, const double *u, double *y) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!u || !y) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] { ctx->c->apply_level(level, u, y); });
+}
+
+int femh_level_dims(femh_ctx *ctx, int level, int *dims) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!dims) return FEMH_BAD_ARG;
+  femh::Context &c = *ctx->c;
+  if (!c.h64.ready || level < 0 || level >= c.nlevels) return FEMH_BAD_ARG;
+  dims[0] = c.h64.lv[level].g.nx;
+  dims[1] = c.h64.lv[level].g.ny;
+  dims[2] = c.h64.lv[level].g.nz;
+  return FEMH_OK;
+}
+
+int femh_last_error(femh_ctx *ctx, char *buf, size_t len) {
+  if (!ctx) return FEMH_BAD_ARG;
+  if (buf && len) {
+    std::strncpy(buf, ctx->err.c_str(), len - 1);
+    buf[len - 1] = '\0';
+  }
+  return ctx->last_status;
+}
+
+}  // extern "C"

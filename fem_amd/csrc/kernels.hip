@@ -1,0 +1,849 @@
+// kernels.hip -- gfx950 kernels of the voxel homogenization engine.
+//
+// Every per-node kernel runs a grid-stride loop over the V nodes of one
+// right-hand side (blockIdx.y selects the RHS, skipped when its mask bit is
+// clear).  Block size is 256 (four wave64s).  FP64 reductions are
+// deterministic: per-block partials + last-block-done summation in a fixed
+// order (see Reduce in kernels.hpp).
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace femh {
+
+// ------------------------------------------------------------------------
+// reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum over a 256-thread block; result valid in every thread.
+__device__ double block_sum(double v) {
+  __shared__ double sh[kBlock / 64];
+  __shared__ double res;
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) res = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return res;
+}
+
+// Publish a block partial for (slot, idx); the last block sums all partials
+// in index order and stores the scalar.  All threads must call.
+__device__ void reduce_finish(const Reduce &red, int slot, int idx, double bsum) {
+  __shared__ int am_last;
+  const size_t base = ((size_t)slot * kSlotWidth + idx) * kMaxPartials;
+  unsigned *ctr = red.counters + slot * kSlotWidth + idx;
+  if (threadIdx.x == 0) {
+    red.partials[base + blockIdx.x] = bsum;
+    __threadfence();
+    unsigned t = atomicAdd(ctr, 1u);
+    am_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (am_last) {
+    __threadfence();
+    const volatile double *pp = red.partials + base;
+    double s = 0;
+    // fixed order: thread t sums t, t+256, ... then the block tree
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) s += pp[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+      red.scalars[slot * kSlotWidth + idx] = s;
+      *ctr = 0u;
+    }
+  }
+}
+
+__device__ __forceinline__ double scal(const Reduce &red, int slot, int idx) {
+  return red.scalars[slot * kSlotWidth + idx];
+}
+
+static inline int grid_for(int V) {
+  int b = (V + kBlock - 1) / kBlock;
+  return b < kMaxPartials ? (b < 1 ? 1 : b) : kMaxPartials;
+}
+
+__device__ __forceinline__ void node_ijk(const Dims &g, int n, int &i, int &j, int &k) {
+  k = n % g.nz;
+  int t = n / g.nz;
+  j = t % g.ny;
+  i = t / g.ny;
+}
+
+// ------------------------------------------------------------------------
+// fine level: matrix-free periodic K u, node gather over the 8 incident voxels
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fine_apply(FineOp<T> op, const T *__restrict__ u,
+                                                       T *__restrict__ y, const T *__restrict__ b,
+                                                       int mode, unsigned mask, Reduce red, int slot,
+                                                       int dot) {
+  const int r = blockIdx.y;
+  if (!((mask >> r) & 1u)) return;
+  const Dims g = op.g;
+  const size_t off = (size_t)r * 3 * g.V;
+  const T *__restrict__ ur = u + off;
+  const T *__restrict__ Ke = op.Ke;
+  double dv = 0;
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    const int I[3] = {i == 0 ? g.nx - 1 : i - 1, i, i == g.nx - 1 ? 0 : i + 1};
+    const int J[3] = {j == 0 ? g.ny - 1 : j - 1, j, j == g.ny - 1 ? 0 : j + 1};
+    const int K[3] = {k == 0 ? g.nz - 1 : k - 1, k, k == g.nz - 1 ? 0 : k + 1};
+    T acc0 = 0, acc1 = 0, acc2 = 0;
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const int ax = (a >> 2) & 1, ay = (a >> 1) & 1, az = a & 1;
+      const T Ee = op.E[(I[1 - ax] * g.ny + J[1 - ay]) * g.nz + K[1 - az]];
+      T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+      for (int bb = 0; bb < 8; bb++) {
+        const int bx = (bb >> 2) & 1, by = (bb >> 1) & 1, bz = bb & 1;
+        const int nb = (I[1 + bx - ax] * g.ny + J[1 + by - ay]) * g.nz + K[1 + bz - az];
+        const T u0 = ur[nb], u1 = ur[g.V + nb], u2 = ur[2 * g.V + nb];
+        const T *row = Ke + (3 * a) * 24 + 3 * bb;
+        s0 += row[0] * u0 + row[1] * u1 + row[2] * u2;
+        s1 += row[24] * u0 + row[25] * u1 + row[26] * u2;
+        s2 += row[48] * u0 + row[49] * u1 + row[50] * u2;
+      }
+      acc0 += Ee * s0;
+      acc1 += Ee * s1;
+      acc2 += Ee * s2;
+    }
+    T *yr = y + off;
+    if (mode == 1) {
+      const T *br = b + off;
+      acc0 = br[n] - acc0;
+      acc1 = br[g.V + n] - acc1;
+      acc2 = br[2 * g.V + n] - acc2;
+    }
+    yr[n] = acc0;
+    yr[g.V + n] = acc1;
+    yr[2 * g.V + n] = acc2;
+    if (dot)
+      dv += (double)ur[n] * (double)acc0 + (double)ur[g.V + n] * (double)acc1 +
+            (double)ur[2 * g.V + n] * (double)acc2;
+  }
+  if (dot) {
+    double s = block_sum(dv);
+    reduce_finish(red, slot, r, s);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fine_diag(FineOp<T> op, T *__restrict__ d) {
+  const Dims g = op.g;
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    double acc[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const int ei = (i - ((a >> 2) & 1) + g.nx) % g.nx, ej = (j - ((a >> 1) & 1) + g.ny) % g.ny,
+                ek = (k - (a & 1) + g.nz) % g.nz;
+      const double Ee = (double)op.E[(ei * g.ny + ej) * g.nz + ek];
+#pragma unroll
+      for (int c = 0; c < 3; c++) acc[c] += Ee * (double)op.Ke[(3 * a + c) * 25];
+    }
+    d[n] = (T)acc[0];
+    d[g.V + n] = (T)acc[1];
+    d[2 * g.V + n] = (T)acc[2];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fine_rhs(FineOp<T> op, const double *__restrict__ gtab,
+                                                     T *__restrict__ b) {
+  const Dims g = op.g;
+  const int lc = blockIdx.y;
+  T *br = b + (size_t)lc * 3 * g.V;
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    double acc[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const int ei = (i - ((a >> 2) & 1) + g.nx) % g.nx, ej = (j - ((a >> 1) & 1) + g.ny) % g.ny,
+                ek = (k - (a & 1) + g.nz) % g.nz;
+      const double Ee = (double)op.E[(ei * g.ny + ej) * g.nz + ek];
+#pragma unroll
+      for (int c = 0; c < 3; c++) acc[c] -= Ee * gtab[lc * 24 + 3 * a + c];
+    }
+    br[n] = (T)acc[0];
+    br[g.V + n] = (T)acc[1];
+    br[2 * g.V + n] = (T)acc[2];
+  }
+}
+
+// ------------------------------------------------------------------------
+// coarse levels: 27-point block stencils
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_stencil_apply(StencilOp<T> op, const T *__restrict__ u,
+                                                          T *__restrict__ y,
+                                                          const T *__restrict__ b, int mode,
+                                                          unsigned mask) {
+  const int r = blockIdx.y;
+  if (!((mask >> r) & 1u)) return;
+  const Dims g = op.g;
+  const size_t off = (size_t)r * 3 * g.V;
+  const T *__restrict__ ur = u + off;
+  const T *__restrict__ S = op.S;
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    const int I[3] = {i == 0 ? g.nx - 1 : i - 1, i, i == g.nx - 1 ? 0 : i + 1};
+    const int J[3] = {j == 0 ? g.ny - 1 : j - 1, j, j == g.ny - 1 ? 0 : j + 1};
+    const int K[3] = {k == 0 ? g.nz - 1 : k - 1, k, k == g.nz - 1 ? 0 : k + 1};
+    T acc0 = 0, acc1 = 0, acc2 = 0;
+#pragma unroll 3
+    for (int o = 0; o < 27; o++) {
+      const int nb = (I[o / 9] * g.ny + J[(o / 3) % 3]) * g.nz + K[o % 3];
+      const T u0 = ur[nb], u1 = ur[g.V + nb], u2 = ur[2 * g.V + nb];
+      const T *s = S + (size_t)(o * 9) * g.V + n;
+      acc0 += s[0] * u0 + s[(size_t)1 * g.V] * u1 + s[(size_t)2 * g.V] * u2;
+      acc1 += s[(size_t)3 * g.V] * u0 + s[(size_t)4 * g.V] * u1 + s[(size_t)5 * g.V] * u2;
+      acc2 += s[(size_t)6 * g.V] * u0 + s[(size_t)7 * g.V] * u1 + s[(size_t)8 * g.V] * u2;
+    }
+    T *yr = y + off;
+    if (mode == 1) {
+      const T *br = b + off;
+      acc0 = br[n] - acc0;
+      acc1 = br[g.V + n] - acc1;
+      acc2 = br[2 * g.V + n] - acc2;
+    }
+    yr[n] = acc0;
+    yr[g.V + n] = acc1;
+    yr[2 * g.V + n] = acc2;
+  }
+}
+
+// Level-1 Galerkin stencil straight from the fine moduli: the coarse element
+// E_c (2x2x2 voxels) has matrix sum_f E_f M_f; thread per (coarse node, offset).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_build_level1(Dims fg, Dims cg, const T *__restrict__ E,
+                                                         const double *__restrict__ M,
+                                                         T *__restrict__ S) {
+  const int o = blockIdx.y;
+  const int di = o / 9 - 1, dj = (o / 3) % 3 - 1, dk = o % 3 - 1;
+  for (int N = blockIdx.x * kBlock + threadIdx.x; N < cg.V; N += gridDim.x * kBlock) {
+    int I, J, K;
+    node_ijk(cg, N, I, J, K);
+    double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int A = 0; A < 8; A++) {
+      const int ax = (A >> 2) & 1, ay = (A >> 1) & 1, az = A & 1;
+      const int bx = ax + di, by = ay + dj, bz = az + dk;
+      if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+      const int B = 4 * bx + 2 * by + bz;
+      const int ei = (I - ax + cg.nx) % cg.nx, ej = (J - ay + cg.ny) % cg.ny,
+                ek = (K - az + cg.nz) % cg.nz;
+      for (int f = 0; f < 8; f++) {
+        const int fi = 2 * ei + ((f >> 2) & 1), fj = 2 * ej + ((f >> 1) & 1), fk = 2 * ek + (f & 1);
+        const double Ef = (double)E[(fi * fg.ny + fj) * fg.nz + fk];
+        const double *m = M + f * 576 + (3 * A) * 24 + 3 * B;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+          for (int d = 0; d < 3; d++) blk[3 * c + d] += Ef * m[c * 24 + d];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) S[(size_t)(o * 9 + q) * cg.V + N] = (T)blk[q];
+  }
+}
+
+// Galerkin triple product on stencils: A_c[N, N+D] = sum P[n,N] A_f[n,m] P[m,N+D]
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_galerkin(Dims fg, Dims cg, const T *__restrict__ Sf,
+                                                     T *__restrict__ Sc) {
+  const int o = blockIdx.y;
+  const int D[3] = {o / 9 - 1, (o / 3) % 3 - 1, o % 3 - 1};
+  for (int N = blockIdx.x * kBlock + threadIdx.x; N < cg.V; N += gridDim.x * kBlock) {
+    int I, J, K;
+    node_ijk(cg, N, I, J, K);
+    double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int on = 0; on < 27; on++) {
+      const int oi = on / 9 - 1, oj = (on / 3) % 3 - 1, ok = on % 3 - 1;
+      const double wn = (oi ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
+      const int fi = (2 * I + oi + fg.nx) % fg.nx, fj = (2 * J + oj + fg.ny) % fg.ny,
+                fk = (2 * K + ok + fg.nz) % fg.nz;
+      const int n = (fi * fg.ny + fj) * fg.nz + fk;
+      for (int dm = 0; dm < 27; dm++) {
+        const int t[3] = {oi + dm / 9 - 1, oj + (dm / 3) % 3 - 1, ok + dm % 3 - 1};
+        double wm = 1;
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+          const int ta = t[ax];
+          double wa;
+          if ((ta & 1) == 0) wa = (ta / 2 == D[ax]) ? 1.0 : 0.0;
+          else wa = (D[ax] == (ta - 1) / 2 || D[ax] == (ta + 1) / 2) ? 0.5 : 0.0;
+          wm *= wa;
+        }
+        if (wm == 0.0) continue;
+        const double w = wn * wm;
+        const T *s = Sf + (size_t)(dm * 9) * fg.V + n;
+#pragma unroll
+        for (int q = 0; q < 9; q++) blk[q] += w * (double)s[(size_t)q * fg.V];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) Sc[(size_t)(o * 9 + q) * cg.V + N] = (T)blk[q];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_stencil_diag(StencilOp<T> op, T *__restrict__ d) {
+  const Dims g = op.g;
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) d[(size_t)c * g.V + n] = op.S[(size_t)(13 * 9 + 4 * c) * g.V + n];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict(Dims fg, Dims cg, const T *__restrict__ rf,
+                                                     T *__restrict__ rc, unsigned mask) {
+  const int r = blockIdx.y;
+  if (!((mask >> r) & 1u)) return;
+  const T *f = rf + (size_t)r * 3 * fg.V;
+  T *c = rc + (size_t)r * 3 * cg.V;
+  for (int N = blockIdx.x * kBlock + threadIdx.x; N < cg.V; N += gridDim.x * kBlock) {
+    int I, J, K;
+    node_ijk(cg, N, I, J, K);
+    double acc[3] = {0, 0, 0};
+    for (int on = 0; on < 27; on++) {
+      const int oi = on / 9 - 1, oj = (on / 3) % 3 - 1, ok = on % 3 - 1;
+      const double w = (oi ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
+      const int n = (((2 * I + oi + fg.nx) % fg.nx) * fg.ny + (2 * J + oj + fg.ny) % fg.ny) * fg.nz +
+                    (2 * K + ok + fg.nz) % fg.nz;
+#pragma unroll
+      for (int q = 0; q < 3; q++) acc[q] += w * (double)f[(size_t)q * fg.V + n];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) c[(size_t)q * cg.V + N] = (T)acc[q];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_prolong_add(Dims fg, Dims cg, const T *__restrict__ ec,
+                                                        T *__restrict__ ef, unsigned mask) {
+  const int r = blockIdx.y;
+  if (!((mask >> r) & 1u)) return;
+  const T *c = ec + (size_t)r * 3 * cg.V;
+  T *f = ef + (size_t)r * 3 * fg.V;
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < fg.V; n += gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(fg, n, i, j, k);
+    const int ci0 = i >> 1, ci1 = ((i + 1) >> 1) % cg.nx;
+    const int cj0 = j >> 1, cj1 = ((j + 1) >> 1) % cg.ny;
+    const int ck0 = k >> 1, ck1 = ((k + 1) >> 1) % cg.nz;
+    const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+    const double w = ((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((k & 1) ? 0.5 : 1.0);
+    double acc[3] = {0, 0, 0};
+    for (int a = 0; a < ni; a++)
+      for (int b2 = 0; b2 < nj; b2++)
+        for (int d = 0; d < nk; d++) {
+          const int N = ((a ? ci1 : ci0) * cg.ny + (b2 ? cj1 : cj0)) * cg.nz + (d ? ck1 : ck0);
+#pragma unroll
+          for (int q = 0; q < 3; q++) acc[q] += (double)c[(size_t)q * cg.V + N];
+        }
+#pragma unroll
+    for (int q = 0; q < 3; q++) f[(size_t)q * fg.V + n] += (T)(w * acc[q]);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dense_solve(int n, const T *__restrict__ Ainv,
+                                                        const T *__restrict__ b, T *__restrict__ x,
+                                                        unsigned mask) {
+  const int r = blockIdx.y;
+  if (!((mask >> r) & 1u)) return;
+  const T *br = b + (size_t)r * n;
+  T *xr = x + (size_t)r * n;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double acc = 0;
+    const T *row = Ainv + (size_t)i * n;
+    for (int j = 0; j < n; j++) acc += (double)row[j] * (double)br[j];
+    xr[i] = (T)acc;
+  }
+}
+
+// ------------------------------------------------------------------------
+// vector kernels
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cheb_first(int V, const T *__restrict__ r,
+                                                       const T *__restrict__ D, T *__restrict__ d,
+                                                       T *__restrict__ x, double inv_theta,
+                                                       int x_zero, unsigned mask) {
+  const int rr = blockIdx.y;
+  if (!((mask >> rr) & 1u)) return;
+  const size_t off = (size_t)rr * 3 * V;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) {
+    const T dv = (T)((double)r[off + t] * inv_theta / (double)D[t]);
+    d[off + t] = dv;
+    x[off + t] = x_zero ? dv : x[off + t] + dv;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cheb_next(int V, const T *rsrc, T *r,
+                                                      const T *__restrict__ Ad,
+                                                      const T *__restrict__ D, T *__restrict__ d,
+                                                      T *__restrict__ x, double c1, double c2,
+                                                      unsigned mask) {
+  const int rr = blockIdx.y;
+  if (!((mask >> rr) & 1u)) return;
+  const size_t off = (size_t)rr * 3 * V;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) {
+    const T rv = rsrc[off + t] - Ad[off + t];
+    r[off + t] = rv;
+    const T dv = (T)(c1 * (double)d[off + t] + c2 * (double)rv / (double)D[t]);
+    d[off + t] = dv;
+    x[off + t] += dv;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int V, T *__restrict__ x, T *__restrict__ r,
+                                                       const T *__restrict__ p,
+                                                       const T *__restrict__ q,
+                                                       const T *__restrict__ D, T *__restrict__ z,
+                                                       int rzslot, int rz_out, unsigned mask,
+                                                       Reduce red) {
+  const int rr = blockIdx.y;
+  if (!((mask >> rr) & 1u)) return;
+  const double pq = scal(red, S_PQ, rr);
+  const double alpha = pq != 0.0 ? scal(red, rzslot, rr) / pq : 0.0;
+  const T a = (T)alpha;
+  const size_t off = (size_t)rr * 3 * V;
+  double srr = 0, srz = 0;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) {
+    x[off + t] += a * p[off + t];
+    const T rv = r[off + t] - a * q[off + t];
+    r[off + t] = rv;
+    srr += (double)rv * (double)rv;
+    if (D) {
+      const T zv = rv / D[t];
+      z[off + t] = zv;
+      srz += (double)rv * (double)zv;
+    }
+  }
+  double s = block_sum(srr);
+  reduce_finish(red, S_RR, rr, s);
+  if (D) {
+    s = block_sum(srz);
+    reduce_finish(red, rz_out, rr, s);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_p_update(int V, T *__restrict__ p,
+                                                     const T *__restrict__ z, int rz_new,
+                                                     int rz_old, int first, unsigned mask,
+                                                     Reduce red) {
+  const int rr = blockIdx.y;
+  if (!((mask >> rr) & 1u)) return;
+  double beta = 0;
+  if (!first) {
+    const double den = scal(red, rz_old, rr);
+    beta = den != 0.0 ? scal(red, rz_new, rr) / den : 0.0;
+  }
+  const T bt = (T)beta;
+  const size_t off = (size_t)rr * 3 * V;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
+    p[off + t] = first ? z[off + t] : z[off + t] + bt * p[off + t];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot(int V, const T *__restrict__ a,
+                                                const T *__restrict__ b, int slot, unsigned mask,
+                                                Reduce red) {
+  const int rr = blockIdx.y;
+  if (!((mask >> rr) & 1u)) return;
+  const size_t off = (size_t)rr * 3 * V;
+  double s = 0;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
+    s += (double)a[off + t] * (double)b[off + t];
+  s = block_sum(s);
+  reduce_finish(red, slot, rr, s);
+}
+
+template <typename Ts, typename Td>
+__global__ __launch_bounds__(kBlock) void k_convert(size_t n, const Ts *__restrict__ src,
+                                                    Td *__restrict__ dst) {
+  for (size_t t = blockIdx.x * (size_t)kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock)
+    dst[t] = (Td)src[t];
+}
+
+__global__ __launch_bounds__(kBlock) void k_add_f32_to_f64(int V, const float *__restrict__ d,
+                                                           double *__restrict__ x, unsigned mask) {
+  const int rr = blockIdx.y;
+  if (!((mask >> rr) & 1u)) return;
+  const size_t off = (size_t)rr * 3 * V;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
+    x[off + t] += (double)d[off + t];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mean_sum(int V, const T *__restrict__ x, unsigned mask,
+                                                     Reduce red) {
+  const int rr = blockIdx.y / 3, c = blockIdx.y % 3;
+  if (!((mask >> rr) & 1u)) return;
+  const T *xc = x + (size_t)rr * 3 * V + (size_t)c * V;
+  double s = 0;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock) s += (double)xc[t];
+  s = block_sum(s);
+  reduce_finish(red, S_AUX, blockIdx.y, s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mean_sub(int V, T *__restrict__ x, unsigned mask,
+                                                     Reduce red) {
+  const int rr = blockIdx.y / 3, c = blockIdx.y % 3;
+  if (!((mask >> rr) & 1u)) return;
+  const T m = (T)(scal(red, S_AUX, blockIdx.y) / (double)V);
+  T *xc = x + (size_t)rr * 3 * V + (size_t)c * V;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock) xc[t] -= m;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill_hash(size_t n, T *__restrict__ x,
+                                                      unsigned long long seed) {
+  for (size_t t = blockIdx.x * (size_t)kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock) {
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (t + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    x[t] = (T)((double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale_dinv_norm(int V, T *__restrict__ y,
+                                                            const T *__restrict__ D, Reduce red) {
+  double s = 0;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) {
+    const T v = y[t] / D[t];
+    y[t] = v;
+    s += (double)v * (double)v;
+  }
+  s = block_sum(s);
+  reduce_finish(red, S_AUX, 0, s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_normalize(int V, const T *__restrict__ y,
+                                                      T *__restrict__ x, Reduce red) {
+  const double nrm = sqrt(scal(red, S_AUX, 0));
+  const double inv = nrm > 0 ? 1.0 / nrm : 0.0;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
+    x[t] = (T)((double)y[t] * inv);
+}
+
+// ------------------------------------------------------------------------
+// material, C_H, adjoint
+__device__ __forceinline__ double interp_E(double rho, int interp, double E0, double Emin, double p) {
+  if (interp == 0) return Emin + (E0 - Emin) * pow(rho, p);
+  if (interp == 1) return E0 * rho / (3.0 - 2.0 * rho);
+  return rho;
+}
+
+__device__ __forceinline__ double interp_dE(double rho, int interp, double E0, double Emin,
+                                            double p) {
+  if (interp == 0) return (E0 - Emin) * p * pow(rho, p - 1.0);
+  if (interp == 1) {
+    const double den = 3.0 - 2.0 * rho;
+    return E0 * 3.0 / (den * den);
+  }
+  return 1.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_material(int V, const double *__restrict__ rho,
+                                                     double *__restrict__ E, int interp, double E0,
+                                                     double Emin, double p) {
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock)
+    E[t] = interp_E(rho[t], interp, E0, Emin, p);
+}
+
+// chi_i of voxel (i,j,k) for load case lc: affine local field + gathered fluctuation
+__device__ __forceinline__ void gather_chi(const Dims &g, const int I[2], const int J[2],
+                                           const int K[2], const double *__restrict__ u,
+                                           const double *__restrict__ uaff, int lc, double *chi) {
+  const double *ul = u + (size_t)lc * 3 * g.V;
+#pragma unroll
+  for (int v = 0; v < 8; v++) {
+    const int n = (I[(v >> 2) & 1] * g.ny + J[(v >> 1) & 1]) * g.nz + K[v & 1];
+#pragma unroll
+    for (int c = 0; c < 3; c++) chi[3 * v + c] = uaff[lc * 24 + 3 * v + c] + ul[(size_t)c * g.V + n];
+  }
+}
+
+// W_p(e) = chi_i^T K0 chi_j for the 21 pairs i<=j (not divided by V)
+__device__ void element_energy(const Dims &g, int e, const double *__restrict__ Ke,
+                               const double *__restrict__ uaff, const double *__restrict__ u,
+                               double *W) {
+  int i, j, k;
+  node_ijk(g, e, i, j, k);
+  const int I[2] = {i, i == g.nx - 1 ? 0 : i + 1};
+  const int J[2] = {j, j == g.ny - 1 ? 0 : j + 1};
+  const int K[2] = {k, k == g.nz - 1 ? 0 : k + 1};
+  int p = 0;
+  for (int a = 0; a < 6; a++) {
+    double chi[24], w[24];
+    gather_chi(g, I, J, K, u, uaff, a, chi);
+#pragma unroll
+    for (int r = 0; r < 24; r++) {
+      double s = 0;
+#pragma unroll
+      for (int q = 0; q < 24; q++) s += Ke[r * 24 + q] * chi[q];
+      w[r] = s;
+    }
+    for (int b = a; b < 6; b++) {
+      double chib[24];
+      if (b == a) {
+#pragma unroll
+        for (int q = 0; q < 24; q++) chib[q] = chi[q];
+      } else {
+        gather_chi(g, I, J, K, u, uaff, b, chib);
+      }
+      double s = 0;
+#pragma unroll
+      for (int q = 0; q < 24; q++) s += w[q] * chib[q];
+      W[p++] = s;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ch_energy(Dims g, const double *__restrict__ E,
+                                                      const double *__restrict__ Ke,
+                                                      const double *__restrict__ uaff,
+                                                      const double *__restrict__ u,
+                                                      double *__restrict__ partials) {
+  double acc[21];
+#pragma unroll
+  for (int p = 0; p < 21; p++) acc[p] = 0;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < g.V; e += gridDim.x * kBlock) {
+    double W[21];
+    element_energy(g, e, Ke, uaff, u, W);
+    const double Ee = E[e];
+#pragma unroll
+    for (int p = 0; p < 21; p++) acc[p] += Ee * W[p];
+  }
+  for (int p = 0; p < 21; p++) {
+    double s = block_sum(acc[p]);
+    if (threadIdx.x == 0) partials[(size_t)p * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restrict__ partials,
+                                                         int nblk, double *__restrict__ out) {
+  const int p = blockIdx.x;
+  double s = 0;
+  for (int t = threadIdx.x; t < nblk; t += kBlock) s += partials[(size_t)p * nblk + t];
+  s = block_sum(s);
+  if (threadIdx.x == 0) out[p] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gradient(Dims g, const double *__restrict__ rho,
+                                                     const double *__restrict__ Ke,
+                                                     const double *__restrict__ uaff,
+                                                     const double *__restrict__ u,
+                                                     const double *__restrict__ Gs, int interp,
+                                                     double E0, double Emin, double pexp,
+                                                     int energy_only, double *__restrict__ out) {
+  const double invV = 1.0 / (double)g.V;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < g.V; e += gridDim.x * kBlock) {
+    double W[21];
+    element_energy(g, e, Ke, uaff, u, W);
+    if (energy_only) {
+#pragma unroll
+      for (int p = 0; p < 21; p++) out[(size_t)e * 21 + p] = W[p] * invV;
+    } else {
+      double s = 0;
+#pragma unroll
+      for (int p = 0; p < 21; p++) s += Gs[p] * W[p];
+      out[e] = s * invV * interp_dE(rho[e], interp, E0, Emin, pexp);
+    }
+  }
+}
+
+__global__ void k_reduce_init(unsigned *ctr, double *scal, int n) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    ctr[t] = 0u;
+    scal[t] = 0.0;
+  }
+}
+
+// ------------------------------------------------------------------------
+// launch wrappers
+static inline dim3 grid2(int V, int y) { return dim3(grid_for(V), y); }
+
+template <typename T>
+void launch_fine_apply(const FineOp<T> &op, const T *u, T *y, const T *b, int mode, unsigned mask,
+                       const Reduce *red, int slot, hipStream_t s) {
+  Reduce rr = red ? *red : Reduce{nullptr, nullptr, nullptr};
+  hipLaunchKernelGGL(k_fine_apply<T>, grid2(op.g.V, kRHS), dim3(kBlock), 0, s, op, u, y, b, mode,
+                     mask, rr, slot, red ? 1 : 0);
+}
+template <typename T>
+void launch_fine_diag(const FineOp<T> &op, T *d, hipStream_t s) {
+  hipLaunchKernelGGL(k_fine_diag<T>, dim3(grid_for(op.g.V)), dim3(kBlock), 0, s, op, d);
+}
+template <typename T>
+void launch_fine_rhs(const FineOp<T> &op, const double *g, T *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_fine_rhs<T>, grid2(op.g.V, kRHS), dim3(kBlock), 0, s, op, g, b);
+}
+template <typename T>
+void launch_stencil_apply(const StencilOp<T> &op, const T *u, T *y, const T *b, int mode,
+                          unsigned mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_stencil_apply<T>, grid2(op.g.V, kRHS), dim3(kBlock), 0, s, op, u, y, b,
+                     mode, mask);
+}
+template <typename T>
+void launch_build_level1(Dims fine, Dims coarse, const T *E, const double *M, T *S, hipStream_t s) {
+  hipLaunchKernelGGL(k_build_level1<T>, grid2(coarse.V, 27), dim3(kBlock), 0, s, fine, coarse, E,
+                     M, S);
+}
+template <typename T>
+void launch_galerkin(Dims fine, Dims coarse, const T *Sf, T *Sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_galerkin<T>, grid2(coarse.V, 27), dim3(kBlock), 0, s, fine, coarse, Sf, Sc);
+}
+template <typename T>
+void launch_stencil_diag(const StencilOp<T> &op, T *d, hipStream_t s) {
+  hipLaunchKernelGGL(k_stencil_diag<T>, dim3(grid_for(op.g.V)), dim3(kBlock), 0, s, op, d);
+}
+template <typename T>
+void launch_restrict(Dims fine, Dims coarse, const T *rf, T *rc, unsigned mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_restrict<T>, grid2(coarse.V, kRHS), dim3(kBlock), 0, s, fine, coarse, rf,
+                     rc, mask);
+}
+template <typename T>
+void launch_prolong_add(Dims fine, Dims coarse, const T *ec, T *ef, unsigned mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_prolong_add<T>, grid2(fine.V, kRHS), dim3(kBlock), 0, s, fine, coarse, ec,
+                     ef, mask);
+}
+template <typename T>
+void launch_dense_solve(int n, const T *Ainv, const T *b, T *x, unsigned mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_dense_solve<T>, grid2(n, kRHS), dim3(kBlock), 0, s, n, Ainv, b, x, mask);
+}
+template <typename T>
+void launch_cheb_first(int V, const T *r, const T *D, T *d, T *x, double inv_theta, int x_zero,
+                       unsigned mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_cheb_first<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, r, D, d, x,
+                     inv_theta, x_zero, mask);
+}
+template <typename T>
+void launch_cheb_next(int V, const T *rsrc, T *r, const T *Ad, const T *D, T *d, T *x, double c1,
+                      double c2, unsigned mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_cheb_next<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, rsrc, r, Ad, D, d,
+                     x, c1, c2, mask);
+}
+template <typename T>
+void launch_pcg_update(int V, T *x, T *r, const T *p, const T *q, const T *D, T *z, int rzslot,
+                       int rz_out, unsigned mask, const Reduce &red, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_update<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, x, r, p, q, D, z,
+                     rzslot, rz_out, mask, red);
+}
+template <typename T>
+void launch_p_update(int V, T *p, const T *z, int rz_new, int rz_old, int first, unsigned mask,
+                     const Reduce &red, hipStream_t s) {
+  hipLaunchKernelGGL(k_p_update<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, p, z, rz_new,
+                     rz_old, first, mask, red);
+}
+template <typename T>
+void launch_dot(int V, const T *a, const T *b, int slot, unsigned mask, const Reduce &red,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_dot<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, a, b, slot, mask, red);
+}
+template <typename Ts, typename Td>
+void launch_convert(size_t n, const Ts *src, Td *dst, hipStream_t s) {
+  size_t b = (n + kBlock - 1) / kBlock;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  hipLaunchKernelGGL((k_convert<Ts, Td>), dim3((unsigned)b), dim3(kBlock), 0, s, n, src, dst);
+}
+void launch_add_f32_to_f64(size_t, const float *d, double *x, unsigned mask, int V, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_f32_to_f64, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, d, x, mask);
+}
+template <typename T>
+void launch_remove_mean(int V, T *x, unsigned mask, const Reduce &red, hipStream_t s) {
+  hipLaunchKernelGGL(k_mean_sum<T>, grid2(V, 3 * kRHS), dim3(kBlock), 0, s, V, x, mask, red);
+  hipLaunchKernelGGL(k_mean_sub<T>, grid2(V, 3 * kRHS), dim3(kBlock), 0, s, V, x, mask, red);
+}
+template <typename T>
+void launch_fill_hash(size_t n, T *x, unsigned long long seed, hipStream_t s) {
+  size_t b = (n + kBlock - 1) / kBlock;
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(k_fill_hash<T>, dim3((unsigned)b), dim3(kBlock), 0, s, n, x, seed);
+}
+template <typename T>
+void launch_scale_by_dinv_norm(int V, T *y, const T *D, const Reduce &red, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_dinv_norm<T>, dim3(grid_for(3 * V)), dim3(kBlock), 0, s, V, y, D, red);
+}
+template <typename T>
+void launch_normalize(int V, const T *y, T *x, const Reduce &red, hipStream_t s) {
+  hipLaunchKernelGGL(k_normalize<T>, dim3(grid_for(3 * V)), dim3(kBlock), 0, s, V, y, x, red);
+}
+void launch_material(int V, const double *rho, double *E, int interp, double E0, double Emin,
+                     double p, hipStream_t s) {
+  hipLaunchKernelGGL(k_material, dim3(grid_for(V)), dim3(kBlock), 0, s, V, rho, E, interp, E0,
+                     Emin, p);
+}
+void launch_ch_energy(Dims g, const double *E, const double *Ke, const double *uaff, const double *u,
+                      double *partials, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL(k_ch_energy, dim3(nblk), dim3(kBlock), 0, s, g, E, Ke, uaff, u, partials);
+}
+void launch_sum_partials(const double *partials, int nsum, int nblk, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_partials, dim3(nsum), dim3(kBlock), 0, s, partials, nblk, out);
+}
+void launch_gradient(Dims g, const double *rho, const double *Ke, const double *uaff,
+                     const double *u, const double *Gs, int interp, double E0, double Emin,
+                     double p, int energy_only, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gradient, dim3(grid_for(g.V)), dim3(kBlock), 0, s, g, rho, Ke, uaff, u, Gs,
+                     interp, E0, Emin, p, energy_only, out);
+}
+void reduce_init(Reduce &red, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_init, dim3(1), dim3(kBlock), 0, s, red.counters, red.scalars,
+                     kNumSlots * kSlotWidth);
+}
+
+#define FEMH_INST(T)                                                                             \
+  template void launch_fine_apply<T>(const FineOp<T> &, const T *, T *, const T *, int, unsigned, \
+                                     const Reduce *, int, hipStream_t);                           \
+  template void launch_fine_diag<T>(const FineOp<T> &, T *, hipStream_t);                         \
+  template void launch_fine_rhs<T>(const FineOp<T> &, const double *, T *, hipStream_t);          \
+  template void launch_stencil_apply<T>(const StencilOp<T> &, const T *, T *, const T *, int,     \
+                                        unsigned, hipStream_t);                                   \
+  template void launch_build_level1<T>(Dims, Dims, const T *, const double *, T *, hipStream_t);  \
+  template void launch_galerkin<T>(Dims, Dims, const T *, T *, hipStream_t);                      \
+  template void launch_stencil_diag<T>(const StencilOp<T> &, T *, hipStream_t);                   \
+  template void launch_restrict<T>(Dims, Dims, const T *, T *, unsigned, hipStream_t);            \
+  template void launch_prolong_add<T>(Dims, Dims, const T *, T *, unsigned, hipStream_t);         \
+  template void launch_dense_solve<T>(int, const T *, const T *, T *, unsigned, hipStream_t);     \
+  template void launch_cheb_first<T>(int, const T *, const T *, T *, T *, double, int, unsigned,  \
+                                     hipStream_t);                                                \
+  template void launch_cheb_next<T>(int, const T *, T *, const T *, const T *, T *, T *, double,  \
+                                    double, unsigned, hipStream_t);                               \
+  template void launch_pcg_update<T>(int, T *, T *, const T *, const T *, const T *, T *, int,    \
+                                     int, unsigned, const Reduce &, hipStream_t);                 \
+  template void launch_p_update<T>(int, T *, const T *, int, int, int, unsigned, const Reduce &,  \
+                                   hipStream_t);                                                  \
+  template void launch_dot<T>(int, const T *, const T *, int, unsigned, const Reduce &,           \
+                              hipStream_t);                                                       \
+  template void launch_remove_mean<T>(int, T *, unsigned, const Reduce &, hipStream_t);           \
+  template void launch_fill_hash<T>(size_t, T *, unsigned long long, hipStream_t);                \
+  template void launch_scale_by_dinv_norm<T>(int, T *, const T *, const Reduce &, hipStream_t);   \
+  template void launch_normalize<T>(int, const T *, T *, const Reduce &, hipStream_t);
+
+FEMH_INST(double)
+FEMH_INST(float)
+template void launch_convert<double, float>(size_t, const double *, float *, hipStream_t);
+template void launch_convert<float, double>(size_t, const float *, double *, hipStream_t);
+template void launch_convert<double, double>(size_t, const double *, double *, hipStream_t);
+
+}  // namespace femh

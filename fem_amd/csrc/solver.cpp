@@ -1,0 +1,582 @@
+// solver.cpp -- host orchestration of the device solver.
+//
+// Method (restating desaic/fem's strain-driven periodic homogenization,
+// SamplesGenerator/NumericalCoarseningUsingMultiGrid.cpp:221-313):
+//   for each unit macro strain eps_lc (Voigt xx,yy,zz,yz,xz,xy):
+//     solve K u_lc = -sum_e E_e K0 u_aff(eps_lc)   on periodic fluctuations
+//   C_ij = 1/V sum_e E_e (u_aff,i + u_i)_e^T K0 (u_aff,j + u_j)_e
+// Solver: PCG (pcgsolver/pcg_solver.h:243-285 loop) with a Jacobi or a
+// Galerkin geometric-multigrid V(nu1,nu2) preconditioner (Chebyshev smoothing,
+// dense coarsest solve), all six load cases advanced together.
+// Mixed precision: FP32 multigrid-PCG corrections inside an FP64 iterative
+// refinement loop (FP64 residual, FP64 dot products, FP64 fields and C_H).
+#include "solver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace femh {
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    int st = (e == hipErrorOutOfMemory) ? FEMH_OOM : FEMH_DEVICE_ERROR;
+    throw Error(st, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+void DBuf::alloc(size_t n) {
+  if (n <= bytes && p) return;
+  release();
+  if (n == 0) return;
+  FEMH_HIP(hipMalloc(&p, n));
+  bytes = n;
+}
+
+void DBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+static size_t vec_bytes(const Dims &g, size_t s) { return (size_t)kRHS * 3 * g.V * s; }
+
+Context::Context(const femh_opts &opts) : o(opts) {
+  if (o.nx < 2 || o.ny < 2 || o.nz < 2) throw Error(FEMH_BAD_ARG, "grid dimensions must be >= 2");
+  if ((long long)o.nx * o.ny * o.nz > (1ll << 30)) throw Error(FEMH_BAD_ARG, "grid too large");
+  if (!(o.nu > -1.0 && o.nu < 0.5)) throw Error(FEMH_BAD_ARG, "Poisson ratio must lie in (-1, 0.5)");
+  if (!(o.E0 > 0)) throw Error(FEMH_BAD_ARG, "E0 must be positive");
+  if (o.interp < 0 || o.interp > 2) throw Error(FEMH_BAD_ARG, "unknown interpolation");
+  if (o.precond != FEMH_PRECOND_JACOBI && o.precond != FEMH_PRECOND_MG)
+    throw Error(FEMH_BAD_ARG, "unknown preconditioner");
+  if (o.precision != FEMH_FP64 && o.precision != FEMH_MIXED)
+    throw Error(FEMH_BAD_ARG, "unknown precision");
+  if (o.nu1 < 1 || o.nu2 < 1 || o.nu1 > 16 || o.nu2 > 16)
+    throw Error(FEMH_BAD_ARG, "smoothing degrees must lie in [1,16]");
+  if (!(o.cheby_ratio > 1.0)) throw Error(FEMH_BAD_ARG, "cheby_ratio must exceed 1");
+  if (!(o.tol > 0) || o.max_iter < 1) throw Error(FEMH_BAD_ARG, "bad tolerance / iteration cap");
+  if (o.check_every < 1) o.check_every = 1;
+  g = Dims{o.nx, o.ny, o.nz, o.nx * o.ny * o.nz};
+  FEMH_HIP(hipSetDevice(o.device));
+  FEMH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  own_stream = true;
+  tab = build_tables(o.nu);
+
+  rho.alloc(sizeof(double) * g.V);
+  E64.alloc(sizeof(double) * g.V);
+  Ke64.alloc(sizeof(double) * 576);
+  gtab.alloc(sizeof(double) * 6 * 24);
+  uaff.alloc(sizeof(double) * 6 * 24);
+  Mchild.alloc(sizeof(double) * 8 * 576);
+  Gs.alloc(sizeof(double) * 21);
+  FEMH_HIP(hipMemcpyAsync(Ke64.p, tab.Ke.data(), sizeof(double) * 576, hipMemcpyHostToDevice, stream));
+  FEMH_HIP(hipMemcpyAsync(gtab.p, tab.g.data(), sizeof(double) * 144, hipMemcpyHostToDevice, stream));
+  FEMH_HIP(hipMemcpyAsync(uaff.p, tab.u_aff.data(), sizeof(double) * 144, hipMemcpyHostToDevice, stream));
+  FEMH_HIP(hipMemcpyAsync(Mchild.p, tab.Mchild.data(), sizeof(double) * 8 * 576, hipMemcpyHostToDevice,
+                          stream));
+  for (DBuf *bf : {&b, &x, &r, &z, &p, &q}) bf->alloc(vec_bytes(g, sizeof(double)));
+  if (o.precision == FEMH_MIXED)
+    for (DBuf *bf : {&b32, &r32, &x32, &z32, &p32, &q32}) bf->alloc(vec_bytes(g, sizeof(float)));
+  red_part.alloc(sizeof(double) * kNumSlots * kSlotWidth * kMaxPartials);
+  red_ctr.alloc(sizeof(unsigned) * kNumSlots * kSlotWidth);
+  red_scal.alloc(sizeof(double) * kNumSlots * kSlotWidth);
+  red = Reduce{red_part.as<double>(), red_ctr.as<unsigned>(), red_scal.as<double>()};
+  reduce_init(red, stream);
+  ch_part.alloc(sizeof(double) * 21 * 1024);
+  ch_out.alloc(sizeof(double) * 21);
+  FEMH_HIP(hipGetLastError());
+  FEMH_HIP(hipStreamSynchronize(stream));
+}
+
+Context::~Context() {
+  if (stream && own_stream) (void)hipStreamDestroy(stream);
+}
+
+void Context::read_scalars(int slot, double *out, int n) {
+  FEMH_HIP(hipMemcpyAsync(out, red.scalars + slot * kSlotWidth, sizeof(double) * n,
+                          hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+}
+
+void Context::set_modulus_from_rho(const double *src, bool device_src) {
+  FEMH_HIP(hipMemcpyAsync(rho.p, src, sizeof(double) * g.V,
+                          device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
+  launch_material(g.V, rho.as<double>(), E64.as<double>(), o.interp, o.E0, o.Emin, o.penal, stream);
+  FEMH_HIP(hipGetLastError());
+}
+
+void Context::set_modulus(const double *E_host) {
+  FEMH_HIP(hipMemcpyAsync(E64.p, E_host, sizeof(double) * g.V, hipMemcpyHostToDevice, stream));
+  FEMH_HIP(hipMemcpyAsync(rho.p, E_host, sizeof(double) * g.V, hipMemcpyHostToDevice, stream));
+}
+
+// ---------------------------------------------------------------------------
+// hierarchy
+template <typename T>
+void Context::apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *bb, int mode, unsigned mask,
+                    const Reduce *rd, int slot) {
+  if (l == 0) {
+    FineOp<T> op{h.lv[0].g, h.E.template as<T>(), h.Ke.template as<T>()};
+    launch_fine_apply<T>(op, u, y, bb, mode, mask, rd, slot, stream);
+  } else {
+    StencilOp<T> op{h.lv[l].g, h.lv[l].S.template as<T>()};
+    launch_stencil_apply<T>(op, u, y, bb, mode, mask, stream);
+  }
+}
+
+template <typename T>
+void Context::build_levels(Hierarchy<T> &h) {
+  // level dimensions: halve while every dimension is even and >= 4
+  std::vector<Dims> dims{g};
+  int maxL = (o.precond == FEMH_PRECOND_MG) ? (o.levels > 0 ? o.levels : 32) : 1;
+  while ((int)dims.size() < maxL) {
+    Dims d = dims.back();
+    if ((d.nx & 1) || (d.ny & 1) || (d.nz & 1) || d.nx < 4 || d.ny < 4 || d.nz < 4) break;
+    Dims c{d.nx / 2, d.ny / 2, d.nz / 2, 0};
+    c.V = c.nx * c.ny * c.nz;
+    dims.push_back(c);
+  }
+  h.lv.resize(dims.size());
+  for (size_t l = 0; l < dims.size(); l++) {
+    Level<T> &L = h.lv[l];
+    L.g = dims[l];
+    L.D.alloc(sizeof(T) * 3 * L.g.V);
+    for (DBuf *bf : {&L.x, &L.b, &L.r, &L.d, &L.Ad}) bf->alloc(vec_bytes(L.g, sizeof(T)));
+    if (l > 0) L.S.alloc(sizeof(T) * 243 * (size_t)L.g.V);
+  }
+  h.E.alloc(sizeof(T) * g.V);
+  h.Ke.alloc(sizeof(T) * 576);
+}
+
+void Context::estimate_lmax(Hierarchy<double> &h) {
+  for (size_t l = 0; l < h.lv.size(); l++) {
+    Level<double> &L = h.lv[l];
+    const int V = L.g.V;
+    double *xv = L.x.as<double>(), *yv = L.Ad.as<double>();
+    launch_fill_hash<double>((size_t)3 * V, xv, 0x51ED27ull + l, stream);
+    for (int it = 0; it < 12; it++) {
+      apply<double>(h, (int)l, xv, yv, nullptr, 0, 1u, nullptr, 0);
+      launch_scale_by_dinv_norm<double>(V, yv, L.D.as<double>(), red, stream);
+      launch_normalize<double>(V, yv, xv, red, stream);
+    }
+    FEMH_HIP(hipGetLastError());
+    double n2;
+    read_scalars(S_AUX, &n2, 1);
+    L.lmax = 1.1 * std::sqrt(std::max(n2, 0.0));
+    if (!(L.lmax > 0) || !std::isfinite(L.lmax)) throw Error(FEMH_NONFINITE, "operator is not finite");
+  }
+}
+
+// Dense inverse of the regularised coarsest operator A + s Z Z^T (Z = the
+// three constant displacement fields): for right-hand sides orthogonal to Z it
+// returns the zero-mean solution of A x = b.
+void Context::dense_coarse(Hierarchy<double> &h) {
+  Level<double> &L = h.lv.back();
+  const int V = L.g.V, n = 3 * V;
+  h.ndense = 0;
+  if (n > 648) return;
+  std::vector<double> A((size_t)n * n, 0.0), Sh;
+  if (h.lv.size() == 1) {
+    // single level: assemble by applying the fine operator to unit vectors
+    std::vector<double> e((size_t)n * kRHS, 0.0), col((size_t)n * kRHS);
+    DBuf de, dc;
+    de.alloc(sizeof(double) * kRHS * n);
+    dc.alloc(sizeof(double) * kRHS * n);
+    for (int t0 = 0; t0 < n; t0 += kRHS) {
+      std::fill(e.begin(), e.end(), 0.0);
+      unsigned mask = 0;
+      for (int r = 0; r < kRHS && t0 + r < n; r++) {
+        e[(size_t)r * n + t0 + r] = 1.0;
+        mask |= 1u << r;
+      }
+      FEMH_HIP(hipMemcpyAsync(de.p, e.data(), sizeof(double) * kRHS * n, hipMemcpyHostToDevice, stream));
+      apply<double>(h, 0, de.as<double>(), dc.as<double>(), nullptr, 0, mask, nullptr, 0);
+      FEMH_HIP(hipMemcpyAsync(col.data(), dc.p, sizeof(double) * kRHS * n, hipMemcpyDeviceToHost, stream));
+      FEMH_HIP(hipStreamSynchronize(stream));
+      for (int r = 0; r < kRHS && t0 + r < n; r++)
+        for (int s = 0; s < n; s++) A[(size_t)s * n + t0 + r] = col[(size_t)r * n + s];
+    }
+  } else {
+    Sh.resize((size_t)243 * V);
+    FEMH_HIP(hipMemcpyAsync(Sh.data(), L.S.p, sizeof(double) * 243 * V, hipMemcpyDeviceToHost, stream));
+    FEMH_HIP(hipStreamSynchronize(stream));
+    const Dims &d = L.g;
+    for (int N = 0; N < V; N++) {
+      const int k = N % d.nz, j = (N / d.nz) % d.ny, i = N / (d.nz * d.ny);
+      for (int off = 0; off < 27; off++) {
+        const int ni = (i + off / 9 - 1 + d.nx) % d.nx, nj = (j + (off / 3) % 3 - 1 + d.ny) % d.ny,
+                  nk = (k + off % 3 - 1 + d.nz) % d.nz;
+        const int M = (ni * d.ny + nj) * d.nz + nk;
+        for (int c = 0; c < 3; c++)
+          for (int dd = 0; dd < 3; dd++)
+            A[(size_t)(c * V + N) * n + dd * V + M] += Sh[(size_t)(off * 9 + 3 * c + dd) * V + N];
+      }
+    }
+  }
+  double dmean = 0;
+  for (int t = 0; t < n; t++) dmean += A[(size_t)t * n + t];
+  dmean /= n;
+  for (int c = 0; c < 3; c++)
+    for (int a = 0; a < V; a++)
+      for (int bb = 0; bb < V; bb++) A[(size_t)(c * V + a) * n + c * V + bb] += dmean / V;
+  // Cholesky A = L L^T (lower, in place)
+  for (int j = 0; j < n; j++) {
+    double s = A[(size_t)j * n + j];
+    for (int k = 0; k < j; k++) s -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(s > 0)) throw Error(FEMH_NONFINITE, "coarsest operator is not positive definite");
+    s = std::sqrt(s);
+    A[(size_t)j * n + j] = s;
+    for (int i = j + 1; i < n; i++) {
+      double t = A[(size_t)i * n + j];
+      for (int k = 0; k < j; k++) t -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = t / s;
+    }
+  }
+  // inverse column by column
+  std::vector<double> inv((size_t)n * n), col(n);
+  for (int c = 0; c < n; c++) {
+    std::fill(col.begin(), col.end(), 0.0);
+    col[c] = 1.0;
+    for (int i = 0; i < n; i++) {
+      double s = col[i];
+      for (int k = 0; k < i; k++) s -= A[(size_t)i * n + k] * col[k];
+      col[i] = s / A[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      double s = col[i];
+      for (int k = i + 1; k < n; k++) s -= A[(size_t)k * n + i] * col[k];
+      col[i] = s / A[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; i++) inv[(size_t)i * n + c] = col[i];
+  }
+  h.Ainv.alloc(sizeof(double) * (size_t)n * n);
+  FEMH_HIP(hipMemcpyAsync(h.Ainv.p, inv.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice,
+                          stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+  h.ndense = n;
+}
+
+void Context::build_hierarchy() {
+  build_levels<double>(h64);
+  nlevels = (int)h64.lv.size();
+  FEMH_HIP(hipMemcpyAsync(h64.E.p, E64.p, sizeof(double) * g.V, hipMemcpyDeviceToDevice, stream));
+  FEMH_HIP(hipMemcpyAsync(h64.Ke.p, Ke64.p, sizeof(double) * 576, hipMemcpyDeviceToDevice, stream));
+  FineOp<double> f0{g, h64.E.as<double>(), h64.Ke.as<double>()};
+  launch_fine_diag<double>(f0, h64.lv[0].D.as<double>(), stream);
+  for (int l = 1; l < nlevels; l++) {
+    Level<double> &L = h64.lv[l];
+    if (l == 1)
+      launch_build_level1<double>(g, L.g, E64.as<double>(), Mchild.as<double>(), L.S.as<double>(), stream);
+    else
+      launch_galerkin<double>(h64.lv[l - 1].g, L.g, h64.lv[l - 1].S.as<double>(), L.S.as<double>(),
+                              stream);
+    StencilOp<double> so{L.g, L.S.as<double>()};
+    launch_stencil_diag<double>(so, L.D.as<double>(), stream);
+  }
+  FEMH_HIP(hipGetLastError());
+  if (o.precond == FEMH_PRECOND_MG) {
+    estimate_lmax(h64);
+    if (nlevels > 1) {
+      dense_coarse(h64);
+      if (h64.ndense == 0)
+        throw Error(FEMH_BAD_ARG, "coarsest multigrid level too large for the dense solve (" +
+                                      std::to_string(3 * h64.lv.back().g.V) +
+                                      " dofs); allow more levels or use a grid divisible by 2^k");
+    }
+  }
+  h64.ready = true;
+  if (o.precision == FEMH_MIXED) {
+    build_levels<float>(h32);
+    launch_convert<double, float>(g.V, E64.as<double>(), h32.E.as<float>(), stream);
+    launch_convert<double, float>(576, Ke64.as<double>(), h32.Ke.as<float>(), stream);
+    for (int l = 0; l < nlevels; l++) {
+      launch_convert<double, float>((size_t)3 * h64.lv[l].g.V, h64.lv[l].D.as<double>(),
+                                    h32.lv[l].D.as<float>(), stream);
+      if (l > 0)
+        launch_convert<double, float>((size_t)243 * h64.lv[l].g.V, h64.lv[l].S.as<double>(),
+                                      h32.lv[l].S.as<float>(), stream);
+      h32.lv[l].lmax = h64.lv[l].lmax;
+    }
+    if (h64.ndense) {
+      h32.Ainv.alloc(sizeof(float) * (size_t)h64.ndense * h64.ndense);
+      launch_convert<double, float>((size_t)h64.ndense * h64.ndense, h64.Ainv.as<double>(),
+                                    h32.Ainv.as<float>(), stream);
+      h32.ndense = h64.ndense;
+    }
+    FEMH_HIP(hipGetLastError());
+    h32.ready = true;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// multigrid V-cycle with Chebyshev smoothing
+template <typename T>
+void Context::smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, int degree, bool x_zero,
+                     unsigned mask) {
+  Level<T> &L = h.lv[l];
+  const int V = L.g.V;
+  T *rr = L.r.template as<T>(), *dd = L.d.template as<T>(), *Ad = L.Ad.template as<T>();
+  const T *D = L.D.template as<T>();
+  const double hi = L.lmax, lo = L.lmax / o.cheby_ratio;
+  const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  const T *rsrc;
+  if (x_zero) {
+    rsrc = bb;
+  } else {
+    apply<T>(h, l, xx, rr, bb, 1, mask, nullptr, 0);
+    rsrc = rr;
+  }
+  launch_cheb_first<T>(V, rsrc, D, dd, xx, 1.0 / theta, x_zero ? 1 : 0, mask, stream);
+  for (int s = 1; s < degree; s++) {
+    apply<T>(h, l, dd, Ad, nullptr, 0, mask, nullptr, 0);
+    const double rho_new = 1.0 / (2.0 * sigma - rho);
+    launch_cheb_next<T>(V, rsrc, rr, Ad, D, dd, xx, rho_new * rho, 2.0 * rho_new / delta, mask, stream);
+    rsrc = rr;
+    rho = rho_new;
+  }
+}
+
+template <typename T>
+void Context::vcycle(Hierarchy<T> &h, int l, const T *bb, T *xx, unsigned mask) {
+  const int last = (int)h.lv.size() - 1;
+  if (l == last) {
+    launch_dense_solve<T>(h.ndense, h.Ainv.template as<T>(), bb, xx, mask, stream);
+    return;
+  }
+  Level<T> &L = h.lv[l];
+  Level<T> &C = h.lv[l + 1];
+  T *rr = L.r.template as<T>();
+  smooth<T>(h, l, bb, xx, o.nu1, true, mask);
+  apply<T>(h, l, xx, rr, bb, 1, mask, nullptr, 0);
+  launch_restrict<T>(L.g, C.g, rr, C.b.template as<T>(), mask, stream);
+  vcycle<T>(h, l + 1, C.b.template as<T>(), C.x.template as<T>(), mask);
+  launch_prolong_add<T>(L.g, C.g, C.x.template as<T>(), xx, mask, stream);
+  smooth<T>(h, l, bb, xx, o.nu2, false, mask);
+}
+
+template <typename T>
+void Context::precond(Hierarchy<T> &h, const T *rv, T *zv, unsigned mask) {
+  if (o.precond == FEMH_PRECOND_MG && h.lv.size() > 1) {
+    vcycle<T>(h, 0, rv, zv, mask);
+  } else {
+    // Jacobi: handled inside the PCG update kernel; this path only serves the
+    // first iteration
+    launch_cheb_first<T>(h.lv[0].g.V, rv, h.lv[0].D.template as<T>(), zv, zv, 1.0, 1, mask, stream);
+  }
+}
+
+// PCG over the six load cases.  Returns the mask of RHS still unconverged.
+template <typename T>
+int Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *qv, unsigned mask,
+                 double tol, int max_iter, const double *bnorm2, int *iters, double *relres) {
+  const int V = g.V;
+  const bool jac = !(o.precond == FEMH_PRECOND_MG && h.lv.size() > 1);
+  const T *D0 = h.lv[0].D.template as<T>();
+  FEMH_HIP(hipMemsetAsync(xv, 0, vec_bytes(g, sizeof(T)), stream));
+  FEMH_HIP(hipMemcpyAsync(rv, bvec, vec_bytes(g, sizeof(T)), hipMemcpyDeviceToDevice, stream));
+  unsigned active = mask;
+  for (int r = 0; r < kRHS; r++) {
+    iters[r] = 0;
+    relres[r] = 0.0;
+  }
+  if (!active) return 0;
+  precond<T>(h, rv, zv, active);
+  launch_dot<T>(V, rv, zv, S_RZ0, active, red, stream);
+  launch_p_update<T>(V, pv, zv, S_RZ0, S_RZ0, 1, active, red, stream);
+  int cur = 0;
+  double rr[kRHS];
+  for (int it = 1; it <= max_iter; it++) {
+    const int rzc = cur ? S_RZ1 : S_RZ0, rzn = cur ? S_RZ0 : S_RZ1;
+    apply<T>(h, 0, pv, qv, nullptr, 0, active, &red, S_PQ);
+    launch_pcg_update<T>(V, xv, rv, pv, qv, jac ? D0 : nullptr, zv, rzc, rzn, active, red, stream);
+    FEMH_HIP(hipGetLastError());
+    if (it % o.check_every == 0 || it == max_iter) {
+      read_scalars(S_RR, rr);
+      for (int r = 0; r < kRHS; r++) {
+        if (!((active >> r) & 1u)) continue;
+        if (!std::isfinite(rr[r])) throw Error(FEMH_NONFINITE, "residual became non-finite");
+        const double rel = std::sqrt(rr[r] / bnorm2[r]);
+        iters[r] = it;
+        relres[r] = rel;
+        if (rel <= tol) active &= ~(1u << r);
+      }
+      if (!active) break;
+    }
+    if (!jac) {
+      vcycle<T>(h, 0, rv, zv, active);
+      launch_dot<T>(V, rv, zv, rzn, active, red, stream);
+    }
+    launch_p_update<T>(V, pv, zv, rzn, rzc, 0, active, red, stream);
+    cur ^= 1;
+  }
+  FEMH_HIP(hipGetLastError());
+  return (int)active;
+}
+
+// ---------------------------------------------------------------------------
+void Context::compute_ch(const double *fields, double *C) {
+  const int nblk = 1024;
+  launch_ch_energy(g, E64.as<double>(), Ke64.as<double>(), uaff.as<double>(), fields,
+                   ch_part.as<double>(), nblk, stream);
+  launch_sum_partials(ch_part.as<double>(), 21, nblk, ch_out.as<double>(), stream);
+  FEMH_HIP(hipGetLastError());
+  double s[21];
+  FEMH_HIP(hipMemcpyAsync(s, ch_out.p, sizeof(s), hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+  int p = 0;
+  for (int a = 0; a < 6; a++)
+    for (int bb = a; bb < 6; bb++) {
+      C[6 * a + bb] = C[6 * bb + a] = s[p] / (double)g.V;
+      p++;
+    }
+}
+
+void Context::homogenize(double *C, femh_stats *st) {
+  auto t0 = std::chrono::steady_clock::now();
+  hipEvent_t ev[3];
+  for (auto &e : ev) FEMH_HIP(hipEventCreate(&e));
+  FEMH_HIP(hipEventRecord(ev[0], stream));
+  have_fields = false;
+  build_hierarchy();
+  FineOp<double> f0{g, E64.as<double>(), Ke64.as<double>()};
+  launch_fine_rhs<double>(f0, gtab.as<double>(), b.as<double>(), stream);
+  launch_dot<double>(g.V, b.as<double>(), b.as<double>(), S_BB, 0x3Fu, red, stream);
+  FEMH_HIP(hipGetLastError());
+  double bb[kRHS];
+  read_scalars(S_BB, bb);
+  // right-hand sides that vanish (the affine field already balances, e.g. a
+  // homogeneous cell) are solved by u = 0
+  double emax = 0;
+  {
+    std::vector<double> Eh(g.V);
+    FEMH_HIP(hipMemcpyAsync(Eh.data(), E64.p, sizeof(double) * g.V, hipMemcpyDeviceToHost, stream));
+    FEMH_HIP(hipStreamSynchronize(stream));
+    for (double e : Eh) {
+      if (!std::isfinite(e)) throw Error(FEMH_NONFINITE, "non-finite modulus");
+      if (e < 0) throw Error(FEMH_BAD_ARG, "negative modulus");
+      emax = std::max(emax, e);
+    }
+    if (!(emax > 0)) throw Error(FEMH_BAD_ARG, "all-void cell (every modulus is zero)");
+  }
+  const double zero_thresh = 1e-13 * emax * std::sqrt((double)g.V);
+  unsigned mask = 0;
+  for (int r = 0; r < kRHS; r++)
+    if (std::sqrt(bb[r]) > zero_thresh) mask |= 1u << r;
+  FEMH_HIP(hipEventRecord(ev[1], stream));
+
+  int iters[kRHS] = {0}, outer[kRHS] = {0};
+  double relres[kRHS] = {0};
+  int status = FEMH_OK;
+  if (o.precision == FEMH_FP64) {
+    unsigned left = pcg<double>(h64, b.as<double>(), x.as<double>(), r.as<double>(), z.as<double>(),
+                                p.as<double>(), q.as<double>(), mask, o.tol, o.max_iter, bb, iters,
+                                relres);
+    for (int k = 0; k < kRHS; k++) outer[k] = (mask >> k) & 1u;
+    if (left) status = FEMH_NOT_CONVERGED;
+  } else {
+    // FP64 iterative refinement around FP32 multigrid-PCG corrections
+    FEMH_HIP(hipMemsetAsync(x.p, 0, vec_bytes(g, sizeof(double)), stream));
+    FEMH_HIP(hipMemcpyAsync(r.p, b.p, vec_bytes(g, sizeof(double)), hipMemcpyDeviceToDevice, stream));
+    unsigned active = mask;
+    const double inner_tol = 1e-4;
+    const int max_outer = 12;
+    double rr[kRHS];
+    for (int k = 0; k < kRHS; k++) rr[k] = bb[k];
+    for (int sweep = 0; sweep < max_outer && active; sweep++) {
+      launch_convert<double, float>((size_t)kRHS * 3 * g.V, r.as<double>(), b32.as<float>(), stream);
+      int it_in[kRHS];
+      double rel_in[kRHS];
+      pcg<float>(h32, b32.as<float>(), x32.as<float>(), r32.as<float>(), z32.as<float>(),
+                 p32.as<float>(), q32.as<float>(), active, inner_tol, o.max_iter, rr, it_in, rel_in);
+      launch_add_f32_to_f64(0, x32.as<float>(), x.as<double>(), active, g.V, stream);
+      apply<double>(h64, 0, x.as<double>(), r.as<double>(), b.as<double>(), 1, active, nullptr, 0);
+      launch_dot<double>(g.V, r.as<double>(), r.as<double>(), S_RR, active, red, stream);
+      FEMH_HIP(hipGetLastError());
+      double rn[kRHS];
+      read_scalars(S_RR, rn);
+      for (int k = 0; k < kRHS; k++) {
+        if (!((active >> k) & 1u)) continue;
+        if (!std::isfinite(rn[k])) throw Error(FEMH_NONFINITE, "residual became non-finite");
+        iters[k] += it_in[k];
+        outer[k]++;
+        rr[k] = rn[k];
+        relres[k] = std::sqrt(rn[k] / bb[k]);
+        if (relres[k] <= o.tol) active &= ~(1u << k);
+      }
+    }
+    if (active) status = FEMH_NOT_CONVERGED;
+  }
+  launch_remove_mean<double>(g.V, x.as<double>(), 0x3Fu, red, stream);
+  FEMH_HIP(hipEventRecord(ev[2], stream));
+  compute_ch(x.as<double>(), C);
+  have_fields = true;
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    for (int k = 0; k < kRHS; k++) {
+      st->iters[k] = iters[k];
+      st->outer[k] = outer[k];
+      st->relres[k] = relres[k];
+    }
+    float ms1 = 0, ms2 = 0;
+    FEMH_HIP(hipEventElapsedTime(&ms1, ev[0], ev[1]));
+    FEMH_HIP(hipEventElapsedTime(&ms2, ev[1], ev[2]));
+    st->ms_setup = ms1;
+    st->ms_solve = ms2;
+    st->levels = nlevels;
+    st->ms_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 36; k++)
+    if (!std::isfinite(C[k])) throw Error(FEMH_NONFINITE, "C_H is not finite");
+  if (status != FEMH_OK) throw Error(status, "PCG did not reach the tolerance within max_iter");
+}
+
+void Context::gradient(const double *dJdC, double *out_host, bool energy_only) {
+  if (!have_fields) throw Error(FEMH_NO_FIELDS, "no solved fields: call femh_homogenize first");
+  double Gsym[21];
+  int p = 0;
+  for (int a = 0; a < 6; a++)
+    for (int bb = a; bb < 6; bb++) {
+      Gsym[p++] = energy_only ? 0.0 : (a == bb ? dJdC[6 * a + a] : dJdC[6 * a + bb] + dJdC[6 * bb + a]);
+    }
+  FEMH_HIP(hipMemcpyAsync(Gs.p, Gsym, sizeof(Gsym), hipMemcpyHostToDevice, stream));
+  size_t n = energy_only ? (size_t)21 * g.V : (size_t)g.V;
+  DBuf out;
+  out.alloc(sizeof(double) * n);
+  launch_gradient(g, rho.as<double>(), Ke64.as<double>(), uaff.as<double>(), x.as<double>(),
+                  Gs.as<double>(), o.interp, o.E0, o.Emin, o.penal, energy_only ? 1 : 0,
+                  out.as<double>(), stream);
+  FEMH_HIP(hipGetLastError());
+  FEMH_HIP(hipMemcpyAsync(out_host, out.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+}
+
+void Context::get_fields(double *u) {
+  if (!have_fields) throw Error(FEMH_NO_FIELDS, "no solved fields: call femh_homogenize first");
+  FEMH_HIP(hipMemcpyAsync(u, x.p, vec_bytes(g, sizeof(double)), hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+}
+
+void Context::apply_level(int level, const double *u, double *y) {
+  if (!h64.ready) throw Error(FEMH_NO_FIELDS, "no hierarchy: call femh_prepare first");
+  if (level < 0 || level >= nlevels) throw Error(FEMH_BAD_ARG, "level out of range");
+  Level<double> &L = h64.lv[level];
+  const size_t n = (size_t)3 * L.g.V;
+  FEMH_HIP(hipMemcpyAsync(L.x.p, u, sizeof(double) * n, hipMemcpyHostToDevice, stream));
+  apply<double>(h64, level, L.x.as<double>(), L.Ad.as<double>(), nullptr, 0, 1u, nullptr, 0);
+  FEMH_HIP(hipGetLastError());
+  FEMH_HIP(hipMemcpyAsync(y, L.Ad.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+}
+
+template int Context::pcg<double>(Hierarchy<double> &, const double *, double *, double *, double *,
+                                  double *, double *, unsigned, double, int, const double *, int *,
+                                  double *);
+template int Context::pcg<float>(Hierarchy<float> &, const float *, float *, float *, float *,
+                                 float *, float *, unsigned, double, int, const double *, int *,
+                                 double *);
+
+}  // namespace femh

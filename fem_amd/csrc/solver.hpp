@@ -1,0 +1,122 @@
+// solver.hpp -- device-resident homogenization context behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/femh.h"
+#include "kernels.hpp"
+#include "tables.hpp"
+
+namespace femh {
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int st, const std::string &m) : std::runtime_error(m), status(st) {}
+};
+
+void hip_check(hipError_t e, const char *what);
+#define FEMH_HIP(x) ::femh::hip_check((x), #x)
+
+// owning device allocation
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DBuf &operator=(DBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t n);
+  void release();
+  template <typename T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+template <typename T>
+struct Level {
+  Dims g{};
+  DBuf S;            // stencil (levels >= 1)
+  DBuf D;            // point diagonal [3V]
+  double lmax = 0;   // lambda_max(D^-1 A) estimate (with margin)
+  DBuf x, b, r, d, Ad;  // V-cycle work, [6][3][V]
+};
+
+template <typename T>
+struct Hierarchy {
+  std::vector<Level<T>> lv;
+  DBuf E;      // fine moduli in T
+  DBuf Ke;     // unit stiffness in T
+  DBuf Ainv;   // dense (regularised) inverse of the coarsest operator
+  int ndense = 0;
+  bool ready = false;
+};
+
+struct Context {
+  femh_opts o{};
+  Dims g{};
+  ElementTables tab;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  int last_status = FEMH_OK;
+
+  // material / tables on device
+  DBuf rho, E64, Ke64, gtab, uaff, Mchild, Gs;
+  // FP64 solver state, each [6][3][V]
+  DBuf b, x, r, z, p, q;
+  // FP32 inner solver state (mixed)
+  DBuf b32, r32, x32, z32, p32, q32;
+  DBuf red_part, red_ctr, red_scal;
+  Reduce red{};
+  DBuf ch_part, ch_out;
+  Hierarchy<double> h64;
+  Hierarchy<float> h32;
+  bool have_fields = false;
+  int nlevels = 1;
+
+  explicit Context(const femh_opts &opts);
+  ~Context();
+
+  void set_modulus_from_rho(const double *d_rho_src, bool device_src);
+  void set_modulus(const double *E_host);
+  void build_hierarchy();
+  void homogenize(double *C, femh_stats *st);
+  void gradient(const double *dJdC, double *out_host, bool energy_only);
+  void get_fields(double *u);
+  void apply_level(int level, const double *u, double *y);
+
+ private:
+  template <typename T>
+  void build_levels(Hierarchy<T> &h);
+  template <typename T>
+  void apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *b, int mode, unsigned mask,
+             const Reduce *red, int slot);
+  template <typename T>
+  void smooth(Hierarchy<T> &h, int l, const T *b, T *x, int degree, bool x_zero, unsigned mask);
+  template <typename T>
+  void vcycle(Hierarchy<T> &h, int l, const T *b, T *x, unsigned mask);
+  template <typename T>
+  void precond(Hierarchy<T> &h, const T *r, T *z, unsigned mask);
+  template <typename T>
+  int pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *qv, unsigned mask,
+          double tol, int max_iter, const double *bnorm2, int *iters, double *relres);
+  void estimate_lmax(Hierarchy<double> &h);
+  void dense_coarse(Hierarchy<double> &h);
+  void read_scalars(int slot, double *out, int n = kRHS);
+  void compute_ch(const double *fields, double *C);
+};
+
+}  // namespace femh

@@ -1,0 +1,228 @@
+"""ctypes binding of the engine's C ABI (include/femh.h).
+
+This is the Python face of the same entry points the C++ host facade uses:
+density (or modulus) grid in, 6x6 homogenized tensor C_H out, dJ/drho from the
+element-energy adjoint.  Arrays are numpy float64, voxel order (i, j, k) with
+k fastest.  The library is the in-tree fem_amd/libfemh.so; there is no CPU
+fallback -- a missing or unloadable library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfemh.so")
+_LIB = None
+
+OK, BAD_ARG, NOT_CONVERGED, NONFINITE, DEVICE_ERROR, OOM, NO_FIELDS = range(7)
+_STATUS = {0: "OK", 1: "BAD_ARG", 2: "NOT_CONVERGED", 3: "NONFINITE", 4: "DEVICE_ERROR",
+           5: "OOM", 6: "NO_FIELDS"}
+INTERP = {"simp": 0, "fastfem": 1, "modulus": 2}
+PRECOND = {"jacobi": 0, "mg": 1}
+PRECISION = {"fp64": 0, "mixed": 1}
+
+# Every symbol include/femh.h declares.
+EXPORTS = ("femh_abi_version", "femh_default_opts", "femh_create", "femh_destroy",
+           "femh_homogenize", "femh_homogenize_modulus", "femh_homogenize_device",
+           "femh_gradient", "femh_energy_terms", "femh_get_fields", "femh_homogenize_batch",
+           "femh_prepare", "femh_apply_level", "femh_level_dims", "femh_last_error")
+
+
+class FemhError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"femh {_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("E0", ctypes.c_double), ("Emin", ctypes.c_double), ("nu", ctypes.c_double),
+                ("penal", ctypes.c_double), ("interp", ctypes.c_int), ("precond", ctypes.c_int),
+                ("levels", ctypes.c_int), ("nu1", ctypes.c_int), ("nu2", ctypes.c_int),
+                ("cheby_ratio", ctypes.c_double), ("coarse_iters", ctypes.c_int),
+                ("tol", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("precision", ctypes.c_int), ("ch_tol", ctypes.c_double),
+                ("check_every", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int * 6), ("outer", ctypes.c_int * 6),
+                ("relres", ctypes.c_double * 6), ("ch_err_est", ctypes.c_double),
+                ("ms_setup", ctypes.c_double), ("ms_solve", ctypes.c_double),
+                ("ms_total", ctypes.c_double), ("levels", ctypes.c_int)]
+
+    def as_dict(self):
+        return dict(iters=list(self.iters), outer=list(self.outer), relres=list(self.relres),
+                    ch_err_est=self.ch_err_est, ms_setup=self.ms_setup, ms_solve=self.ms_solve,
+                    ms_total=self.ms_total, levels=self.levels)
+
+
+def lib():
+    """Load libfemh.so (importing torch first when present so that a single
+    HIP runtime serves both)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        try:
+            import torch  # noqa: F401  (pins torch's libamdhip64 before ours)
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.femh_create.argtypes = [ctypes.POINTER(Opts), ctypes.POINTER(vp)]
+        L.femh_destroy.argtypes = [vp]
+        L.femh_destroy.restype = None
+        L.femh_default_opts.argtypes = [ctypes.POINTER(Opts)]
+        L.femh_default_opts.restype = None
+        for name in ("femh_homogenize", "femh_homogenize_modulus"):
+            getattr(L, name).argtypes = [vp, vp, vp, vp]
+        L.femh_homogenize_device.argtypes = [vp, vp, vp, vp, vp]
+        L.femh_gradient.argtypes = [vp, vp, vp]
+        L.femh_energy_terms.argtypes = [vp, vp]
+        L.femh_get_fields.argtypes = [vp, vp]
+        L.femh_homogenize_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
+        L.femh_prepare.argtypes = [vp, vp]
+        L.femh_apply_level.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.femh_level_dims.argtypes = [vp, ctypes.c_int, vp]
+        L.femh_last_error.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond="mg",
+              levels=0, nu1=2, nu2=2, cheby_ratio=8.0, coarse_iters=64, tol=1e-10,
+              max_iter=5000, precision="fp64", ch_tol=0.0, check_every=1, device=0):
+    o = Opts()
+    lib().femh_default_opts(ctypes.byref(o))
+    o.nx, o.ny, o.nz = (int(s) for s in shape)
+    o.E0, o.Emin, o.nu, o.penal = E0, Emin, nu, penal
+    o.interp = INTERP[interp]
+    o.precond = PRECOND[precond]
+    o.levels, o.nu1, o.nu2 = levels, nu1, nu2
+    o.cheby_ratio, o.coarse_iters = cheby_ratio, coarse_iters
+    o.tol, o.max_iter = tol, max_iter
+    o.precision = PRECISION[precision]
+    o.ch_tol, o.check_every, o.device = ch_tol, check_every, device
+    return o
+
+
+class Homogenizer:
+    """One device context for grids of a fixed shape (mirrors one
+    NumericalCoarseningUsingMultigrid instance of the reference)."""
+
+    def __init__(self, shape, **kw):
+        self.shape = tuple(int(s) for s in shape)
+        self.V = int(np.prod(self.shape))
+        self.opts = make_opts(self.shape, **kw)
+        h = ctypes.c_void_p()
+        st = lib().femh_create(ctypes.byref(self.opts), ctypes.byref(h))
+        self._h = h
+        if st != OK:
+            msg = self._err()
+            self.close()
+            raise FemhError(st, msg)
+
+    def _err(self):
+        buf = ctypes.create_string_buffer(512)
+        if self._h:
+            lib().femh_last_error(self._h, buf, 512)
+        return buf.value.decode(errors="replace")
+
+    def _check(self, st):
+        if st != OK:
+            raise FemhError(st, self._err())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().femh_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _grid(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != self.V:
+            raise ValueError(f"expected {self.V} voxels, got {a.size}")
+        return a
+
+    def homogenize(self, rho, check=True):
+        rho = self._grid(rho)
+        C = np.zeros(36)
+        s = Stats()
+        st = lib().femh_homogenize(self._h, _ptr(rho), _ptr(C), ctypes.byref(s))
+        if check:
+            self._check(st)
+        return C.reshape(6, 6), s.as_dict()
+
+    def homogenize_modulus(self, E, check=True):
+        E = self._grid(E)
+        C = np.zeros(36)
+        s = Stats()
+        st = lib().femh_homogenize_modulus(self._h, _ptr(E), _ptr(C), ctypes.byref(s))
+        if check:
+            self._check(st)
+        return C.reshape(6, 6), s.as_dict()
+
+    def homogenize_device(self, d_rho_ptr, d_C_ptr, stream_ptr=None):
+        s = Stats()
+        self._check(lib().femh_homogenize_device(self._h, ctypes.c_void_p(d_rho_ptr),
+                                                 ctypes.c_void_p(d_C_ptr),
+                                                 ctypes.c_void_p(stream_ptr or 0), ctypes.byref(s)))
+        return s.as_dict()
+
+    def gradient(self, dJdC):
+        G = np.ascontiguousarray(dJdC, dtype=np.float64).reshape(36)
+        out = np.zeros(self.V)
+        self._check(lib().femh_gradient(self._h, _ptr(G), _ptr(out)))
+        return out.reshape(self.shape)
+
+    def energy_terms(self):
+        W = np.zeros((self.V, 21))
+        self._check(lib().femh_energy_terms(self._h, _ptr(W)))
+        return W
+
+    def fields(self):
+        u = np.zeros((6, 3, self.V))
+        self._check(lib().femh_get_fields(self._h, _ptr(u)))
+        return u
+
+    def prepare(self, E):
+        self._check(lib().femh_prepare(self._h, _ptr(self._grid(E))))
+
+    def level_dims(self, level):
+        d = np.zeros(3, dtype=np.int32)
+        st = lib().femh_level_dims(self._h, level, _ptr(d))
+        return None if st != OK else tuple(int(v) for v in d)
+
+    def apply_level(self, level, u):
+        d = self.level_dims(level)
+        if d is None:
+            raise FemhError(BAD_ARG, f"no level {level}")
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        if u.size != 3 * int(np.prod(d)):
+            raise ValueError("field size does not match the level")
+        y = np.zeros_like(u)
+        self._check(lib().femh_apply_level(self._h, level, _ptr(u), _ptr(y)))
+        return y
+
+
+def homogenize_batch(grids, **kw):
+    """Batch over modulus grids of uneven shapes.  Returns (C[n,6,6], stats)."""
+    grids = [np.ascontiguousarray(g, dtype=np.float64) for g in grids]
+    n = len(grids)
+    if n == 0:
+        return np.zeros((0, 6, 6)), {}
+    dims = np.array([g.shape for g in grids], dtype=np.int32).reshape(-1)
+    ptrs = (ctypes.c_void_p * n)(*[g.ctypes.data for g in grids])
+    C = np.zeros((n, 36))
+    h = Homogenizer(grids[0].shape, interp="modulus", **kw)
+    s = Stats()
+    h._check(lib().femh_homogenize_batch(h._h, n, _ptr(dims), ptrs, _ptr(C), ctypes.byref(s)))
+    h.close()
+    return C.reshape(n, 6, 6), s.as_dict()

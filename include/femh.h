@@ -1,0 +1,154 @@
+/*
+ * femh.h -- C ABI of the MI355X voxel-FEM homogenization engine.
+ *
+ * One entry point family replaces the solver-facing surface of desaic/fem's
+ * "density grid in -> homogenized tensor C_H (6x6) and dC_H/drho out" path:
+ *
+ *   femh_create / femh_destroy
+ *       replace NumericalCoarseningUsingMultigrid::init
+ *       (SamplesGenerator/NumericalCoarseningUsingMultigrid.h:27,
+ *        SamplesGenerator/NumericalCoarseningUsingMultiGrid.cpp:134-219) and
+ *       NumericalCoarsening::init (SamplesGenerator/NumericalCoarsening.h:26,
+ *        NumericalCoarsening.cpp:116-133): grid, base material, solver knobs.
+ *   femh_homogenize / femh_homogenize_modulus
+ *       replace computeCoarsenedElasticityTensorAndParameters3D
+ *       (NumericalCoarseningUsingMultigrid.h:29-30,
+ *        NumericalCoarseningUsingMultiGrid.cpp:221-313) and the solve+G part of
+ *       FEM3DFun::setParam (fast_fem/FEM3DFun.cpp:128-196).
+ *   femh_gradient
+ *       replaces the element-energy adjoint of FEM3DFun::df
+ *       (fast_fem/FEM3DFun.cpp:222-268) with dJ/dC supplied by the caller.
+ *   femh_get_fields
+ *       exposes the 6 solved periodic fields (the reference's
+ *       Homogenization::unit_test_u consumed at NumericalCoarseningUsingMultiGrid
+ *       .cpp:275-288, fast_fem/FEM3DFun.cpp:159-161).
+ *   femh_homogenize_batch
+ *       replaces the OpenMP sample loop of SamplesGeneratorImpl::
+ *       computeParametersAndTensorValues (SamplesGenerator/SamplesGeneratorImpl
+ *       .cpp:3707-3731) with one device-side batch.
+ *
+ * Conventions
+ *   voxel index   (i*ny + j)*nz + k, z fastest (femlib/include/ElementRegGrid.hpp:28)
+ *   periodic      node index == voxel index; the (n+1)^3 expansion of
+ *                 copyPeriodicU is left to the caller (host facade)
+ *   C_H           row-major 6x6, Voigt order (xx, yy, zz, yz, xz, xy),
+ *                 engineering shear strains, energy form
+ *                 C_ij = 1/V sum_e E_e chi_i^T K0 chi_j (V = voxel count, h = 1)
+ *   fields        u[lc][c][idx]: 6 load cases x 3 components x V, zero mean
+ *   status        int, 0 = FEMH_OK; femh_last_error() gives the message
+ * All buffers are caller-owned.  A context is used by one thread at a time
+ * (one context per thread, as the reference keeps one homogenizer per OpenMP
+ * thread); calls on different contexts may run concurrently.
+ */
+#ifndef FEMH_H
+#define FEMH_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEMH_ABI_VERSION 1
+
+enum femh_status {
+  FEMH_OK = 0,
+  FEMH_BAD_ARG = 1,
+  FEMH_NOT_CONVERGED = 2,
+  FEMH_NONFINITE = 3,
+  FEMH_DEVICE_ERROR = 4,
+  FEMH_OOM = 5,
+  FEMH_NO_FIELDS = 6
+};
+
+enum femh_precond { FEMH_PRECOND_JACOBI = 0, FEMH_PRECOND_MG = 1 };
+enum femh_precision { FEMH_FP64 = 0, FEMH_MIXED = 1 };
+
+/* Material interpolation E(rho):
+ *   FEMH_INTERP_SIMP     E = Emin + (E0 - Emin) * rho^penal
+ *                        (MaterialOpt/src/FEM3DFun.cpp:244-257 uses Emin=1e-3 E0, p=3)
+ *   FEMH_INTERP_FASTFEM  E = E0 * rho / (3 - 2 rho)  (fast_fem/FEM3DFun.cpp:20-27)
+ *   FEMH_INTERP_MODULUS  rho already holds E per voxel (binary grids etc.) */
+enum femh_interp { FEMH_INTERP_SIMP = 0, FEMH_INTERP_FASTFEM = 1, FEMH_INTERP_MODULUS = 2 };
+
+typedef struct femh_opts {
+  int nx, ny, nz;       /* grid (voxels) */
+  double E0, Emin, nu;  /* base material */
+  double penal;         /* SIMP exponent */
+  int interp;           /* femh_interp */
+  int precond;          /* femh_precond */
+  int levels;           /* multigrid levels incl. fine (0 = as deep as the grid allows) */
+  int nu1, nu2;         /* Chebyshev smoothing degree before / after coarse correction */
+  double cheby_ratio;   /* smoothing interval [lmax/ratio, lmax] of D^-1 A */
+  int coarse_iters;     /* inner Jacobi-PCG iterations when the coarsest level is large */
+  double tol;           /* ||r||/||b|| target (FP64) / outer refinement target (mixed) */
+  int max_iter;         /* outer iteration cap per load case */
+  int precision;        /* femh_precision */
+  double ch_tol;        /* mixed: stop when the estimated relative C_H error of
+                           every load case is below this (0 = use tol only) */
+  int check_every;      /* iterations between host convergence checks */
+  int device;           /* HIP device ordinal */
+} femh_opts;
+
+typedef struct femh_stats {
+  int iters[6];         /* PCG iterations per load case (inner FP32 iterations in mixed) */
+  int outer[6];         /* refinement sweeps per load case (mixed), 1 in FP64 */
+  double relres[6];     /* final FP64 relative residual per load case */
+  double ch_err_est;    /* a-posteriori relative C_H error estimate (energy form) */
+  double ms_setup;      /* device time: hierarchy and right-hand sides */
+  double ms_solve;      /* device time: PCG */
+  double ms_total;      /* wall time of the call */
+  int levels;           /* multigrid levels used */
+} femh_stats;
+
+void femh_default_opts(femh_opts *o);
+
+int femh_create(const femh_opts *o, struct femh_ctx **out);
+void femh_destroy(struct femh_ctx *ctx);
+
+/* Homogenize one density grid (host memory, V = nx*ny*nz doubles, interpreted
+ * through opts.interp).  C[36] receives C_H.  stats may be NULL. */
+int femh_homogenize(struct femh_ctx *ctx, const double *rho, double *C, femh_stats *stats);
+
+/* Same, with the Young's modulus of every voxel given directly. */
+int femh_homogenize_modulus(struct femh_ctx *ctx, const double *E, double *C, femh_stats *stats);
+
+/* Device-pointer variant: d_rho is device memory (V doubles), d_C device
+ * memory (36 doubles); work is queued on `stream` (a hipStream_t, NULL =
+ * default stream) and the call returns after it completes. */
+int femh_homogenize_device(struct femh_ctx *ctx, const double *d_rho, double *d_C, void *stream,
+                           femh_stats *stats);
+
+/* dJ/drho for every voxel from the fields of the last homogenize call:
+ * dJdrho[e] = sum_ij dJdC[6i+j] * E'(rho_e) * chi_i^T K0 chi_j / V.
+ * Returns FEMH_NO_FIELDS when no solve has been done. */
+int femh_gradient(struct femh_ctx *ctx, const double *dJdC, double *dJdrho);
+
+/* Per-voxel energy cross terms W[e*21 + p] = chi_i^T K0 chi_j / V, p over i<=j
+ * (the unscaled adjoint building block). */
+int femh_energy_terms(struct femh_ctx *ctx, double *W);
+
+/* Copies the 6 solved periodic fluctuation fields (6*3*V doubles). */
+int femh_get_fields(struct femh_ctx *ctx, double *u);
+
+/* Batch: n samples, sample s has dims[3s..3s+2] and modulus grid E[s] (host).
+ * C receives 36*n doubles.  Material nu/Emin from the context options.
+ * stats (may be NULL) receives aggregate timings; iters[0] = max iterations. */
+int femh_homogenize_batch(struct femh_ctx *ctx, int n, const int *dims, const double *const *E,
+                          double *C, femh_stats *stats);
+
+/* Test hooks (operator-level parity, no solve): y = K u on the fine grid or
+ * on multigrid level `level` of the hierarchy built from the last
+ * homogenize/prepare call; u,y host, component-major 3*V_level doubles. */
+int femh_prepare(struct femh_ctx *ctx, const double *E);
+int femh_apply_level(struct femh_ctx *ctx, int level, const double *u, double *y);
+int femh_level_dims(struct femh_ctx *ctx, int level, int *dims);
+
+/* Copies the last error message (NUL-terminated) into buf. Returns the
+ * status of the last failed call on this context. */
+int femh_last_error(struct femh_ctx *ctx, char *buf, size_t len);
+
+int femh_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
