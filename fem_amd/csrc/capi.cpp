@@ -46,6 +46,13 @@ extern "C" {
 
 int femh_abi_version(void) { return FEMH_ABI_VERSION; }
 
+int femh_element_stiffness(double E, double nu, double *Ke) {
+  if (!Ke || !(nu > -1.0 && nu < 0.5) || !(E >= 0)) return FEMH_BAD_ARG;
+  femh::ElementTables t = femh::build_tables(nu);
+  for (int q = 0; q < 576; q++) Ke[q] = E * t.Ke[q];
+  return FEMH_OK;
+}
+
 void femh_default_opts(femh_opts *o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
@@ -67,6 +74,9 @@ void femh_default_opts(femh_opts *o) {
   o->ch_tol = 0.0;
   o->check_every = 1;
   o->device = 0;
+  o->inner_tol = 1e-4;
+  o->max_outer = 12;
+  o->apply_kernel = 0;
 }
 
 int femh_create(const femh_opts *o, femh_ctx **out) {
@@ -214,6 +224,25 @@ int femh_level_dims(femh_ctx *ctx, int level, int *dims) {
   dims[1] = c.h64.lv[level].g.ny;
   dims[2] = c.h64.lv[level].g.nz;
   return FEMH_OK;
+}
+
+int femh_set_profiling(femh_ctx *ctx, int enable) {
+  if (int s = need_ctx(ctx)) return s;
+  return guarded(ctx, [&] { ctx->c->set_profiling(enable != 0); });
+}
+
+int femh_kernel_stats(femh_ctx *ctx, int which, long long *launches, double *ms, double *bytes,
+                      double *flops) {
+  if (int s = need_ctx(ctx)) return s;
+  if (which < 0 || which > 1) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] {
+    ctx->c->flush_profile();
+    const auto &k = ctx->c->kst[which];
+    if (launches) *launches = k.launches;
+    if (ms) *ms = k.ms;
+    if (bytes) *bytes = k.bytes;
+    if (flops) *flops = k.flops;
+  });
 }
 
 int femh_last_error(femh_ctx *ctx, char *buf, size_t len) {

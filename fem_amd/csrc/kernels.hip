@@ -33,40 +33,26 @@ __device__ double block_sum(double v) {
   return res;
 }
 
-// Publish a block partial for (slot, idx); the last block sums all partials
-// in index order and stores the scalar.  All threads must call.
-__device__ void reduce_finish(const Reduce &red, int slot, int idx, double bsum) {
-  __shared__ int am_last;
-  const size_t base = ((size_t)slot * kSlotWidth + idx) * kMaxPartials;
-  unsigned *ctr = red.counters + slot * kSlotWidth + idx;
-  if (threadIdx.x == 0) {
-    red.partials[base + blockIdx.x] = bsum;
-    __threadfence();
-    unsigned t = atomicAdd(ctr, 1u);
-    am_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (am_last) {
-    __threadfence();
-    const volatile double *pp = red.partials + base;
-    double s = 0;
-    // fixed order: thread t sums t, t+256, ... then the block tree
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) s += pp[i];
-    s = block_sum(s);
-    if (threadIdx.x == 0) {
-      red.scalars[slot * kSlotWidth + idx] = s;
-      *ctr = 0u;
-    }
-  }
+// Block partial of a reduction: thread 0 stores the block's sum at
+// partials[slot][idx][pidx].  No atomics and no device-scope fences: the
+// partials are consumed by a later kernel (kernel boundaries order them).
+__device__ __forceinline__ void block_partial(const Reduce &red, int slot, int idx, int pidx,
+                                              double bsum) {
+  if (threadIdx.x == 0) red.partials[((size_t)slot * kSlotWidth + idx) * kMaxPartials + pidx] = bsum;
 }
 
-__device__ __forceinline__ double scal(const Reduce &red, int slot, int idx) {
-  return red.scalars[slot * kSlotWidth + idx];
+// Sum of the first np partials of (slot, idx) in a fixed order; every thread
+// of the block receives the value.  All threads must call.
+__device__ double sum_parts(const Reduce &red, int slot, int idx, int np) {
+  const double *pp = red.partials + ((size_t)slot * kSlotWidth + idx) * kMaxPartials;
+  double s = 0;
+  for (int i = threadIdx.x; i < np; i += kBlock) s += pp[i];
+  return block_sum(s);
 }
 
 static inline int grid_for(int V) {
   int b = (V + kBlock - 1) / kBlock;
-  return b < kMaxPartials ? (b < 1 ? 1 : b) : kMaxPartials;
+  return b < kRedBlocks ? (b < 1 ? 1 : b) : kRedBlocks;
 }
 
 __device__ __forceinline__ void node_ijk(const Dims &g, int n, int &i, int &j, int &k) {
@@ -132,7 +118,199 @@ __global__ __launch_bounds__(kBlock) void k_fine_apply(FineOp<T> op, const T *__
   }
   if (dot) {
     double s = block_sum(dv);
-    reduce_finish(red, slot, r, s);
+    block_partial(red, slot, r, blockIdx.x, s);
+  }
+}
+
+
+// ------------------------------------------------------------------------
+// fine level, tiled Walsh-Hadamard form (default).
+//
+// K0 = H^T Lambda H per voxel (see tables.hpp).  A workgroup owns a 16x16
+// tile of node columns (j,k) and marches over a chunk of TI planes in i.
+// Per element slab (planes s, s+1):
+//   1. the next node plane of the tile plus a one-node halo on the low j/k
+//      sides (18x18 nodes, periodic wrap) is staged into LDS;
+//   2. each of the 17x17 voxels touching the tile transforms its 24 corner
+//      values to cube modes (3 butterfly stages per component), applies the
+//      45-term mode coupling scaled by E, transforms back, and leaves its 24
+//      corner forces in LDS;
+//   3. each tile node sums the 4 in-plane voxels' corner forces: plane s is
+//      then complete (the slab below was added in the previous step) and is
+//      written; the plane s+1 share is carried in registers.
+// The j/k halo voxels are recomputed by neighbouring tiles (17^2/16^2 work),
+// the i halo slab once per chunk ((TI+1)/TI).
+constexpr int kTile = 16;
+constexpr int kTN = kTile + 2;        // staged nodes per row
+constexpr int kTE = kTile + 1;        // voxels per row
+constexpr int kEOStride = 25;         // LDS stride of one voxel's 24 forces (odd: no bank conflicts)
+
+template <typename T>
+struct ModeT {
+  T diag[3][8];
+  T cross[3][3][8];
+};
+
+template <typename T>
+__device__ __forceinline__ void hadamard_voxel(const T *__restrict__ u, T E, const ModeT<T> &m,
+                                               T *__restrict__ f) {
+  T w[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int v = 0; v < 8; v++) w[c][v] = u[3 * v + c];
+  // forward: per axis (a, b) -> (a + b, b - a)
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int mb = 4; mb >= 1; mb >>= 1)
+#pragma unroll
+      for (int v = 0; v < 8; v++)
+        if (!(v & mb)) {
+          const T a = w[c][v], b = w[c][v | mb];
+          w[c][v] = a + b;
+          w[c][v | mb] = b - a;
+        }
+  // mode coupling (mode 0 is a rigid translation: no force)
+  T z[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    z[c][0] = 0;
+#pragma unroll
+    for (int sm = 1; sm < 8; sm++) {
+      T acc = m.diag[c][sm] * w[c][sm];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        if (d == c) continue;
+        const int bc = (sm >> (2 - c)) & 1, bd = (sm >> (2 - d)) & 1;
+        if (bc == bd) continue;
+        const int t = sm ^ (1 << (2 - c)) ^ (1 << (2 - d));
+        acc += m.cross[c][d][sm] * w[d][t];
+      }
+      z[c][sm] = E * acc;
+    }
+  }
+  // inverse: per axis (a, b) -> (a - b, a + b)
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int mb = 4; mb >= 1; mb >>= 1)
+#pragma unroll
+      for (int v = 0; v < 8; v++)
+        if (!(v & mb)) {
+          const T a = z[c][v], b = z[c][v | mb];
+          z[c][v] = a - b;
+          z[c][v | mb] = a + b;
+        }
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int v = 0; v < 8; v++) f[3 * v + c] = z[c][v];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fine_apply_tiled(Dims g, const T *__restrict__ E,
+                                                             ModeT<T> mc, const T *__restrict__ u,
+                                                             T *__restrict__ y,
+                                                             const T *__restrict__ b, int mode,
+                                                             unsigned mask, Reduce red, int slot,
+                                                             int dot, int TI, int tiles_k) {
+  const int r = blockIdx.z;
+  if (!((mask >> r) & 1u)) return;
+  __shared__ T up[2][3][kTN * kTN];
+  __shared__ T eo[kTE * kTE * kEOStride];
+  const int tid = threadIdx.x;
+  const int j0 = (blockIdx.x / tiles_k) * kTile, k0 = (blockIdx.x % tiles_k) * kTile;
+  const int i0 = blockIdx.y * TI;
+  const int iend = min(i0 + TI, g.nx);
+  const size_t off = (size_t)r * 3 * g.V;
+  const T *__restrict__ ur = u + off;
+  const int plane = g.ny * g.nz;
+
+  auto stage = [&](int buf, int ip) {
+    const int iw = ((ip % g.nx) + g.nx) % g.nx;
+    for (int t = tid; t < kTN * kTN; t += kBlock) {
+      const int nj = t / kTN, nk = t - nj * kTN;
+      const int jj = (j0 - 1 + nj + g.ny) % g.ny, kk = (k0 - 1 + nk + g.nz) % g.nz;
+      const int n = iw * plane + jj * g.nz + kk;
+      up[buf][0][t] = ur[n];
+      up[buf][1][t] = ur[g.V + n];
+      up[buf][2][t] = ur[2 * g.V + n];
+    }
+  };
+
+  // node owned by this thread
+  const int nj = tid / kTile, nk = tid % kTile;  // output tile coords
+  const int gj = j0 + nj, gk = k0 + nk;
+  const bool owns = (gj < g.ny) && (gk < g.nz);
+  T acc[3] = {0, 0, 0};
+  double dv = 0;
+
+  stage(0, i0 - 1);
+  for (int sl = i0 - 1, p = 0; sl < iend; sl++, p++) {
+    const int cur = p & 1, nxt = cur ^ 1;
+    stage(nxt, sl + 1);
+    __syncthreads();
+    const int sw = ((sl % g.nx) + g.nx) % g.nx;
+    for (int e = tid; e < kTE * kTE; e += kBlock) {
+      const int ej = e / kTE, ek = e - ej * kTE;
+      T uv[24], fv[24];
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        const int bi = (v >> 2) & 1, bj = (v >> 1) & 1, bk = v & 1;
+        const int t = (ej + bj) * kTN + ek + bk;
+        const int bb = bi ? nxt : cur;
+        uv[3 * v] = up[bb][0][t];
+        uv[3 * v + 1] = up[bb][1][t];
+        uv[3 * v + 2] = up[bb][2][t];
+      }
+      const int jj = (j0 - 1 + ej + g.ny) % g.ny, kk = (k0 - 1 + ek + g.nz) % g.nz;
+      const T Ee = E[sw * plane + jj * g.nz + kk];
+      hadamard_voxel<T>(uv, Ee, mc, fv);
+      T *dst = eo + e * kEOStride;
+#pragma unroll
+      for (int q = 0; q < 24; q++) dst[q] = fv[q];
+    }
+    __syncthreads();
+    // node phase: node tile coords (nj+1, nk+1); voxels (nj+dj, nk+dk), corner (1-dj, 1-dk)
+    T f0[3] = {0, 0, 0}, f1[3] = {0, 0, 0};
+#pragma unroll
+    for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+      for (int dk = 0; dk < 2; dk++) {
+        const T *src = eo + ((nj + dj) * kTE + nk + dk) * kEOStride;
+        const int v0 = 2 * (1 - dj) + (1 - dk);  // bi = 0
+        const int v1 = 4 + v0;                   // bi = 1
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          f0[c] += src[3 * v0 + c];
+          f1[c] += src[3 * v1 + c];
+        }
+      }
+    if (sl >= i0 && owns) {
+      const int n = sl * plane + gj * g.nz + gk;
+      T yv[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) yv[c] = acc[c] + f0[c];
+      if (mode == 1) {
+        const T *br = b + off;
+#pragma unroll
+        for (int c = 0; c < 3; c++) yv[c] = br[(size_t)c * g.V + n] - yv[c];
+      }
+      T *yr = y + off;
+#pragma unroll
+      for (int c = 0; c < 3; c++) yr[(size_t)c * g.V + n] = yv[c];
+      if (dot) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) dv += (double)ur[(size_t)c * g.V + n] * (double)yv[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[c] = f1[c];
+  }
+  if (dot) {
+    const double s = block_sum(dv);
+    block_partial(red, slot, r, blockIdx.y * gridDim.x + blockIdx.x, s);
   }
 }
 
@@ -151,6 +329,11 @@ __global__ __launch_bounds__(kBlock) void k_fine_diag(FineOp<T> op, T *__restric
 #pragma unroll
       for (int c = 0; c < 3; c++) acc[c] += Ee * (double)op.Ke[(3 * a + c) * 25];
     }
+    // a node enclosed by zero-modulus voxels carries no stiffness: unit
+    // diagonal keeps Jacobi/Chebyshev finite there (its force is always 0)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      if (!(acc[c] > 0.0)) acc[c] = 1.0;
     d[n] = (T)acc[0];
     d[g.V + n] = (T)acc[1];
     d[2 * g.V + n] = (T)acc[2];
@@ -301,7 +484,10 @@ __global__ __launch_bounds__(kBlock) void k_stencil_diag(StencilOp<T> op, T *__r
   const Dims g = op.g;
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
 #pragma unroll
-    for (int c = 0; c < 3; c++) d[(size_t)c * g.V + n] = op.S[(size_t)(13 * 9 + 4 * c) * g.V + n];
+    for (int c = 0; c < 3; c++) {
+      const T v = op.S[(size_t)(13 * 9 + 4 * c) * g.V + n];
+      d[(size_t)c * g.V + n] = v > (T)0 ? v : (T)1;
+    }
   }
 }
 
@@ -414,11 +600,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int V, T *__restrict__ x,
                                                        const T *__restrict__ q,
                                                        const T *__restrict__ D, T *__restrict__ z,
                                                        int rzslot, int rz_out, unsigned mask,
-                                                       Reduce red) {
+                                                       Reduce red, int np_pq, int np_rz) {
   const int rr = blockIdx.y;
   if (!((mask >> rr) & 1u)) return;
-  const double pq = scal(red, S_PQ, rr);
-  const double alpha = pq != 0.0 ? scal(red, rzslot, rr) / pq : 0.0;
+  const double pq = sum_parts(red, S_PQ, rr, np_pq);
+  const double rz = sum_parts(red, rzslot, rr, np_rz);
+  const double alpha = pq != 0.0 ? rz / pq : 0.0;
   const T a = (T)alpha;
   const size_t off = (size_t)rr * 3 * V;
   double srr = 0, srz = 0;
@@ -434,10 +621,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int V, T *__restrict__ x,
     }
   }
   double s = block_sum(srr);
-  reduce_finish(red, S_RR, rr, s);
+  block_partial(red, S_RR, rr, blockIdx.x, s);
   if (D) {
     s = block_sum(srz);
-    reduce_finish(red, rz_out, rr, s);
+    block_partial(red, rz_out, rr, blockIdx.x, s);
   }
 }
 
@@ -445,13 +632,14 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_p_update(int V, T *__restrict__ p,
                                                      const T *__restrict__ z, int rz_new,
                                                      int rz_old, int first, unsigned mask,
-                                                     Reduce red) {
+                                                     Reduce red, int np) {
   const int rr = blockIdx.y;
   if (!((mask >> rr) & 1u)) return;
   double beta = 0;
   if (!first) {
-    const double den = scal(red, rz_old, rr);
-    beta = den != 0.0 ? scal(red, rz_new, rr) / den : 0.0;
+    const double den = sum_parts(red, rz_old, rr, np);
+    const double num = sum_parts(red, rz_new, rr, np);
+    beta = den != 0.0 ? num / den : 0.0;
   }
   const T bt = (T)beta;
   const size_t off = (size_t)rr * 3 * V;
@@ -470,7 +658,7 @@ __global__ __launch_bounds__(kBlock) void k_dot(int V, const T *__restrict__ a,
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
     s += (double)a[off + t] * (double)b[off + t];
   s = block_sum(s);
-  reduce_finish(red, slot, rr, s);
+  block_partial(red, slot, rr, blockIdx.x, s);
 }
 
 template <typename Ts, typename Td>
@@ -498,15 +686,15 @@ __global__ __launch_bounds__(kBlock) void k_mean_sum(int V, const T *__restrict_
   double s = 0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock) s += (double)xc[t];
   s = block_sum(s);
-  reduce_finish(red, S_AUX, blockIdx.y, s);
+  block_partial(red, S_AUX, blockIdx.y, blockIdx.x, s);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mean_sub(int V, T *__restrict__ x, unsigned mask,
-                                                     Reduce red) {
+                                                     Reduce red, int np) {
   const int rr = blockIdx.y / 3, c = blockIdx.y % 3;
   if (!((mask >> rr) & 1u)) return;
-  const T m = (T)(scal(red, S_AUX, blockIdx.y) / (double)V);
+  const T m = (T)(sum_parts(red, S_AUX, blockIdx.y, np) / (double)V);
   T *xc = x + (size_t)rr * 3 * V + (size_t)c * V;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock) xc[t] -= m;
 }
@@ -533,13 +721,13 @@ __global__ __launch_bounds__(kBlock) void k_scale_dinv_norm(int V, T *__restrict
     s += (double)v * (double)v;
   }
   s = block_sum(s);
-  reduce_finish(red, S_AUX, 0, s);
+  block_partial(red, S_AUX, 0, blockIdx.x, s);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_normalize(int V, const T *__restrict__ y,
-                                                      T *__restrict__ x, Reduce red) {
-  const double nrm = sqrt(scal(red, S_AUX, 0));
+                                                      T *__restrict__ x, Reduce red, int np) {
+  const double nrm = sqrt(sum_parts(red, S_AUX, 0, np));
   const double inv = nrm > 0 ? 1.0 / nrm : 0.0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
     x[t] = (T)((double)y[t] * inv);
@@ -561,6 +749,26 @@ __device__ __forceinline__ double interp_dE(double rho, int interp, double E0, d
     return E0 * 3.0 / (den * den);
   }
   return 1.0;
+}
+
+// partials into S_AUX: idx 0 sum E, idx 1 non-finite count, idx 2 negative count
+__global__ __launch_bounds__(kBlock) void k_modulus_stats(int V, const double *__restrict__ E,
+                                                          Reduce red) {
+  double s = 0, nf = 0, ng = 0;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock) {
+    const double e = E[t];
+    if (!isfinite(e)) nf += 1.0;
+    else {
+      s += e;
+      if (e < 0) ng += 1.0;
+    }
+  }
+  double b = block_sum(s);
+  block_partial(red, S_AUX, 0, blockIdx.x, b);
+  b = block_sum(nf);
+  block_partial(red, S_AUX, 1, blockIdx.x, b);
+  b = block_sum(ng);
+  block_partial(red, S_AUX, 2, blockIdx.x, b);
 }
 
 __global__ __launch_bounds__(kBlock) void k_material(int V, const double *__restrict__ rho,
@@ -672,6 +880,12 @@ __global__ __launch_bounds__(kBlock) void k_gradient(Dims g, const double *__res
   }
 }
 
+// scalars[slot][idx] = sum of the np partials, for idx < nidx (host reads)
+__global__ __launch_bounds__(kBlock) void k_finalize(Reduce red, int slot, int np) {
+  const double s = sum_parts(red, slot, blockIdx.x, np);
+  if (threadIdx.x == 0) red.scalars[slot * kSlotWidth + blockIdx.x] = s;
+}
+
 __global__ void k_reduce_init(unsigned *ctr, double *scal, int n) {
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     ctr[t] = 0u;
@@ -683,12 +897,44 @@ __global__ void k_reduce_init(unsigned *ctr, double *scal, int n) {
 // launch wrappers
 static inline dim3 grid2(int V, int y) { return dim3(grid_for(V), y); }
 
+int fine_tiled_ichunk(const Dims &g) {
+  const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
+  // enough workgroups to fill the chip, at most kMaxPartials per RHS
+  int chunks = (1024 + tiles - 1) / tiles;
+  if (chunks > g.nx) chunks = g.nx;
+  while (tiles * chunks > kMaxPartials) chunks--;
+  if (chunks < 1) chunks = 1;
+  return (g.nx + chunks - 1) / chunks;  // planes per chunk
+}
+
+int fine_apply_parts(const Dims &g, int kernel) {
+  if (kernel == 1) return grid_for(g.V);
+  const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
+  const int TI = fine_tiled_ichunk(g);
+  return tiles * ((g.nx + TI - 1) / TI);
+}
+
 template <typename T>
 void launch_fine_apply(const FineOp<T> &op, const T *u, T *y, const T *b, int mode, unsigned mask,
                        const Reduce *red, int slot, hipStream_t s) {
   Reduce rr = red ? *red : Reduce{nullptr, nullptr, nullptr};
-  hipLaunchKernelGGL(k_fine_apply<T>, grid2(op.g.V, kRHS), dim3(kBlock), 0, s, op, u, y, b, mode,
-                     mask, rr, slot, red ? 1 : 0);
+  if (op.kernel == 1) {
+    hipLaunchKernelGGL(k_fine_apply<T>, grid2(op.g.V, kRHS), dim3(kBlock), 0, s, op, u, y, b, mode,
+                       mask, rr, slot, red ? 1 : 0);
+    return;
+  }
+  const Dims &g = op.g;
+  const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
+  const int TI = fine_tiled_ichunk(g);
+  ModeT<T> mc;
+  for (int c = 0; c < 3; c++)
+    for (int v = 0; v < 8; v++) {
+      mc.diag[c][v] = (T)op.mode->diag[c][v];
+      for (int d = 0; d < 3; d++) mc.cross[c][d][v] = (T)op.mode->cross[c][d][v];
+    }
+  dim3 grid(tiles_j * tiles_k, (g.nx + TI - 1) / TI, kRHS);
+  hipLaunchKernelGGL(k_fine_apply_tiled<T>, grid, dim3(kBlock), 0, s, g, op.E, mc, u, y, b, mode,
+                     mask, rr, slot, red ? 1 : 0, TI, tiles_k);
 }
 template <typename T>
 void launch_fine_diag(const FineOp<T> &op, T *d, hipStream_t s) {
@@ -745,15 +991,15 @@ void launch_cheb_next(int V, const T *rsrc, T *r, const T *Ad, const T *D, T *d,
 }
 template <typename T>
 void launch_pcg_update(int V, T *x, T *r, const T *p, const T *q, const T *D, T *z, int rzslot,
-                       int rz_out, unsigned mask, const Reduce &red, hipStream_t s) {
+                       int rz_out, unsigned mask, const Reduce &red, int np_pq, hipStream_t s) {
   hipLaunchKernelGGL(k_pcg_update<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, x, r, p, q, D, z,
-                     rzslot, rz_out, mask, red);
+                     rzslot, rz_out, mask, red, np_pq, grid_for(3 * V));
 }
 template <typename T>
 void launch_p_update(int V, T *p, const T *z, int rz_new, int rz_old, int first, unsigned mask,
                      const Reduce &red, hipStream_t s) {
   hipLaunchKernelGGL(k_p_update<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, p, z, rz_new,
-                     rz_old, first, mask, red);
+                     rz_old, first, mask, red, grid_for(3 * V));
 }
 template <typename T>
 void launch_dot(int V, const T *a, const T *b, int slot, unsigned mask, const Reduce &red,
@@ -773,7 +1019,8 @@ void launch_add_f32_to_f64(size_t, const float *d, double *x, unsigned mask, int
 template <typename T>
 void launch_remove_mean(int V, T *x, unsigned mask, const Reduce &red, hipStream_t s) {
   hipLaunchKernelGGL(k_mean_sum<T>, grid2(V, 3 * kRHS), dim3(kBlock), 0, s, V, x, mask, red);
-  hipLaunchKernelGGL(k_mean_sub<T>, grid2(V, 3 * kRHS), dim3(kBlock), 0, s, V, x, mask, red);
+  hipLaunchKernelGGL(k_mean_sub<T>, grid2(V, 3 * kRHS), dim3(kBlock), 0, s, V, x, mask, red,
+                     grid_for(V));
 }
 template <typename T>
 void launch_fill_hash(size_t n, T *x, unsigned long long seed, hipStream_t s) {
@@ -787,13 +1034,19 @@ void launch_scale_by_dinv_norm(int V, T *y, const T *D, const Reduce &red, hipSt
 }
 template <typename T>
 void launch_normalize(int V, const T *y, T *x, const Reduce &red, hipStream_t s) {
-  hipLaunchKernelGGL(k_normalize<T>, dim3(grid_for(3 * V)), dim3(kBlock), 0, s, V, y, x, red);
+  hipLaunchKernelGGL(k_normalize<T>, dim3(grid_for(3 * V)), dim3(kBlock), 0, s, V, y, x, red,
+                     grid_for(3 * V));
 }
 void launch_material(int V, const double *rho, double *E, int interp, double E0, double Emin,
                      double p, hipStream_t s) {
   hipLaunchKernelGGL(k_material, dim3(grid_for(V)), dim3(kBlock), 0, s, V, rho, E, interp, E0,
                      Emin, p);
 }
+int launch_modulus_stats(int V, const double *E, const Reduce &red, hipStream_t s) {
+  hipLaunchKernelGGL(k_modulus_stats, dim3(grid_for(V)), dim3(kBlock), 0, s, V, E, red);
+  return grid_for(V);
+}
+
 void launch_ch_energy(Dims g, const double *E, const double *Ke, const double *uaff, const double *u,
                       double *partials, int nblk, hipStream_t s) {
   hipLaunchKernelGGL(k_ch_energy, dim3(nblk), dim3(kBlock), 0, s, g, E, Ke, uaff, u, partials);
@@ -807,6 +1060,12 @@ void launch_gradient(Dims g, const double *rho, const double *Ke, const double *
   hipLaunchKernelGGL(k_gradient, dim3(grid_for(g.V)), dim3(kBlock), 0, s, g, rho, Ke, uaff, u, Gs,
                      interp, E0, Emin, p, energy_only, out);
 }
+int vec_parts(int V) { return grid_for(3 * V); }
+
+void launch_finalize(const Reduce &red, int slot, int nidx, int np, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(nidx), dim3(kBlock), 0, s, red, slot, np);
+}
+
 void reduce_init(Reduce &red, hipStream_t s) {
   hipLaunchKernelGGL(k_reduce_init, dim3(1), dim3(kBlock), 0, s, red.counters, red.scalars,
                      kNumSlots * kSlotWidth);
@@ -830,7 +1089,7 @@ void reduce_init(Reduce &red, hipStream_t s) {
   template void launch_cheb_next<T>(int, const T *, T *, const T *, const T *, T *, T *, double,  \
                                     double, unsigned, hipStream_t);                               \
   template void launch_pcg_update<T>(int, T *, T *, const T *, const T *, const T *, T *, int,    \
-                                     int, unsigned, const Reduce &, hipStream_t);                 \
+                                     int, unsigned, const Reduce &, int, hipStream_t);            \
   template void launch_p_update<T>(int, T *, const T *, int, int, int, unsigned, const Reduce &,  \
                                    hipStream_t);                                                  \
   template void launch_dot<T>(int, const T *, const T *, int, unsigned, const Reduce &,           \

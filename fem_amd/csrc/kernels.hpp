@@ -7,9 +7,9 @@
 //   coarse operators  27-point block stencils, S[(off*9 + 3*c + d)*V + node],
 //                     off = (di+1)*9 + (dj+1)*3 + (dk+1) for the neighbour
 //                     (i+di, j+dj, k+dk) with periodic wrap
-// Reductions are FP64 and deterministic: each block writes a partial sum; the
-// last block to finish (device-scope counter) sums the partials in a fixed
-// order into a per-RHS scalar slot.
+// Reductions are FP64 and deterministic: each block writes a partial sum and
+// the consuming kernel (or launch_finalize, for host reads) sums the partials
+// in a fixed order.  No atomics or device-scope fences are involved.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -20,6 +20,7 @@ namespace femh {
 constexpr int kRHS = 6;
 constexpr int kBlock = 256;
 constexpr int kMaxPartials = 1024;  // partial sums per RHS per reduction
+constexpr int kRedBlocks = 256;     // blocks per RHS of the vector reductions
 constexpr int kSlotWidth = 18;      // scalars per slot: RHS x component
 
 struct Dims {
@@ -35,7 +36,7 @@ enum ScalarSlot : int {
   S_RR = 3,   // r . r
   S_BB = 4,   // b . b
   S_AUX = 5,  // power iteration / misc
-  S_RZOLD = 6,
+  S_XR = 6,   // x . r (C_H estimate)
   S_BX = 7,   // b . x (C_H estimate)
   kNumSlots = 8
 };
@@ -55,8 +56,10 @@ struct ModeCoef {
 template <typename T>
 struct FineOp {
   Dims g;
-  const T *E;   // [V]
-  const T *Ke;  // [576] unit stiffness in precision T
+  const T *E;              // [V]
+  const T *Ke;             // [576] unit stiffness in precision T (dense kernel)
+  const ModeCoef *mode;    // host: cube-mode coupling (tiled kernel)
+  int kernel;              // 0 tiled Walsh-Hadamard (default), 1 dense node gather
 };
 
 template <typename T>
@@ -68,6 +71,9 @@ struct StencilOp {
 // ---- fine level ---------------------------------------------------------
 // y = A u (mode 0) or y = b - A u (mode 1) for the RHS set in mask.
 // dot != 0: also reduces u . y into slot `slot`.
+int fine_apply_parts(const Dims &g, int kernel);  // partials written by a dot-apply
+int vec_parts(int V);                             // partials of 3V-vector reductions
+void launch_finalize(const Reduce &red, int slot, int nidx, int np, hipStream_t s);
 template <typename T>
 void launch_fine_apply(const FineOp<T> &op, const T *u, T *y, const T *b, int mode, unsigned mask,
                        const Reduce *red, int slot, hipStream_t s);
@@ -109,7 +115,7 @@ void launch_cheb_next(int V, const T *rsrc, T *r, const T *Ad, const T *D, T *d,
 //      jacobi: z = r / D and rz -> slot rz_out
 template <typename T>
 void launch_pcg_update(int V, T *x, T *r, const T *p, const T *q, const T *D, T *z, int rzslot,
-                       int rz_out, unsigned mask, const Reduce &red, hipStream_t s);
+                       int rz_out, unsigned mask, const Reduce &red, int np_pq, hipStream_t s);
 // p = z + beta p, beta = rz[new]/rz[old] (first: p = z)
 template <typename T>
 void launch_p_update(int V, T *p, const T *z, int rz_new, int rz_old, int first, unsigned mask,
@@ -139,6 +145,8 @@ void launch_normalize(int V, const T *y, T *x, const Reduce &red, hipStream_t s)
 // E = Emin + (E0-Emin) rho^p (interp 0) | E0 rho/(3-2rho) (1) | rho (2)
 void launch_material(int V, const double *rho, double *E, int interp, double E0, double Emin,
                      double p, hipStream_t s);
+// sum / non-finite count / negative count of E into S_AUX[0..2]; returns the partial count
+int launch_modulus_stats(int V, const double *E, const Reduce &red, hipStream_t s);
 // Energy-form C_H: C[6x6] accumulated over voxels; FP64 throughout.
 // out: 21 upper-triangle sums (not yet divided by V), via partials.
 void launch_ch_energy(Dims g, const double *E, const double *Ke, const double *uaff, const double *u,

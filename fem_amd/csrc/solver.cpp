@@ -62,6 +62,9 @@ Context::Context(const femh_opts &opts) : o(opts) {
   FEMH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   own_stream = true;
   tab = build_tables(o.nu);
+  std::memcpy(modecoef.diag, tab.L.diag, sizeof(modecoef.diag));
+  std::memcpy(modecoef.cross, tab.L.cross, sizeof(modecoef.cross));
+  if (o.apply_kernel < 0 || o.apply_kernel > 1) throw Error(FEMH_BAD_ARG, "unknown apply kernel");
 
   rho.alloc(sizeof(double) * g.V);
   E64.alloc(sizeof(double) * g.V);
@@ -93,7 +96,9 @@ Context::~Context() {
   if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
-void Context::read_scalars(int slot, double *out, int n) {
+void Context::read_scalars(int slot, double *out, int n, int np) {
+  launch_finalize(red, slot, n, np, stream);
+  FEMH_HIP(hipGetLastError());
   FEMH_HIP(hipMemcpyAsync(out, red.scalars + slot * kSlotWidth, sizeof(double) * n,
                           hipMemcpyDeviceToHost, stream));
   FEMH_HIP(hipStreamSynchronize(stream));
@@ -117,8 +122,22 @@ template <typename T>
 void Context::apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *bb, int mode, unsigned mask,
                     const Reduce *rd, int slot) {
   if (l == 0) {
-    FineOp<T> op{h.lv[0].g, h.E.template as<T>(), h.Ke.template as<T>()};
-    launch_fine_apply<T>(op, u, y, bb, mode, mask, rd, slot, stream);
+    FineOp<T> op{h.lv[0].g, h.E.template as<T>(), h.Ke.template as<T>(), &modecoef, o.apply_kernel};
+    if (prof) {
+      Pending pe;
+      pe.a = take_event();
+      pe.b = take_event();
+      pe.which = sizeof(T) == 8 ? 0 : 1;
+      const double R = (double)__builtin_popcount(mask & 0x3Fu), V = (double)op.g.V, sz = sizeof(T);
+      pe.bytes = V * (3.0 * sz * R * (mode == 1 ? 3.0 : 2.0) + sz);
+      pe.flops = V * R * 1152.0;
+      FEMH_HIP(hipEventRecord(pe.a, stream));
+      launch_fine_apply<T>(op, u, y, bb, mode, mask, rd, slot, stream);
+      FEMH_HIP(hipEventRecord(pe.b, stream));
+      pending.push_back(pe);
+    } else {
+      launch_fine_apply<T>(op, u, y, bb, mode, mask, rd, slot, stream);
+    }
   } else {
     StencilOp<T> op{h.lv[l].g, h.lv[l].S.template as<T>()};
     launch_stencil_apply<T>(op, u, y, bb, mode, mask, stream);
@@ -162,7 +181,7 @@ void Context::estimate_lmax(Hierarchy<double> &h) {
     }
     FEMH_HIP(hipGetLastError());
     double n2;
-    read_scalars(S_AUX, &n2, 1);
+    read_scalars(S_AUX, &n2, 1, vec_parts(V));
     L.lmax = 1.1 * std::sqrt(std::max(n2, 0.0));
     if (!(L.lmax > 0) || !std::isfinite(L.lmax)) throw Error(FEMH_NONFINITE, "operator is not finite");
   }
@@ -262,7 +281,7 @@ void Context::build_hierarchy() {
   nlevels = (int)h64.lv.size();
   FEMH_HIP(hipMemcpyAsync(h64.E.p, E64.p, sizeof(double) * g.V, hipMemcpyDeviceToDevice, stream));
   FEMH_HIP(hipMemcpyAsync(h64.Ke.p, Ke64.p, sizeof(double) * 576, hipMemcpyDeviceToDevice, stream));
-  FineOp<double> f0{g, h64.E.as<double>(), h64.Ke.as<double>()};
+  FineOp<double> f0{g, h64.E.as<double>(), h64.Ke.as<double>(), &modecoef, o.apply_kernel};
   launch_fine_diag<double>(f0, h64.lv[0].D.as<double>(), stream);
   for (int l = 1; l < nlevels; l++) {
     Level<double> &L = h64.lv[l];
@@ -370,7 +389,8 @@ void Context::precond(Hierarchy<T> &h, const T *rv, T *zv, unsigned mask) {
 // PCG over the six load cases.  Returns the mask of RHS still unconverged.
 template <typename T>
 int Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *qv, unsigned mask,
-                 double tol, int max_iter, const double *bnorm2, int *iters, double *relres) {
+                 double tol, int max_iter, const double *bnorm2, int *iters, double *relres,
+                 bool z_ready) {
   const int V = g.V;
   const bool jac = !(o.precond == FEMH_PRECOND_MG && h.lv.size() > 1);
   const T *D0 = h.lv[0].D.template as<T>();
@@ -382,18 +402,22 @@ int Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *
     relres[r] = 0.0;
   }
   if (!active) return 0;
-  precond<T>(h, rv, zv, active);
-  launch_dot<T>(V, rv, zv, S_RZ0, active, red, stream);
+  if (!z_ready) {
+    // z_ready: the caller already left z = M^-1 b and r.z in S_RZ0
+    precond<T>(h, rv, zv, active);
+    launch_dot<T>(V, rv, zv, S_RZ0, active, red, stream);
+  }
   launch_p_update<T>(V, pv, zv, S_RZ0, S_RZ0, 1, active, red, stream);
   int cur = 0;
   double rr[kRHS];
   for (int it = 1; it <= max_iter; it++) {
     const int rzc = cur ? S_RZ1 : S_RZ0, rzn = cur ? S_RZ0 : S_RZ1;
     apply<T>(h, 0, pv, qv, nullptr, 0, active, &red, S_PQ);
-    launch_pcg_update<T>(V, xv, rv, pv, qv, jac ? D0 : nullptr, zv, rzc, rzn, active, red, stream);
+    launch_pcg_update<T>(V, xv, rv, pv, qv, jac ? D0 : nullptr, zv, rzc, rzn, active, red,
+                         fine_apply_parts(g, o.apply_kernel), stream);
     FEMH_HIP(hipGetLastError());
     if (it % o.check_every == 0 || it == max_iter) {
-      read_scalars(S_RR, rr);
+      read_scalars(S_RR, rr, kRHS, vec_parts(V));
       for (int r = 0; r < kRHS; r++) {
         if (!((active >> r) & 1u)) continue;
         if (!std::isfinite(rr[r])) throw Error(FEMH_NONFINITE, "residual became non-finite");
@@ -416,6 +440,41 @@ int Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *
 }
 
 // ---------------------------------------------------------------------------
+hipEvent_t Context::take_event() {
+  if (!ev_pool.empty()) {
+    hipEvent_t e = ev_pool.back();
+    ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  FEMH_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void Context::set_profiling(bool on) {
+  flush_profile();
+  prof = on;
+  kst[0] = KStat{};
+  kst[1] = KStat{};
+}
+
+void Context::flush_profile() {
+  if (pending.empty()) return;
+  FEMH_HIP(hipStreamSynchronize(stream));
+  for (Pending &pe : pending) {
+    float ms = 0;
+    FEMH_HIP(hipEventElapsedTime(&ms, pe.a, pe.b));
+    KStat &k = kst[pe.which];
+    k.launches++;
+    k.ms += ms;
+    k.bytes += pe.bytes;
+    k.flops += pe.flops;
+    ev_pool.push_back(pe.a);
+    ev_pool.push_back(pe.b);
+  }
+  pending.clear();
+}
+
 void Context::compute_ch(const double *fields, double *C) {
   const int nblk = 1024;
   launch_ch_energy(g, E64.as<double>(), Ke64.as<double>(), uaff.as<double>(), fields,
@@ -439,28 +498,26 @@ void Context::homogenize(double *C, femh_stats *st) {
   for (auto &e : ev) FEMH_HIP(hipEventCreate(&e));
   FEMH_HIP(hipEventRecord(ev[0], stream));
   have_fields = false;
+  // validate the moduli before any device work depends on them
+  double mst[3];
+  {
+    const int np = launch_modulus_stats(g.V, E64.as<double>(), red, stream);
+    read_scalars(S_AUX, mst, 3, np);
+  }
+  if (mst[1] > 0) throw Error(FEMH_NONFINITE, "non-finite modulus");
+  if (mst[2] > 0) throw Error(FEMH_BAD_ARG, "negative modulus");
+  if (!(mst[0] > 0)) throw Error(FEMH_BAD_ARG, "all-void cell (every modulus is zero)");
+  const double sumE = mst[0];
   build_hierarchy();
-  FineOp<double> f0{g, E64.as<double>(), Ke64.as<double>()};
+  FineOp<double> f0{g, E64.as<double>(), Ke64.as<double>(), &modecoef, o.apply_kernel};
   launch_fine_rhs<double>(f0, gtab.as<double>(), b.as<double>(), stream);
   launch_dot<double>(g.V, b.as<double>(), b.as<double>(), S_BB, 0x3Fu, red, stream);
   FEMH_HIP(hipGetLastError());
   double bb[kRHS];
-  read_scalars(S_BB, bb);
+  read_scalars(S_BB, bb, kRHS, vec_parts(g.V));
   // right-hand sides that vanish (the affine field already balances, e.g. a
   // homogeneous cell) are solved by u = 0
-  double emax = 0;
-  {
-    std::vector<double> Eh(g.V);
-    FEMH_HIP(hipMemcpyAsync(Eh.data(), E64.p, sizeof(double) * g.V, hipMemcpyDeviceToHost, stream));
-    FEMH_HIP(hipStreamSynchronize(stream));
-    for (double e : Eh) {
-      if (!std::isfinite(e)) throw Error(FEMH_NONFINITE, "non-finite modulus");
-      if (e < 0) throw Error(FEMH_BAD_ARG, "negative modulus");
-      emax = std::max(emax, e);
-    }
-    if (!(emax > 0)) throw Error(FEMH_BAD_ARG, "all-void cell (every modulus is zero)");
-  }
-  const double zero_thresh = 1e-13 * emax * std::sqrt((double)g.V);
+  const double zero_thresh = 1e-13 * (sumE / g.V) * std::sqrt((double)g.V);
   unsigned mask = 0;
   for (int r = 0; r < kRHS; r++)
     if (std::sqrt(bb[r]) > zero_thresh) mask |= 1u << r;
@@ -469,6 +526,7 @@ void Context::homogenize(double *C, femh_stats *st) {
   int iters[kRHS] = {0}, outer[kRHS] = {0};
   double relres[kRHS] = {0};
   int status = FEMH_OK;
+  double ch_est = 0;
   if (o.precision == FEMH_FP64) {
     unsigned left = pcg<double>(h64, b.as<double>(), x.as<double>(), r.as<double>(), z.as<double>(),
                                 p.as<double>(), q.as<double>(), mask, o.tol, o.max_iter, bb, iters,
@@ -476,32 +534,70 @@ void Context::homogenize(double *C, femh_stats *st) {
     for (int k = 0; k < kRHS; k++) outer[k] = (mask >> k) & 1u;
     if (left) status = FEMH_NOT_CONVERGED;
   } else {
-    // FP64 iterative refinement around FP32 multigrid-PCG corrections
+    // FP64 iterative refinement around FP32 multigrid-PCG corrections.
+    // Before each sweep the preconditioned residual r.M^-1 r of every load
+    // case estimates its energy-norm error ||e||_K^2, and the energy-form C_H
+    // error is the Gram matrix of those errors: dC = E^T K E / V (PSD), so
+    // ||dC||_F <= trace(dC) = sum_i ||e_i||_K^2 / V.  With C_ii estimated as
+    // (a_ii sum E - b.x - x.r)/V, est = sqrt(6) sum_i rz_i / (V trace C) bounds
+    // the relative Frobenius error when M^-1 ~ K^-1.
     FEMH_HIP(hipMemsetAsync(x.p, 0, vec_bytes(g, sizeof(double)), stream));
     FEMH_HIP(hipMemcpyAsync(r.p, b.p, vec_bytes(g, sizeof(double)), hipMemcpyDeviceToDevice, stream));
     unsigned active = mask;
-    const double inner_tol = 1e-4;
-    const int max_outer = 12;
     double rr[kRHS];
     for (int k = 0; k < kRHS; k++) rr[k] = bb[k];
+    double aii[kRHS];
+    for (int k = 0; k < kRHS; k++) {
+      double acc = 0;
+      for (int t = 0; t < 24; t++) acc += tab.u_aff[k * 24 + t] * tab.g[k * 24 + t];
+      aii[k] = acc;
+    }
+    double bx[kRHS] = {0}, xr[kRHS] = {0};
+    const int max_outer = o.max_outer > 0 ? o.max_outer : 12;
     for (int sweep = 0; sweep < max_outer && active; sweep++) {
       launch_convert<double, float>((size_t)kRHS * 3 * g.V, r.as<double>(), b32.as<float>(), stream);
+      // z = M^-1 r and r.z for the estimate; reused as the first PCG step
+      FEMH_HIP(hipMemcpyAsync(r32.p, b32.p, vec_bytes(g, sizeof(float)), hipMemcpyDeviceToDevice, stream));
+      precond<float>(h32, r32.as<float>(), z32.as<float>(), active);
+      launch_dot<float>(g.V, r32.as<float>(), z32.as<float>(), S_RZ0, active, red, stream);
+      FEMH_HIP(hipGetLastError());
+      if (o.ch_tol > 0 && sweep > 0) {
+        double rz[kRHS];
+        read_scalars(S_RZ0, rz, kRHS, vec_parts(g.V));
+        double trC = 0, srz = 0;
+        for (int k = 0; k < kRHS; k++) {
+          trC += (aii[k] * sumE - bx[k] - xr[k]) / g.V;
+          if ((active >> k) & 1u) srz += std::max(rz[k], 0.0);
+        }
+        ch_est = (trC > 0) ? std::sqrt(6.0) * srz / (g.V * trC) : 1.0;
+        if (ch_est <= o.ch_tol) {
+          active = 0;
+          break;
+        }
+      }
       int it_in[kRHS];
       double rel_in[kRHS];
       pcg<float>(h32, b32.as<float>(), x32.as<float>(), r32.as<float>(), z32.as<float>(),
-                 p32.as<float>(), q32.as<float>(), active, inner_tol, o.max_iter, rr, it_in, rel_in);
+                 p32.as<float>(), q32.as<float>(), active, o.inner_tol > 0 ? o.inner_tol : 1e-4,
+                 o.max_iter, rr, it_in, rel_in, true);
       launch_add_f32_to_f64(0, x32.as<float>(), x.as<double>(), active, g.V, stream);
       apply<double>(h64, 0, x.as<double>(), r.as<double>(), b.as<double>(), 1, active, nullptr, 0);
       launch_dot<double>(g.V, r.as<double>(), r.as<double>(), S_RR, active, red, stream);
+      launch_dot<double>(g.V, b.as<double>(), x.as<double>(), S_BX, active, red, stream);
+      launch_dot<double>(g.V, x.as<double>(), r.as<double>(), S_XR, active, red, stream);
       FEMH_HIP(hipGetLastError());
-      double rn[kRHS];
-      read_scalars(S_RR, rn);
+      double rn[kRHS], bxn[kRHS], xrn[kRHS];
+      read_scalars(S_RR, rn, kRHS, vec_parts(g.V));
+      read_scalars(S_BX, bxn, kRHS, vec_parts(g.V));
+      read_scalars(S_XR, xrn, kRHS, vec_parts(g.V));
       for (int k = 0; k < kRHS; k++) {
         if (!((active >> k) & 1u)) continue;
         if (!std::isfinite(rn[k])) throw Error(FEMH_NONFINITE, "residual became non-finite");
         iters[k] += it_in[k];
         outer[k]++;
         rr[k] = rn[k];
+        bx[k] = bxn[k];
+        xr[k] = xrn[k];
         relres[k] = std::sqrt(rn[k] / bb[k]);
         if (relres[k] <= o.tol) active &= ~(1u << k);
       }
@@ -511,6 +607,7 @@ void Context::homogenize(double *C, femh_stats *st) {
   launch_remove_mean<double>(g.V, x.as<double>(), 0x3Fu, red, stream);
   FEMH_HIP(hipEventRecord(ev[2], stream));
   compute_ch(x.as<double>(), C);
+  flush_profile();
   have_fields = true;
   if (st) {
     std::memset(st, 0, sizeof(*st));
@@ -525,6 +622,7 @@ void Context::homogenize(double *C, femh_stats *st) {
     st->ms_setup = ms1;
     st->ms_solve = ms2;
     st->levels = nlevels;
+    st->ch_err_est = ch_est;
     st->ms_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -574,9 +672,9 @@ void Context::apply_level(int level, const double *u, double *y) {
 
 template int Context::pcg<double>(Hierarchy<double> &, const double *, double *, double *, double *,
                                   double *, double *, unsigned, double, int, const double *, int *,
-                                  double *);
+                                  double *, bool);
 template int Context::pcg<float>(Hierarchy<float> &, const float *, float *, float *, float *,
                                  float *, float *, unsigned, double, int, const double *, int *,
-                                 double *);
+                                 double *, bool);
 
 }  // namespace femh

@@ -68,6 +68,7 @@ struct Context {
   femh_opts o{};
   Dims g{};
   ElementTables tab;
+  ModeCoef modecoef{};
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string err;
@@ -86,6 +87,24 @@ struct Context {
   Hierarchy<float> h32;
   bool have_fields = false;
   int nlevels = 1;
+
+  // kernel timing
+  struct KStat {
+    long long launches = 0;
+    double ms = 0, bytes = 0, flops = 0;
+  };
+  struct Pending {
+    hipEvent_t a, b;
+    int which;
+    double bytes, flops;
+  };
+  bool prof = false;
+  KStat kst[2];
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> ev_pool;
+  void set_profiling(bool on);
+  void flush_profile();
+  hipEvent_t take_event();
 
   explicit Context(const femh_opts &opts);
   ~Context();
@@ -112,10 +131,11 @@ struct Context {
   void precond(Hierarchy<T> &h, const T *r, T *z, unsigned mask);
   template <typename T>
   int pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *qv, unsigned mask,
-          double tol, int max_iter, const double *bnorm2, int *iters, double *relres);
+          double tol, int max_iter, const double *bnorm2, int *iters, double *relres,
+          bool z_ready = false);
   void estimate_lmax(Hierarchy<double> &h);
   void dense_coarse(Hierarchy<double> &h);
-  void read_scalars(int slot, double *out, int n = kRHS);
+  void read_scalars(int slot, double *out, int n, int np);
   void compute_ch(const double *fields, double *C);
 };
 

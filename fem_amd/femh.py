@@ -26,7 +26,8 @@ PRECISION = {"fp64": 0, "mixed": 1}
 EXPORTS = ("femh_abi_version", "femh_default_opts", "femh_create", "femh_destroy",
            "femh_homogenize", "femh_homogenize_modulus", "femh_homogenize_device",
            "femh_gradient", "femh_energy_terms", "femh_get_fields", "femh_homogenize_batch",
-           "femh_prepare", "femh_apply_level", "femh_level_dims", "femh_last_error")
+           "femh_prepare", "femh_apply_level", "femh_level_dims", "femh_set_profiling",
+           "femh_kernel_stats", "femh_element_stiffness", "femh_last_error")
 
 
 class FemhError(RuntimeError):
@@ -43,7 +44,9 @@ class Opts(ctypes.Structure):
                 ("cheby_ratio", ctypes.c_double), ("coarse_iters", ctypes.c_int),
                 ("tol", ctypes.c_double), ("max_iter", ctypes.c_int),
                 ("precision", ctypes.c_int), ("ch_tol", ctypes.c_double),
-                ("check_every", ctypes.c_int), ("device", ctypes.c_int)]
+                ("check_every", ctypes.c_int), ("device", ctypes.c_int),
+                ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_int),
+                ("apply_kernel", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -87,6 +90,9 @@ def lib():
         L.femh_apply_level.argtypes = [vp, ctypes.c_int, vp, vp]
         L.femh_level_dims.argtypes = [vp, ctypes.c_int, vp]
         L.femh_last_error.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+        L.femh_set_profiling.argtypes = [vp, ctypes.c_int]
+        L.femh_kernel_stats.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
+        L.femh_element_stiffness.argtypes = [ctypes.c_double, ctypes.c_double, vp]
         _LIB = L
     return _LIB
 
@@ -97,7 +103,8 @@ def _ptr(a):
 
 def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond="mg",
               levels=0, nu1=2, nu2=2, cheby_ratio=8.0, coarse_iters=64, tol=1e-10,
-              max_iter=5000, precision="fp64", ch_tol=0.0, check_every=1, device=0):
+              max_iter=5000, precision="fp64", ch_tol=0.0, check_every=1, device=0,
+              inner_tol=1e-4, max_outer=12, apply_kernel="tiled"):
     o = Opts()
     lib().femh_default_opts(ctypes.byref(o))
     o.nx, o.ny, o.nz = (int(s) for s in shape)
@@ -109,6 +116,8 @@ def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond
     o.tol, o.max_iter = tol, max_iter
     o.precision = PRECISION[precision]
     o.ch_tol, o.check_every, o.device = ch_tol, check_every, device
+    o.inner_tol, o.max_outer = inner_tol, max_outer
+    o.apply_kernel = {"tiled": 0, "dense": 1}[apply_kernel]
     return o
 
 
@@ -192,6 +201,17 @@ class Homogenizer:
         self._check(lib().femh_get_fields(self._h, _ptr(u)))
         return u
 
+    def set_profiling(self, on=True):
+        self._check(lib().femh_set_profiling(self._h, int(bool(on))))
+
+    def kernel_stats(self, which):
+        """which: 0 = FP64 fine apply, 1 = FP32 fine apply."""
+        n = ctypes.c_longlong()
+        ms, by, fl = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(lib().femh_kernel_stats(self._h, which, ctypes.byref(n), ctypes.byref(ms),
+                                            ctypes.byref(by), ctypes.byref(fl)))
+        return dict(launches=n.value, ms=ms.value, bytes=by.value, flops=fl.value)
+
     def prepare(self, E):
         self._check(lib().femh_prepare(self._h, _ptr(self._grid(E))))
 
@@ -210,6 +230,15 @@ class Homogenizer:
         y = np.zeros_like(u)
         self._check(lib().femh_apply_level(self._h, level, _ptr(u), _ptr(y)))
         return y
+
+
+def element_stiffness(E=1.0, nu=0.3):
+    """24x24 unit-voxel stiffness from the engine's closed-form tables (host only)."""
+    K = np.zeros(576)
+    st = lib().femh_element_stiffness(E, nu, _ptr(K))
+    if st != OK:
+        raise FemhError(st, "bad material")
+    return K.reshape(24, 24)
 
 
 def homogenize_batch(grids, **kw):

@@ -86,6 +86,9 @@ typedef struct femh_opts {
                            every load case is below this (0 = use tol only) */
   int check_every;      /* iterations between host convergence checks */
   int device;           /* HIP device ordinal */
+  double inner_tol;     /* mixed: relative tolerance of each FP32 correction solve */
+  int max_outer;        /* mixed: refinement sweep cap */
+  int apply_kernel;     /* fine-grid operator: 0 tiled cube-mode (default), 1 dense gather */
 } femh_opts;
 
 typedef struct femh_stats {
@@ -142,9 +145,26 @@ int femh_prepare(struct femh_ctx *ctx, const double *E);
 int femh_apply_level(struct femh_ctx *ctx, int level, const double *u, double *y);
 int femh_level_dims(struct femh_ctx *ctx, int level, int *dims);
 
+/* Kernel timing (instrumentation, no counterpart in the reference): when
+ * enabled, every fine-grid operator launch is bracketed by HIP events on the
+ * context's stream; femh_kernel_stats returns, for which = 0 (FP64 apply) or
+ * 1 (FP32 apply), the launch count, summed device milliseconds and the
+ * algorithmic bytes / flops of those launches (bytes: u read, y written, b
+ * read for residual launches, E read once; flops: 1152 per voxel per RHS).
+ * Enabling resets the counters. */
+int femh_set_profiling(struct femh_ctx *ctx, int enable);
+int femh_kernel_stats(struct femh_ctx *ctx, int which, long long *launches, double *ms,
+                      double *bytes, double *flops);
+
 /* Copies the last error message (NUL-terminated) into buf. Returns the
  * status of the last failed call on this context. */
 int femh_last_error(struct femh_ctx *ctx, char *buf, size_t len);
+
+/* Host-only (no device work): the 24x24 trilinear hex stiffness of a unit
+ * voxel (h = 1) with Young's modulus E and Poisson ratio nu, row-major, local
+ * corner order 4x + 2y + z and dof 3*corner + component -- the matrix
+ * MaterialQuad::getStiffness returns (femlib/src/MaterialQuad.cpp:68-79). */
+int femh_element_stiffness(double E, double nu, double *Ke);
 
 int femh_abi_version(void);
 

@@ -1,0 +1,48 @@
+"""Writes tests/golden/ch_goldens.json: C_H of the seeded workloads computed by
+the CPU FP64 oracle (run here, on the CPU; the GPU tests compare against it).
+
+  binary16_s{seed}  16^3 i.i.d. Bernoulli(0.5) solid/void, E_s=1, E_v=1e-9,
+                    nu=0.3, Jacobi-PCG to ||r||/||b|| <= 1e-12
+  simp64_s0         64^3 SIMP rho = clamp(0.5+0.5 g, 1e-3, 1), g periodic GRF
+                    (corr 4 voxels), E = rho^3, nu=0.3, multigrid-PCG to 1e-12
+  simp16_s{seed}    16^3 SIMP (corr 4), for gradient checks
+Inputs are regenerated from fem_amd.workloads with the stated seeds.
+"""
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from fem_amd import workloads as W  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def main():
+    out = {}
+    for seed in (0, 1):
+        E = W.binary_modulus(W.binary_cube(16, seed))
+        r = O.homogenize(E, 0.3, precond="jacobi", tol=1e-12, max_iter=20000)
+        out[f"binary16_s{seed}"] = dict(n=16, seed=seed, kind="binary", p=0.5, E_void=1e-9, nu=0.3,
+                                        C=r["C"].ravel().tolist(), iters=r["iters"].tolist())
+        print("binary16", seed, r["iters"], flush=True)
+    for seed in (0, 1):
+        E = W.simp_modulus(W.simp_density(16, seed, 4.0))
+        r = O.homogenize(E, 0.3, precond="mg", tol=1e-12)
+        out[f"simp16_s{seed}"] = dict(n=16, seed=seed, kind="simp", corr=4.0, penal=3.0, nu=0.3,
+                                      C=r["C"].ravel().tolist(), iters=r["iters"].tolist())
+    t = time.time()
+    E = W.simp_modulus(W.simp_density(64, 0, 4.0))
+    r = O.homogenize(E, 0.3, precond="mg", tol=1e-12)
+    out["simp64_s0"] = dict(n=64, seed=0, kind="simp", corr=4.0, penal=3.0, nu=0.3,
+                            C=r["C"].ravel().tolist(), iters=r["iters"].tolist(),
+                            cpu_seconds=time.time() - t)
+    print("simp64", r["iters"], flush=True)
+    with open(os.path.join(HERE, "ch_goldens.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,55 @@
+"""CPU checks of the C ABI library: it loads, and exports every symbol
+include/femh.h declares (no device work is launched here)."""
+import os
+import re
+
+import numpy as np
+
+import fem_amd
+from fem_amd import femh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "femh.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(femh_[a-z_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = fem_amd.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(femh.EXPORTS)
+
+
+def test_abi_version_and_default_opts():
+    L = fem_amd.lib()
+    assert L.femh_abi_version() == 1
+    o = femh.make_opts((8, 8, 8))
+    assert o.nu == 0.3 and o.tol == 1e-10 and o.precond == 1 and o.nu1 == 2
+
+
+def test_opts_struct_layout_matches_header():
+    # the Python mirror must have the same field order as femh_opts
+    src = open(os.path.join(ROOT, "include", "femh.h")).read()
+    body = src[src.index("typedef struct femh_opts {"):src.index("} femh_opts;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for line in body.splitlines()[1:]:
+        line = line.strip().rstrip(";")
+        if not line:
+            continue
+        typ, rest = line.split(None, 1)
+        fields += [f.strip() for f in rest.split(",")]
+    assert fields == [f[0] for f in femh.Opts._fields_]
+
+
+def test_host_element_table_is_symmetric_psd():
+    K = femh.element_stiffness(1.0, 0.3)
+    assert np.abs(K - K.T).max() < 1e-15
+    assert np.linalg.eigvalsh(K).min() > -1e-13
+    assert femh.lib().femh_element_stiffness(1.0, 0.7, K.ctypes.data) == femh.BAD_ARG
