@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfemh.so")
 
-SOURCES = ["kernels.hip", "tables.cpp", "solver.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "ops.hip", "tables.cpp", "solver.cpp", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FEMH_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -7,60 +7,10 @@
 // order (see Reduce in kernels.hpp).
 #include <hip/hip_runtime.h>
 
+#include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace femh {
-
-// ------------------------------------------------------------------------
-// reductions
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// Sum over a 256-thread block; result valid in every thread.
-__device__ double block_sum(double v) {
-  __shared__ double sh[kBlock / 64];
-  __shared__ double res;
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) res = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-  __syncthreads();
-  return res;
-}
-
-// Block partial of a reduction: thread 0 stores the block's sum at
-// partials[slot][idx][pidx].  No atomics and no device-scope fences: the
-// partials are consumed by a later kernel (kernel boundaries order them).
-__device__ __forceinline__ void block_partial(const Reduce &red, int slot, int idx, int pidx,
-                                              double bsum) {
-  if (threadIdx.x == 0) red.partials[((size_t)slot * kSlotWidth + idx) * kMaxPartials + pidx] = bsum;
-}
-
-// Sum of the first np partials of (slot, idx) in a fixed order; every thread
-// of the block receives the value.  All threads must call.
-__device__ double sum_parts(const Reduce &red, int slot, int idx, int np) {
-  const double *pp = red.partials + ((size_t)slot * kSlotWidth + idx) * kMaxPartials;
-  double s = 0;
-  for (int i = threadIdx.x; i < np; i += kBlock) s += pp[i];
-  return block_sum(s);
-}
-
-static inline int grid_for(int V) {
-  int b = (V + kBlock - 1) / kBlock;
-  return b < kRedBlocks ? (b < 1 ? 1 : b) : kRedBlocks;
-}
-
-__device__ __forceinline__ void node_ijk(const Dims &g, int n, int &i, int &j, int &k) {
-  k = n % g.nz;
-  int t = n / g.nz;
-  j = t % g.ny;
-  i = t / g.ny;
-}
 
 // ------------------------------------------------------------------------
 // fine level: matrix-free periodic K u, node gather over the 8 incident voxels
@@ -123,197 +73,6 @@ __global__ __launch_bounds__(kBlock) void k_fine_apply(FineOp<T> op, const T *__
 }
 
 
-// ------------------------------------------------------------------------
-// fine level, tiled Walsh-Hadamard form (default).
-//
-// K0 = H^T Lambda H per voxel (see tables.hpp).  A workgroup owns a 16x16
-// tile of node columns (j,k) and marches over a chunk of TI planes in i.
-// Per element slab (planes s, s+1):
-//   1. the next node plane of the tile plus a one-node halo on the low j/k
-//      sides (18x18 nodes, periodic wrap) is staged into LDS;
-//   2. each of the 17x17 voxels touching the tile transforms its 24 corner
-//      values to cube modes (3 butterfly stages per component), applies the
-//      45-term mode coupling scaled by E, transforms back, and leaves its 24
-//      corner forces in LDS;
-//   3. each tile node sums the 4 in-plane voxels' corner forces: plane s is
-//      then complete (the slab below was added in the previous step) and is
-//      written; the plane s+1 share is carried in registers.
-// The j/k halo voxels are recomputed by neighbouring tiles (17^2/16^2 work),
-// the i halo slab once per chunk ((TI+1)/TI).
-constexpr int kTile = 16;
-constexpr int kTN = kTile + 2;        // staged nodes per row
-constexpr int kTE = kTile + 1;        // voxels per row
-constexpr int kEOStride = 25;         // LDS stride of one voxel's 24 forces (odd: no bank conflicts)
-
-template <typename T>
-struct ModeT {
-  T diag[3][8];
-  T cross[3][3][8];
-};
-
-template <typename T>
-__device__ __forceinline__ void hadamard_voxel(const T *__restrict__ u, T E, const ModeT<T> &m,
-                                               T *__restrict__ f) {
-  T w[3][8];
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int v = 0; v < 8; v++) w[c][v] = u[3 * v + c];
-  // forward: per axis (a, b) -> (a + b, b - a)
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int mb = 4; mb >= 1; mb >>= 1)
-#pragma unroll
-      for (int v = 0; v < 8; v++)
-        if (!(v & mb)) {
-          const T a = w[c][v], b = w[c][v | mb];
-          w[c][v] = a + b;
-          w[c][v | mb] = b - a;
-        }
-  // mode coupling (mode 0 is a rigid translation: no force)
-  T z[3][8];
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    z[c][0] = 0;
-#pragma unroll
-    for (int sm = 1; sm < 8; sm++) {
-      T acc = m.diag[c][sm] * w[c][sm];
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        if (d == c) continue;
-        const int bc = (sm >> (2 - c)) & 1, bd = (sm >> (2 - d)) & 1;
-        if (bc == bd) continue;
-        const int t = sm ^ (1 << (2 - c)) ^ (1 << (2 - d));
-        acc += m.cross[c][d][sm] * w[d][t];
-      }
-      z[c][sm] = E * acc;
-    }
-  }
-  // inverse: per axis (a, b) -> (a - b, a + b)
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int mb = 4; mb >= 1; mb >>= 1)
-#pragma unroll
-      for (int v = 0; v < 8; v++)
-        if (!(v & mb)) {
-          const T a = z[c][v], b = z[c][v | mb];
-          z[c][v] = a - b;
-          z[c][v | mb] = a + b;
-        }
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int v = 0; v < 8; v++) f[3 * v + c] = z[c][v];
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_fine_apply_tiled(Dims g, const T *__restrict__ E,
-                                                             ModeT<T> mc, const T *__restrict__ u,
-                                                             T *__restrict__ y,
-                                                             const T *__restrict__ b, int mode,
-                                                             unsigned mask, Reduce red, int slot,
-                                                             int dot, int TI, int tiles_k) {
-  const int r = blockIdx.z;
-  if (!((mask >> r) & 1u)) return;
-  __shared__ T up[2][3][kTN * kTN];
-  __shared__ T eo[kTE * kTE * kEOStride];
-  const int tid = threadIdx.x;
-  const int j0 = (blockIdx.x / tiles_k) * kTile, k0 = (blockIdx.x % tiles_k) * kTile;
-  const int i0 = blockIdx.y * TI;
-  const int iend = min(i0 + TI, g.nx);
-  const size_t off = (size_t)r * 3 * g.V;
-  const T *__restrict__ ur = u + off;
-  const int plane = g.ny * g.nz;
-
-  auto stage = [&](int buf, int ip) {
-    const int iw = ((ip % g.nx) + g.nx) % g.nx;
-    for (int t = tid; t < kTN * kTN; t += kBlock) {
-      const int nj = t / kTN, nk = t - nj * kTN;
-      const int jj = (j0 - 1 + nj + g.ny) % g.ny, kk = (k0 - 1 + nk + g.nz) % g.nz;
-      const int n = iw * plane + jj * g.nz + kk;
-      up[buf][0][t] = ur[n];
-      up[buf][1][t] = ur[g.V + n];
-      up[buf][2][t] = ur[2 * g.V + n];
-    }
-  };
-
-  // node owned by this thread
-  const int nj = tid / kTile, nk = tid % kTile;  // output tile coords
-  const int gj = j0 + nj, gk = k0 + nk;
-  const bool owns = (gj < g.ny) && (gk < g.nz);
-  T acc[3] = {0, 0, 0};
-  double dv = 0;
-
-  stage(0, i0 - 1);
-  for (int sl = i0 - 1, p = 0; sl < iend; sl++, p++) {
-    const int cur = p & 1, nxt = cur ^ 1;
-    stage(nxt, sl + 1);
-    __syncthreads();
-    const int sw = ((sl % g.nx) + g.nx) % g.nx;
-    for (int e = tid; e < kTE * kTE; e += kBlock) {
-      const int ej = e / kTE, ek = e - ej * kTE;
-      T uv[24], fv[24];
-#pragma unroll
-      for (int v = 0; v < 8; v++) {
-        const int bi = (v >> 2) & 1, bj = (v >> 1) & 1, bk = v & 1;
-        const int t = (ej + bj) * kTN + ek + bk;
-        const int bb = bi ? nxt : cur;
-        uv[3 * v] = up[bb][0][t];
-        uv[3 * v + 1] = up[bb][1][t];
-        uv[3 * v + 2] = up[bb][2][t];
-      }
-      const int jj = (j0 - 1 + ej + g.ny) % g.ny, kk = (k0 - 1 + ek + g.nz) % g.nz;
-      const T Ee = E[sw * plane + jj * g.nz + kk];
-      hadamard_voxel<T>(uv, Ee, mc, fv);
-      T *dst = eo + e * kEOStride;
-#pragma unroll
-      for (int q = 0; q < 24; q++) dst[q] = fv[q];
-    }
-    __syncthreads();
-    // node phase: node tile coords (nj+1, nk+1); voxels (nj+dj, nk+dk), corner (1-dj, 1-dk)
-    T f0[3] = {0, 0, 0}, f1[3] = {0, 0, 0};
-#pragma unroll
-    for (int dj = 0; dj < 2; dj++)
-#pragma unroll
-      for (int dk = 0; dk < 2; dk++) {
-        const T *src = eo + ((nj + dj) * kTE + nk + dk) * kEOStride;
-        const int v0 = 2 * (1 - dj) + (1 - dk);  // bi = 0
-        const int v1 = 4 + v0;                   // bi = 1
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          f0[c] += src[3 * v0 + c];
-          f1[c] += src[3 * v1 + c];
-        }
-      }
-    if (sl >= i0 && owns) {
-      const int n = sl * plane + gj * g.nz + gk;
-      T yv[3];
-#pragma unroll
-      for (int c = 0; c < 3; c++) yv[c] = acc[c] + f0[c];
-      if (mode == 1) {
-        const T *br = b + off;
-#pragma unroll
-        for (int c = 0; c < 3; c++) yv[c] = br[(size_t)c * g.V + n] - yv[c];
-      }
-      T *yr = y + off;
-#pragma unroll
-      for (int c = 0; c < 3; c++) yr[(size_t)c * g.V + n] = yv[c];
-      if (dot) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) dv += (double)ur[(size_t)c * g.V + n] * (double)yv[c];
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) acc[c] = f1[c];
-  }
-  if (dot) {
-    const double s = block_sum(dv);
-    block_partial(red, slot, r, blockIdx.y * gridDim.x + blockIdx.x, s);
-  }
-}
-
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fine_diag(FineOp<T> op, T *__restrict__ d) {
   const Dims g = op.g;
@@ -366,46 +125,6 @@ __global__ __launch_bounds__(kBlock) void k_fine_rhs(FineOp<T> op, const double 
 
 // ------------------------------------------------------------------------
 // coarse levels: 27-point block stencils
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_stencil_apply(StencilOp<T> op, const T *__restrict__ u,
-                                                          T *__restrict__ y,
-                                                          const T *__restrict__ b, int mode,
-                                                          unsigned mask) {
-  const int r = blockIdx.y;
-  if (!((mask >> r) & 1u)) return;
-  const Dims g = op.g;
-  const size_t off = (size_t)r * 3 * g.V;
-  const T *__restrict__ ur = u + off;
-  const T *__restrict__ S = op.S;
-  for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
-    int i, j, k;
-    node_ijk(g, n, i, j, k);
-    const int I[3] = {i == 0 ? g.nx - 1 : i - 1, i, i == g.nx - 1 ? 0 : i + 1};
-    const int J[3] = {j == 0 ? g.ny - 1 : j - 1, j, j == g.ny - 1 ? 0 : j + 1};
-    const int K[3] = {k == 0 ? g.nz - 1 : k - 1, k, k == g.nz - 1 ? 0 : k + 1};
-    T acc0 = 0, acc1 = 0, acc2 = 0;
-#pragma unroll 3
-    for (int o = 0; o < 27; o++) {
-      const int nb = (I[o / 9] * g.ny + J[(o / 3) % 3]) * g.nz + K[o % 3];
-      const T u0 = ur[nb], u1 = ur[g.V + nb], u2 = ur[2 * g.V + nb];
-      const T *s = S + (size_t)(o * 9) * g.V + n;
-      acc0 += s[0] * u0 + s[(size_t)1 * g.V] * u1 + s[(size_t)2 * g.V] * u2;
-      acc1 += s[(size_t)3 * g.V] * u0 + s[(size_t)4 * g.V] * u1 + s[(size_t)5 * g.V] * u2;
-      acc2 += s[(size_t)6 * g.V] * u0 + s[(size_t)7 * g.V] * u1 + s[(size_t)8 * g.V] * u2;
-    }
-    T *yr = y + off;
-    if (mode == 1) {
-      const T *br = b + off;
-      acc0 = br[n] - acc0;
-      acc1 = br[g.V + n] - acc1;
-      acc2 = br[2 * g.V + n] - acc2;
-    }
-    yr[n] = acc0;
-    yr[g.V + n] = acc1;
-    yr[2 * g.V + n] = acc2;
-  }
-}
-
 // Level-1 Galerkin stencil straight from the fine moduli: the coarse element
 // E_c (2x2x2 voxels) has matrix sum_f E_f M_f; thread per (coarse node, offset).
 template <typename T>
@@ -477,6 +196,12 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(Dims fg, Dims cg, const T *
 #pragma unroll
     for (int q = 0; q < 9; q++) Sc[(size_t)(o * 9 + q) * cg.V + N] = (T)blk[q];
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_invert(size_t n, const T *__restrict__ d, T *__restrict__ di) {
+  for (size_t t = blockIdx.x * (size_t)kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock)
+    di[t] = (T)1 / d[t];
 }
 
 template <typename T>
@@ -570,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb_first(int V, const T *__restric
   if (!((mask >> rr) & 1u)) return;
   const size_t off = (size_t)rr * 3 * V;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) {
-    const T dv = (T)((double)r[off + t] * inv_theta / (double)D[t]);
+    const T dv = r[off + t] * (T)inv_theta * D[t];
     d[off + t] = dv;
     x[off + t] = x_zero ? dv : x[off + t] + dv;
   }
@@ -588,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb_next(int V, const T *rsrc, T *r
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) {
     const T rv = rsrc[off + t] - Ad[off + t];
     r[off + t] = rv;
-    const T dv = (T)(c1 * (double)d[off + t] + c2 * (double)rv / (double)D[t]);
+    const T dv = (T)c1 * d[off + t] + (T)c2 * rv * D[t];
     d[off + t] = dv;
     x[off + t] += dv;
   }
@@ -615,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int V, T *__restrict__ x,
     r[off + t] = rv;
     srr += (double)rv * (double)rv;
     if (D) {
-      const T zv = rv / D[t];
+      const T zv = rv * D[t];
       z[off + t] = zv;
       srz += (double)rv * (double)zv;
     }
@@ -716,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_scale_dinv_norm(int V, T *__restrict
                                                             const T *__restrict__ D, Reduce red) {
   double s = 0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) {
-    const T v = y[t] / D[t];
+    const T v = y[t] * D[t];
     y[t] = v;
     s += (double)v * (double)v;
   }
@@ -897,45 +622,16 @@ __global__ void k_reduce_init(unsigned *ctr, double *scal, int n) {
 // launch wrappers
 static inline dim3 grid2(int V, int y) { return dim3(grid_for(V), y); }
 
-int fine_tiled_ichunk(const Dims &g) {
-  const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
-  // enough workgroups to fill the chip, at most kMaxPartials per RHS
-  int chunks = (1024 + tiles - 1) / tiles;
-  if (chunks > g.nx) chunks = g.nx;
-  while (tiles * chunks > kMaxPartials) chunks--;
-  if (chunks < 1) chunks = 1;
-  return (g.nx + chunks - 1) / chunks;  // planes per chunk
-}
-
-int fine_apply_parts(const Dims &g, int kernel) {
-  if (kernel == 1) return grid_for(g.V);
-  const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
-  const int TI = fine_tiled_ichunk(g);
-  return tiles * ((g.nx + TI - 1) / TI);
-}
-
 template <typename T>
-void launch_fine_apply(const FineOp<T> &op, const T *u, T *y, const T *b, int mode, unsigned mask,
-                       const Reduce *red, int slot, hipStream_t s) {
+void launch_dense_apply(const FineOp<T> &op, const T *u, T *y, const T *b, int mode, unsigned mask,
+                        const Reduce *red, int slot, hipStream_t s) {
   Reduce rr = red ? *red : Reduce{nullptr, nullptr, nullptr};
-  if (op.kernel == 1) {
-    hipLaunchKernelGGL(k_fine_apply<T>, grid2(op.g.V, kRHS), dim3(kBlock), 0, s, op, u, y, b, mode,
-                       mask, rr, slot, red ? 1 : 0);
-    return;
-  }
-  const Dims &g = op.g;
-  const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
-  const int TI = fine_tiled_ichunk(g);
-  ModeT<T> mc;
-  for (int c = 0; c < 3; c++)
-    for (int v = 0; v < 8; v++) {
-      mc.diag[c][v] = (T)op.mode->diag[c][v];
-      for (int d = 0; d < 3; d++) mc.cross[c][d][v] = (T)op.mode->cross[c][d][v];
-    }
-  dim3 grid(tiles_j * tiles_k, (g.nx + TI - 1) / TI, kRHS);
-  hipLaunchKernelGGL(k_fine_apply_tiled<T>, grid, dim3(kBlock), 0, s, g, op.E, mc, u, y, b, mode,
-                     mask, rr, slot, red ? 1 : 0, TI, tiles_k);
+  hipLaunchKernelGGL(k_fine_apply<T>, grid2(op.g.V, kRHS), dim3(kBlock), 0, s, op, u, y, b, mode,
+                     mask, rr, slot, red ? 1 : 0);
 }
+
+int dense_parts(const Dims &g) { return grid_for(g.V); }
+
 template <typename T>
 void launch_fine_diag(const FineOp<T> &op, T *d, hipStream_t s) {
   hipLaunchKernelGGL(k_fine_diag<T>, dim3(grid_for(op.g.V)), dim3(kBlock), 0, s, op, d);
@@ -943,12 +639,6 @@ void launch_fine_diag(const FineOp<T> &op, T *d, hipStream_t s) {
 template <typename T>
 void launch_fine_rhs(const FineOp<T> &op, const double *g, T *b, hipStream_t s) {
   hipLaunchKernelGGL(k_fine_rhs<T>, grid2(op.g.V, kRHS), dim3(kBlock), 0, s, op, g, b);
-}
-template <typename T>
-void launch_stencil_apply(const StencilOp<T> &op, const T *u, T *y, const T *b, int mode,
-                          unsigned mask, hipStream_t s) {
-  hipLaunchKernelGGL(k_stencil_apply<T>, grid2(op.g.V, kRHS), dim3(kBlock), 0, s, op, u, y, b,
-                     mode, mask);
 }
 template <typename T>
 void launch_build_level1(Dims fine, Dims coarse, const T *E, const double *M, T *S, hipStream_t s) {
@@ -1023,6 +713,13 @@ void launch_remove_mean(int V, T *x, unsigned mask, const Reduce &red, hipStream
                      grid_for(V));
 }
 template <typename T>
+void launch_invert(size_t n, const T *d, T *di, hipStream_t s) {
+  size_t b = (n + kBlock - 1) / kBlock;
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(k_invert<T>, dim3((unsigned)b), dim3(kBlock), 0, s, n, d, di);
+}
+
+template <typename T>
 void launch_fill_hash(size_t n, T *x, unsigned long long seed, hipStream_t s) {
   size_t b = (n + kBlock - 1) / kBlock;
   if (b > 4096) b = 4096;
@@ -1072,12 +769,10 @@ void reduce_init(Reduce &red, hipStream_t s) {
 }
 
 #define FEMH_INST(T)                                                                             \
-  template void launch_fine_apply<T>(const FineOp<T> &, const T *, T *, const T *, int, unsigned, \
-                                     const Reduce *, int, hipStream_t);                           \
+  template void launch_dense_apply<T>(const FineOp<T> &, const T *, T *, const T *, int, unsigned,\
+                                      const Reduce *, int, hipStream_t);                          \
   template void launch_fine_diag<T>(const FineOp<T> &, T *, hipStream_t);                         \
   template void launch_fine_rhs<T>(const FineOp<T> &, const double *, T *, hipStream_t);          \
-  template void launch_stencil_apply<T>(const StencilOp<T> &, const T *, T *, const T *, int,     \
-                                        unsigned, hipStream_t);                                   \
   template void launch_build_level1<T>(Dims, Dims, const T *, const double *, T *, hipStream_t);  \
   template void launch_galerkin<T>(Dims, Dims, const T *, T *, hipStream_t);                      \
   template void launch_stencil_diag<T>(const StencilOp<T> &, T *, hipStream_t);                   \
@@ -1096,6 +791,7 @@ void reduce_init(Reduce &red, hipStream_t s) {
                               hipStream_t);                                                       \
   template void launch_remove_mean<T>(int, T *, unsigned, const Reduce &, hipStream_t);           \
   template void launch_fill_hash<T>(size_t, T *, unsigned long long, hipStream_t);                \
+  template void launch_invert<T>(size_t, const T *, T *, hipStream_t);                            \
   template void launch_scale_by_dinv_norm<T>(int, T *, const T *, const Reduce &, hipStream_t);   \
   template void launch_normalize<T>(int, const T *, T *, const Reduce &, hipStream_t);
 

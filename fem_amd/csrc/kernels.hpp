@@ -71,11 +71,12 @@ struct StencilOp {
 // ---- fine level ---------------------------------------------------------
 // y = A u (mode 0) or y = b - A u (mode 1) for the RHS set in mask.
 // dot != 0: also reduces u . y into slot `slot`.
-int fine_apply_parts(const Dims &g, int kernel);  // partials written by a dot-apply
-int vec_parts(int V);                             // partials of 3V-vector reductions
+int dense_parts(const Dims &g);  // partials written by a dense dot-apply
+int vec_parts(int V);             // partials of 3V-vector reductions
 void launch_finalize(const Reduce &red, int slot, int nidx, int np, hipStream_t s);
+// dense node-gather operator (diagnostic / cross-check of the tiled kernel)
 template <typename T>
-void launch_fine_apply(const FineOp<T> &op, const T *u, T *y, const T *b, int mode, unsigned mask,
+void launch_dense_apply(const FineOp<T> &op, const T *u, T *y, const T *b, int mode, unsigned mask,
                        const Reduce *red, int slot, hipStream_t s);
 template <typename T>
 void launch_fine_diag(const FineOp<T> &op, T *d, hipStream_t s);
@@ -84,9 +85,6 @@ template <typename T>
 void launch_fine_rhs(const FineOp<T> &op, const double *g /*device 6x24*/, T *b, hipStream_t s);
 
 // ---- coarse levels --------------------------------------------------------
-template <typename T>
-void launch_stencil_apply(const StencilOp<T> &op, const T *u, T *y, const T *b, int mode,
-                          unsigned mask, hipStream_t s);
 template <typename T>
 void launch_build_level1(Dims fine, Dims coarse, const T *E, const double *Mchild, T *S,
                          hipStream_t s);
@@ -103,8 +101,8 @@ template <typename T>
 void launch_dense_solve(int n, const T *Ainv, const T *b, T *x, unsigned mask, hipStream_t s);
 
 // ---- vector kernels -------------------------------------------------------
-// Chebyshev smoothing steps: first d = r/(theta D), x (+)= d;
-// next r = rsrc - A d, d = c1 d + c2 r/D, x += d
+// Chebyshev smoothing steps (D arguments are inverse diagonals):
+// first d = r/(theta D), x (+)= d; next r = rsrc - A d, d = c1 d + c2 r/D, x += d
 template <typename T>
 void launch_cheb_first(int V, const T *r, const T *D, T *d, T *x, double inv_theta, int x_zero,
                        unsigned mask, hipStream_t s);
@@ -112,7 +110,7 @@ template <typename T>
 void launch_cheb_next(int V, const T *rsrc, T *r, const T *Ad, const T *D, T *d, T *x, double c1,
                       double c2, unsigned mask, hipStream_t s);
 // PCG: alpha = rz[rzslot]/pq ; x += alpha p ; r -= alpha q ; rr -> S_RR
-//      jacobi: z = r / D and rz -> slot rz_out
+//      jacobi (D = inverse diagonal): z = r / D and rz -> slot rz_out
 template <typename T>
 void launch_pcg_update(int V, T *x, T *r, const T *p, const T *q, const T *D, T *z, int rzslot,
                        int rz_out, unsigned mask, const Reduce &red, int np_pq, hipStream_t s);
@@ -133,7 +131,10 @@ void launch_add_f32_to_f64(size_t n, const float *d, double *x, unsigned mask, i
 // subtract per-RHS, per-component mean (uses reductions slot S_AUX)
 template <typename T>
 void launch_remove_mean(int V, T *x, unsigned mask, const Reduce &red, hipStream_t s);
-// power iteration helpers (RHS slot 0 only)
+// di = 1 / d
+template <typename T>
+void launch_invert(size_t n, const T *d, T *di, hipStream_t s);
+// power iteration helpers (RHS slot 0 only; D arguments are inverse diagonals)
 template <typename T>
 void launch_fill_hash(size_t n, T *x, unsigned long long seed, hipStream_t s);
 template <typename T>

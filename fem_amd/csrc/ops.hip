@@ -1,0 +1,630 @@
+// ops.hip -- fused operator kernels (see ops.hpp for the staging / epilogue
+// contract) and the single-workgroup V-cycle of the small multigrid levels.
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "ops.hpp"
+
+namespace femh {
+
+// ---------------------------------------------------------------------------
+// fine level: tiled cube-mode voxel kernel
+//
+// K0 = H^T Lambda H per voxel (tables.hpp).  A workgroup owns a 16x16 tile of
+// node columns (j,k) and marches over a chunk of TI planes in i.  Per voxel
+// slab (planes s, s+1):
+//   1. node plane s+1 of the tile plus a one-node halo on the low j/k sides
+//      (18x18 nodes, periodic wrap) is staged into LDS (three plane buffers,
+//      so the owned node's staged value stays readable for the epilogue);
+//   2. each of the 17x17 voxels touching the tile transforms its 24 corner
+//      values to cube modes (three butterfly stages per component), applies
+//      the 45-term mode coupling scaled by its modulus and transforms back;
+//   3. the voxels' plane-s corner forces go through LDS to the 4 tile nodes
+//      that share each of them: plane s is then complete and passes through
+//      the epilogue; the plane s+1 forces follow the same way and are carried
+//      in registers into the next slab.
+// The j/k halo voxels are recomputed by neighbouring tiles (17^2/16^2 of the
+// voxel work) and the i halo slab once per chunk ((TI+1)/TI).
+constexpr int kTile = 16;
+constexpr int kTN = kTile + 2;   // staged nodes per row
+constexpr int kTE = kTile + 1;   // voxels per row
+constexpr int kEOH = 13;         // LDS stride of one voxel's 12 plane forces (odd: no bank conflicts)
+
+template <typename T>
+struct ModeT {
+  T diag[3][8];
+  T cross[3][3][8];
+};
+
+template <typename T>
+__device__ __forceinline__ void hadamard_voxel(const T *__restrict__ u, T E, const ModeT<T> &m,
+                                               T *__restrict__ f) {
+  T w[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int v = 0; v < 8; v++) w[c][v] = u[3 * v + c];
+  // forward: per axis (a, b) -> (a + b, b - a)
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int mb = 4; mb >= 1; mb >>= 1)
+#pragma unroll
+      for (int v = 0; v < 8; v++)
+        if (!(v & mb)) {
+          const T a = w[c][v], b = w[c][v | mb];
+          w[c][v] = a + b;
+          w[c][v | mb] = b - a;
+        }
+  // mode coupling (mode 0 is a rigid translation: no force)
+  T z[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    z[c][0] = 0;
+#pragma unroll
+    for (int sm = 1; sm < 8; sm++) {
+      T acc = m.diag[c][sm] * w[c][sm];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        if (d == c) continue;
+        const int bc = (sm >> (2 - c)) & 1, bd = (sm >> (2 - d)) & 1;
+        if (bc == bd) continue;
+        const int t = sm ^ (1 << (2 - c)) ^ (1 << (2 - d));
+        acc += m.cross[c][d][sm] * w[d][t];
+      }
+      z[c][sm] = E * acc;
+    }
+  }
+  // inverse: per axis (a, b) -> (a - b, a + b)
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int mb = 4; mb >= 1; mb >>= 1)
+#pragma unroll
+      for (int v = 0; v < 8; v++)
+        if (!(v & mb)) {
+          const T a = z[c][v], b = z[c][v | mb];
+          z[c][v] = a - b;
+          z[c][v | mb] = a + b;
+        }
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int v = 0; v < 8; v++) f[3 * v + c] = z[c][v];
+}
+
+// epilogue at one owned node (all three components)
+template <typename T, int EP>
+__device__ __forceinline__ void epilogue(const OpArgs<T> &a, size_t off, int V, int n, const T Au[3],
+                                         const T uo[3], double &dv) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const size_t q = off + (size_t)c * V + n;
+    if (EP == EP_AX) {
+      a.y[q] = Au[c];
+      if (a.dot) dv += (double)uo[c] * (double)Au[c];
+    } else if (EP == EP_RES) {
+      a.y[q] = a.b[q] - Au[c];
+    } else if (EP == EP_CHEB0) {
+      const T rv = a.b[q] - Au[c];
+      a.r[q] = rv;
+      a.dout[q] = rv * a.theta_inv * a.Dinv[(size_t)c * V + n];
+      a.x[q] = uo[c];
+    } else {  // EP_STEP
+      const T rv = a.rprev[q] - Au[c];
+      a.r[q] = rv;
+      const T dn = a.c1 * uo[c] + a.c2 * rv * a.Dinv[(size_t)c * V + n];
+      a.dout[q] = dn;
+      T xv = (a.xprev ? a.xprev[q] : (T)0) + uo[c];
+      if (a.last) xv += dn;
+      a.x[q] = xv;
+    }
+  }
+}
+
+static inline int tiled_chunk(const Dims &g) {
+  const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
+  // planes per chunk: about 8 (halo slab overhead 1/8), but enough chunks to
+  // put ~2 workgroups per CU across the six load cases, and at most
+  // kMaxPartials workgroups per load case
+  int TI = 8;
+  while (TI > 2 && (long)tiles * ((g.nx + TI - 1) / TI) * kRHS < 512) TI /= 2;
+  if (TI > g.nx) TI = g.nx;
+  while ((long)tiles * ((g.nx + TI - 1) / TI) > kMaxPartials) TI++;
+  return TI;
+}
+
+int tiled_parts(const Dims &g) {
+  const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
+  const int TI = tiled_chunk(g);
+  return tiles * ((g.nx + TI - 1) / TI);
+}
+
+template <typename T, int ST, int EP>
+__global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ E, ModeT<T> mc,
+                                                  OpArgs<T> a, int TI, int tiles_k) {
+  const int r = blockIdx.z;
+  if (!((a.mask >> r) & 1u)) return;
+  __shared__ T up[3][3][kTN * kTN];
+  __shared__ T eo[kTE * kTE * kEOH];
+  const int tid = threadIdx.x;
+  const int j0 = (blockIdx.x / tiles_k) * kTile, k0 = (blockIdx.x % tiles_k) * kTile;
+  const int i0 = blockIdx.y * TI;
+  const int iend = min(i0 + TI, g.nx);
+  const int V = g.V;
+  const size_t off = (size_t)r * 3 * V;
+  const int plane = g.ny * g.nz;
+
+  // per-thread staging slots (fixed across planes): node offsets in a plane
+  int soff[2], sjc[2][2], skc[2][2];
+  float sjw[2], skw[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int t = tid + q * kBlock;
+    soff[q] = -1;
+    if (t < kTN * kTN) {
+      const int nj = t / kTN, nk = t - nj * kTN;
+      const int jj = ((j0 - 1 + nj) % g.ny + g.ny) % g.ny, kk = ((k0 - 1 + nk) % g.nz + g.nz) % g.nz;
+      soff[q] = jj * g.nz + kk;
+      if (ST == ST_PROLONG) {
+        sjc[q][0] = jj >> 1;
+        sjc[q][1] = ((jj + 1) >> 1) % a.gc.ny;
+        sjw[q] = (jj & 1) ? 0.5f : 1.0f;
+        skc[q][0] = kk >> 1;
+        skc[q][1] = ((kk + 1) >> 1) % a.gc.nz;
+        skw[q] = (kk & 1) ? 0.5f : 1.0f;
+      }
+    }
+  }
+  // per-thread voxels (fixed across slabs): modulus offsets in a plane
+  int eoff[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int e = tid + q * kBlock;
+    eoff[q] = -1;
+    if (e < kTE * kTE) {
+      const int ej = e / kTE, ek = e - ej * kTE;
+      const int jj = ((j0 - 1 + ej) % g.ny + g.ny) % g.ny, kk = ((k0 - 1 + ek) % g.nz + g.nz) % g.nz;
+      eoff[q] = jj * g.nz + kk;
+    }
+  }
+
+  auto stage = [&](int buf, int ip) {
+    const int iw = ((ip % g.nx) + g.nx) % g.nx;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (soff[q] < 0) continue;
+      const int t = tid + q * kBlock;
+      const int n = iw * plane + soff[q];
+      T v[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[c] = a.u[off + (size_t)c * V + n];
+      if (ST == ST_DSCALE) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] *= a.theta_inv * a.Dinv[(size_t)c * V + n];
+      } else if (ST == ST_PROLONG) {
+        const int ci0 = iw >> 1, ci1 = ((iw + 1) >> 1) % a.gc.nx;
+        const float wi = (iw & 1) ? 0.5f : 1.0f;
+        const int ni = (iw & 1) ? 2 : 1, nj = (sjw[q] < 1.0f) ? 2 : 1, nk = (skw[q] < 1.0f) ? 2 : 1;
+        const T w = (T)(wi * sjw[q] * skw[q]);
+        const size_t offc = (size_t)r * 3 * a.gc.V;
+        T s[3] = {0, 0, 0};
+        for (int ai = 0; ai < ni; ai++)
+          for (int aj = 0; aj < nj; aj++)
+            for (int ak = 0; ak < nk; ak++) {
+              const int N = ((ai ? ci1 : ci0) * a.gc.ny + sjc[q][aj]) * a.gc.nz + skc[q][ak];
+#pragma unroll
+              for (int c = 0; c < 3; c++) s[c] += a.xc[offc + (size_t)c * a.gc.V + N];
+            }
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] += w * s[c];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) up[buf][c][t] = v[c];
+    }
+  };
+
+  const int nj = tid / kTile, nk = tid % kTile;  // owned node, output tile coords
+  const int gj = j0 + nj, gk = k0 + nk;
+  const bool owns = (gj < g.ny) && (gk < g.nz);
+  const int ot = (nj + 1) * kTN + nk + 1;  // owned node in the staged plane
+  T acc[3] = {0, 0, 0};
+  double dv = 0;
+
+  stage(0, i0 - 1);
+  for (int sl = i0 - 1, p = 0; sl < iend; sl++, p++) {
+    const int cur = p % 3, nxt = (p + 1) % 3;
+    stage(nxt, sl + 1);
+    __syncthreads();
+    const int sw = ((sl % g.nx) + g.nx) % g.nx;
+    T keep[2][12];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (eoff[q] < 0) continue;
+      const int e = tid + q * kBlock;
+      const int ej = e / kTE, ek = e - ej * kTE;
+      T uv[24], fv[24];
+#pragma unroll
+      for (int v = 0; v < 8; v++) {
+        const int bi = (v >> 2) & 1, bj = (v >> 1) & 1, bk = v & 1;
+        const int t = (ej + bj) * kTN + ek + bk;
+        const int bb = bi ? nxt : cur;
+        uv[3 * v] = up[bb][0][t];
+        uv[3 * v + 1] = up[bb][1][t];
+        uv[3 * v + 2] = up[bb][2][t];
+      }
+      hadamard_voxel<T>(uv, E[sw * plane + eoff[q]], mc, fv);
+      T *dst = eo + e * kEOH;
+#pragma unroll
+      for (int t = 0; t < 12; t++) {
+        dst[t] = fv[t];        // corners with bi = 0 (plane s)
+        keep[q][t] = fv[12 + t];  // corners with bi = 1 (plane s+1)
+      }
+    }
+    __syncthreads();
+    // plane s: voxels (nj+dj, nk+dk) hold this node as corner (0, 1-dj, 1-dk)
+    T f[3] = {0, 0, 0};
+#pragma unroll
+    for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+      for (int dk = 0; dk < 2; dk++) {
+        const T *src = eo + ((nj + dj) * kTE + nk + dk) * kEOH + 3 * (2 * (1 - dj) + (1 - dk));
+#pragma unroll
+        for (int c = 0; c < 3; c++) f[c] += src[c];
+      }
+    if (sl >= i0 && owns) {
+      T Au[3], uo[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        Au[c] = acc[c] + f[c];
+        uo[c] = up[cur][c][ot];
+      }
+      epilogue<T, EP>(a, off, V, sl * plane + gj * g.nz + gk, Au, uo, dv);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (eoff[q] < 0) continue;
+      T *dst = eo + (tid + q * kBlock) * kEOH;
+#pragma unroll
+      for (int t = 0; t < 12; t++) dst[t] = keep[q][t];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[c] = 0;
+#pragma unroll
+    for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+      for (int dk = 0; dk < 2; dk++) {
+        const T *src = eo + ((nj + dj) * kTE + nk + dk) * kEOH + 3 * (2 * (1 - dj) + (1 - dk));
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[c] += src[c];
+      }
+  }
+  if (EP == EP_AX && a.dot) {
+    const double s = block_sum(dv);
+    block_partial(a.red, a.slot, r, blockIdx.y * gridDim.x + blockIdx.x, s);
+  }
+}
+
+template <typename T, int ST, int EP>
+static void tiled_go(const Dims &g, const T *E, const ModeT<T> &mc, const OpArgs<T> &a,
+                     hipStream_t s) {
+  const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
+  const int TI = tiled_chunk(g);
+  dim3 grid(tiles_j * tiles_k, (g.nx + TI - 1) / TI, kRHS);
+  hipLaunchKernelGGL((k_tiled<T, ST, EP>), grid, dim3(kBlock), 0, s, g, E, mc, a, TI, tiles_k);
+}
+
+template <typename T>
+void launch_tiled(int st, int ep, const Dims &g, const T *E, const ModeCoef &m, const OpArgs<T> &a,
+                  hipStream_t s) {
+  ModeT<T> mc;
+  for (int c = 0; c < 3; c++)
+    for (int v = 0; v < 8; v++) {
+      mc.diag[c][v] = (T)m.diag[c][v];
+      for (int d = 0; d < 3; d++) mc.cross[c][d][v] = (T)m.cross[c][d][v];
+    }
+#define GO(S_, E_)                                 \
+  if (st == S_ && ep == E_) {                      \
+    tiled_go<T, S_, E_>(g, E, mc, a, s);           \
+    return;                                        \
+  }
+  GO(ST_PLAIN, EP_AX)
+  GO(ST_PLAIN, EP_RES)
+  GO(ST_DSCALE, EP_STEP)
+  GO(ST_PLAIN, EP_STEP)
+  GO(ST_PROLONG, EP_CHEB0)
+  GO(ST_PLAIN, EP_CHEB0)
+#undef GO
+}
+
+// ---------------------------------------------------------------------------
+// coarse levels: 27-point block stencils, every active RHS per thread (the
+// nine coefficients of a block are loaded once for all load cases)
+template <typename T, int ST>
+__device__ __forceinline__ T stage_at(const OpArgs<T> &a, size_t off, int V, int c, int n) {
+  T v = a.u[off + (size_t)c * V + n];
+  if (ST == ST_DSCALE) v *= a.theta_inv * a.Dinv[(size_t)c * V + n];
+  return v;
+}
+
+template <typename T, int ST, int EP, int NR>
+__device__ __forceinline__ void stencil_node(const Dims &g, const T *__restrict__ S, const OpArgs<T> &a,
+                                             int n, int r0) {
+  int i, j, k;
+  node_ijk(g, n, i, j, k);
+  const int I[3] = {i == 0 ? g.nx - 1 : i - 1, i, i == g.nx - 1 ? 0 : i + 1};
+  const int J[3] = {j == 0 ? g.ny - 1 : j - 1, j, j == g.ny - 1 ? 0 : j + 1};
+  const int K[3] = {k == 0 ? g.nz - 1 : k - 1, k, k == g.nz - 1 ? 0 : k + 1};
+  const int V = g.V;
+  T acc[NR][3];
+#pragma unroll
+  for (int r = 0; r < NR; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0;
+#pragma unroll 3
+  for (int o = 0; o < 27; o++) {
+    const int nb = (I[o / 9] * g.ny + J[(o / 3) % 3]) * g.nz + K[o % 3];
+    T s[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) s[q] = S[(size_t)(o * 9 + q) * V + n];
+#pragma unroll
+    for (int rr = 0; rr < NR; rr++) {
+      const int r = r0 + rr;
+      if (!((a.mask >> r) & 1u)) continue;
+      const size_t off = (size_t)r * 3 * V;
+      const T u0 = stage_at<T, ST>(a, off, V, 0, nb), u1 = stage_at<T, ST>(a, off, V, 1, nb),
+              u2 = stage_at<T, ST>(a, off, V, 2, nb);
+      acc[rr][0] += s[0] * u0 + s[1] * u1 + s[2] * u2;
+      acc[rr][1] += s[3] * u0 + s[4] * u1 + s[5] * u2;
+      acc[rr][2] += s[6] * u0 + s[7] * u1 + s[8] * u2;
+    }
+  }
+#pragma unroll
+  for (int rr = 0; rr < NR; rr++) {
+    const int r = r0 + rr;
+    if (!((a.mask >> r) & 1u)) continue;
+    const size_t off = (size_t)r * 3 * V;
+    T uo[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) uo[c] = stage_at<T, ST>(a, off, V, c, n);
+    double dv = 0;
+    epilogue<T, EP>(a, off, V, n, acc[rr], uo, dv);
+  }
+}
+
+template <typename T, int ST, int EP>
+__global__ __launch_bounds__(kBlock) void k_stencil(Dims g, const T *__restrict__ S, OpArgs<T> a) {
+  const int r = blockIdx.y;  // one load case per thread: 6x the parallelism of a
+  if (!((a.mask >> r) & 1u)) return;  // node-only grid on these small levels
+  for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock)
+    stencil_node<T, ST, EP, 1>(g, S, a, n, r);
+}
+
+template <typename T>
+void launch_stencil(int st, int ep, const Dims &g, const T *S, const OpArgs<T> &a, hipStream_t s) {
+  const int blocks = (g.V + kBlock - 1) / kBlock;
+#define GO(S_, E_)                                                                       \
+  if (st == S_ && ep == E_) {                                                            \
+    hipLaunchKernelGGL((k_stencil<T, S_, E_>), dim3(blocks, kRHS), dim3(kBlock), 0, s, g, S, a); \
+    return;                                                                              \
+  }
+  GO(ST_PLAIN, EP_AX)
+  GO(ST_PLAIN, EP_RES)
+  GO(ST_DSCALE, EP_STEP)
+  GO(ST_PLAIN, EP_STEP)
+  GO(ST_PLAIN, EP_CHEB0)
+#undef GO
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_add2(int V, const T *__restrict__ xn,
+                                                 const T *__restrict__ d, T *__restrict__ x,
+                                                 unsigned mask) {
+  const int r = blockIdx.y;
+  if (!((mask >> r) & 1u)) return;
+  const size_t off = (size_t)r * 3 * V;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
+    x[off + t] = xn[off + t] + d[off + t];
+}
+
+template <typename T>
+void launch_add2(int V, const T *xn, const T *d, T *x, unsigned mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_add2<T>, dim3(grid_for(3 * V), kRHS), dim3(kBlock), 0, s, V, xn, d, x, mask);
+}
+
+// ---------------------------------------------------------------------------
+// small levels: the whole V-cycle from level 0 of `lv` down to the coarsest
+// (dense) level, run by one workgroup for all load cases.  Phases are
+// separated by workgroup barriers; the level vectors live in global memory
+// (L2-resident at these sizes).
+template <typename T, int ST, int EP>
+__device__ void small_apply(const SmallLevel<T> &L, const OpArgs<T> &a) {
+  for (int n = threadIdx.x; n < L.g.V; n += blockDim.x) stencil_node<T, ST, EP, kRHS>(L.g, L.S, a, n, 0);
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void small_smooth(const SmallLevel<T> &L, int degree, bool x_zero, unsigned mask) {
+  OpArgs<T> a;
+  a.Dinv = L.Dinv;
+  a.theta_inv = L.theta_inv;
+  a.mask = mask;
+  const int V = L.g.V;
+  if (degree == 1) {
+    // one Jacobi-like step: x_zero: x = b/(theta D); else x += (b - A x)/(theta D)
+    if (x_zero) {
+      for (int t = threadIdx.x; t < 3 * V; t += blockDim.x)
+        for (int r = 0; r < kRHS; r++)
+          if ((mask >> r) & 1u) {
+            const size_t q = (size_t)r * 3 * V + t;
+            L.x[q] = L.b[q] * L.theta_inv * L.Dinv[t];
+          }
+      __syncthreads();
+    } else {
+      a.u = L.x;
+      a.b = L.b;
+      a.r = L.r;
+      a.dout = L.d;
+      a.x = L.xn;
+      small_apply<T, ST_PLAIN, EP_CHEB0>(L, a);
+      for (int t = threadIdx.x; t < 3 * V; t += blockDim.x)
+        for (int r = 0; r < kRHS; r++)
+          if ((mask >> r) & 1u) {
+            const size_t q = (size_t)r * 3 * V + t;
+            L.x[q] = L.xn[q] + L.d[q];
+          }
+      __syncthreads();
+    }
+    return;
+  }
+  T *dcur = L.d, *dnxt = L.d2;
+  if (x_zero) {
+    // step 1 stages d1 = b/(theta D) directly
+    a.u = L.b;
+    a.rprev = L.b;
+    a.xprev = nullptr;
+    a.r = L.r;
+    a.x = L.x;
+    a.dout = dcur;
+    a.c1 = L.c1[0];
+    a.c2 = L.c2[0];
+    a.last = degree == 2;
+    small_apply<T, ST_DSCALE, EP_STEP>(L, a);
+  } else {
+    a.u = L.x;
+    a.b = L.b;
+    a.r = L.r;
+    a.dout = dnxt;
+    a.x = L.xn;
+    small_apply<T, ST_PLAIN, EP_CHEB0>(L, a);
+    // step 1 from d1 (in dnxt), x_prev = xn
+    a.u = dnxt;
+    a.rprev = L.r;
+    a.xprev = L.xn;
+    a.r = L.r;
+    a.x = L.x;
+    a.dout = dcur;
+    a.c1 = L.c1[0];
+    a.c2 = L.c2[0];
+    a.last = degree == 2;
+    small_apply<T, ST_PLAIN, EP_STEP>(L, a);
+  }
+  for (int s = 2; s < degree; s++) {
+    // dcur holds d_s; produce d_{s+1} into dnxt
+    a.u = dcur;
+    a.rprev = L.r;
+    a.xprev = L.x;
+    a.r = L.r;
+    a.x = L.x;
+    a.dout = dnxt;
+    a.c1 = L.c1[s - 1];
+    a.c2 = L.c2[s - 1];
+    a.last = s == degree - 1;
+    small_apply<T, ST_PLAIN, EP_STEP>(L, a);
+    T *tmp = dcur;
+    dcur = dnxt;
+    dnxt = tmp;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSmallThreads) void k_small_cycle(const SmallLevel<T> *__restrict__ lv,
+                                                               int nlev, int nu1, int nu2,
+                                                               unsigned mask) {
+  // down
+  for (int l = 0; l < nlev - 1; l++) {
+    const SmallLevel<T> &L = lv[l];
+    const SmallLevel<T> &C = lv[l + 1];
+    small_smooth<T>(L, nu1, true, mask);
+    OpArgs<T> a;
+    a.u = L.x;
+    a.b = L.b;
+    a.y = L.r;
+    a.mask = mask;
+    small_apply<T, ST_PLAIN, EP_RES>(L, a);
+    // restrict r -> C.b
+    for (int N = threadIdx.x; N < C.g.V; N += blockDim.x) {
+      int I, J, K;
+      node_ijk(C.g, N, I, J, K);
+      for (int r = 0; r < kRHS; r++) {
+        if (!((mask >> r) & 1u)) continue;
+        double acc[3] = {0, 0, 0};
+        for (int on = 0; on < 27; on++) {
+          const int oi = on / 9 - 1, oj = (on / 3) % 3 - 1, ok = on % 3 - 1;
+          const double w = (oi ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
+          const int n = (((2 * I + oi + L.g.nx) % L.g.nx) * L.g.ny + (2 * J + oj + L.g.ny) % L.g.ny) *
+                            L.g.nz + (2 * K + ok + L.g.nz) % L.g.nz;
+#pragma unroll
+          for (int c = 0; c < 3; c++) acc[c] += w * (double)L.r[(size_t)r * 3 * L.g.V + (size_t)c * L.g.V + n];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) C.b[(size_t)r * 3 * C.g.V + (size_t)c * C.g.V + N] = (T)acc[c];
+      }
+    }
+    __syncthreads();
+  }
+  // coarsest: dense inverse
+  {
+    const SmallLevel<T> &C = lv[nlev - 1];
+    const int n = 3 * C.g.V;
+    for (int t = threadIdx.x; t < kRHS * n; t += blockDim.x) {
+      const int r = t / n, i = t - r * n;
+      if (!((mask >> r) & 1u)) continue;
+      double acc = 0;
+      const T *row = C.Ainv + (size_t)i * n;
+      const T *br = C.b + (size_t)r * n;
+      for (int j = 0; j < n; j++) acc += (double)row[j] * (double)br[j];
+      C.x[(size_t)r * n + i] = (T)acc;
+    }
+    __syncthreads();
+  }
+  // up
+  for (int l = nlev - 2; l >= 0; l--) {
+    const SmallLevel<T> &L = lv[l];
+    const SmallLevel<T> &C = lv[l + 1];
+    for (int n = threadIdx.x; n < L.g.V; n += blockDim.x) {
+      int i, j, k;
+      node_ijk(L.g, n, i, j, k);
+      const int ci0 = i >> 1, ci1 = ((i + 1) >> 1) % C.g.nx;
+      const int cj0 = j >> 1, cj1 = ((j + 1) >> 1) % C.g.ny;
+      const int ck0 = k >> 1, ck1 = ((k + 1) >> 1) % C.g.nz;
+      const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
+      const double w = ((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((k & 1) ? 0.5 : 1.0);
+      for (int r = 0; r < kRHS; r++) {
+        if (!((mask >> r) & 1u)) continue;
+        double acc[3] = {0, 0, 0};
+        for (int ai = 0; ai < ni; ai++)
+          for (int aj = 0; aj < nj; aj++)
+            for (int ak = 0; ak < nk; ak++) {
+              const int N = ((ai ? ci1 : ci0) * C.g.ny + (aj ? cj1 : cj0)) * C.g.nz + (ak ? ck1 : ck0);
+#pragma unroll
+              for (int c = 0; c < 3; c++) acc[c] += (double)C.x[(size_t)r * 3 * C.g.V + (size_t)c * C.g.V + N];
+            }
+#pragma unroll
+        for (int c = 0; c < 3; c++) L.x[(size_t)r * 3 * L.g.V + (size_t)c * L.g.V + n] += (T)(w * acc[c]);
+      }
+    }
+    __syncthreads();
+    small_smooth<T>(L, nu2, false, mask);
+  }
+}
+
+template <typename T>
+void launch_small_cycle(const SmallLevel<T> *d_levels, int nlev, int nu1, int nu2, unsigned mask,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_small_cycle<T>, dim3(1), dim3(kSmallThreads), 0, s, d_levels, nlev, nu1, nu2,
+                     mask);
+}
+
+#define FEMH_OPS_INST(T)                                                                        \
+  template void launch_tiled<T>(int, int, const Dims &, const T *, const ModeCoef &,            \
+                                const OpArgs<T> &, hipStream_t);                                \
+  template void launch_stencil<T>(int, int, const Dims &, const T *, const OpArgs<T> &,         \
+                                  hipStream_t);                                                 \
+  template void launch_add2<T>(int, const T *, const T *, T *, unsigned, hipStream_t);          \
+  template void launch_small_cycle<T>(const SmallLevel<T> *, int, int, int, unsigned, hipStream_t);
+
+FEMH_OPS_INST(double)
+FEMH_OPS_INST(float)
+
+}  // namespace femh

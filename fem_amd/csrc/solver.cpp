@@ -119,29 +119,63 @@ void Context::set_modulus(const double *E_host) {
 // ---------------------------------------------------------------------------
 // hierarchy
 template <typename T>
-void Context::apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *bb, int mode, unsigned mask,
-                    const Reduce *rd, int slot) {
+void Context::op(Hierarchy<T> &h, int l, int st, int ep, const OpArgs<T> &a) {
   if (l == 0) {
-    FineOp<T> op{h.lv[0].g, h.E.template as<T>(), h.Ke.template as<T>(), &modecoef, o.apply_kernel};
+    const Dims &g0 = h.lv[0].g;
+    const bool dense = o.apply_kernel == 1 && st == ST_PLAIN && (ep == EP_AX || ep == EP_RES);
+    auto launch = [&] {
+      if (dense) {
+        FineOp<T> f{g0, h.E.template as<T>(), h.Ke.template as<T>(), &modecoef, 1};
+        launch_dense_apply<T>(f, a.u, a.y, a.b, ep == EP_RES ? 1 : 0, a.mask, a.dot ? &a.red : nullptr,
+                              a.slot, stream);
+      } else {
+        launch_tiled<T>(st, ep, g0, h.E.template as<T>(), modecoef, a, stream);
+      }
+    };
     if (prof) {
       Pending pe;
       pe.a = take_event();
       pe.b = take_event();
       pe.which = sizeof(T) == 8 ? 0 : 1;
-      const double R = (double)__builtin_popcount(mask & 0x3Fu), V = (double)op.g.V, sz = sizeof(T);
-      pe.bytes = V * (3.0 * sz * R * (mode == 1 ? 3.0 : 2.0) + sz);
+      const double R = (double)__builtin_popcount(a.mask & 0x3Fu), V = (double)g0.V, sz = sizeof(T);
+      // algorithmic bytes: staged vector read + outputs written + extra
+      // per-node inputs of the epilogue, E read once
+      double vec_in = 1, vec_out = 1;
+      if (ep == EP_RES) vec_in += 1;
+      if (ep == EP_CHEB0) { vec_in += 1; vec_out = 3; }
+      if (ep == EP_STEP) { vec_in += 1 + (a.xprev ? 1 : 0); vec_out = 3; }
+      if (st == ST_PROLONG) vec_in += 0.125;
+      const bool usesD = st == ST_DSCALE || ep == EP_CHEB0 || ep == EP_STEP;
+      pe.bytes = V * (3.0 * sz * R * (vec_in + vec_out) + sz + (usesD ? 3.0 * sz : 0.0));
       pe.flops = V * R * 1152.0;
       FEMH_HIP(hipEventRecord(pe.a, stream));
-      launch_fine_apply<T>(op, u, y, bb, mode, mask, rd, slot, stream);
+      launch();
       FEMH_HIP(hipEventRecord(pe.b, stream));
       pending.push_back(pe);
     } else {
-      launch_fine_apply<T>(op, u, y, bb, mode, mask, rd, slot, stream);
+      launch();
     }
   } else {
-    StencilOp<T> op{h.lv[l].g, h.lv[l].S.template as<T>()};
-    launch_stencil_apply<T>(op, u, y, bb, mode, mask, stream);
+    launch_stencil<T>(st, ep, h.lv[l].g, h.lv[l].S.template as<T>(), a, stream);
   }
+}
+
+int Context::apply_parts() const { return o.apply_kernel == 1 ? dense_parts(g) : tiled_parts(g); }
+
+template <typename T>
+void Context::apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *bb, int mode, unsigned mask,
+                    const Reduce *rd, int slot) {
+  OpArgs<T> a;
+  a.u = u;
+  a.y = y;
+  a.b = bb;
+  a.mask = mask;
+  if (rd) {
+    a.red = *rd;
+    a.slot = slot;
+    a.dot = 1;
+  }
+  op<T>(h, l, ST_PLAIN, mode == 1 ? EP_RES : EP_AX, a);
 }
 
 template <typename T>
@@ -161,7 +195,8 @@ void Context::build_levels(Hierarchy<T> &h) {
     Level<T> &L = h.lv[l];
     L.g = dims[l];
     L.D.alloc(sizeof(T) * 3 * L.g.V);
-    for (DBuf *bf : {&L.x, &L.b, &L.r, &L.d, &L.Ad}) bf->alloc(vec_bytes(L.g, sizeof(T)));
+    L.Dinv.alloc(sizeof(T) * 3 * L.g.V);
+    for (DBuf *bf : {&L.x, &L.b, &L.r, &L.d, &L.d2, &L.xn, &L.Ad}) bf->alloc(vec_bytes(L.g, sizeof(T)));
     if (l > 0) L.S.alloc(sizeof(T) * 243 * (size_t)L.g.V);
   }
   h.E.alloc(sizeof(T) * g.V);
@@ -174,9 +209,9 @@ void Context::estimate_lmax(Hierarchy<double> &h) {
     const int V = L.g.V;
     double *xv = L.x.as<double>(), *yv = L.Ad.as<double>();
     launch_fill_hash<double>((size_t)3 * V, xv, 0x51ED27ull + l, stream);
-    for (int it = 0; it < 12; it++) {
+    for (int it = 0; it < 10; it++) {
       apply<double>(h, (int)l, xv, yv, nullptr, 0, 1u, nullptr, 0);
-      launch_scale_by_dinv_norm<double>(V, yv, L.D.as<double>(), red, stream);
+      launch_scale_by_dinv_norm<double>(V, yv, L.Dinv.as<double>(), red, stream);
       launch_normalize<double>(V, yv, xv, red, stream);
     }
     FEMH_HIP(hipGetLastError());
@@ -281,7 +316,7 @@ void Context::build_hierarchy() {
   nlevels = (int)h64.lv.size();
   FEMH_HIP(hipMemcpyAsync(h64.E.p, E64.p, sizeof(double) * g.V, hipMemcpyDeviceToDevice, stream));
   FEMH_HIP(hipMemcpyAsync(h64.Ke.p, Ke64.p, sizeof(double) * 576, hipMemcpyDeviceToDevice, stream));
-  FineOp<double> f0{g, h64.E.as<double>(), h64.Ke.as<double>(), &modecoef, o.apply_kernel};
+  FineOp<double> f0{g, h64.E.as<double>(), h64.Ke.as<double>(), &modecoef, 1};
   launch_fine_diag<double>(f0, h64.lv[0].D.as<double>(), stream);
   for (int l = 1; l < nlevels; l++) {
     Level<double> &L = h64.lv[l];
@@ -293,6 +328,9 @@ void Context::build_hierarchy() {
     StencilOp<double> so{L.g, L.S.as<double>()};
     launch_stencil_diag<double>(so, L.D.as<double>(), stream);
   }
+  for (int l = 0; l < nlevels; l++)
+    launch_invert<double>((size_t)3 * h64.lv[l].g.V, h64.lv[l].D.as<double>(), h64.lv[l].Dinv.as<double>(),
+                          stream);
   FEMH_HIP(hipGetLastError());
   if (o.precond == FEMH_PRECOND_MG) {
     estimate_lmax(h64);
@@ -304,6 +342,7 @@ void Context::build_hierarchy() {
                                       " dofs); allow more levels or use a grid divisible by 2^k");
     }
   }
+  if (o.precond == FEMH_PRECOND_MG && nlevels > 1) build_small<double>(h64);
   h64.ready = true;
   if (o.precision == FEMH_MIXED) {
     build_levels<float>(h32);
@@ -312,6 +351,8 @@ void Context::build_hierarchy() {
     for (int l = 0; l < nlevels; l++) {
       launch_convert<double, float>((size_t)3 * h64.lv[l].g.V, h64.lv[l].D.as<double>(),
                                     h32.lv[l].D.as<float>(), stream);
+      launch_convert<double, float>((size_t)3 * h64.lv[l].g.V, h64.lv[l].Dinv.as<double>(),
+                                    h32.lv[l].Dinv.as<float>(), stream);
       if (l > 0)
         launch_convert<double, float>((size_t)243 * h64.lv[l].g.V, h64.lv[l].S.as<double>(),
                                       h32.lv[l].S.as<float>(), stream);
@@ -324,55 +365,168 @@ void Context::build_hierarchy() {
       h32.ndense = h64.ndense;
     }
     FEMH_HIP(hipGetLastError());
+    if (o.precond == FEMH_PRECOND_MG && nlevels > 1) build_small<float>(h32);
     h32.ready = true;
   }
 }
 
 // ---------------------------------------------------------------------------
-// multigrid V-cycle with Chebyshev smoothing
-template <typename T>
-void Context::smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, int degree, bool x_zero,
-                     unsigned mask) {
-  Level<T> &L = h.lv[l];
-  const int V = L.g.V;
-  T *rr = L.r.template as<T>(), *dd = L.d.template as<T>(), *Ad = L.Ad.template as<T>();
-  const T *D = L.D.template as<T>();
-  const double hi = L.lmax, lo = L.lmax / o.cheby_ratio;
+// multigrid V-cycle with Chebyshev smoothing (fused kernels, see ops.hpp)
+static void cheb_coeffs(double lmax, double ratio, double *theta_inv, double *c1, double *c2) {
+  const double hi = lmax, lo = lmax / ratio;
   const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+  *theta_inv = 1.0 / theta;
   double rho = 1.0 / sigma;
-  const T *rsrc;
-  if (x_zero) {
-    rsrc = bb;
-  } else {
-    apply<T>(h, l, xx, rr, bb, 1, mask, nullptr, 0);
-    rsrc = rr;
+  for (int s = 0; s < kMaxCheb; s++) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    c1[s] = rn * rho;
+    c2[s] = 2.0 * rn / delta;
+    rho = rn;
   }
-  launch_cheb_first<T>(V, rsrc, D, dd, xx, 1.0 / theta, x_zero ? 1 : 0, mask, stream);
-  for (int s = 1; s < degree; s++) {
-    apply<T>(h, l, dd, Ad, nullptr, 0, mask, nullptr, 0);
-    const double rho_new = 1.0 / (2.0 * sigma - rho);
-    launch_cheb_next<T>(V, rsrc, rr, Ad, D, dd, xx, rho_new * rho, 2.0 * rho_new / delta, mask, stream);
-    rsrc = rr;
-    rho = rho_new;
+}
+
+template <typename T>
+void Context::build_small(Hierarchy<T> &h) {
+  const int L = (int)h.lv.size();
+  int ls = L - 1;
+  // the fine level is matrix-free and always launched on its own
+  while (ls > 1 && h.lv[ls - 1].g.V <= kSmallMaxV) ls--;
+  h.lsmall = ls;
+  std::vector<SmallLevel<T>> d(L - ls);
+  for (int l = ls; l < L; l++) {
+    Level<T> &V = h.lv[l];
+    SmallLevel<T> &q = d[l - ls];
+    q.g = V.g;
+    q.S = V.S.template as<T>();
+    q.Dinv = V.Dinv.template as<T>();
+    q.x = V.x.template as<T>();
+    q.b = V.b.template as<T>();
+    q.r = V.r.template as<T>();
+    q.d = V.d.template as<T>();
+    q.d2 = V.d2.template as<T>();
+    q.xn = V.xn.template as<T>();
+    double ti, c1[kMaxCheb], c2[kMaxCheb];
+    cheb_coeffs(V.lmax, o.cheby_ratio, &ti, c1, c2);
+    q.theta_inv = (T)ti;
+    for (int k = 0; k < kMaxCheb; k++) {
+      q.c1[k] = (T)c1[k];
+      q.c2[k] = (T)c2[k];
+    }
+    q.Ainv = (l == L - 1) ? h.Ainv.template as<T>() : nullptr;
+  }
+  h.small.alloc(sizeof(SmallLevel<T>) * d.size());
+  FEMH_HIP(hipMemcpyAsync(h.small.p, d.data(), sizeof(SmallLevel<T>) * d.size(), hipMemcpyHostToDevice,
+                          stream));
+}
+
+template <typename T>
+void Context::pre_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, unsigned mask) {
+  Level<T> &L = h.lv[l];
+  const int k = o.nu1;
+  double ti, c1[kMaxCheb], c2[kMaxCheb];
+  cheb_coeffs(L.lmax, o.cheby_ratio, &ti, c1, c2);
+  const T *D = L.Dinv.template as<T>();
+  if (k == 1) {
+    launch_cheb_first<T>(L.g.V, bb, D, L.d.template as<T>(), xx, ti, 1, mask, stream);
+    return;
+  }
+  T *dcur = L.d.template as<T>(), *dnxt = L.d2.template as<T>();
+  OpArgs<T> a;
+  a.Dinv = D;
+  a.theta_inv = (T)ti;
+  a.mask = mask;
+  a.u = bb;
+  a.rprev = bb;
+  a.xprev = nullptr;
+  a.r = L.r.template as<T>();
+  a.x = xx;
+  a.dout = dcur;
+  a.c1 = (T)c1[0];
+  a.c2 = (T)c2[0];
+  a.last = (k == 2);
+  op<T>(h, l, ST_DSCALE, EP_STEP, a);
+  for (int s = 2; s < k; s++) {
+    a.u = dcur;
+    a.rprev = L.r.template as<T>();
+    a.xprev = xx;
+    a.dout = dnxt;
+    a.c1 = (T)c1[s - 1];
+    a.c2 = (T)c2[s - 1];
+    a.last = (s == k - 1);
+    op<T>(h, l, ST_PLAIN, EP_STEP, a);
+    std::swap(dcur, dnxt);
+  }
+}
+
+template <typename T>
+void Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, unsigned mask) {
+  Level<T> &L = h.lv[l];
+  Level<T> &C = h.lv[l + 1];
+  const int k = o.nu2;
+  double ti, c1[kMaxCheb], c2[kMaxCheb];
+  cheb_coeffs(L.lmax, o.cheby_ratio, &ti, c1, c2);
+  OpArgs<T> a;
+  a.Dinv = L.Dinv.template as<T>();
+  a.theta_inv = (T)ti;
+  a.mask = mask;
+  a.u = xx;
+  a.b = bb;
+  a.r = L.r.template as<T>();
+  a.dout = L.d2.template as<T>();
+  a.x = L.xn.template as<T>();
+  if (l == 0 && o.apply_kernel == 0) {
+    a.xc = C.x.template as<T>();
+    a.gc = C.g;
+    op<T>(h, l, ST_PROLONG, EP_CHEB0, a);  // x3 = x + P xc staged, kept in xn
+  } else {
+    launch_prolong_add<T>(L.g, C.g, C.x.template as<T>(), xx, mask, stream);
+    op<T>(h, l, ST_PLAIN, EP_CHEB0, a);
+  }
+  if (k == 1) {
+    launch_add2<T>(L.g.V, L.xn.template as<T>(), L.d2.template as<T>(), xx, mask, stream);
+    return;
+  }
+  T *dcur = L.d2.template as<T>(), *dnxt = L.d.template as<T>();
+  OpArgs<T> s1;
+  s1.Dinv = a.Dinv;
+  s1.theta_inv = (T)ti;
+  s1.mask = mask;
+  s1.r = L.r.template as<T>();
+  s1.rprev = L.r.template as<T>();
+  s1.x = xx;
+  for (int s = 1; s < k; s++) {
+    s1.u = dcur;
+    s1.xprev = (s == 1) ? L.xn.template as<T>() : xx;
+    s1.dout = dnxt;
+    s1.c1 = (T)c1[s - 1];
+    s1.c2 = (T)c2[s - 1];
+    s1.last = (s == k - 1);
+    op<T>(h, l, ST_PLAIN, EP_STEP, s1);
+    std::swap(dcur, dnxt);
   }
 }
 
 template <typename T>
 void Context::vcycle(Hierarchy<T> &h, int l, const T *bb, T *xx, unsigned mask) {
   const int last = (int)h.lv.size() - 1;
-  if (l == last) {
-    launch_dense_solve<T>(h.ndense, h.Ainv.template as<T>(), bb, xx, mask, stream);
+  if (l >= h.lsmall) {
+    // this level and everything below run in one workgroup (its b is in place)
+    launch_small_cycle<T>(h.small.template as<SmallLevel<T>>() + (l - h.lsmall), last - l + 1, o.nu1,
+                          o.nu2, mask, stream);
     return;
   }
   Level<T> &L = h.lv[l];
   Level<T> &C = h.lv[l + 1];
-  T *rr = L.r.template as<T>();
-  smooth<T>(h, l, bb, xx, o.nu1, true, mask);
-  apply<T>(h, l, xx, rr, bb, 1, mask, nullptr, 0);
-  launch_restrict<T>(L.g, C.g, rr, C.b.template as<T>(), mask, stream);
+  pre_smooth<T>(h, l, bb, xx, mask);
+  OpArgs<T> a;
+  a.u = xx;
+  a.b = bb;
+  a.y = L.r.template as<T>();
+  a.mask = mask;
+  op<T>(h, l, ST_PLAIN, EP_RES, a);
+  launch_restrict<T>(L.g, C.g, L.r.template as<T>(), C.b.template as<T>(), mask, stream);
   vcycle<T>(h, l + 1, C.b.template as<T>(), C.x.template as<T>(), mask);
-  launch_prolong_add<T>(L.g, C.g, C.x.template as<T>(), xx, mask, stream);
-  smooth<T>(h, l, bb, xx, o.nu2, false, mask);
+  post_smooth<T>(h, l, bb, xx, mask);
 }
 
 template <typename T>
@@ -382,7 +536,7 @@ void Context::precond(Hierarchy<T> &h, const T *rv, T *zv, unsigned mask) {
   } else {
     // Jacobi: handled inside the PCG update kernel; this path only serves the
     // first iteration
-    launch_cheb_first<T>(h.lv[0].g.V, rv, h.lv[0].D.template as<T>(), zv, zv, 1.0, 1, mask, stream);
+    launch_cheb_first<T>(h.lv[0].g.V, rv, h.lv[0].Dinv.template as<T>(), zv, zv, 1.0, 1, mask, stream);
   }
 }
 
@@ -393,7 +547,7 @@ int Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *
                  bool z_ready) {
   const int V = g.V;
   const bool jac = !(o.precond == FEMH_PRECOND_MG && h.lv.size() > 1);
-  const T *D0 = h.lv[0].D.template as<T>();
+  const T *D0 = h.lv[0].Dinv.template as<T>();
   FEMH_HIP(hipMemsetAsync(xv, 0, vec_bytes(g, sizeof(T)), stream));
   FEMH_HIP(hipMemcpyAsync(rv, bvec, vec_bytes(g, sizeof(T)), hipMemcpyDeviceToDevice, stream));
   unsigned active = mask;
@@ -414,7 +568,7 @@ int Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *
     const int rzc = cur ? S_RZ1 : S_RZ0, rzn = cur ? S_RZ0 : S_RZ1;
     apply<T>(h, 0, pv, qv, nullptr, 0, active, &red, S_PQ);
     launch_pcg_update<T>(V, xv, rv, pv, qv, jac ? D0 : nullptr, zv, rzc, rzn, active, red,
-                         fine_apply_parts(g, o.apply_kernel), stream);
+                         apply_parts(), stream);
     FEMH_HIP(hipGetLastError());
     if (it % o.check_every == 0 || it == max_iter) {
       read_scalars(S_RR, rr, kRHS, vec_parts(V));
@@ -509,7 +663,7 @@ void Context::homogenize(double *C, femh_stats *st) {
   if (!(mst[0] > 0)) throw Error(FEMH_BAD_ARG, "all-void cell (every modulus is zero)");
   const double sumE = mst[0];
   build_hierarchy();
-  FineOp<double> f0{g, E64.as<double>(), Ke64.as<double>(), &modecoef, o.apply_kernel};
+  FineOp<double> f0{g, E64.as<double>(), Ke64.as<double>(), &modecoef, 1};
   launch_fine_rhs<double>(f0, gtab.as<double>(), b.as<double>(), stream);
   launch_dot<double>(g.V, b.as<double>(), b.as<double>(), S_BB, 0x3Fu, red, stream);
   FEMH_HIP(hipGetLastError());

@@ -8,6 +8,7 @@
 
 #include "../../include/femh.h"
 #include "kernels.hpp"
+#include "ops.hpp"
 #include "tables.hpp"
 
 namespace femh {
@@ -50,8 +51,9 @@ struct Level {
   Dims g{};
   DBuf S;            // stencil (levels >= 1)
   DBuf D;            // point diagonal [3V]
+  DBuf Dinv;         // its inverse (what the kernels use)
   double lmax = 0;   // lambda_max(D^-1 A) estimate (with margin)
-  DBuf x, b, r, d, Ad;  // V-cycle work, [6][3][V]
+  DBuf x, b, r, d, d2, xn, Ad;  // V-cycle work, [6][3][V]
 };
 
 template <typename T>
@@ -61,6 +63,8 @@ struct Hierarchy {
   DBuf Ke;     // unit stiffness in T
   DBuf Ainv;   // dense (regularised) inverse of the coarsest operator
   int ndense = 0;
+  int lsmall = 0;   // first level run inside the single-workgroup coarse cycle
+  DBuf small;       // device array of SmallLevel<T> for levels lsmall..L-1
   bool ready = false;
 };
 
@@ -124,9 +128,16 @@ struct Context {
   void apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *b, int mode, unsigned mask,
              const Reduce *red, int slot);
   template <typename T>
-  void smooth(Hierarchy<T> &h, int l, const T *b, T *x, int degree, bool x_zero, unsigned mask);
+  void op(Hierarchy<T> &h, int l, int st, int ep, const OpArgs<T> &a);
+  template <typename T>
+  void pre_smooth(Hierarchy<T> &h, int l, const T *b, T *x, unsigned mask);
+  template <typename T>
+  void post_smooth(Hierarchy<T> &h, int l, const T *b, T *x, unsigned mask);
   template <typename T>
   void vcycle(Hierarchy<T> &h, int l, const T *b, T *x, unsigned mask);
+  template <typename T>
+  void build_small(Hierarchy<T> &h);
+  int apply_parts() const;
   template <typename T>
   void precond(Hierarchy<T> &h, const T *r, T *z, unsigned mask);
   template <typename T>
