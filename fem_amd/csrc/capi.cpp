@@ -1,9 +1,11 @@
 // capi.cpp -- extern "C" entry points declared in include/femh.h.
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <new>
 #include <tuple>
+#include <vector>
 
 #include "../../include/femh.h"
 #include "solver.hpp"
@@ -13,7 +15,7 @@ struct femh_ctx {
   femh_opts opts{};
   std::string err;
   int last_status = FEMH_OK;
-  // per-grid sub-contexts used by the batch entry point
+  // per-grid sub-contexts used by the uneven batch entry point
   std::map<std::tuple<int, int, int>, std::unique_ptr<femh::Context>> sub;
 };
 
@@ -37,6 +39,27 @@ int guarded(femh_ctx *ctx, F &&f) {
     ctx->err = e.what();
     ctx->last_status = FEMH_DEVICE_ERROR;
     return FEMH_DEVICE_ERROR;
+  }
+}
+
+int need_ctx(femh_ctx *ctx) {
+  if (!ctx) return FEMH_BAD_ARG;
+  if (!ctx->c) {
+    ctx->last_status = FEMH_BAD_ARG;
+    if (ctx->err.empty()) ctx->err = "context was not created successfully";
+    return FEMH_BAD_ARG;
+  }
+  return FEMH_OK;
+}
+
+// n samples through a context in chunks of its batch capacity
+void run_many(femh::Context &c, int n, const double *src, bool device_src, bool modulus, double *C,
+              femh_stats *stats) {
+  const size_t V = (size_t)c.g.V;
+  for (int s0 = 0; s0 < n; s0 += c.B) {
+    const int k = std::min(c.B, n - s0);
+    c.load(src + (size_t)s0 * V, k, device_src, modulus);
+    c.homogenize(C + 36 * (size_t)s0, stats ? stats + s0 : nullptr);
   }
 }
 
@@ -77,6 +100,7 @@ void femh_default_opts(femh_opts *o) {
   o->inner_tol = 1e-4;
   o->max_outer = 12;
   o->apply_kernel = 0;
+  o->batch = 1;
 }
 
 int femh_create(const femh_opts *o, femh_ctx **out) {
@@ -85,66 +109,61 @@ int femh_create(const femh_opts *o, femh_ctx **out) {
   femh_ctx *ctx = new (std::nothrow) femh_ctx();
   if (!ctx) return FEMH_OOM;
   ctx->opts = *o;
-  int st = guarded(ctx, [&] { ctx->c.reset(new femh::Context(*o)); });
-  if (st != FEMH_OK) {
-    // keep the context so the caller can read the message; it cannot solve
-    *out = ctx;
-    return st;
-  }
+  const int st = guarded(ctx, [&] { ctx->c.reset(new femh::Context(*o)); });
+  // on failure the context is still returned so the caller can read the message
   *out = ctx;
-  return FEMH_OK;
+  return st;
 }
 
 void femh_destroy(femh_ctx *ctx) { delete ctx; }
 
-static int need_ctx(femh_ctx *ctx) {
-  if (!ctx) return FEMH_BAD_ARG;
-  if (!ctx->c) {
-    ctx->last_status = FEMH_BAD_ARG;
-    if (ctx->err.empty()) ctx->err = "context was not created successfully";
-    return FEMH_BAD_ARG;
-  }
-  return FEMH_OK;
-}
-
 int femh_homogenize(femh_ctx *ctx, const double *rho, double *C, femh_stats *stats) {
   if (int s = need_ctx(ctx)) return s;
   if (!rho || !C) return FEMH_BAD_ARG;
-  return guarded(ctx, [&] {
-    ctx->c->set_modulus_from_rho(rho, false);
-    ctx->c->homogenize(C, stats);
-  });
+  return guarded(ctx, [&] { run_many(*ctx->c, 1, rho, false, false, C, stats); });
 }
 
 int femh_homogenize_modulus(femh_ctx *ctx, const double *E, double *C, femh_stats *stats) {
   if (int s = need_ctx(ctx)) return s;
   if (!E || !C) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] { run_many(*ctx->c, 1, E, false, true, C, stats); });
+}
+
+int femh_homogenize_many(femh_ctx *ctx, int n, const double *rho, double *C, femh_stats *stats) {
+  if (int s = need_ctx(ctx)) return s;
+  if (n < 1 || !rho || !C) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] { run_many(*ctx->c, n, rho, false, false, C, stats); });
+}
+
+int femh_homogenize_many_modulus(femh_ctx *ctx, int n, const double *E, double *C,
+                                 femh_stats *stats) {
+  if (int s = need_ctx(ctx)) return s;
+  if (n < 1 || !E || !C) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] { run_many(*ctx->c, n, E, false, true, C, stats); });
+}
+
+static int many_device(femh_ctx *ctx, int n, const double *d_rho, double *d_C, void *stream,
+                       femh_stats *stats) {
+  if (int s = need_ctx(ctx)) return s;
+  if (n < 1 || !d_rho || !d_C) return FEMH_BAD_ARG;
   return guarded(ctx, [&] {
-    int saved = ctx->c->o.interp;
-    ctx->c->o.interp = FEMH_INTERP_MODULUS;
-    try {
-      ctx->c->set_modulus(E);
-      ctx->c->homogenize(C, stats);
-    } catch (...) {
-      ctx->c->o.interp = saved;
-      throw;
-    }
-    ctx->c->o.interp = saved;
+    FEMH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    std::vector<double> C(36 * (size_t)n);
+    run_many(*ctx->c, n, d_rho, true, false, C.data(), stats);
+    FEMH_HIP(hipMemcpyAsync(d_C, C.data(), sizeof(double) * C.size(), hipMemcpyHostToDevice,
+                            ctx->c->stream));
+    FEMH_HIP(hipStreamSynchronize(ctx->c->stream));
   });
 }
 
 int femh_homogenize_device(femh_ctx *ctx, const double *d_rho, double *d_C, void *stream,
                            femh_stats *stats) {
-  if (int s = need_ctx(ctx)) return s;
-  if (!d_rho || !d_C) return FEMH_BAD_ARG;
-  return guarded(ctx, [&] {
-    FEMH_HIP(hipStreamSynchronize((hipStream_t)stream));
-    ctx->c->set_modulus_from_rho(d_rho, true);
-    double C[36];
-    ctx->c->homogenize(C, stats);
-    FEMH_HIP(hipMemcpyAsync(d_C, C, sizeof(C), hipMemcpyHostToDevice, ctx->c->stream));
-    FEMH_HIP(hipStreamSynchronize(ctx->c->stream));
-  });
+  return many_device(ctx, 1, d_rho, d_C, stream, stats);
+}
+
+int femh_homogenize_many_device(femh_ctx *ctx, int n, const double *d_rho, double *d_C,
+                                void *stream, femh_stats *stats) {
+  return many_device(ctx, n, d_rho, d_C, stream, stats);
 }
 
 int femh_gradient(femh_ctx *ctx, const double *dJdC, double *dJdrho) {
@@ -157,8 +176,8 @@ int femh_energy_terms(femh_ctx *ctx, double *W) {
   if (int s = need_ctx(ctx)) return s;
   if (!W) return FEMH_BAD_ARG;
   return guarded(ctx, [&] {
-    double zero[36] = {0};
-    ctx->c->gradient(zero, W, true);
+    std::vector<double> zero(36 * (size_t)std::max(ctx->c->nb, 1), 0.0);
+    ctx->c->gradient(zero.data(), W, true);
   });
 }
 
@@ -173,24 +192,42 @@ int femh_homogenize_batch(femh_ctx *ctx, int n, const int *dims, const double *c
   if (!ctx) return FEMH_BAD_ARG;
   if (n < 0 || (n > 0 && (!dims || !E || !C))) return FEMH_BAD_ARG;
   return guarded(ctx, [&] {
+    // group the samples by grid size; each group runs through a batched
+    // sub-context, `batch` samples per launch sequence
+    std::map<std::tuple<int, int, int>, std::vector<int>> groups;
+    for (int s = 0; s < n; s++)
+      groups[std::make_tuple(dims[3 * s], dims[3 * s + 1], dims[3 * s + 2])].push_back(s);
     int maxit = 0;
     double ms = 0;
-    for (int s = 0; s < n; s++) {
-      auto key = std::make_tuple(dims[3 * s], dims[3 * s + 1], dims[3 * s + 2]);
-      auto it = ctx->sub.find(key);
+    for (auto &kv : groups) {
+      auto it = ctx->sub.find(kv.first);
       if (it == ctx->sub.end()) {
         femh_opts o = ctx->opts;
-        o.nx = dims[3 * s];
-        o.ny = dims[3 * s + 1];
-        o.nz = dims[3 * s + 2];
+        o.nx = std::get<0>(kv.first);
+        o.ny = std::get<1>(kv.first);
+        o.nz = std::get<2>(kv.first);
         o.interp = FEMH_INTERP_MODULUS;
-        it = ctx->sub.emplace(key, std::unique_ptr<femh::Context>(new femh::Context(o))).first;
+        if (o.batch < 1) o.batch = 1;
+        it = ctx->sub.emplace(kv.first, std::unique_ptr<femh::Context>(new femh::Context(o))).first;
       }
-      femh_stats st{};
-      it->second->set_modulus(E[s]);
-      it->second->homogenize(C + 36 * (size_t)s, &st);
-      for (int k = 0; k < 6; k++) maxit = st.iters[k] > maxit ? st.iters[k] : maxit;
-      ms += st.ms_total;
+      femh::Context &c = *it->second;
+      const std::vector<int> &ids = kv.second;
+      const size_t V = (size_t)c.g.V;
+      std::vector<double> buf;
+      std::vector<double> Cg(36 * (size_t)c.B);
+      std::vector<femh_stats> st(c.B);
+      for (size_t s0 = 0; s0 < ids.size(); s0 += c.B) {
+        const int k = (int)std::min<size_t>(c.B, ids.size() - s0);
+        buf.resize((size_t)k * V);
+        for (int j = 0; j < k; j++) std::memcpy(buf.data() + j * V, E[ids[s0 + j]], sizeof(double) * V);
+        c.load(buf.data(), k, false, true);
+        c.homogenize(Cg.data(), st.data());
+        for (int j = 0; j < k; j++) {
+          std::memcpy(C + 36 * (size_t)ids[s0 + j], Cg.data() + 36 * j, sizeof(double) * 36);
+          for (int q = 0; q < 6; q++) maxit = std::max(maxit, st[j].iters[q]);
+        }
+        ms += st[0].ms_total;
+      }
     }
     if (stats) {
       std::memset(stats, 0, sizeof(*stats));
@@ -204,7 +241,7 @@ int femh_prepare(femh_ctx *ctx, const double *E) {
   if (int s = need_ctx(ctx)) return s;
   if (!E) return FEMH_BAD_ARG;
   return guarded(ctx, [&] {
-    ctx->c->set_modulus(E);
+    ctx->c->load(E, 1, false, true);
     ctx->c->build_hierarchy();
   });
 }

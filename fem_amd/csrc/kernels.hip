@@ -1,10 +1,9 @@
-// kernels.hip -- gfx950 kernels of the voxel homogenization engine.
+// kernels.hip -- gfx950 setup, transfer, vector and reduction kernels of the
+// voxel homogenization engine (the fused operator kernels are in ops.hip).
 //
-// Every per-node kernel runs a grid-stride loop over the V nodes of one
-// right-hand side (blockIdx.y selects the RHS, skipped when its mask bit is
-// clear).  Block size is 256 (four wave64s).  FP64 reductions are
-// deterministic: per-block partials + last-block-done summation in a fixed
-// order (see Reduce in kernels.hpp).
+// Per-RHS kernels run a grid-stride loop over one right-hand side q
+// (blockIdx.y) and return at once when act[q] == 0.  Block size is 256 (four
+// wave64s).  Reductions: see device_common.hpp.
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
@@ -13,17 +12,18 @@
 namespace femh {
 
 // ------------------------------------------------------------------------
-// fine level: matrix-free periodic K u, node gather over the 8 incident voxels
+// fine level, dense node gather over the 8 incident voxels (diagnostic)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fine_apply(FineOp<T> op, const T *__restrict__ u,
                                                        T *__restrict__ y, const T *__restrict__ b,
-                                                       int mode, unsigned mask, Reduce red, int slot,
-                                                       int dot) {
+                                                       int mode, const int *__restrict__ act,
+                                                       Reduce red, int slot, int dot) {
   const int r = blockIdx.y;
-  if (!((mask >> r) & 1u)) return;
+  if (!act[r]) return;
   const Dims g = op.g;
   const size_t off = (size_t)r * 3 * g.V;
   const T *__restrict__ ur = u + off;
+  const T *__restrict__ Es = op.E + (size_t)(r / kRHS) * g.V;
   const T *__restrict__ Ke = op.Ke;
   double dv = 0;
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kBlock) void k_fine_apply(FineOp<T> op, const T *__
 #pragma unroll
     for (int a = 0; a < 8; a++) {
       const int ax = (a >> 2) & 1, ay = (a >> 1) & 1, az = a & 1;
-      const T Ee = op.E[(I[1 - ax] * g.ny + J[1 - ay]) * g.nz + K[1 - az]];
+      const T Ee = Es[(I[1 - ax] * g.ny + J[1 - ay]) * g.nz + K[1 - az]];
       T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
       for (int bb = 0; bb < 8; bb++) {
@@ -72,10 +72,12 @@ __global__ __launch_bounds__(kBlock) void k_fine_apply(FineOp<T> op, const T *__
   }
 }
 
-
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fine_diag(FineOp<T> op, T *__restrict__ d) {
   const Dims g = op.g;
+  const int s = blockIdx.y;
+  const T *Es = op.E + (size_t)s * g.V;
+  T *ds = d + (size_t)s * 3 * g.V;
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
     int i, j, k;
     node_ijk(g, n, i, j, k);
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(kBlock) void k_fine_diag(FineOp<T> op, T *__restric
     for (int a = 0; a < 8; a++) {
       const int ei = (i - ((a >> 2) & 1) + g.nx) % g.nx, ej = (j - ((a >> 1) & 1) + g.ny) % g.ny,
                 ek = (k - (a & 1) + g.nz) % g.nz;
-      const double Ee = (double)op.E[(ei * g.ny + ej) * g.nz + ek];
+      const double Ee = (double)Es[(ei * g.ny + ej) * g.nz + ek];
 #pragma unroll
       for (int c = 0; c < 3; c++) acc[c] += Ee * (double)op.Ke[(3 * a + c) * 25];
     }
@@ -93,9 +95,9 @@ __global__ __launch_bounds__(kBlock) void k_fine_diag(FineOp<T> op, T *__restric
 #pragma unroll
     for (int c = 0; c < 3; c++)
       if (!(acc[c] > 0.0)) acc[c] = 1.0;
-    d[n] = (T)acc[0];
-    d[g.V + n] = (T)acc[1];
-    d[2 * g.V + n] = (T)acc[2];
+    ds[n] = (T)acc[0];
+    ds[g.V + n] = (T)acc[1];
+    ds[2 * g.V + n] = (T)acc[2];
   }
 }
 
@@ -103,8 +105,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fine_rhs(FineOp<T> op, const double *__restrict__ gtab,
                                                      T *__restrict__ b) {
   const Dims g = op.g;
-  const int lc = blockIdx.y;
-  T *br = b + (size_t)lc * 3 * g.V;
+  const int q = blockIdx.y, lc = q % kRHS;
+  const T *Es = op.E + (size_t)(q / kRHS) * g.V;
+  T *br = b + (size_t)q * 3 * g.V;
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
     int i, j, k;
     node_ijk(g, n, i, j, k);
@@ -113,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void k_fine_rhs(FineOp<T> op, const double 
     for (int a = 0; a < 8; a++) {
       const int ei = (i - ((a >> 2) & 1) + g.nx) % g.nx, ej = (j - ((a >> 1) & 1) + g.ny) % g.ny,
                 ek = (k - (a & 1) + g.nz) % g.nz;
-      const double Ee = (double)op.E[(ei * g.ny + ej) * g.nz + ek];
+      const double Ee = (double)Es[(ei * g.ny + ej) * g.nz + ek];
 #pragma unroll
       for (int c = 0; c < 3; c++) acc[c] -= Ee * gtab[lc * 24 + 3 * a + c];
     }
@@ -124,15 +127,17 @@ __global__ __launch_bounds__(kBlock) void k_fine_rhs(FineOp<T> op, const double 
 }
 
 // ------------------------------------------------------------------------
-// coarse levels: 27-point block stencils
+// coarse levels
 // Level-1 Galerkin stencil straight from the fine moduli: the coarse element
 // E_c (2x2x2 voxels) has matrix sum_f E_f M_f; thread per (coarse node, offset).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_build_level1(Dims fg, Dims cg, const T *__restrict__ E,
                                                          const double *__restrict__ M,
                                                          T *__restrict__ S) {
-  const int o = blockIdx.y;
+  const int o = blockIdx.y, s = blockIdx.z;
   const int di = o / 9 - 1, dj = (o / 3) % 3 - 1, dk = o % 3 - 1;
+  const T *Es = E + (size_t)s * fg.V;
+  T *Ss = S + (size_t)s * 243 * cg.V;
   for (int N = blockIdx.x * kBlock + threadIdx.x; N < cg.V; N += gridDim.x * kBlock) {
     int I, J, K;
     node_ijk(cg, N, I, J, K);
@@ -146,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_build_level1(Dims fg, Dims cg, const
                 ek = (K - az + cg.nz) % cg.nz;
       for (int f = 0; f < 8; f++) {
         const int fi = 2 * ei + ((f >> 2) & 1), fj = 2 * ej + ((f >> 1) & 1), fk = 2 * ek + (f & 1);
-        const double Ef = (double)E[(fi * fg.ny + fj) * fg.nz + fk];
+        const double Ef = (double)Es[(fi * fg.ny + fj) * fg.nz + fk];
         const double *m = M + f * 576 + (3 * A) * 24 + 3 * B;
 #pragma unroll
         for (int c = 0; c < 3; c++)
@@ -155,46 +160,72 @@ __global__ __launch_bounds__(kBlock) void k_build_level1(Dims fg, Dims cg, const
       }
     }
 #pragma unroll
-    for (int q = 0; q < 9; q++) S[(size_t)(o * 9 + q) * cg.V + N] = (T)blk[q];
+    for (int q = 0; q < 9; q++) Ss[(size_t)(o * 9 + q) * cg.V + N] = (T)blk[q];
   }
 }
 
 // Galerkin triple product on stencils: A_c[N, N+D] = sum P[n,N] A_f[n,m] P[m,N+D]
+// with fine n = 2N + o and m = n + d.  Per axis the (o, d) pairs that reach
+// coarse offset D with a non-zero weight are enumerated first (at most 7 of
+// the 9), so the inner loop visits only contributing (n, m) pairs.
+struct AxisPairs {
+  int n;
+  int o[9], d[9];
+  double w[9];
+};
+
+__device__ __forceinline__ void axis_pairs(int D, AxisPairs &p) {
+  p.n = 0;
+#pragma unroll
+  for (int o = -1; o <= 1; o++)
+#pragma unroll
+    for (int d = -1; d <= 1; d++) {
+      const int t = o + d;
+      double w;
+      if ((t & 1) == 0) w = (t / 2 == D) ? 1.0 : 0.0;
+      else w = (D == (t - 1) / 2 || D == (t + 1) / 2) ? 0.5 : 0.0;
+      w *= o ? 0.5 : 1.0;
+      if (w != 0.0) {
+        p.o[p.n] = o;
+        p.d[p.n] = d;
+        p.w[p.n] = w;
+        p.n++;
+      }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_galerkin(Dims fg, Dims cg, const T *__restrict__ Sf,
                                                      T *__restrict__ Sc) {
-  const int o = blockIdx.y;
-  const int D[3] = {o / 9 - 1, (o / 3) % 3 - 1, o % 3 - 1};
+  const int oc = blockIdx.y, s = blockIdx.z;
+  const T *Sfs = Sf + (size_t)s * 243 * fg.V;
+  T *Scs = Sc + (size_t)s * 243 * cg.V;
+  AxisPairs px, py, pz;
+  axis_pairs(oc / 9 - 1, px);
+  axis_pairs((oc / 3) % 3 - 1, py);
+  axis_pairs(oc % 3 - 1, pz);
   for (int N = blockIdx.x * kBlock + threadIdx.x; N < cg.V; N += gridDim.x * kBlock) {
     int I, J, K;
     node_ijk(cg, N, I, J, K);
     double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int on = 0; on < 27; on++) {
-      const int oi = on / 9 - 1, oj = (on / 3) % 3 - 1, ok = on % 3 - 1;
-      const double wn = (oi ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
-      const int fi = (2 * I + oi + fg.nx) % fg.nx, fj = (2 * J + oj + fg.ny) % fg.ny,
-                fk = (2 * K + ok + fg.nz) % fg.nz;
-      const int n = (fi * fg.ny + fj) * fg.nz + fk;
-      for (int dm = 0; dm < 27; dm++) {
-        const int t[3] = {oi + dm / 9 - 1, oj + (dm / 3) % 3 - 1, ok + dm % 3 - 1};
-        double wm = 1;
+    for (int a = 0; a < px.n; a++) {
+      const int fi = (2 * I + px.o[a] + fg.nx) % fg.nx;
+      for (int bq = 0; bq < py.n; bq++) {
+        const int fj = (2 * J + py.o[bq] + fg.ny) % fg.ny;
+        const double wab = px.w[a] * py.w[bq];
+        for (int c = 0; c < pz.n; c++) {
+          const int fk = (2 * K + pz.o[c] + fg.nz) % fg.nz;
+          const int n = (fi * fg.ny + fj) * fg.nz + fk;
+          const int dm = (px.d[a] + 1) * 9 + (py.d[bq] + 1) * 3 + (pz.d[c] + 1);
+          const double w = wab * pz.w[c];
+          const T *src = Sfs + (size_t)(dm * 9) * fg.V + n;
 #pragma unroll
-        for (int ax = 0; ax < 3; ax++) {
-          const int ta = t[ax];
-          double wa;
-          if ((ta & 1) == 0) wa = (ta / 2 == D[ax]) ? 1.0 : 0.0;
-          else wa = (D[ax] == (ta - 1) / 2 || D[ax] == (ta + 1) / 2) ? 0.5 : 0.0;
-          wm *= wa;
+          for (int q = 0; q < 9; q++) blk[q] += w * (double)src[(size_t)q * fg.V];
         }
-        if (wm == 0.0) continue;
-        const double w = wn * wm;
-        const T *s = Sf + (size_t)(dm * 9) * fg.V + n;
-#pragma unroll
-        for (int q = 0; q < 9; q++) blk[q] += w * (double)s[(size_t)q * fg.V];
       }
     }
 #pragma unroll
-    for (int q = 0; q < 9; q++) Sc[(size_t)(o * 9 + q) * cg.V + N] = (T)blk[q];
+    for (int q = 0; q < 9; q++) Scs[(size_t)(oc * 9 + q) * cg.V + N] = (T)blk[q];
   }
 }
 
@@ -207,31 +238,41 @@ __global__ __launch_bounds__(kBlock) void k_invert(size_t n, const T *__restrict
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_stencil_diag(StencilOp<T> op, T *__restrict__ d) {
   const Dims g = op.g;
+  const int s = blockIdx.y;
+  const T *S = op.S + (size_t)s * 243 * g.V;
+  T *ds = d + (size_t)s * 3 * g.V;
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const T v = op.S[(size_t)(13 * 9 + 4 * c) * g.V + n];
-      d[(size_t)c * g.V + n] = v > (T)0 ? v : (T)1;
+      const T v = S[(size_t)(13 * 9 + 4 * c) * g.V + n];
+      ds[(size_t)c * g.V + n] = v > (T)0 ? v : (T)1;
     }
   }
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_restrict(Dims fg, Dims cg, const T *__restrict__ rf,
-                                                     T *__restrict__ rc, unsigned mask) {
+                                                     T *__restrict__ rc, const int *__restrict__ act) {
   const int r = blockIdx.y;
-  if (!((mask >> r) & 1u)) return;
+  if (!act[r]) return;
   const T *f = rf + (size_t)r * 3 * fg.V;
   T *c = rc + (size_t)r * 3 * cg.V;
   for (int N = blockIdx.x * kBlock + threadIdx.x; N < cg.V; N += gridDim.x * kBlock) {
     int I, J, K;
     node_ijk(cg, N, I, J, K);
+    int fi[3], fj[3], fk[3];
+#pragma unroll
+    for (int o = 0; o < 3; o++) {
+      fi[o] = (2 * I + o - 1 + fg.nx) % fg.nx;
+      fj[o] = (2 * J + o - 1 + fg.ny) % fg.ny;
+      fk[o] = (2 * K + o - 1 + fg.nz) % fg.nz;
+    }
     double acc[3] = {0, 0, 0};
+#pragma unroll
     for (int on = 0; on < 27; on++) {
-      const int oi = on / 9 - 1, oj = (on / 3) % 3 - 1, ok = on % 3 - 1;
-      const double w = (oi ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
-      const int n = (((2 * I + oi + fg.nx) % fg.nx) * fg.ny + (2 * J + oj + fg.ny) % fg.ny) * fg.nz +
-                    (2 * K + ok + fg.nz) % fg.nz;
+      const int oi = on / 9, oj = (on / 3) % 3, ok = on % 3;
+      const double w = (oi != 1 ? 0.5 : 1.0) * (oj != 1 ? 0.5 : 1.0) * (ok != 1 ? 0.5 : 1.0);
+      const int n = (fi[oi] * fg.ny + fj[oj]) * fg.nz + fk[ok];
 #pragma unroll
       for (int q = 0; q < 3; q++) acc[q] += w * (double)f[(size_t)q * fg.V + n];
     }
@@ -242,9 +283,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Dims fg, Dims cg, const T *
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(Dims fg, Dims cg, const T *__restrict__ ec,
-                                                        T *__restrict__ ef, unsigned mask) {
+                                                        T *__restrict__ ef,
+                                                        const int *__restrict__ act) {
   const int r = blockIdx.y;
-  if (!((mask >> r) & 1u)) return;
+  if (!act[r]) return;
   const T *c = ec + (size_t)r * 3 * cg.V;
   T *f = ef + (size_t)r * 3 * fg.V;
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < fg.V; n += gridDim.x * kBlock) {
@@ -271,14 +313,15 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Dims fg, Dims cg, const 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dense_solve(int n, const T *__restrict__ Ainv,
                                                         const T *__restrict__ b, T *__restrict__ x,
-                                                        unsigned mask) {
+                                                        const int *__restrict__ act) {
   const int r = blockIdx.y;
-  if (!((mask >> r) & 1u)) return;
+  if (!act[r]) return;
+  const T *A = Ainv + (size_t)(r / kRHS) * n * n;
   const T *br = b + (size_t)r * n;
   T *xr = x + (size_t)r * n;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     double acc = 0;
-    const T *row = Ainv + (size_t)i * n;
+    const T *row = A + (size_t)i * n;
     for (int j = 0; j < n; j++) acc += (double)row[j] * (double)br[j];
     xr[i] = (T)acc;
   }
@@ -290,32 +333,15 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cheb_first(int V, const T *__restrict__ r,
                                                        const T *__restrict__ D, T *__restrict__ d,
                                                        T *__restrict__ x, double inv_theta,
-                                                       int x_zero, unsigned mask) {
+                                                       int x_zero, const int *__restrict__ act) {
   const int rr = blockIdx.y;
-  if (!((mask >> rr) & 1u)) return;
+  if (!act[rr]) return;
   const size_t off = (size_t)rr * 3 * V;
+  const T *Ds = D + (size_t)(rr / kRHS) * 3 * V;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) {
-    const T dv = r[off + t] * (T)inv_theta * D[t];
+    const T dv = r[off + t] * (T)inv_theta * Ds[t];
     d[off + t] = dv;
     x[off + t] = x_zero ? dv : x[off + t] + dv;
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cheb_next(int V, const T *rsrc, T *r,
-                                                      const T *__restrict__ Ad,
-                                                      const T *__restrict__ D, T *__restrict__ d,
-                                                      T *__restrict__ x, double c1, double c2,
-                                                      unsigned mask) {
-  const int rr = blockIdx.y;
-  if (!((mask >> rr) & 1u)) return;
-  const size_t off = (size_t)rr * 3 * V;
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) {
-    const T rv = rsrc[off + t] - Ad[off + t];
-    r[off + t] = rv;
-    const T dv = (T)c1 * d[off + t] + (T)c2 * rv * D[t];
-    d[off + t] = dv;
-    x[off + t] += dv;
   }
 }
 
@@ -324,30 +350,32 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int V, T *__restrict__ x,
                                                        const T *__restrict__ p,
                                                        const T *__restrict__ q,
                                                        const T *__restrict__ D, T *__restrict__ z,
-                                                       int rzslot, int rz_out, unsigned mask,
-                                                       Reduce red, int np_pq, int np_rz) {
+                                                       int rzslot, int rz_out,
+                                                       const int *__restrict__ act, Reduce red,
+                                                       int np_pq, int np_rz) {
   const int rr = blockIdx.y;
-  if (!((mask >> rr) & 1u)) return;
+  if (!act[rr]) return;
   const double pq = sum_parts(red, S_PQ, rr, np_pq);
   const double rz = sum_parts(red, rzslot, rr, np_rz);
   const double alpha = pq != 0.0 ? rz / pq : 0.0;
   const T a = (T)alpha;
   const size_t off = (size_t)rr * 3 * V;
+  const T *Ds = D ? D + (size_t)(rr / kRHS) * 3 * V : nullptr;
   double srr = 0, srz = 0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) {
     x[off + t] += a * p[off + t];
     const T rv = r[off + t] - a * q[off + t];
     r[off + t] = rv;
     srr += (double)rv * (double)rv;
-    if (D) {
-      const T zv = rv * D[t];
+    if (Ds) {
+      const T zv = rv * Ds[t];
       z[off + t] = zv;
       srz += (double)rv * (double)zv;
     }
   }
   double s = block_sum(srr);
   block_partial(red, S_RR, rr, blockIdx.x, s);
-  if (D) {
+  if (Ds) {
     s = block_sum(srz);
     block_partial(red, rz_out, rr, blockIdx.x, s);
   }
@@ -356,10 +384,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int V, T *__restrict__ x,
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_p_update(int V, T *__restrict__ p,
                                                      const T *__restrict__ z, int rz_new,
-                                                     int rz_old, int first, unsigned mask,
-                                                     Reduce red, int np) {
+                                                     int rz_old, int first,
+                                                     const int *__restrict__ act, Reduce red,
+                                                     int np) {
   const int rr = blockIdx.y;
-  if (!((mask >> rr) & 1u)) return;
+  if (!act[rr]) return;
   double beta = 0;
   if (!first) {
     const double den = sum_parts(red, rz_old, rr, np);
@@ -374,10 +403,10 @@ __global__ __launch_bounds__(kBlock) void k_p_update(int V, T *__restrict__ p,
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(int V, const T *__restrict__ a,
-                                                const T *__restrict__ b, int slot, unsigned mask,
-                                                Reduce red) {
+                                                const T *__restrict__ b, int slot,
+                                                const int *__restrict__ act, Reduce red) {
   const int rr = blockIdx.y;
-  if (!((mask >> rr) & 1u)) return;
+  if (!act[rr]) return;
   const size_t off = (size_t)rr * 3 * V;
   double s = 0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
@@ -394,19 +423,20 @@ __global__ __launch_bounds__(kBlock) void k_convert(size_t n, const Ts *__restri
 }
 
 __global__ __launch_bounds__(kBlock) void k_add_f32_to_f64(int V, const float *__restrict__ d,
-                                                           double *__restrict__ x, unsigned mask) {
+                                                           double *__restrict__ x,
+                                                           const int *__restrict__ act) {
   const int rr = blockIdx.y;
-  if (!((mask >> rr) & 1u)) return;
+  if (!act[rr]) return;
   const size_t off = (size_t)rr * 3 * V;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
     x[off + t] += (double)d[off + t];
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mean_sum(int V, const T *__restrict__ x, unsigned mask,
-                                                     Reduce red) {
+__global__ __launch_bounds__(kBlock) void k_mean_sum(int V, const T *__restrict__ x,
+                                                     const int *__restrict__ act, Reduce red) {
   const int rr = blockIdx.y / 3, c = blockIdx.y % 3;
-  if (!((mask >> rr) & 1u)) return;
+  if (!act[rr]) return;
   const T *xc = x + (size_t)rr * 3 * V + (size_t)c * V;
   double s = 0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock) s += (double)xc[t];
@@ -415,20 +445,21 @@ __global__ __launch_bounds__(kBlock) void k_mean_sum(int V, const T *__restrict_
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mean_sub(int V, T *__restrict__ x, unsigned mask,
-                                                     Reduce red, int np) {
+__global__ __launch_bounds__(kBlock) void k_mean_sub(int V, T *__restrict__ x,
+                                                     const int *__restrict__ act, Reduce red,
+                                                     int np) {
   const int rr = blockIdx.y / 3, c = blockIdx.y % 3;
-  if (!((mask >> rr) & 1u)) return;
+  if (!act[rr]) return;
   const T m = (T)(sum_parts(red, S_AUX, blockIdx.y, np) / (double)V);
   T *xc = x + (size_t)rr * 3 * V + (size_t)c * V;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock) xc[t] -= m;
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_fill_hash(size_t n, T *__restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_fill_hash(size_t n, size_t period, T *__restrict__ x,
                                                       unsigned long long seed) {
   for (size_t t = blockIdx.x * (size_t)kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock) {
-    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (t + 1);
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (t % period + 1);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
@@ -438,24 +469,34 @@ __global__ __launch_bounds__(kBlock) void k_fill_hash(size_t n, T *__restrict__ 
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scale_dinv_norm(int V, T *__restrict__ y,
-                                                            const T *__restrict__ D, Reduce red) {
+                                                            const T *__restrict__ Dinv,
+                                                            const int *__restrict__ act,
+                                                            Reduce red) {
+  const int rr = blockIdx.y;
+  if (!act[rr]) return;
+  const size_t off = (size_t)rr * 3 * V;
+  const T *Ds = Dinv + (size_t)(rr / kRHS) * 3 * V;
   double s = 0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) {
-    const T v = y[t] * D[t];
-    y[t] = v;
+    const T v = y[off + t] * Ds[t];
+    y[off + t] = v;
     s += (double)v * (double)v;
   }
   s = block_sum(s);
-  block_partial(red, S_AUX, 0, blockIdx.x, s);
+  block_partial(red, S_AUX, rr, blockIdx.x, s);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_normalize(int V, const T *__restrict__ y,
-                                                      T *__restrict__ x, Reduce red, int np) {
-  const double nrm = sqrt(sum_parts(red, S_AUX, 0, np));
+                                                      T *__restrict__ x, const int *__restrict__ act,
+                                                      Reduce red, int np) {
+  const int rr = blockIdx.y;
+  if (!act[rr]) return;
+  const double nrm = sqrt(sum_parts(red, S_AUX, rr, np));
   const double inv = nrm > 0 ? 1.0 / nrm : 0.0;
+  const size_t off = (size_t)rr * 3 * V;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
-    x[t] = (T)((double)y[t] * inv);
+    x[off + t] = (T)((double)y[off + t] * inv);
 }
 
 // ------------------------------------------------------------------------
@@ -476,12 +517,14 @@ __device__ __forceinline__ double interp_dE(double rho, int interp, double E0, d
   return 1.0;
 }
 
-// partials into S_AUX: idx 0 sum E, idx 1 non-finite count, idx 2 negative count
+// partials into S_AUX: idx 3s sum E, 3s+1 non-finite count, 3s+2 negative count
 __global__ __launch_bounds__(kBlock) void k_modulus_stats(int V, const double *__restrict__ E,
                                                           Reduce red) {
+  const int smp = blockIdx.y;
+  const double *Es = E + (size_t)smp * V;
   double s = 0, nf = 0, ng = 0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock) {
-    const double e = E[t];
+    const double e = Es[t];
     if (!isfinite(e)) nf += 1.0;
     else {
       s += e;
@@ -489,25 +532,26 @@ __global__ __launch_bounds__(kBlock) void k_modulus_stats(int V, const double *_
     }
   }
   double b = block_sum(s);
-  block_partial(red, S_AUX, 0, blockIdx.x, b);
+  block_partial(red, S_AUX, 3 * smp, blockIdx.x, b);
   b = block_sum(nf);
-  block_partial(red, S_AUX, 1, blockIdx.x, b);
+  block_partial(red, S_AUX, 3 * smp + 1, blockIdx.x, b);
   b = block_sum(ng);
-  block_partial(red, S_AUX, 2, blockIdx.x, b);
+  block_partial(red, S_AUX, 3 * smp + 2, blockIdx.x, b);
 }
 
-__global__ __launch_bounds__(kBlock) void k_material(int V, const double *__restrict__ rho,
+__global__ __launch_bounds__(kBlock) void k_material(size_t n, const double *__restrict__ rho,
                                                      double *__restrict__ E, int interp, double E0,
                                                      double Emin, double p) {
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock)
+  for (size_t t = blockIdx.x * (size_t)kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock)
     E[t] = interp_E(rho[t], interp, E0, Emin, p);
 }
 
-// chi_i of voxel (i,j,k) for load case lc: affine local field + gathered fluctuation
+// chi of voxel (i,j,k) for load case lc of one sample: affine local field +
+// gathered periodic fluctuation (us = that sample's 6 fields)
 __device__ __forceinline__ void gather_chi(const Dims &g, const int I[2], const int J[2],
-                                           const int K[2], const double *__restrict__ u,
+                                           const int K[2], const double *__restrict__ us,
                                            const double *__restrict__ uaff, int lc, double *chi) {
-  const double *ul = u + (size_t)lc * 3 * g.V;
+  const double *ul = us + (size_t)lc * 3 * g.V;
 #pragma unroll
   for (int v = 0; v < 8; v++) {
     const int n = (I[(v >> 2) & 1] * g.ny + J[(v >> 1) & 1]) * g.nz + K[v & 1];
@@ -518,7 +562,7 @@ __device__ __forceinline__ void gather_chi(const Dims &g, const int I[2], const 
 
 // W_p(e) = chi_i^T K0 chi_j for the 21 pairs i<=j (not divided by V)
 __device__ void element_energy(const Dims &g, int e, const double *__restrict__ Ke,
-                               const double *__restrict__ uaff, const double *__restrict__ u,
+                               const double *__restrict__ uaff, const double *__restrict__ us,
                                double *W) {
   int i, j, k;
   node_ijk(g, e, i, j, k);
@@ -528,7 +572,7 @@ __device__ void element_energy(const Dims &g, int e, const double *__restrict__ 
   int p = 0;
   for (int a = 0; a < 6; a++) {
     double chi[24], w[24];
-    gather_chi(g, I, J, K, u, uaff, a, chi);
+    gather_chi(g, I, J, K, us, uaff, a, chi);
 #pragma unroll
     for (int r = 0; r < 24; r++) {
       double s = 0;
@@ -542,7 +586,7 @@ __device__ void element_energy(const Dims &g, int e, const double *__restrict__ 
 #pragma unroll
         for (int q = 0; q < 24; q++) chib[q] = chi[q];
       } else {
-        gather_chi(g, I, J, K, u, uaff, b, chib);
+        gather_chi(g, I, J, K, us, uaff, b, chib);
       }
       double s = 0;
 #pragma unroll
@@ -557,25 +601,28 @@ __global__ __launch_bounds__(kBlock) void k_ch_energy(Dims g, const double *__re
                                                       const double *__restrict__ uaff,
                                                       const double *__restrict__ u,
                                                       double *__restrict__ partials) {
+  const int smp = blockIdx.y;
+  const double *Es = E + (size_t)smp * g.V;
+  const double *us = u + (size_t)smp * kRHS * 3 * g.V;
   double acc[21];
 #pragma unroll
   for (int p = 0; p < 21; p++) acc[p] = 0;
   for (int e = blockIdx.x * kBlock + threadIdx.x; e < g.V; e += gridDim.x * kBlock) {
     double W[21];
-    element_energy(g, e, Ke, uaff, u, W);
-    const double Ee = E[e];
+    element_energy(g, e, Ke, uaff, us, W);
+    const double Ee = Es[e];
 #pragma unroll
     for (int p = 0; p < 21; p++) acc[p] += Ee * W[p];
   }
   for (int p = 0; p < 21; p++) {
     double s = block_sum(acc[p]);
-    if (threadIdx.x == 0) partials[(size_t)p * gridDim.x + blockIdx.x] = s;
+    if (threadIdx.x == 0) partials[((size_t)smp * 21 + p) * gridDim.x + blockIdx.x] = s;
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restrict__ partials,
                                                          int nblk, double *__restrict__ out) {
-  const int p = blockIdx.x;
+  const int p = blockIdx.x;  // (sample, pair)
   double s = 0;
   for (int t = threadIdx.x; t < nblk; t += kBlock) s += partials[(size_t)p * nblk + t];
   s = block_sum(s);
@@ -589,18 +636,22 @@ __global__ __launch_bounds__(kBlock) void k_gradient(Dims g, const double *__res
                                                      const double *__restrict__ Gs, int interp,
                                                      double E0, double Emin, double pexp,
                                                      int energy_only, double *__restrict__ out) {
+  const int smp = blockIdx.y;
   const double invV = 1.0 / (double)g.V;
+  const double *us = u + (size_t)smp * kRHS * 3 * g.V;
+  const double *rs = rho + (size_t)smp * g.V;
+  const double *G = Gs + (size_t)smp * 21;
   for (int e = blockIdx.x * kBlock + threadIdx.x; e < g.V; e += gridDim.x * kBlock) {
     double W[21];
-    element_energy(g, e, Ke, uaff, u, W);
+    element_energy(g, e, Ke, uaff, us, W);
     if (energy_only) {
 #pragma unroll
-      for (int p = 0; p < 21; p++) out[(size_t)e * 21 + p] = W[p] * invV;
+      for (int p = 0; p < 21; p++) out[((size_t)smp * g.V + e) * 21 + p] = W[p] * invV;
     } else {
       double s = 0;
 #pragma unroll
-      for (int p = 0; p < 21; p++) s += Gs[p] * W[p];
-      out[e] = s * invV * interp_dE(rho[e], interp, E0, Emin, pexp);
+      for (int p = 0; p < 21; p++) s += G[p] * W[p];
+      out[(size_t)smp * g.V + e] = s * invV * interp_dE(rs[e], interp, E0, Emin, pexp);
     }
   }
 }
@@ -623,78 +674,78 @@ __global__ void k_reduce_init(unsigned *ctr, double *scal, int n) {
 static inline dim3 grid2(int V, int y) { return dim3(grid_for(V), y); }
 
 template <typename T>
-void launch_dense_apply(const FineOp<T> &op, const T *u, T *y, const T *b, int mode, unsigned mask,
-                        const Reduce *red, int slot, hipStream_t s) {
+void launch_dense_apply(const FineOp<T> &op, int R, const T *u, T *y, const T *b, int mode,
+                        const int *act, const Reduce *red, int slot, hipStream_t s) {
   Reduce rr = red ? *red : Reduce{nullptr, nullptr, nullptr};
-  hipLaunchKernelGGL(k_fine_apply<T>, grid2(op.g.V, kRHS), dim3(kBlock), 0, s, op, u, y, b, mode,
-                     mask, rr, slot, red ? 1 : 0);
+  hipLaunchKernelGGL(k_fine_apply<T>, grid2(op.g.V, R), dim3(kBlock), 0, s, op, u, y, b, mode, act,
+                     rr, slot, red ? 1 : 0);
 }
 
 int dense_parts(const Dims &g) { return grid_for(g.V); }
 
 template <typename T>
-void launch_fine_diag(const FineOp<T> &op, T *d, hipStream_t s) {
-  hipLaunchKernelGGL(k_fine_diag<T>, dim3(grid_for(op.g.V)), dim3(kBlock), 0, s, op, d);
+void launch_fine_diag(const FineOp<T> &op, int B, T *d, hipStream_t s) {
+  hipLaunchKernelGGL(k_fine_diag<T>, grid2(op.g.V, B), dim3(kBlock), 0, s, op, d);
 }
 template <typename T>
-void launch_fine_rhs(const FineOp<T> &op, const double *g, T *b, hipStream_t s) {
-  hipLaunchKernelGGL(k_fine_rhs<T>, grid2(op.g.V, kRHS), dim3(kBlock), 0, s, op, g, b);
+void launch_fine_rhs(const FineOp<T> &op, int R, const double *g, T *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_fine_rhs<T>, grid2(op.g.V, R), dim3(kBlock), 0, s, op, g, b);
 }
 template <typename T>
-void launch_build_level1(Dims fine, Dims coarse, const T *E, const double *M, T *S, hipStream_t s) {
-  hipLaunchKernelGGL(k_build_level1<T>, grid2(coarse.V, 27), dim3(kBlock), 0, s, fine, coarse, E,
-                     M, S);
+void launch_build_level1(Dims fine, Dims coarse, int B, const T *E, const double *M, T *S,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_build_level1<T>, dim3(grid_for(coarse.V), 27, B), dim3(kBlock), 0, s, fine,
+                     coarse, E, M, S);
 }
 template <typename T>
-void launch_galerkin(Dims fine, Dims coarse, const T *Sf, T *Sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_galerkin<T>, grid2(coarse.V, 27), dim3(kBlock), 0, s, fine, coarse, Sf, Sc);
+void launch_galerkin(Dims fine, Dims coarse, int B, const T *Sf, T *Sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_galerkin<T>, dim3(grid_for(coarse.V), 27, B), dim3(kBlock), 0, s, fine,
+                     coarse, Sf, Sc);
 }
 template <typename T>
-void launch_stencil_diag(const StencilOp<T> &op, T *d, hipStream_t s) {
-  hipLaunchKernelGGL(k_stencil_diag<T>, dim3(grid_for(op.g.V)), dim3(kBlock), 0, s, op, d);
+void launch_stencil_diag(const StencilOp<T> &op, int B, T *d, hipStream_t s) {
+  hipLaunchKernelGGL(k_stencil_diag<T>, grid2(op.g.V, B), dim3(kBlock), 0, s, op, d);
 }
 template <typename T>
-void launch_restrict(Dims fine, Dims coarse, const T *rf, T *rc, unsigned mask, hipStream_t s) {
-  hipLaunchKernelGGL(k_restrict<T>, grid2(coarse.V, kRHS), dim3(kBlock), 0, s, fine, coarse, rf,
-                     rc, mask);
+void launch_restrict(Dims fine, Dims coarse, int R, const T *rf, T *rc, const int *act,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_restrict<T>, grid2(coarse.V, R), dim3(kBlock), 0, s, fine, coarse, rf, rc,
+                     act);
 }
 template <typename T>
-void launch_prolong_add(Dims fine, Dims coarse, const T *ec, T *ef, unsigned mask, hipStream_t s) {
-  hipLaunchKernelGGL(k_prolong_add<T>, grid2(fine.V, kRHS), dim3(kBlock), 0, s, fine, coarse, ec,
-                     ef, mask);
+void launch_prolong_add(Dims fine, Dims coarse, int R, const T *ec, T *ef, const int *act,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_prolong_add<T>, grid2(fine.V, R), dim3(kBlock), 0, s, fine, coarse, ec, ef,
+                     act);
 }
 template <typename T>
-void launch_dense_solve(int n, const T *Ainv, const T *b, T *x, unsigned mask, hipStream_t s) {
-  hipLaunchKernelGGL(k_dense_solve<T>, grid2(n, kRHS), dim3(kBlock), 0, s, n, Ainv, b, x, mask);
+void launch_dense_solve(int n, int R, const T *Ainv, const T *b, T *x, const int *act,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_dense_solve<T>, grid2(n, R), dim3(kBlock), 0, s, n, Ainv, b, x, act);
 }
 template <typename T>
-void launch_cheb_first(int V, const T *r, const T *D, T *d, T *x, double inv_theta, int x_zero,
-                       unsigned mask, hipStream_t s) {
-  hipLaunchKernelGGL(k_cheb_first<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, r, D, d, x,
-                     inv_theta, x_zero, mask);
+void launch_cheb_first(int V, int R, const T *r, const T *D, T *d, T *x, double inv_theta,
+                       int x_zero, const int *act, hipStream_t s) {
+  hipLaunchKernelGGL(k_cheb_first<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, r, D, d, x,
+                     inv_theta, x_zero, act);
 }
 template <typename T>
-void launch_cheb_next(int V, const T *rsrc, T *r, const T *Ad, const T *D, T *d, T *x, double c1,
-                      double c2, unsigned mask, hipStream_t s) {
-  hipLaunchKernelGGL(k_cheb_next<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, rsrc, r, Ad, D, d,
-                     x, c1, c2, mask);
+void launch_pcg_update(int V, int R, T *x, T *r, const T *p, const T *q, const T *D, T *z,
+                       int rzslot, int rz_out, const int *act, const Reduce &red, int np_pq,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_update<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, x, r, p, q, D, z,
+                     rzslot, rz_out, act, red, np_pq, grid_for(3 * V));
 }
 template <typename T>
-void launch_pcg_update(int V, T *x, T *r, const T *p, const T *q, const T *D, T *z, int rzslot,
-                       int rz_out, unsigned mask, const Reduce &red, int np_pq, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_update<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, x, r, p, q, D, z,
-                     rzslot, rz_out, mask, red, np_pq, grid_for(3 * V));
+void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int first,
+                     const int *act, const Reduce &red, hipStream_t s) {
+  hipLaunchKernelGGL(k_p_update<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, p, z, rz_new, rz_old,
+                     first, act, red, grid_for(3 * V));
 }
 template <typename T>
-void launch_p_update(int V, T *p, const T *z, int rz_new, int rz_old, int first, unsigned mask,
-                     const Reduce &red, hipStream_t s) {
-  hipLaunchKernelGGL(k_p_update<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, p, z, rz_new,
-                     rz_old, first, mask, red, grid_for(3 * V));
-}
-template <typename T>
-void launch_dot(int V, const T *a, const T *b, int slot, unsigned mask, const Reduce &red,
+void launch_dot(int V, int R, const T *a, const T *b, int slot, const int *act, const Reduce &red,
                 hipStream_t s) {
-  hipLaunchKernelGGL(k_dot<T>, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, a, b, slot, mask, red);
+  hipLaunchKernelGGL(k_dot<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, a, b, slot, act, red);
 }
 template <typename Ts, typename Td>
 void launch_convert(size_t n, const Ts *src, Td *dst, hipStream_t s) {
@@ -703,59 +754,63 @@ void launch_convert(size_t n, const Ts *src, Td *dst, hipStream_t s) {
   if (b < 1) b = 1;
   hipLaunchKernelGGL((k_convert<Ts, Td>), dim3((unsigned)b), dim3(kBlock), 0, s, n, src, dst);
 }
-void launch_add_f32_to_f64(size_t, const float *d, double *x, unsigned mask, int V, hipStream_t s) {
-  hipLaunchKernelGGL(k_add_f32_to_f64, grid2(3 * V, kRHS), dim3(kBlock), 0, s, V, d, x, mask);
-}
-template <typename T>
-void launch_remove_mean(int V, T *x, unsigned mask, const Reduce &red, hipStream_t s) {
-  hipLaunchKernelGGL(k_mean_sum<T>, grid2(V, 3 * kRHS), dim3(kBlock), 0, s, V, x, mask, red);
-  hipLaunchKernelGGL(k_mean_sub<T>, grid2(V, 3 * kRHS), dim3(kBlock), 0, s, V, x, mask, red,
-                     grid_for(V));
-}
 template <typename T>
 void launch_invert(size_t n, const T *d, T *di, hipStream_t s) {
   size_t b = (n + kBlock - 1) / kBlock;
   if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
   hipLaunchKernelGGL(k_invert<T>, dim3((unsigned)b), dim3(kBlock), 0, s, n, d, di);
 }
-
+void launch_add_f32_to_f64(int V, int R, const float *d, double *x, const int *act, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_f32_to_f64, grid2(3 * V, R), dim3(kBlock), 0, s, V, d, x, act);
+}
 template <typename T>
-void launch_fill_hash(size_t n, T *x, unsigned long long seed, hipStream_t s) {
+void launch_remove_mean(int V, int R, T *x, const int *act, const Reduce &red, hipStream_t s) {
+  hipLaunchKernelGGL(k_mean_sum<T>, grid2(V, 3 * R), dim3(kBlock), 0, s, V, x, act, red);
+  hipLaunchKernelGGL(k_mean_sub<T>, grid2(V, 3 * R), dim3(kBlock), 0, s, V, x, act, red,
+                     grid_for(V));
+}
+template <typename T>
+void launch_fill_hash(size_t n, size_t period, T *x, unsigned long long seed, hipStream_t s) {
   size_t b = (n + kBlock - 1) / kBlock;
   if (b > 4096) b = 4096;
-  hipLaunchKernelGGL(k_fill_hash<T>, dim3((unsigned)b), dim3(kBlock), 0, s, n, x, seed);
+  if (b < 1) b = 1;
+  hipLaunchKernelGGL(k_fill_hash<T>, dim3((unsigned)b), dim3(kBlock), 0, s, n, period, x, seed);
 }
 template <typename T>
-void launch_scale_by_dinv_norm(int V, T *y, const T *D, const Reduce &red, hipStream_t s) {
-  hipLaunchKernelGGL(k_scale_dinv_norm<T>, dim3(grid_for(3 * V)), dim3(kBlock), 0, s, V, y, D, red);
+void launch_scale_by_dinv_norm(int V, int R, T *y, const T *Dinv, const int *act,
+                               const Reduce &red, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_dinv_norm<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, y, Dinv, act,
+                     red);
 }
 template <typename T>
-void launch_normalize(int V, const T *y, T *x, const Reduce &red, hipStream_t s) {
-  hipLaunchKernelGGL(k_normalize<T>, dim3(grid_for(3 * V)), dim3(kBlock), 0, s, V, y, x, red,
+void launch_normalize(int V, int R, const T *y, T *x, const int *act, const Reduce &red,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_normalize<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, y, x, act, red,
                      grid_for(3 * V));
 }
-void launch_material(int V, const double *rho, double *E, int interp, double E0, double Emin,
+void launch_material(size_t n, const double *rho, double *E, int interp, double E0, double Emin,
                      double p, hipStream_t s) {
-  hipLaunchKernelGGL(k_material, dim3(grid_for(V)), dim3(kBlock), 0, s, V, rho, E, interp, E0,
-                     Emin, p);
+  size_t b = (n + kBlock - 1) / kBlock;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  hipLaunchKernelGGL(k_material, dim3((unsigned)b), dim3(kBlock), 0, s, n, rho, E, interp, E0, Emin,
+                     p);
 }
-int launch_modulus_stats(int V, const double *E, const Reduce &red, hipStream_t s) {
-  hipLaunchKernelGGL(k_modulus_stats, dim3(grid_for(V)), dim3(kBlock), 0, s, V, E, red);
+int launch_modulus_stats(int V, int B, const double *E, const Reduce &red, hipStream_t s) {
+  hipLaunchKernelGGL(k_modulus_stats, grid2(V, B), dim3(kBlock), 0, s, V, E, red);
   return grid_for(V);
 }
-
-void launch_ch_energy(Dims g, const double *E, const double *Ke, const double *uaff, const double *u,
-                      double *partials, int nblk, hipStream_t s) {
-  hipLaunchKernelGGL(k_ch_energy, dim3(nblk), dim3(kBlock), 0, s, g, E, Ke, uaff, u, partials);
+void launch_ch_energy(Dims g, int B, const double *E, const double *Ke, const double *uaff,
+                      const double *u, double *partials, int nblk, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_ch_energy, dim3(nblk, B), dim3(kBlock), 0, s, g, E, Ke, uaff, u, partials);
+  hipLaunchKernelGGL(k_sum_partials, dim3(21 * B), dim3(kBlock), 0, s, partials, nblk, out);
 }
-void launch_sum_partials(const double *partials, int nsum, int nblk, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_partials, dim3(nsum), dim3(kBlock), 0, s, partials, nblk, out);
-}
-void launch_gradient(Dims g, const double *rho, const double *Ke, const double *uaff,
+void launch_gradient(Dims g, int B, const double *rho, const double *Ke, const double *uaff,
                      const double *u, const double *Gs, int interp, double E0, double Emin,
                      double p, int energy_only, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_gradient, dim3(grid_for(g.V)), dim3(kBlock), 0, s, g, rho, Ke, uaff, u, Gs,
-                     interp, E0, Emin, p, energy_only, out);
+  hipLaunchKernelGGL(k_gradient, grid2(g.V, B), dim3(kBlock), 0, s, g, rho, Ke, uaff, u, Gs, interp,
+                     E0, Emin, p, energy_only, out);
 }
 int vec_parts(int V) { return grid_for(3 * V); }
 
@@ -768,32 +823,34 @@ void reduce_init(Reduce &red, hipStream_t s) {
                      kNumSlots * kSlotWidth);
 }
 
-#define FEMH_INST(T)                                                                             \
-  template void launch_dense_apply<T>(const FineOp<T> &, const T *, T *, const T *, int, unsigned,\
-                                      const Reduce *, int, hipStream_t);                          \
-  template void launch_fine_diag<T>(const FineOp<T> &, T *, hipStream_t);                         \
-  template void launch_fine_rhs<T>(const FineOp<T> &, const double *, T *, hipStream_t);          \
-  template void launch_build_level1<T>(Dims, Dims, const T *, const double *, T *, hipStream_t);  \
-  template void launch_galerkin<T>(Dims, Dims, const T *, T *, hipStream_t);                      \
-  template void launch_stencil_diag<T>(const StencilOp<T> &, T *, hipStream_t);                   \
-  template void launch_restrict<T>(Dims, Dims, const T *, T *, unsigned, hipStream_t);            \
-  template void launch_prolong_add<T>(Dims, Dims, const T *, T *, unsigned, hipStream_t);         \
-  template void launch_dense_solve<T>(int, const T *, const T *, T *, unsigned, hipStream_t);     \
-  template void launch_cheb_first<T>(int, const T *, const T *, T *, T *, double, int, unsigned,  \
-                                     hipStream_t);                                                \
-  template void launch_cheb_next<T>(int, const T *, T *, const T *, const T *, T *, T *, double,  \
-                                    double, unsigned, hipStream_t);                               \
-  template void launch_pcg_update<T>(int, T *, T *, const T *, const T *, const T *, T *, int,    \
-                                     int, unsigned, const Reduce &, int, hipStream_t);            \
-  template void launch_p_update<T>(int, T *, const T *, int, int, int, unsigned, const Reduce &,  \
-                                   hipStream_t);                                                  \
-  template void launch_dot<T>(int, const T *, const T *, int, unsigned, const Reduce &,           \
+#define FEMH_INST(T)                                                                              \
+  template void launch_dense_apply<T>(const FineOp<T> &, int, const T *, T *, const T *, int,     \
+                                      const int *, const Reduce *, int, hipStream_t);             \
+  template void launch_fine_diag<T>(const FineOp<T> &, int, T *, hipStream_t);                    \
+  template void launch_fine_rhs<T>(const FineOp<T> &, int, const double *, T *, hipStream_t);     \
+  template void launch_build_level1<T>(Dims, Dims, int, const T *, const double *, T *,           \
+                                       hipStream_t);                                              \
+  template void launch_galerkin<T>(Dims, Dims, int, const T *, T *, hipStream_t);                 \
+  template void launch_stencil_diag<T>(const StencilOp<T> &, int, T *, hipStream_t);              \
+  template void launch_restrict<T>(Dims, Dims, int, const T *, T *, const int *, hipStream_t);    \
+  template void launch_prolong_add<T>(Dims, Dims, int, const T *, T *, const int *, hipStream_t); \
+  template void launch_dense_solve<T>(int, int, const T *, const T *, T *, const int *,           \
+                                      hipStream_t);                                               \
+  template void launch_cheb_first<T>(int, int, const T *, const T *, T *, T *, double, int,       \
+                                     const int *, hipStream_t);                                   \
+  template void launch_pcg_update<T>(int, int, T *, T *, const T *, const T *, const T *, T *,    \
+                                     int, int, const int *, const Reduce &, int, hipStream_t);    \
+  template void launch_p_update<T>(int, int, T *, const T *, int, int, int, const int *,          \
+                                   const Reduce &, hipStream_t);                                  \
+  template void launch_dot<T>(int, int, const T *, const T *, int, const int *, const Reduce &,   \
                               hipStream_t);                                                       \
-  template void launch_remove_mean<T>(int, T *, unsigned, const Reduce &, hipStream_t);           \
-  template void launch_fill_hash<T>(size_t, T *, unsigned long long, hipStream_t);                \
+  template void launch_remove_mean<T>(int, int, T *, const int *, const Reduce &, hipStream_t);   \
+  template void launch_fill_hash<T>(size_t, size_t, T *, unsigned long long, hipStream_t);                \
   template void launch_invert<T>(size_t, const T *, T *, hipStream_t);                            \
-  template void launch_scale_by_dinv_norm<T>(int, T *, const T *, const Reduce &, hipStream_t);   \
-  template void launch_normalize<T>(int, const T *, T *, const Reduce &, hipStream_t);
+  template void launch_scale_by_dinv_norm<T>(int, int, T *, const T *, const int *,               \
+                                             const Reduce &, hipStream_t);                        \
+  template void launch_normalize<T>(int, int, const T *, T *, const int *, const Reduce &,        \
+                                    hipStream_t);
 
 FEMH_INST(double)
 FEMH_INST(float)

@@ -1,5 +1,7 @@
 // ops.hip -- fused operator kernels (see ops.hpp for the staging / epilogue
-// contract) and the single-workgroup V-cycle of the small multigrid levels.
+// contract).  Grids carry the right-hand side q = 6 s + lc; per-sample data
+// (moduli, stencils, inverse diagonals, Chebyshev coefficients) is selected
+// by s = q / 6.
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
@@ -95,7 +97,8 @@ __device__ __forceinline__ void hadamard_voxel(const T *__restrict__ u, T E, con
 
 // epilogue at one owned node (all three components)
 template <typename T, int EP>
-__device__ __forceinline__ void epilogue(const OpArgs<T> &a, size_t off, int V, int n, const T Au[3],
+__device__ __forceinline__ void epilogue(const OpArgs<T> &a, const T *__restrict__ Ds, T ti, T c1,
+                                         T c2, size_t off, int V, int n, const T Au[3],
                                          const T uo[3], double &dv) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -108,12 +111,12 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, size_t off, int V, 
     } else if (EP == EP_CHEB0) {
       const T rv = a.b[q] - Au[c];
       a.r[q] = rv;
-      a.dout[q] = rv * a.theta_inv * a.Dinv[(size_t)c * V + n];
+      a.dout[q] = rv * ti * Ds[(size_t)c * V + n];
       a.x[q] = uo[c];
     } else {  // EP_STEP
       const T rv = a.rprev[q] - Au[c];
       a.r[q] = rv;
-      const T dn = a.c1 * uo[c] + a.c2 * rv * a.Dinv[(size_t)c * V + n];
+      const T dn = c1 * uo[c] + c2 * rv * Ds[(size_t)c * V + n];
       a.dout[q] = dn;
       T xv = (a.xprev ? a.xprev[q] : (T)0) + uo[c];
       if (a.last) xv += dn;
@@ -122,21 +125,21 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, size_t off, int V, 
   }
 }
 
-static inline int tiled_chunk(const Dims &g) {
+static inline int tiled_chunk(const Dims &g, int R) {
   const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
   // planes per chunk: about 8 (halo slab overhead 1/8), but enough chunks to
-  // put ~2 workgroups per CU across the six load cases, and at most
-  // kMaxPartials workgroups per load case
+  // put ~2 workgroups per CU across the right-hand sides, and at most
+  // kMaxPartials workgroups per right-hand side
   int TI = 8;
-  while (TI > 2 && (long)tiles * ((g.nx + TI - 1) / TI) * kRHS < 512) TI /= 2;
+  while (TI > 2 && (long)tiles * ((g.nx + TI - 1) / TI) * R < 512) TI /= 2;
   if (TI > g.nx) TI = g.nx;
   while ((long)tiles * ((g.nx + TI - 1) / TI) > kMaxPartials) TI++;
   return TI;
 }
 
-int tiled_parts(const Dims &g) {
+int tiled_parts(const Dims &g, int R) {
   const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
-  const int TI = tiled_chunk(g);
+  const int TI = tiled_chunk(g, R);
   return tiles * ((g.nx + TI - 1) / TI);
 }
 
@@ -144,7 +147,11 @@ template <typename T, int ST, int EP>
 __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ E, ModeT<T> mc,
                                                   OpArgs<T> a, int TI, int tiles_k) {
   const int r = blockIdx.z;
-  if (!((a.mask >> r) & 1u)) return;
+  if (!a.act[r]) return;
+  const int smp = r / kRHS;
+  E += (size_t)smp * g.V;
+  const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * g.V : nullptr;
+  const T ti = a.theta_inv[smp], c1 = a.c1[smp], c2 = a.c2[smp];
   __shared__ T up[3][3][kTN * kTN];
   __shared__ T eo[kTE * kTE * kEOH];
   const int tid = threadIdx.x;
@@ -201,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
       for (int c = 0; c < 3; c++) v[c] = a.u[off + (size_t)c * V + n];
       if (ST == ST_DSCALE) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[c] *= a.theta_inv * a.Dinv[(size_t)c * V + n];
+        for (int c = 0; c < 3; c++) v[c] *= ti * Ds[(size_t)c * V + n];
       } else if (ST == ST_PROLONG) {
         const int ci0 = iw >> 1, ci1 = ((iw + 1) >> 1) % a.gc.nx;
         const float wi = (iw & 1) ? 0.5f : 1.0f;
@@ -279,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
         Au[c] = acc[c] + f[c];
         uo[c] = up[cur][c][ot];
       }
-      epilogue<T, EP>(a, off, V, sl * plane + gj * g.nz + gk, Au, uo, dv);
+      epilogue<T, EP>(a, Ds, ti, c1, c2, off, V, sl * plane + gj * g.nz + gk, Au, uo, dv);
     }
     __syncthreads();
 #pragma unroll
@@ -308,17 +315,17 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
 }
 
 template <typename T, int ST, int EP>
-static void tiled_go(const Dims &g, const T *E, const ModeT<T> &mc, const OpArgs<T> &a,
+static void tiled_go(const Dims &g, int R, const T *E, const ModeT<T> &mc, const OpArgs<T> &a,
                      hipStream_t s) {
   const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
-  const int TI = tiled_chunk(g);
-  dim3 grid(tiles_j * tiles_k, (g.nx + TI - 1) / TI, kRHS);
+  const int TI = tiled_chunk(g, R);
+  dim3 grid(tiles_j * tiles_k, (g.nx + TI - 1) / TI, R);
   hipLaunchKernelGGL((k_tiled<T, ST, EP>), grid, dim3(kBlock), 0, s, g, E, mc, a, TI, tiles_k);
 }
 
 template <typename T>
-void launch_tiled(int st, int ep, const Dims &g, const T *E, const ModeCoef &m, const OpArgs<T> &a,
-                  hipStream_t s) {
+void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCoef &m,
+                  const OpArgs<T> &a, hipStream_t s) {
   ModeT<T> mc;
   for (int c = 0; c < 3; c++)
     for (int v = 0; v < 8; v++) {
@@ -327,7 +334,7 @@ void launch_tiled(int st, int ep, const Dims &g, const T *E, const ModeCoef &m, 
     }
 #define GO(S_, E_)                                 \
   if (st == S_ && ep == E_) {                      \
-    tiled_go<T, S_, E_>(g, E, mc, a, s);           \
+    tiled_go<T, S_, E_>(g, R, E, mc, a, s);        \
     return;                                        \
   }
   GO(ST_PLAIN, EP_AX)
@@ -343,69 +350,62 @@ void launch_tiled(int st, int ep, const Dims &g, const T *E, const ModeCoef &m, 
 // coarse levels: 27-point block stencils, every active RHS per thread (the
 // nine coefficients of a block are loaded once for all load cases)
 template <typename T, int ST>
-__device__ __forceinline__ T stage_at(const OpArgs<T> &a, size_t off, int V, int c, int n) {
+__device__ __forceinline__ T stage_at(const OpArgs<T> &a, const T *__restrict__ Ds, T ti, size_t off,
+                                      int V, int c, int n) {
   T v = a.u[off + (size_t)c * V + n];
-  if (ST == ST_DSCALE) v *= a.theta_inv * a.Dinv[(size_t)c * V + n];
+  if (ST == ST_DSCALE) v *= ti * Ds[(size_t)c * V + n];
   return v;
 }
 
-template <typename T, int ST, int EP, int NR>
+template <typename T, int ST, int EP>
 __device__ __forceinline__ void stencil_node(const Dims &g, const T *__restrict__ S, const OpArgs<T> &a,
-                                             int n, int r0) {
+                                             int n, int r) {
   int i, j, k;
   node_ijk(g, n, i, j, k);
   const int I[3] = {i == 0 ? g.nx - 1 : i - 1, i, i == g.nx - 1 ? 0 : i + 1};
   const int J[3] = {j == 0 ? g.ny - 1 : j - 1, j, j == g.ny - 1 ? 0 : j + 1};
   const int K[3] = {k == 0 ? g.nz - 1 : k - 1, k, k == g.nz - 1 ? 0 : k + 1};
   const int V = g.V;
-  T acc[NR][3];
-#pragma unroll
-  for (int r = 0; r < NR; r++) acc[r][0] = acc[r][1] = acc[r][2] = 0;
+  const int smp = r / kRHS;
+  const T *__restrict__ Ss = S + (size_t)smp * 243 * V;
+  const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * V : nullptr;
+  const T ti = a.theta_inv[smp];
+  const size_t off = (size_t)r * 3 * V;
+  T acc[3] = {0, 0, 0};
 #pragma unroll 3
   for (int o = 0; o < 27; o++) {
     const int nb = (I[o / 9] * g.ny + J[(o / 3) % 3]) * g.nz + K[o % 3];
-    T s[9];
+    T sv[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) s[q] = S[(size_t)(o * 9 + q) * V + n];
-#pragma unroll
-    for (int rr = 0; rr < NR; rr++) {
-      const int r = r0 + rr;
-      if (!((a.mask >> r) & 1u)) continue;
-      const size_t off = (size_t)r * 3 * V;
-      const T u0 = stage_at<T, ST>(a, off, V, 0, nb), u1 = stage_at<T, ST>(a, off, V, 1, nb),
-              u2 = stage_at<T, ST>(a, off, V, 2, nb);
-      acc[rr][0] += s[0] * u0 + s[1] * u1 + s[2] * u2;
-      acc[rr][1] += s[3] * u0 + s[4] * u1 + s[5] * u2;
-      acc[rr][2] += s[6] * u0 + s[7] * u1 + s[8] * u2;
-    }
+    for (int q = 0; q < 9; q++) sv[q] = Ss[(size_t)(o * 9 + q) * V + n];
+    const T u0 = stage_at<T, ST>(a, Ds, ti, off, V, 0, nb), u1 = stage_at<T, ST>(a, Ds, ti, off, V, 1, nb),
+            u2 = stage_at<T, ST>(a, Ds, ti, off, V, 2, nb);
+    acc[0] += sv[0] * u0 + sv[1] * u1 + sv[2] * u2;
+    acc[1] += sv[3] * u0 + sv[4] * u1 + sv[5] * u2;
+    acc[2] += sv[6] * u0 + sv[7] * u1 + sv[8] * u2;
   }
+  T uo[3];
 #pragma unroll
-  for (int rr = 0; rr < NR; rr++) {
-    const int r = r0 + rr;
-    if (!((a.mask >> r) & 1u)) continue;
-    const size_t off = (size_t)r * 3 * V;
-    T uo[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) uo[c] = stage_at<T, ST>(a, off, V, c, n);
-    double dv = 0;
-    epilogue<T, EP>(a, off, V, n, acc[rr], uo, dv);
-  }
+  for (int c = 0; c < 3; c++) uo[c] = stage_at<T, ST>(a, Ds, ti, off, V, c, n);
+  double dv = 0;
+  epilogue<T, EP>(a, Ds, ti, a.c1[smp], a.c2[smp], off, V, n, acc, uo, dv);
 }
 
 template <typename T, int ST, int EP>
 __global__ __launch_bounds__(kBlock) void k_stencil(Dims g, const T *__restrict__ S, OpArgs<T> a) {
-  const int r = blockIdx.y;  // one load case per thread: 6x the parallelism of a
-  if (!((a.mask >> r) & 1u)) return;  // node-only grid on these small levels
+  const int r = blockIdx.y;  // one right-hand side per thread: on these small
+  if (!a.act[r]) return;     // levels parallelism matters more than reuse
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock)
-    stencil_node<T, ST, EP, 1>(g, S, a, n, r);
+    stencil_node<T, ST, EP>(g, S, a, n, r);
 }
 
 template <typename T>
-void launch_stencil(int st, int ep, const Dims &g, const T *S, const OpArgs<T> &a, hipStream_t s) {
+void launch_stencil(int st, int ep, const Dims &g, int R, const T *S, const OpArgs<T> &a,
+                    hipStream_t s) {
   const int blocks = (g.V + kBlock - 1) / kBlock;
 #define GO(S_, E_)                                                                       \
   if (st == S_ && ep == E_) {                                                            \
-    hipLaunchKernelGGL((k_stencil<T, S_, E_>), dim3(blocks, kRHS), dim3(kBlock), 0, s, g, S, a); \
+    hipLaunchKernelGGL((k_stencil<T, S_, E_>), dim3(blocks, R), dim3(kBlock), 0, s, g, S, a); \
     return;                                                                              \
   }
   GO(ST_PLAIN, EP_AX)
@@ -419,210 +419,25 @@ void launch_stencil(int st, int ep, const Dims &g, const T *S, const OpArgs<T> &
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_add2(int V, const T *__restrict__ xn,
                                                  const T *__restrict__ d, T *__restrict__ x,
-                                                 unsigned mask) {
+                                                 const int *__restrict__ act) {
   const int r = blockIdx.y;
-  if (!((mask >> r) & 1u)) return;
+  if (!act[r]) return;
   const size_t off = (size_t)r * 3 * V;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
     x[off + t] = xn[off + t] + d[off + t];
 }
 
 template <typename T>
-void launch_add2(int V, const T *xn, const T *d, T *x, unsigned mask, hipStream_t s) {
-  hipLaunchKernelGGL(k_add2<T>, dim3(grid_for(3 * V), kRHS), dim3(kBlock), 0, s, V, xn, d, x, mask);
-}
-
-// ---------------------------------------------------------------------------
-// small levels: the whole V-cycle from level 0 of `lv` down to the coarsest
-// (dense) level, run by one workgroup for all load cases.  Phases are
-// separated by workgroup barriers; the level vectors live in global memory
-// (L2-resident at these sizes).
-template <typename T, int ST, int EP>
-__device__ void small_apply(const SmallLevel<T> &L, const OpArgs<T> &a) {
-  for (int n = threadIdx.x; n < L.g.V; n += blockDim.x) stencil_node<T, ST, EP, kRHS>(L.g, L.S, a, n, 0);
-  __syncthreads();
-}
-
-template <typename T>
-__device__ void small_smooth(const SmallLevel<T> &L, int degree, bool x_zero, unsigned mask) {
-  OpArgs<T> a;
-  a.Dinv = L.Dinv;
-  a.theta_inv = L.theta_inv;
-  a.mask = mask;
-  const int V = L.g.V;
-  if (degree == 1) {
-    // one Jacobi-like step: x_zero: x = b/(theta D); else x += (b - A x)/(theta D)
-    if (x_zero) {
-      for (int t = threadIdx.x; t < 3 * V; t += blockDim.x)
-        for (int r = 0; r < kRHS; r++)
-          if ((mask >> r) & 1u) {
-            const size_t q = (size_t)r * 3 * V + t;
-            L.x[q] = L.b[q] * L.theta_inv * L.Dinv[t];
-          }
-      __syncthreads();
-    } else {
-      a.u = L.x;
-      a.b = L.b;
-      a.r = L.r;
-      a.dout = L.d;
-      a.x = L.xn;
-      small_apply<T, ST_PLAIN, EP_CHEB0>(L, a);
-      for (int t = threadIdx.x; t < 3 * V; t += blockDim.x)
-        for (int r = 0; r < kRHS; r++)
-          if ((mask >> r) & 1u) {
-            const size_t q = (size_t)r * 3 * V + t;
-            L.x[q] = L.xn[q] + L.d[q];
-          }
-      __syncthreads();
-    }
-    return;
-  }
-  T *dcur = L.d, *dnxt = L.d2;
-  if (x_zero) {
-    // step 1 stages d1 = b/(theta D) directly
-    a.u = L.b;
-    a.rprev = L.b;
-    a.xprev = nullptr;
-    a.r = L.r;
-    a.x = L.x;
-    a.dout = dcur;
-    a.c1 = L.c1[0];
-    a.c2 = L.c2[0];
-    a.last = degree == 2;
-    small_apply<T, ST_DSCALE, EP_STEP>(L, a);
-  } else {
-    a.u = L.x;
-    a.b = L.b;
-    a.r = L.r;
-    a.dout = dnxt;
-    a.x = L.xn;
-    small_apply<T, ST_PLAIN, EP_CHEB0>(L, a);
-    // step 1 from d1 (in dnxt), x_prev = xn
-    a.u = dnxt;
-    a.rprev = L.r;
-    a.xprev = L.xn;
-    a.r = L.r;
-    a.x = L.x;
-    a.dout = dcur;
-    a.c1 = L.c1[0];
-    a.c2 = L.c2[0];
-    a.last = degree == 2;
-    small_apply<T, ST_PLAIN, EP_STEP>(L, a);
-  }
-  for (int s = 2; s < degree; s++) {
-    // dcur holds d_s; produce d_{s+1} into dnxt
-    a.u = dcur;
-    a.rprev = L.r;
-    a.xprev = L.x;
-    a.r = L.r;
-    a.x = L.x;
-    a.dout = dnxt;
-    a.c1 = L.c1[s - 1];
-    a.c2 = L.c2[s - 1];
-    a.last = s == degree - 1;
-    small_apply<T, ST_PLAIN, EP_STEP>(L, a);
-    T *tmp = dcur;
-    dcur = dnxt;
-    dnxt = tmp;
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kSmallThreads) void k_small_cycle(const SmallLevel<T> *__restrict__ lv,
-                                                               int nlev, int nu1, int nu2,
-                                                               unsigned mask) {
-  // down
-  for (int l = 0; l < nlev - 1; l++) {
-    const SmallLevel<T> &L = lv[l];
-    const SmallLevel<T> &C = lv[l + 1];
-    small_smooth<T>(L, nu1, true, mask);
-    OpArgs<T> a;
-    a.u = L.x;
-    a.b = L.b;
-    a.y = L.r;
-    a.mask = mask;
-    small_apply<T, ST_PLAIN, EP_RES>(L, a);
-    // restrict r -> C.b
-    for (int N = threadIdx.x; N < C.g.V; N += blockDim.x) {
-      int I, J, K;
-      node_ijk(C.g, N, I, J, K);
-      for (int r = 0; r < kRHS; r++) {
-        if (!((mask >> r) & 1u)) continue;
-        double acc[3] = {0, 0, 0};
-        for (int on = 0; on < 27; on++) {
-          const int oi = on / 9 - 1, oj = (on / 3) % 3 - 1, ok = on % 3 - 1;
-          const double w = (oi ? 0.5 : 1.0) * (oj ? 0.5 : 1.0) * (ok ? 0.5 : 1.0);
-          const int n = (((2 * I + oi + L.g.nx) % L.g.nx) * L.g.ny + (2 * J + oj + L.g.ny) % L.g.ny) *
-                            L.g.nz + (2 * K + ok + L.g.nz) % L.g.nz;
-#pragma unroll
-          for (int c = 0; c < 3; c++) acc[c] += w * (double)L.r[(size_t)r * 3 * L.g.V + (size_t)c * L.g.V + n];
-        }
-#pragma unroll
-        for (int c = 0; c < 3; c++) C.b[(size_t)r * 3 * C.g.V + (size_t)c * C.g.V + N] = (T)acc[c];
-      }
-    }
-    __syncthreads();
-  }
-  // coarsest: dense inverse
-  {
-    const SmallLevel<T> &C = lv[nlev - 1];
-    const int n = 3 * C.g.V;
-    for (int t = threadIdx.x; t < kRHS * n; t += blockDim.x) {
-      const int r = t / n, i = t - r * n;
-      if (!((mask >> r) & 1u)) continue;
-      double acc = 0;
-      const T *row = C.Ainv + (size_t)i * n;
-      const T *br = C.b + (size_t)r * n;
-      for (int j = 0; j < n; j++) acc += (double)row[j] * (double)br[j];
-      C.x[(size_t)r * n + i] = (T)acc;
-    }
-    __syncthreads();
-  }
-  // up
-  for (int l = nlev - 2; l >= 0; l--) {
-    const SmallLevel<T> &L = lv[l];
-    const SmallLevel<T> &C = lv[l + 1];
-    for (int n = threadIdx.x; n < L.g.V; n += blockDim.x) {
-      int i, j, k;
-      node_ijk(L.g, n, i, j, k);
-      const int ci0 = i >> 1, ci1 = ((i + 1) >> 1) % C.g.nx;
-      const int cj0 = j >> 1, cj1 = ((j + 1) >> 1) % C.g.ny;
-      const int ck0 = k >> 1, ck1 = ((k + 1) >> 1) % C.g.nz;
-      const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
-      const double w = ((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((k & 1) ? 0.5 : 1.0);
-      for (int r = 0; r < kRHS; r++) {
-        if (!((mask >> r) & 1u)) continue;
-        double acc[3] = {0, 0, 0};
-        for (int ai = 0; ai < ni; ai++)
-          for (int aj = 0; aj < nj; aj++)
-            for (int ak = 0; ak < nk; ak++) {
-              const int N = ((ai ? ci1 : ci0) * C.g.ny + (aj ? cj1 : cj0)) * C.g.nz + (ak ? ck1 : ck0);
-#pragma unroll
-              for (int c = 0; c < 3; c++) acc[c] += (double)C.x[(size_t)r * 3 * C.g.V + (size_t)c * C.g.V + N];
-            }
-#pragma unroll
-        for (int c = 0; c < 3; c++) L.x[(size_t)r * 3 * L.g.V + (size_t)c * L.g.V + n] += (T)(w * acc[c]);
-      }
-    }
-    __syncthreads();
-    small_smooth<T>(L, nu2, false, mask);
-  }
-}
-
-template <typename T>
-void launch_small_cycle(const SmallLevel<T> *d_levels, int nlev, int nu1, int nu2, unsigned mask,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(k_small_cycle<T>, dim3(1), dim3(kSmallThreads), 0, s, d_levels, nlev, nu1, nu2,
-                     mask);
+void launch_add2(int V, int R, const T *xn, const T *d, T *x, const int *act, hipStream_t s) {
+  hipLaunchKernelGGL(k_add2<T>, dim3(grid_for(3 * V), R), dim3(kBlock), 0, s, V, xn, d, x, act);
 }
 
 #define FEMH_OPS_INST(T)                                                                        \
-  template void launch_tiled<T>(int, int, const Dims &, const T *, const ModeCoef &,            \
+  template void launch_tiled<T>(int, int, const Dims &, int, const T *, const ModeCoef &,       \
                                 const OpArgs<T> &, hipStream_t);                                \
-  template void launch_stencil<T>(int, int, const Dims &, const T *, const OpArgs<T> &,         \
+  template void launch_stencil<T>(int, int, const Dims &, int, const T *, const OpArgs<T> &,    \
                                   hipStream_t);                                                 \
-  template void launch_add2<T>(int, const T *, const T *, T *, unsigned, hipStream_t);          \
-  template void launch_small_cycle<T>(const SmallLevel<T> *, int, int, int, unsigned, hipStream_t);
+  template void launch_add2<T>(int, int, const T *, const T *, T *, const int *, hipStream_t);
 
 FEMH_OPS_INST(double)
 FEMH_OPS_INST(float)

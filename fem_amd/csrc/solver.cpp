@@ -7,7 +7,8 @@
 //   C_ij = 1/V sum_e E_e (u_aff,i + u_i)_e^T K0 (u_aff,j + u_j)_e
 // Solver: PCG (pcgsolver/pcg_solver.h:243-285 loop) with a Jacobi or a
 // Galerkin geometric-multigrid V(nu1,nu2) preconditioner (Chebyshev smoothing,
-// dense coarsest solve), all six load cases advanced together.
+// dense coarsest solve); all load cases of all samples of a call advance
+// together, each right-hand side with its own PCG scalars and stopping test.
 // Mixed precision: FP32 multigrid-PCG corrections inside an FP64 iterative
 // refinement loop (FP64 residual, FP64 dot products, FP64 fields and C_H).
 #include "solver.hpp"
@@ -40,7 +41,8 @@ void DBuf::release() {
   bytes = 0;
 }
 
-static size_t vec_bytes(const Dims &g, size_t s) { return (size_t)kRHS * 3 * g.V * s; }
+// bytes of an [R][3][V] field for capacity B
+static size_t field_bytes(const Dims &g, int B, size_t s) { return (size_t)kRHS * B * 3 * g.V * s; }
 
 Context::Context(const femh_opts &opts) : o(opts) {
   if (o.nx < 2 || o.ny < 2 || o.nz < 2) throw Error(FEMH_BAD_ARG, "grid dimensions must be >= 2");
@@ -52,11 +54,15 @@ Context::Context(const femh_opts &opts) : o(opts) {
     throw Error(FEMH_BAD_ARG, "unknown preconditioner");
   if (o.precision != FEMH_FP64 && o.precision != FEMH_MIXED)
     throw Error(FEMH_BAD_ARG, "unknown precision");
-  if (o.nu1 < 1 || o.nu2 < 1 || o.nu1 > 16 || o.nu2 > 16)
+  if (o.nu1 < 1 || o.nu2 < 1 || o.nu1 > kMaxCheb || o.nu2 > kMaxCheb)
     throw Error(FEMH_BAD_ARG, "smoothing degrees must lie in [1,16]");
   if (!(o.cheby_ratio > 1.0)) throw Error(FEMH_BAD_ARG, "cheby_ratio must exceed 1");
   if (!(o.tol > 0) || o.max_iter < 1) throw Error(FEMH_BAD_ARG, "bad tolerance / iteration cap");
+  if (o.apply_kernel < 0 || o.apply_kernel > 1) throw Error(FEMH_BAD_ARG, "unknown apply kernel");
+  if (o.batch < 0 || o.batch > kMaxB)
+    throw Error(FEMH_BAD_ARG, "batch must lie in [1," + std::to_string(kMaxB) + "]");
   if (o.check_every < 1) o.check_every = 1;
+  B = o.batch > 0 ? o.batch : 1;
   g = Dims{o.nx, o.ny, o.nz, o.nx * o.ny * o.nz};
   FEMH_HIP(hipSetDevice(o.device));
   FEMH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -64,36 +70,50 @@ Context::Context(const femh_opts &opts) : o(opts) {
   tab = build_tables(o.nu);
   std::memcpy(modecoef.diag, tab.L.diag, sizeof(modecoef.diag));
   std::memcpy(modecoef.cross, tab.L.cross, sizeof(modecoef.cross));
-  if (o.apply_kernel < 0 || o.apply_kernel > 1) throw Error(FEMH_BAD_ARG, "unknown apply kernel");
 
-  rho.alloc(sizeof(double) * g.V);
-  E64.alloc(sizeof(double) * g.V);
+  rho.alloc(sizeof(double) * B * g.V);
+  E64.alloc(sizeof(double) * B * g.V);
   Ke64.alloc(sizeof(double) * 576);
   gtab.alloc(sizeof(double) * 6 * 24);
   uaff.alloc(sizeof(double) * 6 * 24);
   Mchild.alloc(sizeof(double) * 8 * 576);
-  Gs.alloc(sizeof(double) * 21);
+  Gs.alloc(sizeof(double) * 21 * B);
   FEMH_HIP(hipMemcpyAsync(Ke64.p, tab.Ke.data(), sizeof(double) * 576, hipMemcpyHostToDevice, stream));
   FEMH_HIP(hipMemcpyAsync(gtab.p, tab.g.data(), sizeof(double) * 144, hipMemcpyHostToDevice, stream));
   FEMH_HIP(hipMemcpyAsync(uaff.p, tab.u_aff.data(), sizeof(double) * 144, hipMemcpyHostToDevice, stream));
   FEMH_HIP(hipMemcpyAsync(Mchild.p, tab.Mchild.data(), sizeof(double) * 8 * 576, hipMemcpyHostToDevice,
                           stream));
-  for (DBuf *bf : {&b, &x, &r, &z, &p, &q}) bf->alloc(vec_bytes(g, sizeof(double)));
+  for (DBuf *bf : {&b, &x, &r, &z, &p, &q}) bf->alloc(field_bytes(g, B, sizeof(double)));
   if (o.precision == FEMH_MIXED)
-    for (DBuf *bf : {&b32, &r32, &x32, &z32, &p32, &q32}) bf->alloc(vec_bytes(g, sizeof(float)));
+    for (DBuf *bf : {&b32, &r32, &x32, &z32, &p32, &q32}) bf->alloc(field_bytes(g, B, sizeof(float)));
   red_part.alloc(sizeof(double) * kNumSlots * kSlotWidth * kMaxPartials);
   red_ctr.alloc(sizeof(unsigned) * kNumSlots * kSlotWidth);
   red_scal.alloc(sizeof(double) * kNumSlots * kSlotWidth);
   red = Reduce{red_part.as<double>(), red_ctr.as<unsigned>(), red_scal.as<double>()};
   reduce_init(red, stream);
-  ch_part.alloc(sizeof(double) * 21 * 1024);
-  ch_out.alloc(sizeof(double) * 21);
+  ch_part.alloc(sizeof(double) * 21 * 1024 * B);
+  ch_out.alloc(sizeof(double) * 21 * B);
+  act.alloc(sizeof(int) * kActSlots * kMaxR);
   FEMH_HIP(hipGetLastError());
   FEMH_HIP(hipStreamSynchronize(stream));
 }
 
 Context::~Context() {
+  for (auto &pe : pending) {
+    (void)hipEventDestroy(pe.a);
+    (void)hipEventDestroy(pe.b);
+  }
+  for (auto e : ev_pool) (void)hipEventDestroy(e);
   if (stream && own_stream) (void)hipStreamDestroy(stream);
+}
+
+void Context::set_act(int slot, const std::vector<int> &v) {
+  int h[kMaxR] = {0};
+  for (size_t t = 0; t < v.size() && t < (size_t)kMaxR; t++) h[t] = v[t] ? 1 : 0;
+  // pageable source: the copy completes before the call returns, so the
+  // stack buffer may go away afterwards
+  FEMH_HIP(hipMemcpyAsync(act.as<int>() + slot * kMaxR, h, sizeof(h), hipMemcpyHostToDevice, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
 }
 
 void Context::read_scalars(int slot, double *out, int n, int np) {
@@ -104,32 +124,40 @@ void Context::read_scalars(int slot, double *out, int n, int np) {
   FEMH_HIP(hipStreamSynchronize(stream));
 }
 
-void Context::set_modulus_from_rho(const double *src, bool device_src) {
-  FEMH_HIP(hipMemcpyAsync(rho.p, src, sizeof(double) * g.V,
-                          device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
-  launch_material(g.V, rho.as<double>(), E64.as<double>(), o.interp, o.E0, o.Emin, o.penal, stream);
+void Context::load(const double *src, int n, bool device_src, bool modulus) {
+  if (n < 1 || n > B) throw Error(FEMH_BAD_ARG, "sample count exceeds the context's batch");
+  nb = n;
+  R = kRHS * n;
+  const size_t bytes = sizeof(double) * (size_t)n * g.V;
+  const hipMemcpyKind kind = device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  FEMH_HIP(hipMemcpyAsync(rho.p, src, bytes, kind, stream));
+  if (modulus)
+    FEMH_HIP(hipMemcpyAsync(E64.p, src, bytes, kind, stream));
+  else
+    launch_material((size_t)n * g.V, rho.as<double>(), E64.as<double>(), o.interp, o.E0, o.Emin,
+                    o.penal, stream);
   FEMH_HIP(hipGetLastError());
-}
-
-void Context::set_modulus(const double *E_host) {
-  FEMH_HIP(hipMemcpyAsync(E64.p, E_host, sizeof(double) * g.V, hipMemcpyHostToDevice, stream));
-  FEMH_HIP(hipMemcpyAsync(rho.p, E_host, sizeof(double) * g.V, hipMemcpyHostToDevice, stream));
+  modulus_input = modulus;
+  h64.ready = h32.ready = false;
+  have_fields = false;
 }
 
 // ---------------------------------------------------------------------------
-// hierarchy
+// operator dispatch (with optional kernel timing of the fine level)
 template <typename T>
-void Context::op(Hierarchy<T> &h, int l, int st, int ep, const OpArgs<T> &a) {
+void Context::op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a) {
+  const Level<T> &L = h.lv[l];
+  if (!a.Dinv) a.Dinv = L.Dinv.template as<T>();
   if (l == 0) {
-    const Dims &g0 = h.lv[0].g;
+    const Dims &g0 = L.g;
     const bool dense = o.apply_kernel == 1 && st == ST_PLAIN && (ep == EP_AX || ep == EP_RES);
     auto launch = [&] {
       if (dense) {
-        FineOp<T> f{g0, h.E.template as<T>(), h.Ke.template as<T>(), &modecoef, 1};
-        launch_dense_apply<T>(f, a.u, a.y, a.b, ep == EP_RES ? 1 : 0, a.mask, a.dot ? &a.red : nullptr,
-                              a.slot, stream);
+        FineOp<T> f{g0, h.E.template as<T>(), h.Ke.template as<T>()};
+        launch_dense_apply<T>(f, R, a.u, a.y, a.b, ep == EP_RES ? 1 : 0, a.act,
+                              a.dot ? &a.red : nullptr, a.slot, stream);
       } else {
-        launch_tiled<T>(st, ep, g0, h.E.template as<T>(), modecoef, a, stream);
+        launch_tiled<T>(st, ep, g0, R, h.E.template as<T>(), modecoef, a, stream);
       }
     };
     if (prof) {
@@ -137,17 +165,32 @@ void Context::op(Hierarchy<T> &h, int l, int st, int ep, const OpArgs<T> &a) {
       pe.a = take_event();
       pe.b = take_event();
       pe.which = sizeof(T) == 8 ? 0 : 1;
-      const double R = (double)__builtin_popcount(a.mask & 0x3Fu), V = (double)g0.V, sz = sizeof(T);
+      int na = 0;
+      {
+        int hact[kMaxR];
+        FEMH_HIP(hipMemcpyAsync(hact, a.act, sizeof(int) * R, hipMemcpyDeviceToHost, stream));
+        FEMH_HIP(hipStreamSynchronize(stream));
+        for (int t = 0; t < R; t++) na += hact[t] != 0;
+      }
+      const double Rn = na, V = (double)g0.V, sz = sizeof(T);
       // algorithmic bytes: staged vector read + outputs written + extra
-      // per-node inputs of the epilogue, E read once
+      // per-node inputs of the epilogue; per sample E read once and, where
+      // the smoother needs it, the inverse diagonal
       double vec_in = 1, vec_out = 1;
       if (ep == EP_RES) vec_in += 1;
-      if (ep == EP_CHEB0) { vec_in += 1; vec_out = 3; }
-      if (ep == EP_STEP) { vec_in += 1 + (a.xprev ? 1 : 0); vec_out = 3; }
+      if (ep == EP_CHEB0) {
+        vec_in += 1;
+        vec_out = 3;
+      }
+      if (ep == EP_STEP) {
+        vec_in += 1 + (a.xprev ? 1 : 0);
+        vec_out = 3;
+      }
       if (st == ST_PROLONG) vec_in += 0.125;
       const bool usesD = st == ST_DSCALE || ep == EP_CHEB0 || ep == EP_STEP;
-      pe.bytes = V * (3.0 * sz * R * (vec_in + vec_out) + sz + (usesD ? 3.0 * sz : 0.0));
-      pe.flops = V * R * 1152.0;
+      const double samples = std::ceil(Rn / kRHS);
+      pe.bytes = V * (3.0 * sz * Rn * (vec_in + vec_out) + samples * sz * (usesD ? 4.0 : 1.0));
+      pe.flops = V * Rn * 1152.0;
       FEMH_HIP(hipEventRecord(pe.a, stream));
       launch();
       FEMH_HIP(hipEventRecord(pe.b, stream));
@@ -156,20 +199,20 @@ void Context::op(Hierarchy<T> &h, int l, int st, int ep, const OpArgs<T> &a) {
       launch();
     }
   } else {
-    launch_stencil<T>(st, ep, h.lv[l].g, h.lv[l].S.template as<T>(), a, stream);
+    launch_stencil<T>(st, ep, L.g, R, L.S.template as<T>(), a, stream);
   }
 }
 
-int Context::apply_parts() const { return o.apply_kernel == 1 ? dense_parts(g) : tiled_parts(g); }
+int Context::apply_parts() const { return o.apply_kernel == 1 ? dense_parts(g) : tiled_parts(g, R); }
 
 template <typename T>
-void Context::apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *bb, int mode, unsigned mask,
-                    const Reduce *rd, int slot) {
+void Context::apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *bb, int mode,
+                    const int *act_, const Reduce *rd, int slot) {
   OpArgs<T> a;
   a.u = u;
   a.y = y;
   a.b = bb;
-  a.mask = mask;
+  a.act = act_;
   if (rd) {
     a.red = *rd;
     a.slot = slot;
@@ -178,6 +221,8 @@ void Context::apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *bb, int m
   op<T>(h, l, ST_PLAIN, mode == 1 ? EP_RES : EP_AX, a);
 }
 
+// ---------------------------------------------------------------------------
+// hierarchy
 template <typename T>
 void Context::build_levels(Hierarchy<T> &h) {
   // level dimensions: halve while every dimension is even and >= 4
@@ -194,68 +239,62 @@ void Context::build_levels(Hierarchy<T> &h) {
   for (size_t l = 0; l < dims.size(); l++) {
     Level<T> &L = h.lv[l];
     L.g = dims[l];
-    L.D.alloc(sizeof(T) * 3 * L.g.V);
-    L.Dinv.alloc(sizeof(T) * 3 * L.g.V);
-    for (DBuf *bf : {&L.x, &L.b, &L.r, &L.d, &L.d2, &L.xn, &L.Ad}) bf->alloc(vec_bytes(L.g, sizeof(T)));
-    if (l > 0) L.S.alloc(sizeof(T) * 243 * (size_t)L.g.V);
+    L.D.alloc(sizeof(T) * 3 * B * L.g.V);
+    L.Dinv.alloc(sizeof(T) * 3 * B * L.g.V);
+    L.lmax.assign(B, 0.0);
+    for (DBuf *bf : {&L.x, &L.b, &L.r, &L.d, &L.d2, &L.xn, &L.Ad})
+      bf->alloc(field_bytes(L.g, B, sizeof(T)));
+    if (l > 0) L.S.alloc(sizeof(T) * 243 * B * (size_t)L.g.V);
   }
-  h.E.alloc(sizeof(T) * g.V);
+  h.E.alloc(sizeof(T) * B * g.V);
   h.Ke.alloc(sizeof(T) * 576);
 }
 
 void Context::estimate_lmax(Hierarchy<double> &h) {
+  // power iteration on D^-1 A, one vector per sample (RHS 6 s)
+  std::vector<int> pw(R, 0);
+  for (int s = 0; s < nb; s++) pw[kRHS * s] = 1;
+  set_act(A_POW, pw);
+  const int *ap = actp(A_POW);
   for (size_t l = 0; l < h.lv.size(); l++) {
     Level<double> &L = h.lv[l];
     const int V = L.g.V;
     double *xv = L.x.as<double>(), *yv = L.Ad.as<double>();
-    launch_fill_hash<double>((size_t)3 * V, xv, 0x51ED27ull + l, stream);
+    // every sample starts from the same vector (batched == one-sample results)
+    launch_fill_hash<double>((size_t)R * 3 * V, (size_t)kRHS * 3 * V, xv, 0x51ED27ull + l, stream);
     for (int it = 0; it < 10; it++) {
-      apply<double>(h, (int)l, xv, yv, nullptr, 0, 1u, nullptr, 0);
-      launch_scale_by_dinv_norm<double>(V, yv, L.Dinv.as<double>(), red, stream);
-      launch_normalize<double>(V, yv, xv, red, stream);
+      apply<double>(h, (int)l, xv, yv, nullptr, 0, ap, nullptr, 0);
+      launch_scale_by_dinv_norm<double>(V, R, yv, L.Dinv.as<double>(), ap, red, stream);
+      launch_normalize<double>(V, R, yv, xv, ap, red, stream);
     }
     FEMH_HIP(hipGetLastError());
-    double n2;
-    read_scalars(S_AUX, &n2, 1, vec_parts(V));
-    L.lmax = 1.1 * std::sqrt(std::max(n2, 0.0));
-    if (!(L.lmax > 0) || !std::isfinite(L.lmax)) throw Error(FEMH_NONFINITE, "operator is not finite");
+    std::vector<double> n2(R);
+    read_scalars(S_AUX, n2.data(), R, vec_parts(V));
+    for (int s = 0; s < nb; s++) {
+      L.lmax[s] = 1.1 * std::sqrt(std::max(n2[kRHS * s], 0.0));
+      if (!(L.lmax[s] > 0) || !std::isfinite(L.lmax[s]))
+        throw Error(FEMH_NONFINITE, "operator is not finite");
+    }
   }
 }
 
-// Dense inverse of the regularised coarsest operator A + s Z Z^T (Z = the
-// three constant displacement fields): for right-hand sides orthogonal to Z it
-// returns the zero-mean solution of A x = b.
+// Dense inverse of each sample's regularised coarsest operator A + s Z Z^T
+// (Z = the three constant displacement fields): for right-hand sides
+// orthogonal to Z it returns the zero-mean solution of A x = b.
 void Context::dense_coarse(Hierarchy<double> &h) {
   Level<double> &L = h.lv.back();
   const int V = L.g.V, n = 3 * V;
   h.ndense = 0;
-  if (n > 648) return;
-  std::vector<double> A((size_t)n * n, 0.0), Sh;
-  if (h.lv.size() == 1) {
-    // single level: assemble by applying the fine operator to unit vectors
-    std::vector<double> e((size_t)n * kRHS, 0.0), col((size_t)n * kRHS);
-    DBuf de, dc;
-    de.alloc(sizeof(double) * kRHS * n);
-    dc.alloc(sizeof(double) * kRHS * n);
-    for (int t0 = 0; t0 < n; t0 += kRHS) {
-      std::fill(e.begin(), e.end(), 0.0);
-      unsigned mask = 0;
-      for (int r = 0; r < kRHS && t0 + r < n; r++) {
-        e[(size_t)r * n + t0 + r] = 1.0;
-        mask |= 1u << r;
-      }
-      FEMH_HIP(hipMemcpyAsync(de.p, e.data(), sizeof(double) * kRHS * n, hipMemcpyHostToDevice, stream));
-      apply<double>(h, 0, de.as<double>(), dc.as<double>(), nullptr, 0, mask, nullptr, 0);
-      FEMH_HIP(hipMemcpyAsync(col.data(), dc.p, sizeof(double) * kRHS * n, hipMemcpyDeviceToHost, stream));
-      FEMH_HIP(hipStreamSynchronize(stream));
-      for (int r = 0; r < kRHS && t0 + r < n; r++)
-        for (int s = 0; s < n; s++) A[(size_t)s * n + t0 + r] = col[(size_t)r * n + s];
-    }
-  } else {
-    Sh.resize((size_t)243 * V);
-    FEMH_HIP(hipMemcpyAsync(Sh.data(), L.S.p, sizeof(double) * 243 * V, hipMemcpyDeviceToHost, stream));
-    FEMH_HIP(hipStreamSynchronize(stream));
-    const Dims &d = L.g;
+  if (n > 648 || h.lv.size() < 2) return;
+  std::vector<double> Sh((size_t)243 * V * nb);
+  FEMH_HIP(hipMemcpyAsync(Sh.data(), L.S.p, sizeof(double) * Sh.size(), hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+  std::vector<double> inv((size_t)n * n * nb);
+  std::vector<double> A((size_t)n * n), col(n);
+  const Dims &d = L.g;
+  for (int smp = 0; smp < nb; smp++) {
+    const double *S = Sh.data() + (size_t)smp * 243 * V;
+    std::fill(A.begin(), A.end(), 0.0);
     for (int N = 0; N < V; N++) {
       const int k = N % d.nz, j = (N / d.nz) % d.ny, i = N / (d.nz * d.ny);
       for (int off = 0; off < 27; off++) {
@@ -264,48 +303,47 @@ void Context::dense_coarse(Hierarchy<double> &h) {
         const int M = (ni * d.ny + nj) * d.nz + nk;
         for (int c = 0; c < 3; c++)
           for (int dd = 0; dd < 3; dd++)
-            A[(size_t)(c * V + N) * n + dd * V + M] += Sh[(size_t)(off * 9 + 3 * c + dd) * V + N];
+            A[(size_t)(c * V + N) * n + dd * V + M] += S[(size_t)(off * 9 + 3 * c + dd) * V + N];
       }
     }
-  }
-  double dmean = 0;
-  for (int t = 0; t < n; t++) dmean += A[(size_t)t * n + t];
-  dmean /= n;
-  for (int c = 0; c < 3; c++)
-    for (int a = 0; a < V; a++)
-      for (int bb = 0; bb < V; bb++) A[(size_t)(c * V + a) * n + c * V + bb] += dmean / V;
-  // Cholesky A = L L^T (lower, in place)
-  for (int j = 0; j < n; j++) {
-    double s = A[(size_t)j * n + j];
-    for (int k = 0; k < j; k++) s -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
-    if (!(s > 0)) throw Error(FEMH_NONFINITE, "coarsest operator is not positive definite");
-    s = std::sqrt(s);
-    A[(size_t)j * n + j] = s;
-    for (int i = j + 1; i < n; i++) {
-      double t = A[(size_t)i * n + j];
-      for (int k = 0; k < j; k++) t -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
-      A[(size_t)i * n + j] = t / s;
+    double dmean = 0;
+    for (int t = 0; t < n; t++) dmean += A[(size_t)t * n + t];
+    dmean /= n;
+    for (int c = 0; c < 3; c++)
+      for (int a = 0; a < V; a++)
+        for (int bb = 0; bb < V; bb++) A[(size_t)(c * V + a) * n + c * V + bb] += dmean / V;
+    // Cholesky A = L L^T (lower, in place)
+    for (int j = 0; j < n; j++) {
+      double s = A[(size_t)j * n + j];
+      for (int k = 0; k < j; k++) s -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+      if (!(s > 0)) throw Error(FEMH_NONFINITE, "coarsest operator is not positive definite");
+      s = std::sqrt(s);
+      A[(size_t)j * n + j] = s;
+      for (int i = j + 1; i < n; i++) {
+        double t = A[(size_t)i * n + j];
+        for (int k = 0; k < j; k++) t -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+        A[(size_t)i * n + j] = t / s;
+      }
+    }
+    double *Ai = inv.data() + (size_t)smp * n * n;
+    for (int c = 0; c < n; c++) {
+      std::fill(col.begin(), col.end(), 0.0);
+      col[c] = 1.0;
+      for (int i = 0; i < n; i++) {
+        double s = col[i];
+        for (int k = 0; k < i; k++) s -= A[(size_t)i * n + k] * col[k];
+        col[i] = s / A[(size_t)i * n + i];
+      }
+      for (int i = n - 1; i >= 0; i--) {
+        double s = col[i];
+        for (int k = i + 1; k < n; k++) s -= A[(size_t)k * n + i] * col[k];
+        col[i] = s / A[(size_t)i * n + i];
+      }
+      for (int i = 0; i < n; i++) Ai[(size_t)i * n + c] = col[i];
     }
   }
-  // inverse column by column
-  std::vector<double> inv((size_t)n * n), col(n);
-  for (int c = 0; c < n; c++) {
-    std::fill(col.begin(), col.end(), 0.0);
-    col[c] = 1.0;
-    for (int i = 0; i < n; i++) {
-      double s = col[i];
-      for (int k = 0; k < i; k++) s -= A[(size_t)i * n + k] * col[k];
-      col[i] = s / A[(size_t)i * n + i];
-    }
-    for (int i = n - 1; i >= 0; i--) {
-      double s = col[i];
-      for (int k = i + 1; k < n; k++) s -= A[(size_t)k * n + i] * col[k];
-      col[i] = s / A[(size_t)i * n + i];
-    }
-    for (int i = 0; i < n; i++) inv[(size_t)i * n + c] = col[i];
-  }
-  h.Ainv.alloc(sizeof(double) * (size_t)n * n);
-  FEMH_HIP(hipMemcpyAsync(h.Ainv.p, inv.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice,
+  h.Ainv.alloc(sizeof(double) * (size_t)n * n * B);
+  FEMH_HIP(hipMemcpyAsync(h.Ainv.p, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice,
                           stream));
   FEMH_HIP(hipStreamSynchronize(stream));
   h.ndense = n;
@@ -314,23 +352,24 @@ void Context::dense_coarse(Hierarchy<double> &h) {
 void Context::build_hierarchy() {
   build_levels<double>(h64);
   nlevels = (int)h64.lv.size();
-  FEMH_HIP(hipMemcpyAsync(h64.E.p, E64.p, sizeof(double) * g.V, hipMemcpyDeviceToDevice, stream));
+  FEMH_HIP(hipMemcpyAsync(h64.E.p, E64.p, sizeof(double) * nb * g.V, hipMemcpyDeviceToDevice, stream));
   FEMH_HIP(hipMemcpyAsync(h64.Ke.p, Ke64.p, sizeof(double) * 576, hipMemcpyDeviceToDevice, stream));
-  FineOp<double> f0{g, h64.E.as<double>(), h64.Ke.as<double>(), &modecoef, 1};
-  launch_fine_diag<double>(f0, h64.lv[0].D.as<double>(), stream);
+  FineOp<double> f0{g, h64.E.as<double>(), h64.Ke.as<double>()};
+  launch_fine_diag<double>(f0, nb, h64.lv[0].D.as<double>(), stream);
   for (int l = 1; l < nlevels; l++) {
     Level<double> &L = h64.lv[l];
     if (l == 1)
-      launch_build_level1<double>(g, L.g, E64.as<double>(), Mchild.as<double>(), L.S.as<double>(), stream);
+      launch_build_level1<double>(g, L.g, nb, E64.as<double>(), Mchild.as<double>(), L.S.as<double>(),
+                                  stream);
     else
-      launch_galerkin<double>(h64.lv[l - 1].g, L.g, h64.lv[l - 1].S.as<double>(), L.S.as<double>(),
+      launch_galerkin<double>(h64.lv[l - 1].g, L.g, nb, h64.lv[l - 1].S.as<double>(), L.S.as<double>(),
                               stream);
     StencilOp<double> so{L.g, L.S.as<double>()};
-    launch_stencil_diag<double>(so, L.D.as<double>(), stream);
+    launch_stencil_diag<double>(so, nb, L.D.as<double>(), stream);
   }
   for (int l = 0; l < nlevels; l++)
-    launch_invert<double>((size_t)3 * h64.lv[l].g.V, h64.lv[l].D.as<double>(), h64.lv[l].Dinv.as<double>(),
-                          stream);
+    launch_invert<double>((size_t)3 * nb * h64.lv[l].g.V, h64.lv[l].D.as<double>(),
+                          h64.lv[l].Dinv.as<double>(), stream);
   FEMH_HIP(hipGetLastError());
   if (o.precond == FEMH_PRECOND_MG) {
     estimate_lmax(h64);
@@ -342,30 +381,27 @@ void Context::build_hierarchy() {
                                       " dofs); allow more levels or use a grid divisible by 2^k");
     }
   }
-  if (o.precond == FEMH_PRECOND_MG && nlevels > 1) build_small<double>(h64);
   h64.ready = true;
   if (o.precision == FEMH_MIXED) {
     build_levels<float>(h32);
-    launch_convert<double, float>(g.V, E64.as<double>(), h32.E.as<float>(), stream);
+    launch_convert<double, float>((size_t)nb * g.V, E64.as<double>(), h32.E.as<float>(), stream);
     launch_convert<double, float>(576, Ke64.as<double>(), h32.Ke.as<float>(), stream);
     for (int l = 0; l < nlevels; l++) {
-      launch_convert<double, float>((size_t)3 * h64.lv[l].g.V, h64.lv[l].D.as<double>(),
-                                    h32.lv[l].D.as<float>(), stream);
-      launch_convert<double, float>((size_t)3 * h64.lv[l].g.V, h64.lv[l].Dinv.as<double>(),
-                                    h32.lv[l].Dinv.as<float>(), stream);
+      const size_t nd = (size_t)3 * nb * h64.lv[l].g.V;
+      launch_convert<double, float>(nd, h64.lv[l].D.as<double>(), h32.lv[l].D.as<float>(), stream);
+      launch_convert<double, float>(nd, h64.lv[l].Dinv.as<double>(), h32.lv[l].Dinv.as<float>(), stream);
       if (l > 0)
-        launch_convert<double, float>((size_t)243 * h64.lv[l].g.V, h64.lv[l].S.as<double>(),
+        launch_convert<double, float>((size_t)243 * nb * h64.lv[l].g.V, h64.lv[l].S.as<double>(),
                                       h32.lv[l].S.as<float>(), stream);
       h32.lv[l].lmax = h64.lv[l].lmax;
     }
     if (h64.ndense) {
-      h32.Ainv.alloc(sizeof(float) * (size_t)h64.ndense * h64.ndense);
-      launch_convert<double, float>((size_t)h64.ndense * h64.ndense, h64.Ainv.as<double>(),
+      h32.Ainv.alloc(sizeof(float) * (size_t)h64.ndense * h64.ndense * B);
+      launch_convert<double, float>((size_t)h64.ndense * h64.ndense * nb, h64.Ainv.as<double>(),
                                     h32.Ainv.as<float>(), stream);
       h32.ndense = h64.ndense;
     }
     FEMH_HIP(hipGetLastError());
-    if (o.precond == FEMH_PRECOND_MG && nlevels > 1) build_small<float>(h32);
     h32.ready = true;
   }
 }
@@ -385,112 +421,91 @@ static void cheb_coeffs(double lmax, double ratio, double *theta_inv, double *c1
   }
 }
 
+// per-sample 1/theta and the coefficients of Chebyshev step `step` (1-based)
 template <typename T>
-void Context::build_small(Hierarchy<T> &h) {
-  const int L = (int)h.lv.size();
-  int ls = L - 1;
-  // the fine level is matrix-free and always launched on its own
-  while (ls > 1 && h.lv[ls - 1].g.V <= kSmallMaxV) ls--;
-  h.lsmall = ls;
-  std::vector<SmallLevel<T>> d(L - ls);
-  for (int l = ls; l < L; l++) {
-    Level<T> &V = h.lv[l];
-    SmallLevel<T> &q = d[l - ls];
-    q.g = V.g;
-    q.S = V.S.template as<T>();
-    q.Dinv = V.Dinv.template as<T>();
-    q.x = V.x.template as<T>();
-    q.b = V.b.template as<T>();
-    q.r = V.r.template as<T>();
-    q.d = V.d.template as<T>();
-    q.d2 = V.d2.template as<T>();
-    q.xn = V.xn.template as<T>();
+void Context::set_cheb(const Hierarchy<T> &h, int l, OpArgs<T> &a, int step) {
+  for (int s = 0; s < nb; s++) {
     double ti, c1[kMaxCheb], c2[kMaxCheb];
-    cheb_coeffs(V.lmax, o.cheby_ratio, &ti, c1, c2);
-    q.theta_inv = (T)ti;
-    for (int k = 0; k < kMaxCheb; k++) {
-      q.c1[k] = (T)c1[k];
-      q.c2[k] = (T)c2[k];
+    cheb_coeffs(h.lv[l].lmax[s], o.cheby_ratio, &ti, c1, c2);
+    a.theta_inv[s] = (T)ti;
+    if (step >= 1) {
+      a.c1[s] = (T)c1[step - 1];
+      a.c2[s] = (T)c2[step - 1];
     }
-    q.Ainv = (l == L - 1) ? h.Ainv.template as<T>() : nullptr;
   }
-  h.small.alloc(sizeof(SmallLevel<T>) * d.size());
-  FEMH_HIP(hipMemcpyAsync(h.small.p, d.data(), sizeof(SmallLevel<T>) * d.size(), hipMemcpyHostToDevice,
-                          stream));
 }
 
 template <typename T>
-void Context::pre_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, unsigned mask) {
+void Context::pre_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_) {
   Level<T> &L = h.lv[l];
   const int k = o.nu1;
-  double ti, c1[kMaxCheb], c2[kMaxCheb];
-  cheb_coeffs(L.lmax, o.cheby_ratio, &ti, c1, c2);
-  const T *D = L.Dinv.template as<T>();
   if (k == 1) {
-    launch_cheb_first<T>(L.g.V, bb, D, L.d.template as<T>(), xx, ti, 1, mask, stream);
+    // x = b / (theta D): per-sample theta folded by launching per sample is
+    // avoided by one fused DSCALE pass with c1 = c2 = 0
+    OpArgs<T> a;
+    a.act = act_;
+    a.u = bb;
+    a.rprev = bb;
+    a.r = L.r.template as<T>();
+    a.x = xx;
+    a.dout = L.d.template as<T>();
+    a.last = 0;
+    set_cheb(h, l, a, 0);
+    op<T>(h, l, ST_DSCALE, EP_STEP, a);
     return;
   }
   T *dcur = L.d.template as<T>(), *dnxt = L.d2.template as<T>();
   OpArgs<T> a;
-  a.Dinv = D;
-  a.theta_inv = (T)ti;
-  a.mask = mask;
+  a.act = act_;
   a.u = bb;
   a.rprev = bb;
   a.xprev = nullptr;
   a.r = L.r.template as<T>();
   a.x = xx;
   a.dout = dcur;
-  a.c1 = (T)c1[0];
-  a.c2 = (T)c2[0];
   a.last = (k == 2);
+  set_cheb(h, l, a, 1);
   op<T>(h, l, ST_DSCALE, EP_STEP, a);
   for (int s = 2; s < k; s++) {
     a.u = dcur;
     a.rprev = L.r.template as<T>();
     a.xprev = xx;
     a.dout = dnxt;
-    a.c1 = (T)c1[s - 1];
-    a.c2 = (T)c2[s - 1];
     a.last = (s == k - 1);
+    set_cheb(h, l, a, s);
     op<T>(h, l, ST_PLAIN, EP_STEP, a);
     std::swap(dcur, dnxt);
   }
 }
 
 template <typename T>
-void Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, unsigned mask) {
+void Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_) {
   Level<T> &L = h.lv[l];
   Level<T> &C = h.lv[l + 1];
   const int k = o.nu2;
-  double ti, c1[kMaxCheb], c2[kMaxCheb];
-  cheb_coeffs(L.lmax, o.cheby_ratio, &ti, c1, c2);
   OpArgs<T> a;
-  a.Dinv = L.Dinv.template as<T>();
-  a.theta_inv = (T)ti;
-  a.mask = mask;
+  a.act = act_;
   a.u = xx;
   a.b = bb;
   a.r = L.r.template as<T>();
   a.dout = L.d2.template as<T>();
   a.x = L.xn.template as<T>();
-  if (l == 0 && o.apply_kernel == 0) {
+  set_cheb(h, l, a, 0);
+  if (l == 0) {
     a.xc = C.x.template as<T>();
     a.gc = C.g;
     op<T>(h, l, ST_PROLONG, EP_CHEB0, a);  // x3 = x + P xc staged, kept in xn
   } else {
-    launch_prolong_add<T>(L.g, C.g, C.x.template as<T>(), xx, mask, stream);
+    launch_prolong_add<T>(L.g, C.g, R, C.x.template as<T>(), xx, act_, stream);
     op<T>(h, l, ST_PLAIN, EP_CHEB0, a);
   }
   if (k == 1) {
-    launch_add2<T>(L.g.V, L.xn.template as<T>(), L.d2.template as<T>(), xx, mask, stream);
+    launch_add2<T>(L.g.V, R, L.xn.template as<T>(), L.d2.template as<T>(), xx, act_, stream);
     return;
   }
   T *dcur = L.d2.template as<T>(), *dnxt = L.d.template as<T>();
   OpArgs<T> s1;
-  s1.Dinv = a.Dinv;
-  s1.theta_inv = (T)ti;
-  s1.mask = mask;
+  s1.act = act_;
   s1.r = L.r.template as<T>();
   s1.rprev = L.r.template as<T>();
   s1.x = xx;
@@ -498,99 +513,100 @@ void Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, unsigned m
     s1.u = dcur;
     s1.xprev = (s == 1) ? L.xn.template as<T>() : xx;
     s1.dout = dnxt;
-    s1.c1 = (T)c1[s - 1];
-    s1.c2 = (T)c2[s - 1];
     s1.last = (s == k - 1);
+    set_cheb(h, l, s1, s);
     op<T>(h, l, ST_PLAIN, EP_STEP, s1);
     std::swap(dcur, dnxt);
   }
 }
 
 template <typename T>
-void Context::vcycle(Hierarchy<T> &h, int l, const T *bb, T *xx, unsigned mask) {
+void Context::vcycle(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_) {
   const int last = (int)h.lv.size() - 1;
-  if (l >= h.lsmall) {
-    // this level and everything below run in one workgroup (its b is in place)
-    launch_small_cycle<T>(h.small.template as<SmallLevel<T>>() + (l - h.lsmall), last - l + 1, o.nu1,
-                          o.nu2, mask, stream);
+  if (l == last) {
+    launch_dense_solve<T>(h.ndense, R, h.Ainv.template as<T>(), bb, xx, act_, stream);
     return;
   }
   Level<T> &L = h.lv[l];
   Level<T> &C = h.lv[l + 1];
-  pre_smooth<T>(h, l, bb, xx, mask);
-  OpArgs<T> a;
-  a.u = xx;
-  a.b = bb;
-  a.y = L.r.template as<T>();
-  a.mask = mask;
-  op<T>(h, l, ST_PLAIN, EP_RES, a);
-  launch_restrict<T>(L.g, C.g, L.r.template as<T>(), C.b.template as<T>(), mask, stream);
-  vcycle<T>(h, l + 1, C.b.template as<T>(), C.x.template as<T>(), mask);
-  post_smooth<T>(h, l, bb, xx, mask);
+  pre_smooth<T>(h, l, bb, xx, act_);
+  apply<T>(h, l, xx, L.r.template as<T>(), bb, 1, act_, nullptr, 0);
+  launch_restrict<T>(L.g, C.g, R, L.r.template as<T>(), C.b.template as<T>(), act_, stream);
+  vcycle<T>(h, l + 1, C.b.template as<T>(), C.x.template as<T>(), act_);
+  post_smooth<T>(h, l, bb, xx, act_);
 }
 
 template <typename T>
-void Context::precond(Hierarchy<T> &h, const T *rv, T *zv, unsigned mask) {
+void Context::precond(Hierarchy<T> &h, const T *rv, T *zv, const int *act_) {
   if (o.precond == FEMH_PRECOND_MG && h.lv.size() > 1) {
-    vcycle<T>(h, 0, rv, zv, mask);
+    vcycle<T>(h, 0, rv, zv, act_);
   } else {
-    // Jacobi: handled inside the PCG update kernel; this path only serves the
-    // first iteration
-    launch_cheb_first<T>(h.lv[0].g.V, rv, h.lv[0].Dinv.template as<T>(), zv, zv, 1.0, 1, mask, stream);
+    // Jacobi: inside the PCG update kernel; this path serves the first step
+    launch_cheb_first<T>(h.lv[0].g.V, R, rv, h.lv[0].Dinv.template as<T>(), zv, zv, 1.0, 1, act_,
+                         stream);
   }
 }
 
-// PCG over the six load cases.  Returns the mask of RHS still unconverged.
+// PCG over all right-hand sides flagged in `active` (updated in place as
+// they converge).
 template <typename T>
-int Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *qv, unsigned mask,
-                 double tol, int max_iter, const double *bnorm2, int *iters, double *relres,
-                 bool z_ready) {
+void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *qv,
+                  std::vector<int> &active, double tol, int max_iter, const double *bnorm2,
+                  int *iters, double *relres, bool z_ready) {
   const int V = g.V;
   const bool jac = !(o.precond == FEMH_PRECOND_MG && h.lv.size() > 1);
   const T *D0 = h.lv[0].Dinv.template as<T>();
-  FEMH_HIP(hipMemsetAsync(xv, 0, vec_bytes(g, sizeof(T)), stream));
-  FEMH_HIP(hipMemcpyAsync(rv, bvec, vec_bytes(g, sizeof(T)), hipMemcpyDeviceToDevice, stream));
-  unsigned active = mask;
-  for (int r = 0; r < kRHS; r++) {
-    iters[r] = 0;
-    relres[r] = 0.0;
+  FEMH_HIP(hipMemsetAsync(xv, 0, field_bytes(g, nb, sizeof(T)), stream));
+  FEMH_HIP(hipMemcpyAsync(rv, bvec, field_bytes(g, nb, sizeof(T)), hipMemcpyDeviceToDevice, stream));
+  for (int t = 0; t < R; t++) {
+    iters[t] = 0;
+    relres[t] = 0.0;
   }
-  if (!active) return 0;
+  int nact = 0;
+  for (int t = 0; t < R; t++) nact += active[t] != 0;
+  if (!nact) return;
+  set_act(A_PCG, active);
+  const int *ap = actp(A_PCG);
   if (!z_ready) {
     // z_ready: the caller already left z = M^-1 b and r.z in S_RZ0
-    precond<T>(h, rv, zv, active);
-    launch_dot<T>(V, rv, zv, S_RZ0, active, red, stream);
+    precond<T>(h, rv, zv, ap);
+    launch_dot<T>(V, R, rv, zv, S_RZ0, ap, red, stream);
   }
-  launch_p_update<T>(V, pv, zv, S_RZ0, S_RZ0, 1, active, red, stream);
+  launch_p_update<T>(V, R, pv, zv, S_RZ0, S_RZ0, 1, ap, red, stream);
   int cur = 0;
-  double rr[kRHS];
+  std::vector<double> rr(R);
   for (int it = 1; it <= max_iter; it++) {
     const int rzc = cur ? S_RZ1 : S_RZ0, rzn = cur ? S_RZ0 : S_RZ1;
-    apply<T>(h, 0, pv, qv, nullptr, 0, active, &red, S_PQ);
-    launch_pcg_update<T>(V, xv, rv, pv, qv, jac ? D0 : nullptr, zv, rzc, rzn, active, red,
+    apply<T>(h, 0, pv, qv, nullptr, 0, ap, &red, S_PQ);
+    launch_pcg_update<T>(V, R, xv, rv, pv, qv, jac ? D0 : nullptr, zv, rzc, rzn, ap, red,
                          apply_parts(), stream);
     FEMH_HIP(hipGetLastError());
     if (it % o.check_every == 0 || it == max_iter) {
-      read_scalars(S_RR, rr, kRHS, vec_parts(V));
-      for (int r = 0; r < kRHS; r++) {
-        if (!((active >> r) & 1u)) continue;
-        if (!std::isfinite(rr[r])) throw Error(FEMH_NONFINITE, "residual became non-finite");
-        const double rel = std::sqrt(rr[r] / bnorm2[r]);
-        iters[r] = it;
-        relres[r] = rel;
-        if (rel <= tol) active &= ~(1u << r);
+      read_scalars(S_RR, rr.data(), R, vec_parts(V));
+      bool changed = false;
+      for (int t = 0; t < R; t++) {
+        if (!active[t]) continue;
+        if (!std::isfinite(rr[t])) throw Error(FEMH_NONFINITE, "residual became non-finite");
+        const double rel = std::sqrt(rr[t] / bnorm2[t]);
+        iters[t] = it;
+        relres[t] = rel;
+        if (rel <= tol) {
+          active[t] = 0;
+          changed = true;
+          nact--;
+        }
       }
-      if (!active) break;
+      if (!nact) break;
+      if (changed) set_act(A_PCG, active);
     }
     if (!jac) {
-      vcycle<T>(h, 0, rv, zv, active);
-      launch_dot<T>(V, rv, zv, rzn, active, red, stream);
+      vcycle<T>(h, 0, rv, zv, ap);
+      launch_dot<T>(V, R, rv, zv, rzn, ap, red, stream);
     }
-    launch_p_update<T>(V, pv, zv, rzn, rzc, 0, active, red, stream);
+    launch_p_update<T>(V, R, pv, zv, rzn, rzc, 0, ap, red, stream);
     cur ^= 1;
   }
   FEMH_HIP(hipGetLastError());
-  return (int)active;
 }
 
 // ---------------------------------------------------------------------------
@@ -631,62 +647,71 @@ void Context::flush_profile() {
 
 void Context::compute_ch(const double *fields, double *C) {
   const int nblk = 1024;
-  launch_ch_energy(g, E64.as<double>(), Ke64.as<double>(), uaff.as<double>(), fields,
-                   ch_part.as<double>(), nblk, stream);
-  launch_sum_partials(ch_part.as<double>(), 21, nblk, ch_out.as<double>(), stream);
+  launch_ch_energy(g, nb, E64.as<double>(), Ke64.as<double>(), uaff.as<double>(), fields,
+                   ch_part.as<double>(), nblk, ch_out.as<double>(), stream);
   FEMH_HIP(hipGetLastError());
-  double s[21];
-  FEMH_HIP(hipMemcpyAsync(s, ch_out.p, sizeof(s), hipMemcpyDeviceToHost, stream));
+  std::vector<double> s(21 * nb);
+  FEMH_HIP(hipMemcpyAsync(s.data(), ch_out.p, sizeof(double) * s.size(), hipMemcpyDeviceToHost, stream));
   FEMH_HIP(hipStreamSynchronize(stream));
-  int p = 0;
-  for (int a = 0; a < 6; a++)
-    for (int bb = a; bb < 6; bb++) {
-      C[6 * a + bb] = C[6 * bb + a] = s[p] / (double)g.V;
-      p++;
-    }
+  for (int smp = 0; smp < nb; smp++) {
+    int p = 0;
+    double *Cs = C + 36 * smp;
+    for (int a = 0; a < 6; a++)
+      for (int bb = a; bb < 6; bb++) {
+        Cs[6 * a + bb] = Cs[6 * bb + a] = s[21 * smp + p] / (double)g.V;
+        p++;
+      }
+  }
 }
 
 void Context::homogenize(double *C, femh_stats *st) {
+  if (nb < 1) throw Error(FEMH_BAD_ARG, "no input loaded");
   auto t0 = std::chrono::steady_clock::now();
   hipEvent_t ev[3];
   for (auto &e : ev) FEMH_HIP(hipEventCreate(&e));
   FEMH_HIP(hipEventRecord(ev[0], stream));
   have_fields = false;
   // validate the moduli before any device work depends on them
-  double mst[3];
+  std::vector<double> mst(3 * nb);
   {
-    const int np = launch_modulus_stats(g.V, E64.as<double>(), red, stream);
-    read_scalars(S_AUX, mst, 3, np);
+    const int np = launch_modulus_stats(g.V, nb, E64.as<double>(), red, stream);
+    read_scalars(S_AUX, mst.data(), 3 * nb, np);
   }
-  if (mst[1] > 0) throw Error(FEMH_NONFINITE, "non-finite modulus");
-  if (mst[2] > 0) throw Error(FEMH_BAD_ARG, "negative modulus");
-  if (!(mst[0] > 0)) throw Error(FEMH_BAD_ARG, "all-void cell (every modulus is zero)");
-  const double sumE = mst[0];
+  for (int s = 0; s < nb; s++) {
+    if (mst[3 * s + 1] > 0) throw Error(FEMH_NONFINITE, "non-finite modulus");
+    if (mst[3 * s + 2] > 0) throw Error(FEMH_BAD_ARG, "negative modulus");
+    if (!(mst[3 * s] > 0)) throw Error(FEMH_BAD_ARG, "all-void cell (every modulus is zero)");
+  }
+  std::vector<int> all(R, 1);
+  set_act(A_ALL, all);
   build_hierarchy();
-  FineOp<double> f0{g, E64.as<double>(), Ke64.as<double>(), &modecoef, 1};
-  launch_fine_rhs<double>(f0, gtab.as<double>(), b.as<double>(), stream);
-  launch_dot<double>(g.V, b.as<double>(), b.as<double>(), S_BB, 0x3Fu, red, stream);
+  FineOp<double> f0{g, E64.as<double>(), Ke64.as<double>()};
+  launch_fine_rhs<double>(f0, R, gtab.as<double>(), b.as<double>(), stream);
+  launch_dot<double>(g.V, R, b.as<double>(), b.as<double>(), S_BB, actp(A_ALL), red, stream);
   FEMH_HIP(hipGetLastError());
-  double bb[kRHS];
-  read_scalars(S_BB, bb, kRHS, vec_parts(g.V));
+  std::vector<double> bb(R);
+  read_scalars(S_BB, bb.data(), R, vec_parts(g.V));
   // right-hand sides that vanish (the affine field already balances, e.g. a
   // homogeneous cell) are solved by u = 0
-  const double zero_thresh = 1e-13 * (sumE / g.V) * std::sqrt((double)g.V);
-  unsigned mask = 0;
-  for (int r = 0; r < kRHS; r++)
-    if (std::sqrt(bb[r]) > zero_thresh) mask |= 1u << r;
+  std::vector<int> active(R, 0);
+  for (int t = 0; t < R; t++) {
+    const double sumE = mst[3 * (t / kRHS)];
+    const double zero_thresh = 1e-13 * (sumE / g.V) * std::sqrt((double)g.V);
+    active[t] = std::sqrt(bb[t]) > zero_thresh;
+  }
   FEMH_HIP(hipEventRecord(ev[1], stream));
 
-  int iters[kRHS] = {0}, outer[kRHS] = {0};
-  double relres[kRHS] = {0};
+  std::vector<int> iters(R, 0), outer(R, 0);
+  std::vector<double> relres(R, 0.0), ch_est(nb, 0.0);
   int status = FEMH_OK;
-  double ch_est = 0;
   if (o.precision == FEMH_FP64) {
-    unsigned left = pcg<double>(h64, b.as<double>(), x.as<double>(), r.as<double>(), z.as<double>(),
-                                p.as<double>(), q.as<double>(), mask, o.tol, o.max_iter, bb, iters,
-                                relres);
-    for (int k = 0; k < kRHS; k++) outer[k] = (mask >> k) & 1u;
-    if (left) status = FEMH_NOT_CONVERGED;
+    std::vector<int> act_pcg = active;
+    pcg<double>(h64, b.as<double>(), x.as<double>(), r.as<double>(), z.as<double>(), p.as<double>(),
+                q.as<double>(), act_pcg, o.tol, o.max_iter, bb.data(), iters.data(), relres.data());
+    for (int t = 0; t < R; t++) {
+      outer[t] = active[t];
+      if (act_pcg[t]) status = FEMH_NOT_CONVERGED;
+    }
   } else {
     // FP64 iterative refinement around FP32 multigrid-PCG corrections.
     // Before each sweep the preconditioned residual r.M^-1 r of every load
@@ -694,112 +719,137 @@ void Context::homogenize(double *C, femh_stats *st) {
     // error is the Gram matrix of those errors: dC = E^T K E / V (PSD), so
     // ||dC||_F <= trace(dC) = sum_i ||e_i||_K^2 / V.  With C_ii estimated as
     // (a_ii sum E - b.x - x.r)/V, est = sqrt(6) sum_i rz_i / (V trace C) bounds
-    // the relative Frobenius error when M^-1 ~ K^-1.
-    FEMH_HIP(hipMemsetAsync(x.p, 0, vec_bytes(g, sizeof(double)), stream));
-    FEMH_HIP(hipMemcpyAsync(r.p, b.p, vec_bytes(g, sizeof(double)), hipMemcpyDeviceToDevice, stream));
-    unsigned active = mask;
-    double rr[kRHS];
-    for (int k = 0; k < kRHS; k++) rr[k] = bb[k];
+    // the relative Frobenius error of a sample when M^-1 ~ K^-1.
+    FEMH_HIP(hipMemsetAsync(x.p, 0, field_bytes(g, nb, sizeof(double)), stream));
+    FEMH_HIP(hipMemcpyAsync(r.p, b.p, field_bytes(g, nb, sizeof(double)), hipMemcpyDeviceToDevice,
+                            stream));
+    std::vector<double> rr = bb;
     double aii[kRHS];
     for (int k = 0; k < kRHS; k++) {
       double acc = 0;
       for (int t = 0; t < 24; t++) acc += tab.u_aff[k * 24 + t] * tab.g[k * 24 + t];
       aii[k] = acc;
     }
-    double bx[kRHS] = {0}, xr[kRHS] = {0};
+    std::vector<double> bx(R, 0.0), xr(R, 0.0), rz(R), rn(R), bxn(R), xrn(R);
+    std::vector<int> it_in(R);
+    std::vector<double> rel_in(R);
     const int max_outer = o.max_outer > 0 ? o.max_outer : 12;
-    for (int sweep = 0; sweep < max_outer && active; sweep++) {
-      launch_convert<double, float>((size_t)kRHS * 3 * g.V, r.as<double>(), b32.as<float>(), stream);
+    int nact = 0;
+    for (int t = 0; t < R; t++) nact += active[t];
+    for (int sweep = 0; sweep < max_outer && nact; sweep++) {
+      set_act(A_OUT, active);
+      const int *ao = actp(A_OUT);
+      launch_convert<double, float>((size_t)R * 3 * g.V, r.as<double>(), b32.as<float>(), stream);
       // z = M^-1 r and r.z for the estimate; reused as the first PCG step
-      FEMH_HIP(hipMemcpyAsync(r32.p, b32.p, vec_bytes(g, sizeof(float)), hipMemcpyDeviceToDevice, stream));
-      precond<float>(h32, r32.as<float>(), z32.as<float>(), active);
-      launch_dot<float>(g.V, r32.as<float>(), z32.as<float>(), S_RZ0, active, red, stream);
+      FEMH_HIP(hipMemcpyAsync(r32.p, b32.p, field_bytes(g, nb, sizeof(float)), hipMemcpyDeviceToDevice,
+                              stream));
+      precond<float>(h32, r32.as<float>(), z32.as<float>(), ao);
+      launch_dot<float>(g.V, R, r32.as<float>(), z32.as<float>(), S_RZ0, ao, red, stream);
       FEMH_HIP(hipGetLastError());
       if (o.ch_tol > 0 && sweep > 0) {
-        double rz[kRHS];
-        read_scalars(S_RZ0, rz, kRHS, vec_parts(g.V));
-        double trC = 0, srz = 0;
-        for (int k = 0; k < kRHS; k++) {
-          trC += (aii[k] * sumE - bx[k] - xr[k]) / g.V;
-          if ((active >> k) & 1u) srz += std::max(rz[k], 0.0);
+        read_scalars(S_RZ0, rz.data(), R, vec_parts(g.V));
+        for (int s = 0; s < nb; s++) {
+          double trC = 0, srz = 0;
+          bool any = false;
+          for (int k = 0; k < kRHS; k++) {
+            const int t = kRHS * s + k;
+            trC += (aii[k] * mst[3 * s] - bx[t] - xr[t]) / g.V;
+            if (active[t]) {
+              srz += std::max(rz[t], 0.0);
+              any = true;
+            }
+          }
+          if (!any) continue;
+          ch_est[s] = (trC > 0) ? std::sqrt(6.0) * srz / (g.V * trC) : 1.0;
+          if (ch_est[s] <= o.ch_tol)
+            for (int k = 0; k < kRHS; k++)
+              if (active[kRHS * s + k]) {
+                active[kRHS * s + k] = 0;
+                nact--;
+              }
         }
-        ch_est = (trC > 0) ? std::sqrt(6.0) * srz / (g.V * trC) : 1.0;
-        if (ch_est <= o.ch_tol) {
-          active = 0;
-          break;
-        }
+        if (!nact) break;
       }
-      int it_in[kRHS];
-      double rel_in[kRHS];
+      std::vector<int> act_in = active;
       pcg<float>(h32, b32.as<float>(), x32.as<float>(), r32.as<float>(), z32.as<float>(),
-                 p32.as<float>(), q32.as<float>(), active, o.inner_tol > 0 ? o.inner_tol : 1e-4,
-                 o.max_iter, rr, it_in, rel_in, true);
-      launch_add_f32_to_f64(0, x32.as<float>(), x.as<double>(), active, g.V, stream);
-      apply<double>(h64, 0, x.as<double>(), r.as<double>(), b.as<double>(), 1, active, nullptr, 0);
-      launch_dot<double>(g.V, r.as<double>(), r.as<double>(), S_RR, active, red, stream);
-      launch_dot<double>(g.V, b.as<double>(), x.as<double>(), S_BX, active, red, stream);
-      launch_dot<double>(g.V, x.as<double>(), r.as<double>(), S_XR, active, red, stream);
+                 p32.as<float>(), q32.as<float>(), act_in, o.inner_tol > 0 ? o.inner_tol : 1e-4,
+                 o.max_iter, rr.data(), it_in.data(), rel_in.data(), true);
+      set_act(A_OUT, active);
+      launch_add_f32_to_f64(g.V, R, x32.as<float>(), x.as<double>(), ao, stream);
+      apply<double>(h64, 0, x.as<double>(), r.as<double>(), b.as<double>(), 1, ao, nullptr, 0);
+      launch_dot<double>(g.V, R, r.as<double>(), r.as<double>(), S_RR, ao, red, stream);
+      launch_dot<double>(g.V, R, b.as<double>(), x.as<double>(), S_BX, ao, red, stream);
+      launch_dot<double>(g.V, R, x.as<double>(), r.as<double>(), S_XR, ao, red, stream);
       FEMH_HIP(hipGetLastError());
-      double rn[kRHS], bxn[kRHS], xrn[kRHS];
-      read_scalars(S_RR, rn, kRHS, vec_parts(g.V));
-      read_scalars(S_BX, bxn, kRHS, vec_parts(g.V));
-      read_scalars(S_XR, xrn, kRHS, vec_parts(g.V));
-      for (int k = 0; k < kRHS; k++) {
-        if (!((active >> k) & 1u)) continue;
-        if (!std::isfinite(rn[k])) throw Error(FEMH_NONFINITE, "residual became non-finite");
-        iters[k] += it_in[k];
-        outer[k]++;
-        rr[k] = rn[k];
-        bx[k] = bxn[k];
-        xr[k] = xrn[k];
-        relres[k] = std::sqrt(rn[k] / bb[k]);
-        if (relres[k] <= o.tol) active &= ~(1u << k);
+      read_scalars(S_RR, rn.data(), R, vec_parts(g.V));
+      read_scalars(S_BX, bxn.data(), R, vec_parts(g.V));
+      read_scalars(S_XR, xrn.data(), R, vec_parts(g.V));
+      for (int t = 0; t < R; t++) {
+        if (!active[t]) continue;
+        if (!std::isfinite(rn[t])) throw Error(FEMH_NONFINITE, "residual became non-finite");
+        iters[t] += it_in[t];
+        outer[t]++;
+        rr[t] = rn[t];
+        bx[t] = bxn[t];
+        xr[t] = xrn[t];
+        relres[t] = std::sqrt(rn[t] / bb[t]);
+        if (relres[t] <= o.tol) {
+          active[t] = 0;
+          nact--;
+        }
       }
     }
-    if (active) status = FEMH_NOT_CONVERGED;
+    if (nact) status = FEMH_NOT_CONVERGED;
   }
-  launch_remove_mean<double>(g.V, x.as<double>(), 0x3Fu, red, stream);
+  launch_remove_mean<double>(g.V, R, x.as<double>(), actp(A_ALL), red, stream);
   FEMH_HIP(hipEventRecord(ev[2], stream));
   compute_ch(x.as<double>(), C);
   flush_profile();
   have_fields = true;
+  float ms1 = 0, ms2 = 0;
+  FEMH_HIP(hipEventElapsedTime(&ms1, ev[0], ev[1]));
+  FEMH_HIP(hipEventElapsedTime(&ms2, ev[1], ev[2]));
+  const double ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (st) {
-    std::memset(st, 0, sizeof(*st));
-    for (int k = 0; k < kRHS; k++) {
-      st->iters[k] = iters[k];
-      st->outer[k] = outer[k];
-      st->relres[k] = relres[k];
+    for (int s = 0; s < nb; s++) {
+      femh_stats &o_ = st[s];
+      std::memset(&o_, 0, sizeof(o_));
+      for (int k = 0; k < kRHS; k++) {
+        o_.iters[k] = iters[kRHS * s + k];
+        o_.outer[k] = outer[kRHS * s + k];
+        o_.relres[k] = relres[kRHS * s + k];
+      }
+      o_.ch_err_est = ch_est[s];
+      o_.ms_setup = ms1;
+      o_.ms_solve = ms2;
+      o_.ms_total = ms_total;
+      o_.levels = nlevels;
     }
-    float ms1 = 0, ms2 = 0;
-    FEMH_HIP(hipEventElapsedTime(&ms1, ev[0], ev[1]));
-    FEMH_HIP(hipEventElapsedTime(&ms2, ev[1], ev[2]));
-    st->ms_setup = ms1;
-    st->ms_solve = ms2;
-    st->levels = nlevels;
-    st->ch_err_est = ch_est;
-    st->ms_total =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
-  for (int k = 0; k < 36; k++)
+  for (int k = 0; k < 36 * nb; k++)
     if (!std::isfinite(C[k])) throw Error(FEMH_NONFINITE, "C_H is not finite");
   if (status != FEMH_OK) throw Error(status, "PCG did not reach the tolerance within max_iter");
 }
 
 void Context::gradient(const double *dJdC, double *out_host, bool energy_only) {
   if (!have_fields) throw Error(FEMH_NO_FIELDS, "no solved fields: call femh_homogenize first");
-  double Gsym[21];
-  int p = 0;
-  for (int a = 0; a < 6; a++)
-    for (int bb = a; bb < 6; bb++) {
-      Gsym[p++] = energy_only ? 0.0 : (a == bb ? dJdC[6 * a + a] : dJdC[6 * a + bb] + dJdC[6 * bb + a]);
-    }
-  FEMH_HIP(hipMemcpyAsync(Gs.p, Gsym, sizeof(Gsym), hipMemcpyHostToDevice, stream));
-  size_t n = energy_only ? (size_t)21 * g.V : (size_t)g.V;
+  std::vector<double> Gsym(21 * nb);
+  for (int s = 0; s < nb; s++) {
+    const double *G = dJdC + 36 * s;
+    int p = 0;
+    for (int a = 0; a < 6; a++)
+      for (int bb = a; bb < 6; bb++)
+        Gsym[21 * s + p++] = energy_only ? 0.0 : (a == bb ? G[6 * a + a] : G[6 * a + bb] + G[6 * bb + a]);
+  }
+  FEMH_HIP(hipMemcpyAsync(Gs.p, Gsym.data(), sizeof(double) * Gsym.size(), hipMemcpyHostToDevice, stream));
+  const size_t n = (energy_only ? (size_t)21 : 1) * g.V * nb;
   DBuf out;
   out.alloc(sizeof(double) * n);
-  launch_gradient(g, rho.as<double>(), Ke64.as<double>(), uaff.as<double>(), x.as<double>(),
-                  Gs.as<double>(), o.interp, o.E0, o.Emin, o.penal, energy_only ? 1 : 0,
+  launch_gradient(g, nb, rho.as<double>(), Ke64.as<double>(), uaff.as<double>(), x.as<double>(),
+                  Gs.as<double>(), modulus_input ? FEMH_INTERP_MODULUS : o.interp, o.E0, o.Emin,
+                  o.penal, energy_only ? 1 : 0,
                   out.as<double>(), stream);
   FEMH_HIP(hipGetLastError());
   FEMH_HIP(hipMemcpyAsync(out_host, out.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
@@ -808,7 +858,7 @@ void Context::gradient(const double *dJdC, double *out_host, bool energy_only) {
 
 void Context::get_fields(double *u) {
   if (!have_fields) throw Error(FEMH_NO_FIELDS, "no solved fields: call femh_homogenize first");
-  FEMH_HIP(hipMemcpyAsync(u, x.p, vec_bytes(g, sizeof(double)), hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipMemcpyAsync(u, x.p, field_bytes(g, nb, sizeof(double)), hipMemcpyDeviceToHost, stream));
   FEMH_HIP(hipStreamSynchronize(stream));
 }
 
@@ -817,18 +867,21 @@ void Context::apply_level(int level, const double *u, double *y) {
   if (level < 0 || level >= nlevels) throw Error(FEMH_BAD_ARG, "level out of range");
   Level<double> &L = h64.lv[level];
   const size_t n = (size_t)3 * L.g.V;
+  std::vector<int> one(R, 0);
+  one[0] = 1;
+  set_act(A_POW, one);
   FEMH_HIP(hipMemcpyAsync(L.x.p, u, sizeof(double) * n, hipMemcpyHostToDevice, stream));
-  apply<double>(h64, level, L.x.as<double>(), L.Ad.as<double>(), nullptr, 0, 1u, nullptr, 0);
+  apply<double>(h64, level, L.x.as<double>(), L.Ad.as<double>(), nullptr, 0, actp(A_POW), nullptr, 0);
   FEMH_HIP(hipGetLastError());
   FEMH_HIP(hipMemcpyAsync(y, L.Ad.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
   FEMH_HIP(hipStreamSynchronize(stream));
 }
 
-template int Context::pcg<double>(Hierarchy<double> &, const double *, double *, double *, double *,
-                                  double *, double *, unsigned, double, int, const double *, int *,
-                                  double *, bool);
-template int Context::pcg<float>(Hierarchy<float> &, const float *, float *, float *, float *,
-                                 float *, float *, unsigned, double, int, const double *, int *,
-                                 double *, bool);
+template void Context::pcg<double>(Hierarchy<double> &, const double *, double *, double *, double *,
+                                   double *, double *, std::vector<int> &, double, int,
+                                   const double *, int *, double *, bool);
+template void Context::pcg<float>(Hierarchy<float> &, const float *, float *, float *, float *,
+                                  float *, float *, std::vector<int> &, double, int, const double *,
+                                  int *, double *, bool);
 
 }  // namespace femh

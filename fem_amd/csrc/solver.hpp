@@ -1,4 +1,8 @@
 // solver.hpp -- device-resident homogenization context behind the C ABI.
+//
+// A context owns device storage for up to B samples of one grid size and
+// solves the 6 load cases of every sample together (R = 6 nb right-hand
+// sides for a call with nb <= B samples).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -28,7 +32,10 @@ struct DBuf {
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
-  DBuf(DBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DBuf(DBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
   DBuf &operator=(DBuf &&o) noexcept {
     if (this != &o) {
       release();
@@ -43,53 +50,58 @@ struct DBuf {
   void alloc(size_t n);
   void release();
   template <typename T>
-  T *as() const { return static_cast<T *>(p); }
+  T *as() const {
+    return static_cast<T *>(p);
+  }
 };
 
 template <typename T>
 struct Level {
   Dims g{};
-  DBuf S;            // stencil (levels >= 1)
-  DBuf D;            // point diagonal [3V]
-  DBuf Dinv;         // its inverse (what the kernels use)
-  double lmax = 0;   // lambda_max(D^-1 A) estimate (with margin)
-  DBuf x, b, r, d, d2, xn, Ad;  // V-cycle work, [6][3][V]
+  DBuf S;                    // stencil [B][243][V] (levels >= 1)
+  DBuf D, Dinv;              // point diagonal and its inverse [B][3][V]
+  std::vector<double> lmax;  // per sample: lambda_max(D^-1 A) estimate (with margin)
+  DBuf x, b, r, d, d2, xn, Ad;  // V-cycle work, [R][3][V]
 };
 
 template <typename T>
 struct Hierarchy {
   std::vector<Level<T>> lv;
-  DBuf E;      // fine moduli in T
-  DBuf Ke;     // unit stiffness in T
-  DBuf Ainv;   // dense (regularised) inverse of the coarsest operator
+  DBuf E;     // fine moduli [B][V] in T
+  DBuf Ke;    // unit stiffness in T
+  DBuf Ainv;  // per sample: dense (regularised) inverse of the coarsest operator
   int ndense = 0;
-  int lsmall = 0;   // first level run inside the single-workgroup coarse cycle
-  DBuf small;       // device array of SmallLevel<T> for levels lsmall..L-1
   bool ready = false;
 };
+
+// activity-array slots (device int[kMaxR] each)
+enum ActSlot : int { A_ALL = 0, A_PCG = 1, A_OUT = 2, A_POW = 3, kActSlots = 4 };
 
 struct Context {
   femh_opts o{};
   Dims g{};
+  int B = 1;   // capacity (samples)
+  int nb = 0;  // samples of the current call
+  int R = 0;   // 6 nb
   ElementTables tab;
   ModeCoef modecoef{};
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  std::string err;
-  int last_status = FEMH_OK;
 
   // material / tables on device
   DBuf rho, E64, Ke64, gtab, uaff, Mchild, Gs;
-  // FP64 solver state, each [6][3][V]
+  // FP64 solver state, each [R][3][V]
   DBuf b, x, r, z, p, q;
   // FP32 inner solver state (mixed)
   DBuf b32, r32, x32, z32, p32, q32;
   DBuf red_part, red_ctr, red_scal;
   Reduce red{};
   DBuf ch_part, ch_out;
+  DBuf act;
   Hierarchy<double> h64;
   Hierarchy<float> h32;
   bool have_fields = false;
+  bool modulus_input = false;  // last load gave moduli (dE/drho = 1)
   int nlevels = 1;
 
   // kernel timing
@@ -113,37 +125,39 @@ struct Context {
   explicit Context(const femh_opts &opts);
   ~Context();
 
-  void set_modulus_from_rho(const double *d_rho_src, bool device_src);
-  void set_modulus(const double *E_host);
+  // inputs: n samples (<= B) of density (interp applied) or modulus
+  void load(const double *src, int n, bool device_src, bool modulus);
   void build_hierarchy();
-  void homogenize(double *C, femh_stats *st);
+  void homogenize(double *C, femh_stats *st);  // C[36 nb], st[nb] (may be null)
   void gradient(const double *dJdC, double *out_host, bool energy_only);
   void get_fields(double *u);
   void apply_level(int level, const double *u, double *y);
 
  private:
+  const int *actp(int slot) const { return act.as<int>() + slot * kMaxR; }
+  void set_act(int slot, const std::vector<int> &v);
   template <typename T>
   void build_levels(Hierarchy<T> &h);
   template <typename T>
-  void apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *b, int mode, unsigned mask,
+  void op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a);
+  template <typename T>
+  void apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *b, int mode, const int *act_,
              const Reduce *red, int slot);
   template <typename T>
-  void op(Hierarchy<T> &h, int l, int st, int ep, const OpArgs<T> &a);
+  void pre_smooth(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_);
   template <typename T>
-  void pre_smooth(Hierarchy<T> &h, int l, const T *b, T *x, unsigned mask);
+  void post_smooth(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_);
   template <typename T>
-  void post_smooth(Hierarchy<T> &h, int l, const T *b, T *x, unsigned mask);
+  void vcycle(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_);
   template <typename T>
-  void vcycle(Hierarchy<T> &h, int l, const T *b, T *x, unsigned mask);
+  void precond(Hierarchy<T> &h, const T *r, T *z, const int *act_);
   template <typename T>
-  void build_small(Hierarchy<T> &h);
+  void pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *qv,
+           std::vector<int> &active, double tol, int max_iter, const double *bnorm2, int *iters,
+           double *relres, bool z_ready = false);
+  template <typename T>
+  void set_cheb(const Hierarchy<T> &h, int l, OpArgs<T> &a, int step);
   int apply_parts() const;
-  template <typename T>
-  void precond(Hierarchy<T> &h, const T *r, T *z, unsigned mask);
-  template <typename T>
-  int pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *qv, unsigned mask,
-          double tol, int max_iter, const double *bnorm2, int *iters, double *relres,
-          bool z_ready = false);
   void estimate_lmax(Hierarchy<double> &h);
   void dense_coarse(Hierarchy<double> &h);
   void read_scalars(int slot, double *out, int n, int np);

@@ -25,6 +25,7 @@ PRECISION = {"fp64": 0, "mixed": 1}
 # Every symbol include/femh.h declares.
 EXPORTS = ("femh_abi_version", "femh_default_opts", "femh_create", "femh_destroy",
            "femh_homogenize", "femh_homogenize_modulus", "femh_homogenize_device",
+           "femh_homogenize_many", "femh_homogenize_many_modulus", "femh_homogenize_many_device",
            "femh_gradient", "femh_energy_terms", "femh_get_fields", "femh_homogenize_batch",
            "femh_prepare", "femh_apply_level", "femh_level_dims", "femh_set_profiling",
            "femh_kernel_stats", "femh_element_stiffness", "femh_last_error")
@@ -46,7 +47,7 @@ class Opts(ctypes.Structure):
                 ("precision", ctypes.c_int), ("ch_tol", ctypes.c_double),
                 ("check_every", ctypes.c_int), ("device", ctypes.c_int),
                 ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_int),
-                ("apply_kernel", ctypes.c_int)]
+                ("apply_kernel", ctypes.c_int), ("batch", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -82,6 +83,9 @@ def lib():
         for name in ("femh_homogenize", "femh_homogenize_modulus"):
             getattr(L, name).argtypes = [vp, vp, vp, vp]
         L.femh_homogenize_device.argtypes = [vp, vp, vp, vp, vp]
+        for name in ("femh_homogenize_many", "femh_homogenize_many_modulus"):
+            getattr(L, name).argtypes = [vp, ctypes.c_int, vp, vp, vp]
+        L.femh_homogenize_many_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.femh_gradient.argtypes = [vp, vp, vp]
         L.femh_energy_terms.argtypes = [vp, vp]
         L.femh_get_fields.argtypes = [vp, vp]
@@ -104,7 +108,7 @@ def _ptr(a):
 def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond="mg",
               levels=0, nu1=2, nu2=2, cheby_ratio=8.0, coarse_iters=64, tol=1e-10,
               max_iter=5000, precision="fp64", ch_tol=0.0, check_every=1, device=0,
-              inner_tol=1e-4, max_outer=12, apply_kernel="tiled"):
+              inner_tol=1e-4, max_outer=12, apply_kernel="tiled", batch=1):
     o = Opts()
     lib().femh_default_opts(ctypes.byref(o))
     o.nx, o.ny, o.nz = (int(s) for s in shape)
@@ -118,6 +122,7 @@ def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond
     o.ch_tol, o.check_every, o.device = ch_tol, check_every, device
     o.inner_tol, o.max_outer = inner_tol, max_outer
     o.apply_kernel = {"tiled": 0, "dense": 1}[apply_kernel]
+    o.batch = batch
     return o
 
 
@@ -185,21 +190,44 @@ class Homogenizer:
                                                  ctypes.c_void_p(stream_ptr or 0), ctypes.byref(s)))
         return s.as_dict()
 
-    def gradient(self, dJdC):
-        G = np.ascontiguousarray(dJdC, dtype=np.float64).reshape(36)
-        out = np.zeros(self.V)
+    def homogenize_many(self, grids, modulus=False, check=True):
+        """n grids of this shape (array [n, *shape]), `batch` per launch
+        sequence.  Returns (C[n,6,6], list of stats dicts)."""
+        a = np.ascontiguousarray(grids, dtype=np.float64).reshape(-1)
+        if a.size % self.V:
+            raise ValueError("grid count does not divide the array")
+        n = a.size // self.V
+        C = np.zeros((n, 36))
+        S = (Stats * n)()
+        fn = lib().femh_homogenize_many_modulus if modulus else lib().femh_homogenize_many
+        st = fn(self._h, n, _ptr(a), _ptr(C), S)
+        if check:
+            self._check(st)
+        return C.reshape(n, 6, 6), [s.as_dict() for s in S]
+
+    def homogenize_many_device(self, n, d_rho_ptr, d_C_ptr, stream_ptr=None):
+        S = (Stats * n)()
+        self._check(lib().femh_homogenize_many_device(self._h, n, ctypes.c_void_p(d_rho_ptr),
+                                                      ctypes.c_void_p(d_C_ptr),
+                                                      ctypes.c_void_p(stream_ptr or 0), S))
+        return [s.as_dict() for s in S]
+
+    def gradient(self, dJdC, nb=1):
+        """dJ/drho of the last batch of nb samples (dJdC [nb,6,6])."""
+        G = np.ascontiguousarray(dJdC, dtype=np.float64).reshape(36 * nb)
+        out = np.zeros(self.V * nb)
         self._check(lib().femh_gradient(self._h, _ptr(G), _ptr(out)))
-        return out.reshape(self.shape)
+        return out.reshape(self.shape) if nb == 1 else out.reshape((nb,) + self.shape)
 
-    def energy_terms(self):
-        W = np.zeros((self.V, 21))
+    def energy_terms(self, nb=1):
+        W = np.zeros((nb * self.V, 21))
         self._check(lib().femh_energy_terms(self._h, _ptr(W)))
-        return W
+        return W if nb == 1 else W.reshape(nb, self.V, 21)
 
-    def fields(self):
-        u = np.zeros((6, 3, self.V))
+    def fields(self, nb=1):
+        u = np.zeros((nb * 6, 3, self.V))
         self._check(lib().femh_get_fields(self._h, _ptr(u)))
-        return u
+        return u if nb == 1 else u.reshape(nb, 6, 3, self.V)
 
     def set_profiling(self, on=True):
         self._check(lib().femh_set_profiling(self._h, int(bool(on))))

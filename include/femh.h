@@ -89,6 +89,7 @@ typedef struct femh_opts {
   double inner_tol;     /* mixed: relative tolerance of each FP32 correction solve */
   int max_outer;        /* mixed: refinement sweep cap */
   int apply_kernel;     /* fine-grid operator: 0 tiled cube-mode (default), 1 dense gather */
+  int batch;            /* samples solved together per launch sequence (1..8) */
 } femh_opts;
 
 typedef struct femh_stats {
@@ -120,8 +121,21 @@ int femh_homogenize_modulus(struct femh_ctx *ctx, const double *E, double *C, fe
 int femh_homogenize_device(struct femh_ctx *ctx, const double *d_rho, double *d_C, void *stream,
                            femh_stats *stats);
 
+/* n samples of the context's grid, solved `batch` at a time: rho holds n*V
+ * doubles (sample after sample), C receives 36*n, stats (may be NULL) n
+ * entries.  The _modulus form takes Young's moduli, the _device form device
+ * pointers (densities) and a stream. */
+int femh_homogenize_many(struct femh_ctx *ctx, int n, const double *rho, double *C,
+                         femh_stats *stats);
+int femh_homogenize_many_modulus(struct femh_ctx *ctx, int n, const double *E, double *C,
+                                 femh_stats *stats);
+int femh_homogenize_many_device(struct femh_ctx *ctx, int n, const double *d_rho, double *d_C,
+                                void *stream, femh_stats *stats);
+
 /* dJ/drho for every voxel from the fields of the last homogenize call:
  * dJdrho[e] = sum_ij dJdC[6i+j] * E'(rho_e) * chi_i^T K0 chi_j / V.
+ * After a call with nb samples (the last batch of a _many call), dJdC holds
+ * 36*nb values and dJdrho receives nb*V.
  * Returns FEMH_NO_FIELDS when no solve has been done. */
 int femh_gradient(struct femh_ctx *ctx, const double *dJdC, double *dJdrho);
 
@@ -129,7 +143,8 @@ int femh_gradient(struct femh_ctx *ctx, const double *dJdC, double *dJdrho);
  * (the unscaled adjoint building block). */
 int femh_energy_terms(struct femh_ctx *ctx, double *W);
 
-/* Copies the 6 solved periodic fluctuation fields (6*3*V doubles). */
+/* Copies the 6 solved periodic fluctuation fields (6*3*V doubles per sample
+ * of the last batch). */
 int femh_get_fields(struct femh_ctx *ctx, double *u);
 
 /* Batch: n samples, sample s has dims[3s..3s+2] and modulus grid E[s] (host).

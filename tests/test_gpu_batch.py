@@ -1,0 +1,129 @@
+"""GPU tests of batched solves: several samples of one grid share every launch
+(each load case of each sample keeps its own PCG scalars, stopping test and
+smoother spectrum), so a batched result must equal the one-sample result."""
+import numpy as np
+import pytest
+
+from fem_amd import FemhError, Homogenizer, homogenize_batch
+from fem_amd import workloads as W
+from fem_amd.femh import BAD_ARG, NONFINITE
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+NU = 0.3
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a).ravel() - np.asarray(b).ravel()) / np.linalg.norm(b))
+
+
+def simp_set(n, edge, seed0=20, corr=3.0):
+    return np.stack([W.simp_density(edge, seed0 + s, corr) for s in range(n)])
+
+
+@pytest.mark.parametrize("batch", [2, 4, 8])
+def test_fp64_batch_equals_single_sample_solves(batch):
+    rho = simp_set(6, 16)
+    hb = Homogenizer((16, 16, 16), nu=NU, tol=1e-11, batch=batch)
+    Cb, stb = hb.homogenize_many(rho)
+    h1 = Homogenizer((16, 16, 16), nu=NU, tol=1e-11)
+    for s in range(6):
+        C1, st1 = h1.homogenize(rho[s])
+        assert rel(Cb[s], C1) <= 1e-13
+        assert stb[s]["iters"] == st1["iters"]
+
+
+def test_batch_against_oracle():
+    rho = simp_set(3, 12, seed0=40)
+    h = Homogenizer((12, 12, 12), nu=NU, tol=1e-12, batch=3)
+    C, _ = h.homogenize_many(rho)
+    for s in range(3):
+        ref = O.homogenize(W.simp_modulus(rho[s]), NU, precond="mg", tol=1e-13)
+        assert rel(C[s], ref["C"]) <= 1e-10
+
+
+def test_batch_mixes_trivial_and_hard_samples():
+    """A homogeneous cell (all right-hand sides vanish), a laminate and a
+    near-void binary cell in one launch sequence."""
+    from tests.test_oracle_pins import c_iso, laminate_closed_form
+    n = 8
+    Es = np.array([1.0, 0.05, 0.3, 0.3, 1.0, 1.0, 0.01, 0.5])
+    lam = np.broadcast_to(Es[:, None, None], (n, n, n)).copy()
+    binary = W.binary_modulus(W.binary_cube(n, 3), 1.0, 1e-6)
+    E = np.stack([2.0 * np.ones((n, n, n)), lam, binary])
+    h = Homogenizer((n, n, n), nu=NU, tol=1e-12, max_iter=20000, batch=4)
+    C, st = h.homogenize_many(E, modulus=True)
+    assert np.abs(C[0] - c_iso(2.0, NU)).max() <= 1e-13
+    assert st[0]["iters"] == [0] * 6
+    Cr = laminate_closed_form(Es, np.full(8, 1 / 8), NU)
+    assert np.abs(C[1] - Cr).max() <= 1e-10 * np.abs(Cr).max()
+    h1 = Homogenizer((n, n, n), nu=NU, tol=1e-12, max_iter=20000)
+    C2, _ = h1.homogenize_modulus(binary)
+    assert rel(C[2], C2) <= 1e-12
+
+
+def test_mixed_batch_meets_1e6_and_estimates():
+    rho = simp_set(5, 32, seed0=60, corr=4.0)
+    hm = Homogenizer((32, 32, 32), nu=NU, precision="mixed", ch_tol=1e-8, batch=4)
+    Cm, stm = hm.homogenize_many(rho)
+    h64 = Homogenizer((32, 32, 32), nu=NU, tol=1e-12, batch=4)
+    C64, _ = h64.homogenize_many(rho)
+    for s in range(5):
+        err = rel(Cm[s], C64[s])
+        assert err <= 1e-6, (s, err)
+        assert 0 < stm[s]["ch_err_est"] <= 1e-8
+        # the estimate bounds the error it reports (allowing a 10x slack)
+        assert err <= 10 * stm[s]["ch_err_est"] + 1e-12
+
+
+def test_batched_gradient_equals_single_sample_gradients():
+    rho = simp_set(3, 8, seed0=80, corr=2.0)
+    G = np.random.default_rng(3).standard_normal((3, 6, 6))
+    hb = Homogenizer((8, 8, 8), nu=NU, tol=1e-12, batch=3)
+    hb.homogenize_many(rho)
+    gb = hb.gradient(G, nb=3)
+    Wb = hb.energy_terms(nb=3)
+    ub = hb.fields(nb=3)
+    h1 = Homogenizer((8, 8, 8), nu=NU, tol=1e-12)
+    for s in range(3):
+        h1.homogenize(rho[s])
+        g1 = h1.gradient(G[s])
+        assert np.abs(gb[s] - g1).max() <= 1e-12 * np.abs(g1).max()
+        assert np.abs(Wb[s] - h1.energy_terms()).max() <= 1e-12 * np.abs(Wb[s]).max()
+        assert np.abs(ub[s] - h1.fields()).max() <= 1e-12 * np.abs(ub[s]).max()
+
+
+def test_many_device_entry_point():
+    torch = pytest.importorskip("torch")
+    rho = simp_set(3, 16, seed0=90)
+    h = Homogenizer((16, 16, 16), nu=NU, tol=1e-11, batch=2)
+    Ch, _ = h.homogenize_many(rho)
+    d_rho = torch.from_numpy(rho).to("cuda:0")
+    d_C = torch.zeros(3, 36, dtype=torch.float64, device="cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    h.homogenize_many_device(3, d_rho.data_ptr(), d_C.data_ptr(), stream)
+    torch.cuda.synchronize()
+    assert np.abs(d_C.cpu().numpy().reshape(3, 6, 6) - Ch).max() == 0.0
+
+
+def test_uneven_batch_uses_batched_groups():
+    grids = W.batch_uneven(11, 5, edges=(8, 12, 16))
+    C, _ = homogenize_batch(grids, nu=NU, tol=1e-11, batch=4)
+    for g, c in zip(grids, C):
+        ci, _ = Homogenizer(g.shape, nu=NU, tol=1e-11).homogenize_modulus(g)
+        assert rel(c, ci) <= 1e-13
+
+
+def test_batch_errors():
+    with pytest.raises(FemhError) as e:
+        Homogenizer((8, 8, 8), batch=9)
+    assert e.value.status == BAD_ARG
+    h = Homogenizer((8, 8, 8), nu=NU, batch=2)
+    rho = simp_set(2, 8)
+    rho[1, 2, 3, 4] = np.inf
+    with pytest.raises(FemhError) as e:
+        h.homogenize_many(rho)
+    assert e.value.status == NONFINITE
+    rho[1, 2, 3, 4] = 0.5
+    C, _ = h.homogenize_many(rho)
+    assert np.isfinite(C).all()
