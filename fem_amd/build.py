@@ -1,7 +1,10 @@
-"""Build recipe of the engine's shared library (libfemh.so, in-tree).
+"""Build recipe of the engine's shared libraries (in-tree):
 
-hipcc cross-compiles the HIP kernels for gfx950 (no GPU needed).  Packed FP32
-instructions are disabled for all device code.
+  libfemh.so       HIP kernels + solver + C ABI (hipcc, gfx950; packed FP32
+                   instructions are disabled for all device code)
+  libfemh_host.so  C++ host facade with the reference's class names, over the
+                   C ABI only (g++)
+  _build/test_host C++ test program of the facade (run by a GPU test)
 """
 import os
 import subprocess
@@ -11,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfemh.so")
+HOST_LIB = os.path.join(HERE, "libfemh_host.so")
+CXX = os.environ.get("CXX", "g++")
 
 SOURCES = ["kernels.hip", "ops.hip", "tables.cpp", "solver.cpp", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -46,7 +51,21 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    build_host(objdir)
     return LIB
+
+
+def build_host(objdir):
+    """The facade links only against libfemh.so's C ABI."""
+    host = os.path.join(HERE, "host")
+    tmp = HOST_LIB + ".tmp"
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", tmp,
+          os.path.join(host, "femh_host.cpp"), "-L" + HERE, "-lfemh", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_LIB)
+    test_src = os.path.join(ROOT, "tests", "cpp", "test_host.cpp")
+    if os.path.exists(test_src):
+        _run([CXX, "-O2", "-std=c++17", "-Wall", "-o", os.path.join(objdir, "test_host"), test_src,
+              "-L" + HERE, "-lfemh_host", "-lfemh", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/.."])
 
 
 if __name__ == "__main__":

@@ -58,4 +58,72 @@ static __device__ __forceinline__ void node_ijk(const Dims &g, int n, int &i, in
   i = t / g.ny;
 }
 
+// ------------------------------------------------------------------------
+// unit-modulus element operator in cube modes: f = E K0 u for one voxel's 24
+// corner values (u[3 v + c], corner v = 4 bx + 2 by + bz), K0 = H^T Lambda H
+// with H the 8-point Walsh-Hadamard transform per component and Lambda the
+// 45-term mode coupling (tables.cpp)
+template <typename T>
+struct ModeT {
+  T diag[3][8];
+  T cross[3][3][8];
+};
+
+template <typename T>
+__device__ __forceinline__ void hadamard_voxel(const T *__restrict__ u, T E, const ModeT<T> &m,
+                                               T *__restrict__ f) {
+  T w[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int v = 0; v < 8; v++) w[c][v] = u[3 * v + c];
+  // forward: per axis (a, b) -> (a + b, b - a)
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int mb = 4; mb >= 1; mb >>= 1)
+#pragma unroll
+      for (int v = 0; v < 8; v++)
+        if (!(v & mb)) {
+          const T a = w[c][v], b = w[c][v | mb];
+          w[c][v] = a + b;
+          w[c][v | mb] = b - a;
+        }
+  // mode coupling (mode 0 is a rigid translation: no force)
+  T z[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    z[c][0] = 0;
+#pragma unroll
+    for (int sm = 1; sm < 8; sm++) {
+      T acc = m.diag[c][sm] * w[c][sm];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        if (d == c) continue;
+        const int bc = (sm >> (2 - c)) & 1, bd = (sm >> (2 - d)) & 1;
+        if (bc == bd) continue;
+        const int t = sm ^ (1 << (2 - c)) ^ (1 << (2 - d));
+        acc += m.cross[c][d][sm] * w[d][t];
+      }
+      z[c][sm] = E * acc;
+    }
+  }
+  // inverse: per axis (a, b) -> (a - b, a + b)
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int mb = 4; mb >= 1; mb >>= 1)
+#pragma unroll
+      for (int v = 0; v < 8; v++)
+        if (!(v & mb)) {
+          const T a = z[c][v], b = z[c][v | mb];
+          z[c][v] = a - b;
+          z[c][v | mb] = a + b;
+        }
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int v = 0; v < 8; v++) f[3 * v + c] = z[c][v];
+}
+
 }  // namespace femh

@@ -168,64 +168,54 @@ __global__ __launch_bounds__(kBlock) void k_build_level1(Dims fg, Dims cg, const
 // with fine n = 2N + o and m = n + d.  Per axis the (o, d) pairs that reach
 // coarse offset D with a non-zero weight are enumerated first (at most 7 of
 // the 9), so the inner loop visits only contributing (n, m) pairs.
-struct AxisPairs {
-  int n;
-  int o[9], d[9];
-  double w[9];
-};
-
-__device__ __forceinline__ void axis_pairs(int D, AxisPairs &p) {
-  p.n = 0;
-#pragma unroll
-  for (int o = -1; o <= 1; o++)
-#pragma unroll
-    for (int d = -1; d <= 1; d++) {
-      const int t = o + d;
-      double w;
-      if ((t & 1) == 0) w = (t / 2 == D) ? 1.0 : 0.0;
-      else w = (D == (t - 1) / 2 || D == (t + 1) / 2) ? 0.5 : 0.0;
-      w *= o ? 0.5 : 1.0;
-      if (w != 0.0) {
-        p.o[p.n] = o;
-        p.d[p.n] = d;
-        p.w[p.n] = w;
-        p.n++;
-      }
-    }
+// Weight of the pair (o, d) along one axis for coarse offset D: a coarse
+// node I touches fine nodes 2I+o (P weight 1 for o = 0, 1/2 for o = +-1); the
+// fine column 2I+o+d belongs to coarse node I+D with P weight 1 (2I+o+d even)
+// or 1/2 (odd, shared by two coarse nodes).
+__device__ __forceinline__ double pair_weight(int o, int d, int D) {
+  const int t = o + d;
+  double w;
+  if ((t & 1) == 0)
+    w = (t / 2 == D) ? 1.0 : 0.0;
+  else
+    w = (D == (t - 1) / 2 || D == (t + 1) / 2) ? 0.5 : 0.0;
+  return o ? 0.5 * w : w;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_galerkin(Dims fg, Dims cg, const T *__restrict__ Sf,
-                                                     T *__restrict__ Sc) {
+// One axis of the Galerkin product P^T A P with P = Px (x) Py (x) Pz: the
+// stencil `Sin` on grid gi is coarsened along AXIS into `Sout` on grid go (gi
+// with that dimension halved); the other two axes are untouched.  Three passes
+// (x, y, z) give the coarse operator with at most 7 terms per entry and pass.
+template <typename T, int AXIS>
+__global__ __launch_bounds__(kBlock) void k_galerkin_axis(Dims gi, Dims go, const T *__restrict__ Sin,
+                                                          T *__restrict__ Sout) {
   const int oc = blockIdx.y, s = blockIdx.z;
-  const T *Sfs = Sf + (size_t)s * 243 * fg.V;
-  T *Scs = Sc + (size_t)s * 243 * cg.V;
-  AxisPairs px, py, pz;
-  axis_pairs(oc / 9 - 1, px);
-  axis_pairs((oc / 3) % 3 - 1, py);
-  axis_pairs(oc % 3 - 1, pz);
-  for (int N = blockIdx.x * kBlock + threadIdx.x; N < cg.V; N += gridDim.x * kBlock) {
-    int I, J, K;
-    node_ijk(cg, N, I, J, K);
+  const int D = AXIS == 0 ? oc / 9 - 1 : (AXIS == 1 ? (oc / 3) % 3 - 1 : oc % 3 - 1);
+  constexpr int st = AXIS == 0 ? 9 : (AXIS == 1 ? 3 : 1);
+  const T *Sis = Sin + (size_t)s * 243 * gi.V;
+  T *Sos = Sout + (size_t)s * 243 * go.V;
+  for (int N = blockIdx.x * kBlock + threadIdx.x; N < go.V; N += gridDim.x * kBlock) {
+    int i, j, k;
+    node_ijk(go, N, i, j, k);
     double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int a = 0; a < px.n; a++) {
-      const int fi = (2 * I + px.o[a] + fg.nx) % fg.nx;
-      for (int bq = 0; bq < py.n; bq++) {
-        const int fj = (2 * J + py.o[bq] + fg.ny) % fg.ny;
-        const double wab = px.w[a] * py.w[bq];
-        for (int c = 0; c < pz.n; c++) {
-          const int fk = (2 * K + pz.o[c] + fg.nz) % fg.nz;
-          const int n = (fi * fg.ny + fj) * fg.nz + fk;
-          const int dm = (px.d[a] + 1) * 9 + (py.d[bq] + 1) * 3 + (pz.d[c] + 1);
-          const double w = wab * pz.w[c];
-          const T *src = Sfs + (size_t)(dm * 9) * fg.V + n;
 #pragma unroll
-          for (int q = 0; q < 9; q++) blk[q] += w * (double)src[(size_t)q * fg.V];
-        }
+    for (int o = -1; o <= 1; o++) {
+      int fi = i, fj = j, fk = k;
+      if (AXIS == 0) fi = (2 * i + o + gi.nx) % gi.nx;
+      if (AXIS == 1) fj = (2 * j + o + gi.ny) % gi.ny;
+      if (AXIS == 2) fk = (2 * k + o + gi.nz) % gi.nz;
+      const int n = (fi * gi.ny + fj) * gi.nz + fk;
+#pragma unroll
+      for (int d = -1; d <= 1; d++) {
+        const double w = pair_weight(o, d, D);
+        if (w == 0.0) continue;
+        const T *src = Sis + (size_t)((oc + (d - D) * st) * 9) * gi.V + n;
+#pragma unroll
+        for (int q = 0; q < 9; q++) blk[q] += w * (double)src[(size_t)q * gi.V];
       }
     }
 #pragma unroll
-    for (int q = 0; q < 9; q++) Scs[(size_t)(oc * 9 + q) * cg.V + N] = (T)blk[q];
+    for (int q = 0; q < 9; q++) Sos[(size_t)(oc * 9 + q) * go.V + N] = (T)blk[q];
   }
 }
 
@@ -281,6 +271,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Dims fg, Dims cg, const T *
   }
 }
 
+// ef += P ec (trilinear interpolation).  Along each axis a fine node 2I takes
+// coarse node I with weight 1, a fine node 2I+1 takes I and I+1 with 1/2
+// each; every lane reads the same 8 coarse nodes with per-axis weights
+// (w0, w1) = (1, 0) or (1/2, 1/2), so the loop does not diverge on parity.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(Dims fg, Dims cg, const T *__restrict__ ec,
                                                         T *__restrict__ ef,
@@ -292,21 +286,26 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Dims fg, Dims cg, const 
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < fg.V; n += gridDim.x * kBlock) {
     int i, j, k;
     node_ijk(fg, n, i, j, k);
-    const int ci0 = i >> 1, ci1 = ((i + 1) >> 1) % cg.nx;
-    const int cj0 = j >> 1, cj1 = ((j + 1) >> 1) % cg.ny;
-    const int ck0 = k >> 1, ck1 = ((k + 1) >> 1) % cg.nz;
-    const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
-    const double w = ((i & 1) ? 0.5 : 1.0) * ((j & 1) ? 0.5 : 1.0) * ((k & 1) ? 0.5 : 1.0);
-    double acc[3] = {0, 0, 0};
-    for (int a = 0; a < ni; a++)
-      for (int b2 = 0; b2 < nj; b2++)
-        for (int d = 0; d < nk; d++) {
-          const int N = ((a ? ci1 : ci0) * cg.ny + (b2 ? cj1 : cj0)) * cg.nz + (d ? ck1 : ck0);
+    const int ci[2] = {i >> 1, ((i + 1) >> 1) % cg.nx};
+    const int cj[2] = {j >> 1, ((j + 1) >> 1) % cg.ny};
+    const int ck[2] = {k >> 1, ((k + 1) >> 1) % cg.nz};
+    const T wi[2] = {(i & 1) ? (T)0.5 : (T)1, (i & 1) ? (T)0.5 : (T)0};
+    const T wj[2] = {(j & 1) ? (T)0.5 : (T)1, (j & 1) ? (T)0.5 : (T)0};
+    const T wk[2] = {(k & 1) ? (T)0.5 : (T)1, (k & 1) ? (T)0.5 : (T)0};
+    T acc[3] = {0, 0, 0};
 #pragma unroll
-          for (int q = 0; q < 3; q++) acc[q] += (double)c[(size_t)q * cg.V + N];
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b2 = 0; b2 < 2; b2++)
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+          const int N = (ci[a] * cg.ny + cj[b2]) * cg.nz + ck[d];
+          const T w = wi[a] * wj[b2] * wk[d];
+#pragma unroll
+          for (int q = 0; q < 3; q++) acc[q] += w * c[(size_t)q * cg.V + N];
         }
 #pragma unroll
-    for (int q = 0; q < 3; q++) f[(size_t)q * fg.V + n] += (T)(w * acc[q]);
+    for (int q = 0; q < 3; q++) f[(size_t)q * fg.V + n] += acc[q];
   }
 }
 
@@ -560,63 +559,54 @@ __device__ __forceinline__ void gather_chi(const Dims &g, const int I[2], const 
   }
 }
 
-// W_p(e) = chi_i^T K0 chi_j for the 21 pairs i<=j (not divided by V)
-__device__ void element_energy(const Dims &g, int e, const double *__restrict__ Ke,
-                               const double *__restrict__ uaff, const double *__restrict__ us,
-                               double *W) {
+// W_p(e) = chi_a^T K0 chi_b for the 21 pairs a <= b (row order, p = 0..20; not
+// divided by V), handed to sink(p, W_p) one at a time.  K0 chi_a goes through
+// the cube-mode factorization.  The pair loops stay rolled (register
+// pressure), so sinks index memory with the runtime p.
+template <typename F>
+__device__ __forceinline__ void element_energy(const Dims &g, int e, const ModeT<double> &mc,
+                                               const double *__restrict__ uaff,
+                                               const double *__restrict__ us, F &&sink) {
   int i, j, k;
   node_ijk(g, e, i, j, k);
   const int I[2] = {i, i == g.nx - 1 ? 0 : i + 1};
   const int J[2] = {j, j == g.ny - 1 ? 0 : j + 1};
   const int K[2] = {k, k == g.nz - 1 ? 0 : k + 1};
-  int p = 0;
+#pragma unroll 1
   for (int a = 0; a < 6; a++) {
     double chi[24], w[24];
     gather_chi(g, I, J, K, us, uaff, a, chi);
-#pragma unroll
-    for (int r = 0; r < 24; r++) {
-      double s = 0;
-#pragma unroll
-      for (int q = 0; q < 24; q++) s += Ke[r * 24 + q] * chi[q];
-      w[r] = s;
-    }
+    hadamard_voxel<double>(chi, 1.0, mc, w);
+#pragma unroll 1
     for (int b = a; b < 6; b++) {
       double chib[24];
-      if (b == a) {
-#pragma unroll
-        for (int q = 0; q < 24; q++) chib[q] = chi[q];
-      } else {
-        gather_chi(g, I, J, K, us, uaff, b, chib);
-      }
+      gather_chi(g, I, J, K, us, uaff, b, chib);
       double s = 0;
 #pragma unroll
       for (int q = 0; q < 24; q++) s += w[q] * chib[q];
-      W[p++] = s;
+      sink(a * 6 - a * (a - 1) / 2 + (b - a), s);
     }
   }
 }
 
 __global__ __launch_bounds__(kBlock) void k_ch_energy(Dims g, const double *__restrict__ E,
-                                                      const double *__restrict__ Ke,
+                                                      ModeT<double> mc,
                                                       const double *__restrict__ uaff,
                                                       const double *__restrict__ u,
                                                       double *__restrict__ partials) {
   const int smp = blockIdx.y;
   const double *Es = E + (size_t)smp * g.V;
   const double *us = u + (size_t)smp * kRHS * 3 * g.V;
-  double acc[21];
-#pragma unroll
-  for (int p = 0; p < 21; p++) acc[p] = 0;
-  for (int e = blockIdx.x * kBlock + threadIdx.x; e < g.V; e += gridDim.x * kBlock) {
-    double W[21];
-    element_energy(g, e, Ke, uaff, us, W);
+  __shared__ double acc[21][kBlock];  // per-thread accumulators
+  const int tid = threadIdx.x;
+  for (int p = 0; p < 21; p++) acc[p][tid] = 0;
+  for (int e = blockIdx.x * kBlock + tid; e < g.V; e += gridDim.x * kBlock) {
     const double Ee = Es[e];
-#pragma unroll
-    for (int p = 0; p < 21; p++) acc[p] += Ee * W[p];
+    element_energy(g, e, mc, uaff, us, [&](int p, double w) { acc[p][tid] += Ee * w; });
   }
   for (int p = 0; p < 21; p++) {
-    double s = block_sum(acc[p]);
-    if (threadIdx.x == 0) partials[((size_t)smp * 21 + p) * gridDim.x + blockIdx.x] = s;
+    double s = block_sum(acc[p][tid]);
+    if (tid == 0) partials[((size_t)smp * 21 + p) * gridDim.x + blockIdx.x] = s;
   }
 }
 
@@ -630,7 +620,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restric
 }
 
 __global__ __launch_bounds__(kBlock) void k_gradient(Dims g, const double *__restrict__ rho,
-                                                     const double *__restrict__ Ke,
+                                                     ModeT<double> mc,
                                                      const double *__restrict__ uaff,
                                                      const double *__restrict__ u,
                                                      const double *__restrict__ Gs, int interp,
@@ -642,15 +632,12 @@ __global__ __launch_bounds__(kBlock) void k_gradient(Dims g, const double *__res
   const double *rs = rho + (size_t)smp * g.V;
   const double *G = Gs + (size_t)smp * 21;
   for (int e = blockIdx.x * kBlock + threadIdx.x; e < g.V; e += gridDim.x * kBlock) {
-    double W[21];
-    element_energy(g, e, Ke, uaff, us, W);
     if (energy_only) {
-#pragma unroll
-      for (int p = 0; p < 21; p++) out[((size_t)smp * g.V + e) * 21 + p] = W[p] * invV;
+      double *o = out + ((size_t)smp * g.V + e) * 21;
+      element_energy(g, e, mc, uaff, us, [&](int p, double w) { o[p] = w * invV; });
     } else {
       double s = 0;
-#pragma unroll
-      for (int p = 0; p < 21; p++) s += G[p] * W[p];
+      element_energy(g, e, mc, uaff, us, [&](int p, double w) { s += G[p] * w; });
       out[(size_t)smp * g.V + e] = s * invV * interp_dE(rs[e], interp, E0, Emin, pexp);
     }
   }
@@ -698,9 +685,15 @@ void launch_build_level1(Dims fine, Dims coarse, int B, const T *E, const double
                      coarse, E, M, S);
 }
 template <typename T>
-void launch_galerkin(Dims fine, Dims coarse, int B, const T *Sf, T *Sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_galerkin<T>, dim3(grid_for(coarse.V), 27, B), dim3(kBlock), 0, s, fine,
-                     coarse, Sf, Sc);
+void launch_galerkin(Dims fine, Dims coarse, int B, const T *Sf, T *Sc, T *t1, T *t2, hipStream_t s) {
+  Dims a{coarse.nx, fine.ny, fine.nz, coarse.nx * fine.ny * fine.nz};
+  Dims b{coarse.nx, coarse.ny, fine.nz, coarse.nx * coarse.ny * fine.nz};
+  hipLaunchKernelGGL((k_galerkin_axis<T, 0>), dim3(grid_for(a.V), 27, B), dim3(kBlock), 0, s, fine,
+                     a, Sf, t1);
+  hipLaunchKernelGGL((k_galerkin_axis<T, 1>), dim3(grid_for(b.V), 27, B), dim3(kBlock), 0, s, a, b,
+                     t1, t2);
+  hipLaunchKernelGGL((k_galerkin_axis<T, 2>), dim3(grid_for(coarse.V), 27, B), dim3(kBlock), 0, s, b,
+                     coarse, t2, Sc);
 }
 template <typename T>
 void launch_stencil_diag(const StencilOp<T> &op, int B, T *d, hipStream_t s) {
@@ -801,16 +794,26 @@ int launch_modulus_stats(int V, int B, const double *E, const Reduce &red, hipSt
   hipLaunchKernelGGL(k_modulus_stats, grid2(V, B), dim3(kBlock), 0, s, V, E, red);
   return grid_for(V);
 }
-void launch_ch_energy(Dims g, int B, const double *E, const double *Ke, const double *uaff,
+static ModeT<double> mode_table(const ModeCoef &m) {
+  ModeT<double> t;
+  for (int c = 0; c < 3; c++)
+    for (int v = 0; v < 8; v++) {
+      t.diag[c][v] = m.diag[c][v];
+      for (int d = 0; d < 3; d++) t.cross[c][d][v] = m.cross[c][d][v];
+    }
+  return t;
+}
+void launch_ch_energy(Dims g, int B, const double *E, const ModeCoef &m, const double *uaff,
                       const double *u, double *partials, int nblk, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_ch_energy, dim3(nblk, B), dim3(kBlock), 0, s, g, E, Ke, uaff, u, partials);
+  hipLaunchKernelGGL(k_ch_energy, dim3(nblk, B), dim3(kBlock), 0, s, g, E, mode_table(m), uaff, u,
+                     partials);
   hipLaunchKernelGGL(k_sum_partials, dim3(21 * B), dim3(kBlock), 0, s, partials, nblk, out);
 }
-void launch_gradient(Dims g, int B, const double *rho, const double *Ke, const double *uaff,
+void launch_gradient(Dims g, int B, const double *rho, const ModeCoef &m, const double *uaff,
                      const double *u, const double *Gs, int interp, double E0, double Emin,
                      double p, int energy_only, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_gradient, grid2(g.V, B), dim3(kBlock), 0, s, g, rho, Ke, uaff, u, Gs, interp,
-                     E0, Emin, p, energy_only, out);
+  hipLaunchKernelGGL(k_gradient, grid2(g.V, B), dim3(kBlock), 0, s, g, rho, mode_table(m), uaff, u,
+                     Gs, interp, E0, Emin, p, energy_only, out);
 }
 int vec_parts(int V) { return grid_for(3 * V); }
 
@@ -830,7 +833,7 @@ void reduce_init(Reduce &red, hipStream_t s) {
   template void launch_fine_rhs<T>(const FineOp<T> &, int, const double *, T *, hipStream_t);     \
   template void launch_build_level1<T>(Dims, Dims, int, const T *, const double *, T *,           \
                                        hipStream_t);                                              \
-  template void launch_galerkin<T>(Dims, Dims, int, const T *, T *, hipStream_t);                 \
+  template void launch_galerkin<T>(Dims, Dims, int, const T *, T *, T *, T *, hipStream_t);       \
   template void launch_stencil_diag<T>(const StencilOp<T> &, int, T *, hipStream_t);              \
   template void launch_restrict<T>(Dims, Dims, int, const T *, T *, const int *, hipStream_t);    \
   template void launch_prolong_add<T>(Dims, Dims, int, const T *, T *, const int *, hipStream_t); \

@@ -95,8 +95,10 @@ void launch_fine_rhs(const FineOp<T> &op, int R, const double *g /*device 6x24*/
 template <typename T>
 void launch_build_level1(Dims fine, Dims coarse, int B, const T *E, const double *Mchild, T *S,
                          hipStream_t s);
+// coarse stencil P^T A P by three one-axis passes; t1 holds 243 B V/2 and t2
+// 243 B V/4 values of the fine level's V
 template <typename T>
-void launch_galerkin(Dims fine, Dims coarse, int B, const T *Sf, T *Sc, hipStream_t s);
+void launch_galerkin(Dims fine, Dims coarse, int B, const T *Sf, T *Sc, T *t1, T *t2, hipStream_t s);
 template <typename T>
 void launch_stencil_diag(const StencilOp<T> &op, int B, T *d, hipStream_t s);
 template <typename T>
@@ -156,11 +158,11 @@ void launch_material(size_t n, const double *rho, double *E, int interp, double 
 // per sample: sum / non-finite count / negative count of E into S_AUX[3s..3s+2]
 int launch_modulus_stats(int V, int B, const double *E, const Reduce &red, hipStream_t s);
 // Energy-form C_H per sample: 21 upper-triangle sums (not yet / V) into out[s][21]
-void launch_ch_energy(Dims g, int B, const double *E, const double *Ke, const double *uaff,
+void launch_ch_energy(Dims g, int B, const double *E, const ModeCoef &m, const double *uaff,
                       const double *u, double *partials /*[B][21][nblk]*/, int nblk, double *out,
                       hipStream_t s);
 // dJ/drho_e = dE/drho(rho_e) * sum_p Gs[s][p] W_p(e) / V ; energy_only: W (21/voxel)
-void launch_gradient(Dims g, int B, const double *rho, const double *Ke, const double *uaff,
+void launch_gradient(Dims g, int B, const double *rho, const ModeCoef &m, const double *uaff,
                      const double *u, const double *Gs /*device [B][21]*/, int interp, double E0,
                      double Emin, double p, int energy_only, double *out, hipStream_t s);
 

@@ -32,74 +32,33 @@ constexpr int kTN = kTile + 2;   // staged nodes per row
 constexpr int kTE = kTile + 1;   // voxels per row
 constexpr int kEOH = 13;         // LDS stride of one voxel's 12 plane forces (odd: no bank conflicts)
 
+// epilogue inputs of one owned node (loaded ahead of the operator work so
+// that their latency overlaps it):
+//   EP_RES: e0 = b ; EP_CHEB0: e0 = b, e1 = 1/D ; EP_STEP: e0 = rprev, e1 = 1/D,
+//   e2 = xprev (0 without one)
 template <typename T>
-struct ModeT {
-  T diag[3][8];
-  T cross[3][3][8];
+struct EpiIn {
+  T e0[3], e1[3], e2[3];
 };
 
-template <typename T>
-__device__ __forceinline__ void hadamard_voxel(const T *__restrict__ u, T E, const ModeT<T> &m,
-                                               T *__restrict__ f) {
-  T w[3][8];
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int v = 0; v < 8; v++) w[c][v] = u[3 * v + c];
-  // forward: per axis (a, b) -> (a + b, b - a)
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int mb = 4; mb >= 1; mb >>= 1)
-#pragma unroll
-      for (int v = 0; v < 8; v++)
-        if (!(v & mb)) {
-          const T a = w[c][v], b = w[c][v | mb];
-          w[c][v] = a + b;
-          w[c][v | mb] = b - a;
-        }
-  // mode coupling (mode 0 is a rigid translation: no force)
-  T z[3][8];
+template <typename T, int EP>
+__device__ __forceinline__ void epi_load(const OpArgs<T> &a, const T *__restrict__ Ds, size_t off,
+                                         int V, int n, EpiIn<T> &p) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    z[c][0] = 0;
-#pragma unroll
-    for (int sm = 1; sm < 8; sm++) {
-      T acc = m.diag[c][sm] * w[c][sm];
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        if (d == c) continue;
-        const int bc = (sm >> (2 - c)) & 1, bd = (sm >> (2 - d)) & 1;
-        if (bc == bd) continue;
-        const int t = sm ^ (1 << (2 - c)) ^ (1 << (2 - d));
-        acc += m.cross[c][d][sm] * w[d][t];
-      }
-      z[c][sm] = E * acc;
-    }
+    const size_t q = off + (size_t)c * V + n;
+    if (EP == EP_RES || EP == EP_CHEB0) p.e0[c] = a.b[q];
+    if (EP == EP_STEP) p.e0[c] = a.rprev[q];
+    if (EP == EP_CHEB0 || EP == EP_STEP) p.e1[c] = Ds[(size_t)c * V + n];
+    if (EP == EP_STEP) p.e2[c] = a.xprev ? a.xprev[q] : (T)0;
   }
-  // inverse: per axis (a, b) -> (a - b, a + b)
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int mb = 4; mb >= 1; mb >>= 1)
-#pragma unroll
-      for (int v = 0; v < 8; v++)
-        if (!(v & mb)) {
-          const T a = z[c][v], b = z[c][v | mb];
-          z[c][v] = a - b;
-          z[c][v | mb] = a + b;
-        }
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int v = 0; v < 8; v++) f[3 * v + c] = z[c][v];
 }
 
 // epilogue at one owned node (all three components)
 template <typename T, int EP>
-__device__ __forceinline__ void epilogue(const OpArgs<T> &a, const T *__restrict__ Ds, T ti, T c1,
-                                         T c2, size_t off, int V, int n, const T Au[3],
-                                         const T uo[3], double &dv) {
+__device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, T ti, T c1, T c2,
+                                         size_t off, int V, int n, const T Au[3], const T uo[3],
+                                         double &dv) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const size_t q = off + (size_t)c * V + n;
@@ -107,18 +66,18 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const T *__restrict
       a.y[q] = Au[c];
       if (a.dot) dv += (double)uo[c] * (double)Au[c];
     } else if (EP == EP_RES) {
-      a.y[q] = a.b[q] - Au[c];
+      a.y[q] = p.e0[c] - Au[c];
     } else if (EP == EP_CHEB0) {
-      const T rv = a.b[q] - Au[c];
+      const T rv = p.e0[c] - Au[c];
       a.r[q] = rv;
-      a.dout[q] = rv * ti * Ds[(size_t)c * V + n];
+      a.dout[q] = rv * ti * p.e1[c];
       a.x[q] = uo[c];
     } else {  // EP_STEP
-      const T rv = a.rprev[q] - Au[c];
+      const T rv = p.e0[c] - Au[c];
       a.r[q] = rv;
-      const T dn = c1 * uo[c] + c2 * rv * Ds[(size_t)c * V + n];
+      const T dn = c1 * uo[c] + c2 * rv * p.e1[c];
       a.dout[q] = dn;
-      T xv = (a.xprev ? a.xprev[q] : (T)0) + uo[c];
+      T xv = p.e2[c] + uo[c];
       if (a.last) xv += dn;
       a.x[q] = xv;
     }
@@ -196,7 +155,39 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
     }
   }
 
+  // staging of node plane ip into LDS buffer buf.  PLAIN / DSCALE are
+  // software-pipelined: fetch() issues the global loads of a plane into
+  // registers one slab ahead, put() scales and stores them.  PROLONG (one
+  // sample only) stages directly.
+  T pf[2][3], pd[2][3];
+  auto fetch = [&](int ip) {
+    const int iw = ((ip % g.nx) + g.nx) % g.nx;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (soff[q] < 0) continue;
+      const int n = iw * plane + soff[q];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        pf[q][c] = a.u[off + (size_t)c * V + n];
+        if (ST == ST_DSCALE) pd[q][c] = Ds[(size_t)c * V + n];
+      }
+    }
+  };
+  auto put = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (soff[q] < 0) continue;
+      const int t = tid + q * kBlock;
+#pragma unroll
+      for (int c = 0; c < 3; c++) up[buf][c][t] = (ST == ST_DSCALE) ? pf[q][c] * (ti * pd[q][c]) : pf[q][c];
+    }
+  };
   auto stage = [&](int buf, int ip) {
+    if (ST != ST_PROLONG) {
+      fetch(ip);
+      put(buf);
+      return;
+    }
     const int iw = ((ip % g.nx) + g.nx) % g.nx;
 #pragma unroll
     for (int q = 0; q < 2; q++) {
@@ -206,28 +197,21 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
       T v[3];
 #pragma unroll
       for (int c = 0; c < 3; c++) v[c] = a.u[off + (size_t)c * V + n];
-      if (ST == ST_DSCALE) {
+      const int ci0 = iw >> 1, ci1 = ((iw + 1) >> 1) % a.gc.nx;
+      const float wi = (iw & 1) ? 0.5f : 1.0f;
+      const int ni = (iw & 1) ? 2 : 1, nj = (sjw[q] < 1.0f) ? 2 : 1, nk = (skw[q] < 1.0f) ? 2 : 1;
+      const T w = (T)(wi * sjw[q] * skw[q]);
+      const size_t offc = (size_t)r * 3 * a.gc.V;
+      T sum[3] = {0, 0, 0};
+      for (int ai = 0; ai < ni; ai++)
+        for (int aj = 0; aj < nj; aj++)
+          for (int ak = 0; ak < nk; ak++) {
+            const int N = ((ai ? ci1 : ci0) * a.gc.ny + sjc[q][aj]) * a.gc.nz + skc[q][ak];
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[c] *= ti * Ds[(size_t)c * V + n];
-      } else if (ST == ST_PROLONG) {
-        const int ci0 = iw >> 1, ci1 = ((iw + 1) >> 1) % a.gc.nx;
-        const float wi = (iw & 1) ? 0.5f : 1.0f;
-        const int ni = (iw & 1) ? 2 : 1, nj = (sjw[q] < 1.0f) ? 2 : 1, nk = (skw[q] < 1.0f) ? 2 : 1;
-        const T w = (T)(wi * sjw[q] * skw[q]);
-        const size_t offc = (size_t)r * 3 * a.gc.V;
-        T s[3] = {0, 0, 0};
-        for (int ai = 0; ai < ni; ai++)
-          for (int aj = 0; aj < nj; aj++)
-            for (int ak = 0; ak < nk; ak++) {
-              const int N = ((ai ? ci1 : ci0) * a.gc.ny + sjc[q][aj]) * a.gc.nz + skc[q][ak];
+            for (int c = 0; c < 3; c++) sum[c] += a.xc[offc + (size_t)c * a.gc.V + N];
+          }
 #pragma unroll
-              for (int c = 0; c < 3; c++) s[c] += a.xc[offc + (size_t)c * a.gc.V + N];
-            }
-#pragma unroll
-        for (int c = 0; c < 3; c++) v[c] += w * s[c];
-      }
-#pragma unroll
-      for (int c = 0; c < 3; c++) up[buf][c][t] = v[c];
+      for (int c = 0; c < 3; c++) up[buf][c][t] = v[c] + w * sum[c];
     }
   };
 
@@ -239,11 +223,25 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
   double dv = 0;
 
   stage(0, i0 - 1);
+  if (ST != ST_PROLONG) fetch(i0);
   for (int sl = i0 - 1, p = 0; sl < iend; sl++, p++) {
     const int cur = p % 3, nxt = (p + 1) % 3;
-    stage(nxt, sl + 1);
-    __syncthreads();
     const int sw = ((sl % g.nx) + g.nx) % g.nx;
+    // loads in the order they are consumed: moduli of this slab, epilogue
+    // inputs of plane sl, then plane sl + 2 for the next slab
+    T Ev[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) Ev[q] = eoff[q] >= 0 ? E[sw * plane + eoff[q]] : (T)0;
+    const size_t nown = (size_t)sl * plane + gj * g.nz + gk;
+    EpiIn<T> pin;
+    if (EP != EP_AX && sl >= i0 && owns) epi_load<T, EP>(a, Ds, off, V, (int)nown, pin);
+    if (ST != ST_PROLONG) {
+      put(nxt);
+      if (sl + 2 <= iend) fetch(sl + 2);
+    } else {
+      stage(nxt, sl + 1);
+    }
+    __syncthreads();
     T keep[2][12];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
@@ -260,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
         uv[3 * v + 1] = up[bb][1][t];
         uv[3 * v + 2] = up[bb][2][t];
       }
-      hadamard_voxel<T>(uv, E[sw * plane + eoff[q]], mc, fv);
+      hadamard_voxel<T>(uv, Ev[q], mc, fv);
       T *dst = eo + e * kEOH;
 #pragma unroll
       for (int t = 0; t < 12; t++) {
@@ -286,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
         Au[c] = acc[c] + f[c];
         uo[c] = up[cur][c][ot];
       }
-      epilogue<T, EP>(a, Ds, ti, c1, c2, off, V, sl * plane + gj * g.nz + gk, Au, uo, dv);
+      epilogue<T, EP>(a, pin, ti, c1, c2, off, V, (int)nown, Au, uo, dv);
     }
     __syncthreads();
 #pragma unroll
@@ -357,56 +355,228 @@ __device__ __forceinline__ T stage_at(const OpArgs<T> &a, const T *__restrict__ 
   return v;
 }
 
-template <typename T, int ST, int EP>
+// Coarse-level node update for NR right-hand sides of one sample (load cases
+// lc0 .. lc0+NR-1, those set in `mask`): the 27 3x3 blocks of the node's
+// stencil row are read once and applied to every right-hand side.  With
+// `sym` the 13 lower-offset blocks are taken as the transposed upper blocks
+// of the neighbours, S(n, o)[c][d] = S(n + o, -o)[d][c] (the Galerkin
+// operator is symmetric), so only 14 of the 27 stored blocks are read; a
+// level with a dimension of 2 (where n + o = n - o) reads all 27.
+template <typename T, int ST, int EP, int NR>
 __device__ __forceinline__ void stencil_node(const Dims &g, const T *__restrict__ S, const OpArgs<T> &a,
-                                             int n, int r) {
+                                             int n, int smp, int lc0, unsigned mask, bool sym) {
   int i, j, k;
   node_ijk(g, n, i, j, k);
   const int I[3] = {i == 0 ? g.nx - 1 : i - 1, i, i == g.nx - 1 ? 0 : i + 1};
   const int J[3] = {j == 0 ? g.ny - 1 : j - 1, j, j == g.ny - 1 ? 0 : j + 1};
   const int K[3] = {k == 0 ? g.nz - 1 : k - 1, k, k == g.nz - 1 ? 0 : k + 1};
   const int V = g.V;
-  const int smp = r / kRHS;
   const T *__restrict__ Ss = S + (size_t)smp * 243 * V;
   const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * V : nullptr;
   const T ti = a.theta_inv[smp];
-  const size_t off = (size_t)r * 3 * V;
-  T acc[3] = {0, 0, 0};
+  const size_t off0 = (size_t)(smp * kRHS + lc0) * 3 * V;
+  T acc[NR][3];
+#pragma unroll
+  for (int t = 0; t < NR; t++) acc[t][0] = acc[t][1] = acc[t][2] = 0;
 #pragma unroll 3
   for (int o = 0; o < 27; o++) {
     const int nb = (I[o / 9] * g.ny + J[(o / 3) % 3]) * g.nz + K[o % 3];
     T sv[9];
+    if (sym && o < 13) {
 #pragma unroll
-    for (int q = 0; q < 9; q++) sv[q] = Ss[(size_t)(o * 9 + q) * V + n];
-    const T u0 = stage_at<T, ST>(a, Ds, ti, off, V, 0, nb), u1 = stage_at<T, ST>(a, Ds, ti, off, V, 1, nb),
-            u2 = stage_at<T, ST>(a, Ds, ti, off, V, 2, nb);
-    acc[0] += sv[0] * u0 + sv[1] * u1 + sv[2] * u2;
-    acc[1] += sv[3] * u0 + sv[4] * u1 + sv[5] * u2;
-    acc[2] += sv[6] * u0 + sv[7] * u1 + sv[8] * u2;
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) sv[3 * c + d] = Ss[(size_t)((26 - o) * 9 + 3 * d + c) * V + nb];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; q++) sv[q] = Ss[(size_t)(o * 9 + q) * V + n];
+    }
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+      if (!((mask >> t) & 1u)) continue;
+      const size_t off = off0 + (size_t)t * 3 * V;
+      const T u0 = stage_at<T, ST>(a, Ds, ti, off, V, 0, nb), u1 = stage_at<T, ST>(a, Ds, ti, off, V, 1, nb),
+              u2 = stage_at<T, ST>(a, Ds, ti, off, V, 2, nb);
+      acc[t][0] += sv[0] * u0 + sv[1] * u1 + sv[2] * u2;
+      acc[t][1] += sv[3] * u0 + sv[4] * u1 + sv[5] * u2;
+      acc[t][2] += sv[6] * u0 + sv[7] * u1 + sv[8] * u2;
+    }
   }
-  T uo[3];
 #pragma unroll
-  for (int c = 0; c < 3; c++) uo[c] = stage_at<T, ST>(a, Ds, ti, off, V, c, n);
-  double dv = 0;
-  epilogue<T, EP>(a, Ds, ti, a.c1[smp], a.c2[smp], off, V, n, acc, uo, dv);
+  for (int t = 0; t < NR; t++) {
+    if (!((mask >> t) & 1u)) continue;
+    const size_t off = off0 + (size_t)t * 3 * V;
+    T uo[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) uo[c] = stage_at<T, ST>(a, Ds, ti, off, V, c, n);
+    double dv = 0;
+    EpiIn<T> pin;
+    epi_load<T, EP>(a, Ds, off, V, n, pin);
+    epilogue<T, EP>(a, pin, ti, a.c1[smp], a.c2[smp], off, V, n, acc[t], uo, dv);
+  }
 }
 
-template <typename T, int ST, int EP>
-__global__ __launch_bounds__(kBlock) void k_stencil(Dims g, const T *__restrict__ S, OpArgs<T> a) {
-  const int r = blockIdx.y;  // one right-hand side per thread: on these small
-  if (!a.act[r]) return;     // levels parallelism matters more than reuse
+// NR = 1: one right-hand side per thread (blockIdx.y = RHS), for small levels
+// where parallelism matters more than reuse; NR = 6: all load cases of a
+// sample per thread (blockIdx.y = sample), reading the stencil once.
+template <typename T, int ST, int EP, int NR>
+__global__ __launch_bounds__(kBlock) void k_stencil(Dims g, const T *__restrict__ S, OpArgs<T> a,
+                                                    int sym) {
+  int smp, lc0;
+  unsigned mask = 0;
+  if (NR == 1) {
+    const int r = blockIdx.y;
+    smp = r / kRHS;
+    lc0 = r % kRHS;
+    mask = a.act[r] ? 1u : 0u;
+  } else {
+    smp = blockIdx.y;
+    lc0 = 0;
+#pragma unroll
+    for (int t = 0; t < NR; t++) mask |= a.act[smp * kRHS + t] ? (1u << t) : 0u;
+  }
+  if (!mask) return;
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock)
-    stencil_node<T, ST, EP>(g, S, a, n, r);
+    stencil_node<T, ST, EP, NR>(g, S, a, n, smp, lc0, mask, sym != 0);
+}
+
+// Coarse levels whose planes hold at least a 16x16 tile: LDS-tiled marching
+// form of the stencil kernel.  A workgroup owns a 16x16 (j,k) node tile of
+// one sample and marches over TI planes in i; node plane i+1 of NG load cases
+// (staged as stage_at does) goes with a one-node halo into a three-plane LDS
+// ring, so each staged value is read from memory about (TI+2)/TI * 324/256
+// times instead of once per neighbour, and each node's stencil row is
+// streamed once for the NG load cases.
+constexpr int kStTI = 2;  // planes per workgroup on the tiled coarse path
+
+template <typename T, int ST, int EP, int NG>
+__global__ __launch_bounds__(kBlock) void k_stencil_tiled(Dims g, const T *__restrict__ S, OpArgs<T> a,
+                                                          int tiles_k, int sym) {
+  constexpr int G = kRHS / NG;  // load-case groups per sample
+  const int smp = blockIdx.z / G, lc0 = (blockIdx.z % G) * NG;
+  unsigned mask = 0;
+#pragma unroll
+  for (int t = 0; t < NG; t++) mask |= a.act[smp * kRHS + lc0 + t] ? (1u << t) : 0u;
+  if (!mask) return;
+  __shared__ T us[3][NG][3][kTN * kTN];
+  const int tid = threadIdx.x;
+  const int j0 = (blockIdx.x / tiles_k) * kTile, k0 = (blockIdx.x % tiles_k) * kTile;
+  const int i0 = blockIdx.y * kStTI, iend = min(i0 + kStTI, g.nx);
+  const int V = g.V, plane = g.ny * g.nz;
+  const T *__restrict__ Ss = S + (size_t)smp * 243 * V;
+  const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * V : nullptr;
+  const T ti = a.theta_inv[smp];
+  const size_t off0 = (size_t)(smp * kRHS + lc0) * 3 * V;
+  int soff[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int t = tid + q * kBlock;
+    soff[q] = -1;
+    if (t < kTN * kTN) {
+      const int nj = t / kTN, nk = t - nj * kTN;
+      const int jj = (j0 - 1 + nj + g.ny) % g.ny, kk = (k0 - 1 + nk + g.nz) % g.nz;
+      soff[q] = jj * g.nz + kk;
+    }
+  }
+  auto stage = [&](int buf, int ip) {
+    const int iw = (ip % g.nx + g.nx) % g.nx;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (soff[q] < 0) continue;
+      const int t = tid + q * kBlock;
+      const int n = iw * plane + soff[q];
+#pragma unroll
+      for (int r = 0; r < NG; r++) {
+        if (!((mask >> r) & 1u)) continue;
+#pragma unroll
+        for (int c = 0; c < 3; c++) us[buf][r][c][t] = stage_at<T, ST>(a, Ds, ti, off0 + (size_t)r * 3 * V, V, c, n);
+      }
+    }
+  };
+  const int nj = tid / kTile, nk = tid % kTile;
+  const int gj = j0 + nj, gk = k0 + nk;
+  const bool owns = gj < g.ny && gk < g.nz;
+  const int Jn[3] = {(gj - 1 + g.ny) % g.ny, gj % g.ny, (gj + 1) % g.ny};
+  const int Kn[3] = {(gk - 1 + g.nz) % g.nz, gk % g.nz, (gk + 1) % g.nz};
+  stage(0, i0 - 1);
+  stage(1, i0);
+  for (int i = i0, p = 0; i < iend; i++, p++) {
+    const int bm = p % 3, b0 = (p + 1) % 3, bp = (p + 2) % 3;
+    stage(bp, i + 1);
+    __syncthreads();
+    if (owns) {
+      const int In[3] = {(i - 1 + g.nx) % g.nx, i, (i + 1) % g.nx};
+      const int n = (i * g.ny + gj) * g.nz + gk;
+      T acc[NG][3];
+#pragma unroll
+      for (int t = 0; t < NG; t++) acc[t][0] = acc[t][1] = acc[t][2] = 0;
+#pragma unroll 3
+      for (int o = 0; o < 27; o++) {
+        const int di = o / 9, dj = (o / 3) % 3, dk = o % 3;
+        T sv[9];
+        if (sym && o < 13) {
+          const int nb = (In[di] * g.ny + Jn[dj]) * g.nz + Kn[dk];
+#pragma unroll
+          for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) sv[3 * c + d] = Ss[(size_t)((26 - o) * 9 + 3 * d + c) * V + nb];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 9; q++) sv[q] = Ss[(size_t)(o * 9 + q) * V + n];
+        }
+        const int buf = di == 0 ? bm : (di == 1 ? b0 : bp);
+        const int slot = (nj + dj) * kTN + nk + dk;
+#pragma unroll
+        for (int t = 0; t < NG; t++) {
+          if (!((mask >> t) & 1u)) continue;
+          const T u0 = us[buf][t][0][slot], u1 = us[buf][t][1][slot], u2 = us[buf][t][2][slot];
+          acc[t][0] += sv[0] * u0 + sv[1] * u1 + sv[2] * u2;
+          acc[t][1] += sv[3] * u0 + sv[4] * u1 + sv[5] * u2;
+          acc[t][2] += sv[6] * u0 + sv[7] * u1 + sv[8] * u2;
+        }
+      }
+      const int own = (nj + 1) * kTN + nk + 1;
+#pragma unroll
+      for (int t = 0; t < NG; t++) {
+        if (!((mask >> t) & 1u)) continue;
+        const size_t off = off0 + (size_t)t * 3 * V;
+        T uo[3] = {us[b0][t][0][own], us[b0][t][1][own], us[b0][t][2][own]};
+        double dv = 0;
+        EpiIn<T> pin;
+        epi_load<T, EP>(a, Ds, off, V, n, pin);
+        epilogue<T, EP>(a, pin, ti, a.c1[smp], a.c2[smp], off, V, n, acc[t], uo, dv);
+      }
+    }
+    __syncthreads();
+  }
 }
 
 template <typename T>
 void launch_stencil(int st, int ep, const Dims &g, int R, const T *S, const OpArgs<T> &a,
                     hipStream_t s) {
   const int blocks = (g.V + kBlock - 1) / kBlock;
-#define GO(S_, E_)                                                                       \
-  if (st == S_ && ep == E_) {                                                            \
-    hipLaunchKernelGGL((k_stencil<T, S_, E_>), dim3(blocks, R), dim3(kBlock), 0, s, g, S, a); \
-    return;                                                                              \
+  const int nsmp = (R + kRHS - 1) / kRHS;
+  const int sym = (g.nx > 2 && g.ny > 2 && g.nz > 2) ? 1 : 0;
+  // planes of at least one full 16x16 tile: the LDS-tiled marching kernel
+  // (all load cases of a sample per workgroup in FP32, three in FP64)
+  constexpr int NG = sizeof(T) == 4 ? kRHS : kRHS / 2;
+  const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
+  const bool tiled = g.ny >= kTile && g.nz >= kTile && g.nx >= 3 && R % kRHS == 0;
+  const dim3 tgrid(tiles_j * tiles_k, (g.nx + kStTI - 1) / kStTI, nsmp * (kRHS / NG));
+  // all load cases per thread once that still leaves >= 2 workgroups per CU
+  const bool multi = (long)blocks * nsmp >= 512 && R % kRHS == 0;
+#define GO(S_, E_)                                                                               \
+  if (st == S_ && ep == E_) {                                                                    \
+    if (tiled)                                                                                   \
+      hipLaunchKernelGGL((k_stencil_tiled<T, S_, E_, NG>), tgrid, dim3(kBlock), 0, s, g, S, a,   \
+                         tiles_k, sym);                                                          \
+    else if (multi)                                                                              \
+      hipLaunchKernelGGL((k_stencil<T, S_, E_, kRHS>), dim3(blocks, nsmp), dim3(kBlock), 0, s, g, S, \
+                         a, sym);                                                                \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_stencil<T, S_, E_, 1>), dim3(blocks, R), dim3(kBlock), 0, s, g, S, a,  \
+                         sym);                                                                   \
+    return;                                                                                      \
   }
   GO(ST_PLAIN, EP_AX)
   GO(ST_PLAIN, EP_RES)

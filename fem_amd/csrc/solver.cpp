@@ -246,26 +246,31 @@ void Context::build_levels(Hierarchy<T> &h) {
       bf->alloc(field_bytes(L.g, B, sizeof(T)));
     if (l > 0) L.S.alloc(sizeof(T) * 243 * B * (size_t)L.g.V);
   }
+  if (dims.size() > 2) {
+    h.gt1.alloc(sizeof(T) * 243 * B * (size_t)dims[1].V / 2);
+    h.gt2.alloc(sizeof(T) * 243 * B * (size_t)dims[1].V / 4);
+  }
   h.E.alloc(sizeof(T) * B * g.V);
   h.Ke.alloc(sizeof(T) * 576);
 }
 
-void Context::estimate_lmax(Hierarchy<double> &h) {
+template <typename T>
+void Context::estimate_lmax(Hierarchy<T> &h) {
   // power iteration on D^-1 A, one vector per sample (RHS 6 s)
   std::vector<int> pw(R, 0);
   for (int s = 0; s < nb; s++) pw[kRHS * s] = 1;
   set_act(A_POW, pw);
   const int *ap = actp(A_POW);
   for (size_t l = 0; l < h.lv.size(); l++) {
-    Level<double> &L = h.lv[l];
+    Level<T> &L = h.lv[l];
     const int V = L.g.V;
-    double *xv = L.x.as<double>(), *yv = L.Ad.as<double>();
+    T *xv = L.x.template as<T>(), *yv = L.Ad.template as<T>();
     // every sample starts from the same vector (batched == one-sample results)
-    launch_fill_hash<double>((size_t)R * 3 * V, (size_t)kRHS * 3 * V, xv, 0x51ED27ull + l, stream);
+    launch_fill_hash<T>((size_t)R * 3 * V, (size_t)kRHS * 3 * V, xv, 0x51ED27ull + l, stream);
     for (int it = 0; it < 10; it++) {
-      apply<double>(h, (int)l, xv, yv, nullptr, 0, ap, nullptr, 0);
-      launch_scale_by_dinv_norm<double>(V, R, yv, L.Dinv.as<double>(), ap, red, stream);
-      launch_normalize<double>(V, R, yv, xv, ap, red, stream);
+      apply<T>(h, (int)l, xv, yv, nullptr, 0, ap, nullptr, 0);
+      launch_scale_by_dinv_norm<T>(V, R, yv, L.Dinv.template as<T>(), ap, red, stream);
+      launch_normalize<T>(V, R, yv, xv, ap, red, stream);
     }
     FEMH_HIP(hipGetLastError());
     std::vector<double> n2(R);
@@ -363,7 +368,7 @@ void Context::build_hierarchy() {
                                   stream);
     else
       launch_galerkin<double>(h64.lv[l - 1].g, L.g, nb, h64.lv[l - 1].S.as<double>(), L.S.as<double>(),
-                              stream);
+                              h64.gt1.as<double>(), h64.gt2.as<double>(), stream);
     StencilOp<double> so{L.g, L.S.as<double>()};
     launch_stencil_diag<double>(so, nb, L.D.as<double>(), stream);
   }
@@ -372,14 +377,12 @@ void Context::build_hierarchy() {
                           h64.lv[l].Dinv.as<double>(), stream);
   FEMH_HIP(hipGetLastError());
   if (o.precond == FEMH_PRECOND_MG) {
-    estimate_lmax(h64);
-    if (nlevels > 1) {
-      dense_coarse(h64);
-      if (h64.ndense == 0)
-        throw Error(FEMH_BAD_ARG, "coarsest multigrid level too large for the dense solve (" +
-                                      std::to_string(3 * h64.lv.back().g.V) +
-                                      " dofs); allow more levels or use a grid divisible by 2^k");
-    }
+    // the smoother's spectrum bound: from the operators that will smooth
+    // (the FP32 copies in mixed precision, estimated below)
+    if (o.precision != FEMH_MIXED) estimate_lmax(h64);
+    // the coarsest level is inverted densely when it is small enough, else
+    // solved approximately by a long Chebyshev polynomial (cheb_run)
+    if (nlevels > 1) dense_coarse(h64);
   }
   h64.ready = true;
   if (o.precision == FEMH_MIXED) {
@@ -393,7 +396,6 @@ void Context::build_hierarchy() {
       if (l > 0)
         launch_convert<double, float>((size_t)243 * nb * h64.lv[l].g.V, h64.lv[l].S.as<double>(),
                                       h32.lv[l].S.as<float>(), stream);
-      h32.lv[l].lmax = h64.lv[l].lmax;
     }
     if (h64.ndense) {
       h32.Ainv.alloc(sizeof(float) * (size_t)h64.ndense * h64.ndense * B);
@@ -402,46 +404,54 @@ void Context::build_hierarchy() {
       h32.ndense = h64.ndense;
     }
     FEMH_HIP(hipGetLastError());
+    if (o.precond == FEMH_PRECOND_MG) {
+      estimate_lmax(h32);
+      for (int l = 0; l < nlevels; l++) h64.lv[l].lmax = h32.lv[l].lmax;
+    }
     h32.ready = true;
   }
 }
 
 // ---------------------------------------------------------------------------
 // multigrid V-cycle with Chebyshev smoothing (fused kernels, see ops.hpp)
-static void cheb_coeffs(double lmax, double ratio, double *theta_inv, double *c1, double *c2) {
+// Chebyshev iteration for D^-1 A on [lmax/ratio, lmax]: 1/theta and the
+// coefficients (c1, c2) of step `step` (1-based) of the three-term recurrence
+// d_s = c1 d_{s-1} + c2 D^-1 r_s.
+static void cheb_step(double lmax, double ratio, int step, double *theta_inv, double *c1, double *c2) {
   const double hi = lmax, lo = lmax / ratio;
   const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
   *theta_inv = 1.0 / theta;
   double rho = 1.0 / sigma;
-  for (int s = 0; s < kMaxCheb; s++) {
+  *c1 = *c2 = 0.0;
+  for (int s = 1; s <= step; s++) {
     const double rn = 1.0 / (2.0 * sigma - rho);
-    c1[s] = rn * rho;
-    c2[s] = 2.0 * rn / delta;
+    *c1 = rn * rho;
+    *c2 = 2.0 * rn / delta;
     rho = rn;
   }
 }
 
 // per-sample 1/theta and the coefficients of Chebyshev step `step` (1-based)
 template <typename T>
-void Context::set_cheb(const Hierarchy<T> &h, int l, OpArgs<T> &a, int step) {
+void Context::set_cheb(const Hierarchy<T> &h, int l, OpArgs<T> &a, int step, double ratio) {
   for (int s = 0; s < nb; s++) {
-    double ti, c1[kMaxCheb], c2[kMaxCheb];
-    cheb_coeffs(h.lv[l].lmax[s], o.cheby_ratio, &ti, c1, c2);
+    double ti, c1, c2;
+    cheb_step(h.lv[l].lmax[s], ratio, step, &ti, &c1, &c2);
     a.theta_inv[s] = (T)ti;
     if (step >= 1) {
-      a.c1[s] = (T)c1[step - 1];
-      a.c2[s] = (T)c2[step - 1];
+      a.c1[s] = (T)c1;
+      a.c2[s] = (T)c2;
     }
   }
 }
 
+// k Chebyshev steps on level l from x = 0 (pre-smoothing, and the coarsest
+// solve when that level is too large for the dense inverse)
 template <typename T>
-void Context::pre_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_) {
+void Context::cheb_run(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_, int k, double ratio) {
   Level<T> &L = h.lv[l];
-  const int k = o.nu1;
   if (k == 1) {
-    // x = b / (theta D): per-sample theta folded by launching per sample is
-    // avoided by one fused DSCALE pass with c1 = c2 = 0
+    // x = b / (theta D) as one fused DSCALE pass with c1 = c2 = 0
     OpArgs<T> a;
     a.act = act_;
     a.u = bb;
@@ -450,7 +460,7 @@ void Context::pre_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *
     a.x = xx;
     a.dout = L.d.template as<T>();
     a.last = 0;
-    set_cheb(h, l, a, 0);
+    set_cheb(h, l, a, 0, ratio);
     op<T>(h, l, ST_DSCALE, EP_STEP, a);
     return;
   }
@@ -464,7 +474,7 @@ void Context::pre_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *
   a.x = xx;
   a.dout = dcur;
   a.last = (k == 2);
-  set_cheb(h, l, a, 1);
+  set_cheb(h, l, a, 1, ratio);
   op<T>(h, l, ST_DSCALE, EP_STEP, a);
   for (int s = 2; s < k; s++) {
     a.u = dcur;
@@ -472,10 +482,15 @@ void Context::pre_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *
     a.xprev = xx;
     a.dout = dnxt;
     a.last = (s == k - 1);
-    set_cheb(h, l, a, s);
+    set_cheb(h, l, a, s, ratio);
     op<T>(h, l, ST_PLAIN, EP_STEP, a);
     std::swap(dcur, dnxt);
   }
+}
+
+template <typename T>
+void Context::pre_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_) {
+  cheb_run<T>(h, l, bb, xx, act_, o.nu1, o.cheby_ratio);
 }
 
 template <typename T>
@@ -490,8 +505,11 @@ void Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int 
   a.r = L.r.template as<T>();
   a.dout = L.d2.template as<T>();
   a.x = L.xn.template as<T>();
-  set_cheb(h, l, a, 0);
-  if (l == 0) {
+  set_cheb(h, l, a, 0, o.cheby_ratio);
+  if (l == 0 && R <= kRHS) {
+    // one sample: staging x + P xc inside the operator saves a launch; with
+    // more right-hand sides the recomputed interpolation costs more than the
+    // separate pass
     a.xc = C.x.template as<T>();
     a.gc = C.g;
     op<T>(h, l, ST_PROLONG, EP_CHEB0, a);  // x3 = x + P xc staged, kept in xn
@@ -514,7 +532,7 @@ void Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int 
     s1.xprev = (s == 1) ? L.xn.template as<T>() : xx;
     s1.dout = dnxt;
     s1.last = (s == k - 1);
-    set_cheb(h, l, s1, s);
+    set_cheb(h, l, s1, s, o.cheby_ratio);
     op<T>(h, l, ST_PLAIN, EP_STEP, s1);
     std::swap(dcur, dnxt);
   }
@@ -524,7 +542,10 @@ template <typename T>
 void Context::vcycle(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_) {
   const int last = (int)h.lv.size() - 1;
   if (l == last) {
-    launch_dense_solve<T>(h.ndense, R, h.Ainv.template as<T>(), bb, xx, act_, stream);
+    if (h.ndense)
+      launch_dense_solve<T>(h.ndense, R, h.Ainv.template as<T>(), bb, xx, act_, stream);
+    else
+      cheb_run<T>(h, l, bb, xx, act_, std::max(2, o.coarse_iters), kCoarseChebRatio);
     return;
   }
   Level<T> &L = h.lv[l];
@@ -647,7 +668,7 @@ void Context::flush_profile() {
 
 void Context::compute_ch(const double *fields, double *C) {
   const int nblk = 1024;
-  launch_ch_energy(g, nb, E64.as<double>(), Ke64.as<double>(), uaff.as<double>(), fields,
+  launch_ch_energy(g, nb, E64.as<double>(), modecoef, uaff.as<double>(), fields,
                    ch_part.as<double>(), nblk, ch_out.as<double>(), stream);
   FEMH_HIP(hipGetLastError());
   std::vector<double> s(21 * nb);
@@ -847,7 +868,7 @@ void Context::gradient(const double *dJdC, double *out_host, bool energy_only) {
   const size_t n = (energy_only ? (size_t)21 : 1) * g.V * nb;
   DBuf out;
   out.alloc(sizeof(double) * n);
-  launch_gradient(g, nb, rho.as<double>(), Ke64.as<double>(), uaff.as<double>(), x.as<double>(),
+  launch_gradient(g, nb, rho.as<double>(), modecoef, uaff.as<double>(), x.as<double>(),
                   Gs.as<double>(), modulus_input ? FEMH_INTERP_MODULUS : o.interp, o.E0, o.Emin,
                   o.penal, energy_only ? 1 : 0,
                   out.as<double>(), stream);

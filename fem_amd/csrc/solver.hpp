@@ -70,9 +70,13 @@ struct Hierarchy {
   DBuf E;     // fine moduli [B][V] in T
   DBuf Ke;    // unit stiffness in T
   DBuf Ainv;  // per sample: dense (regularised) inverse of the coarsest operator
+  DBuf gt1, gt2;  // Galerkin intermediates (one- and two-axis coarsened stencils)
   int ndense = 0;
   bool ready = false;
 };
+
+// Chebyshev interval [lmax / ratio, lmax] of the iterative coarsest solve
+constexpr double kCoarseChebRatio = 1000.0;
 
 // activity-array slots (device int[kMaxR] each)
 enum ActSlot : int { A_ALL = 0, A_PCG = 1, A_OUT = 2, A_POW = 3, kActSlots = 4 };
@@ -156,9 +160,12 @@ struct Context {
            std::vector<int> &active, double tol, int max_iter, const double *bnorm2, int *iters,
            double *relres, bool z_ready = false);
   template <typename T>
-  void set_cheb(const Hierarchy<T> &h, int l, OpArgs<T> &a, int step);
+  void set_cheb(const Hierarchy<T> &h, int l, OpArgs<T> &a, int step, double ratio);
+  template <typename T>
+  void cheb_run(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_, int k, double ratio);
   int apply_parts() const;
-  void estimate_lmax(Hierarchy<double> &h);
+  template <typename T>
+  void estimate_lmax(Hierarchy<T> &h);
   void dense_coarse(Hierarchy<double> &h);
   void read_scalars(int slot, double *out, int n, int np);
   void compute_ch(const double *fields, double *C);

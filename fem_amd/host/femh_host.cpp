@@ -1,0 +1,481 @@
+// femh_host.cpp -- C++ host facade over the femh C ABI (see femh_host.hpp).
+#include "femh_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace femh {
+
+namespace {
+
+[[noreturn]] void fail(femh_ctx *ctx, int st, const char *what) {
+  char buf[512] = {0};
+  if (ctx) femh_last_error(ctx, buf, sizeof(buf));
+  throw HostError(st, std::string(what) + ": " + buf);
+}
+
+void check(femh_ctx *ctx, int st, const char *what) {
+  if (st != FEMH_OK) fail(ctx, st, what);
+}
+
+// 6x6 inverse by Gauss-Jordan with partial pivoting
+Mat6 inverse6(const Mat6 &A) {
+  double M[6][12];
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 12; j++) M[i][j] = j < 6 ? A(i, j) : (j - 6 == i ? 1.0 : 0.0);
+  for (int c = 0; c < 6; c++) {
+    int p = c;
+    for (int r = c + 1; r < 6; r++)
+      if (std::fabs(M[r][c]) > std::fabs(M[p][c])) p = r;
+    if (M[p][c] == 0.0) throw HostError(FEMH_BAD_ARG, "singular 6x6 matrix");
+    if (p != c)
+      for (int j = 0; j < 12; j++) std::swap(M[p][j], M[c][j]);
+    const double inv = 1.0 / M[c][c];
+    for (int j = 0; j < 12; j++) M[c][j] *= inv;
+    for (int r = 0; r < 6; r++) {
+      if (r == c) continue;
+      const double f = M[r][c];
+      for (int j = 0; j < 12; j++) M[r][j] -= f * M[c][j];
+    }
+  }
+  Mat6 out;
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) out(i, j) = M[i][j + 6];
+  return out;
+}
+
+}  // namespace
+
+void BaseMaterial::youngPoisson(double &E, double &nu) const {
+  E = mu * (3 * lambda + 2 * mu) / (lambda + mu);
+  nu = lambda / (2 * (lambda + mu));
+}
+
+VectorXd RealFun::df() { throw HostError(FEMH_BAD_ARG, "df() is not implemented by this function"); }
+
+// ---- fields ----------------------------------------------------------------
+void PiecewiseConstant3D::allocate(int nx, int ny, int nz) {
+  gridSize[0] = nx;
+  gridSize[1] = ny;
+  gridSize[2] = nz;
+  param.assign((size_t)nx * ny * nz, 0.0);
+}
+
+void PiecewiseConstant3D::gridIdx(const double x[3], int idx[3]) const {
+  for (int d = 0; d < 3; d++) {
+    int i = (int)(x[d] * gridSize[d]);
+    idx[d] = std::min(std::max(i, 0), gridSize[d] - 1);
+  }
+}
+
+void PiecewiseConstantSym3D::gridIdx(const double x[3], int idx[3]) const {
+  for (int d = 0; d < 3; d++) {
+    double c = 2 * x[d];
+    if (c > 1) c = 2 - c;
+    int i = (int)(c * gridSize[d]);
+    idx[d] = std::min(std::max(i, 0), gridSize[d] - 1);
+  }
+}
+
+void PiecewiseConstantCubic3D::gridIdx(const double x[3], int idx[3]) const {
+  double c[3];
+  for (int d = 0; d < 3; d++) {
+    c[d] = 2 * x[d];
+    if (c[d] > 1) c[d] = 2 - c[d];
+  }
+  // sort descending: the parameter lives on one of the 48 symmetric copies
+  if (c[0] < c[1]) std::swap(c[0], c[1]);
+  if (c[0] < c[2]) std::swap(c[0], c[2]);
+  if (c[1] < c[2]) std::swap(c[1], c[2]);
+  for (int d = 0; d < 3; d++) {
+    int i = (int)(c[d] * gridSize[d]);
+    idx[d] = std::min(std::max(i, 0), gridSize[d] - 1);
+  }
+}
+
+double PiecewiseConstant3D::f(const double x[3]) {
+  int idx[3];
+  gridIdx(x, idx);
+  return param[((size_t)idx[0] * gridSize[1] + idx[1]) * gridSize[2] + idx[2]];
+}
+
+std::vector<std::pair<int, double>> PiecewiseConstant3D::df(const double x[3]) {
+  int idx[3];
+  gridIdx(x, idx);
+  return {{(int)(((size_t)idx[0] * gridSize[1] + idx[1]) * gridSize[2] + idx[2]), 1.0}};
+}
+
+// ---- FEM3DFun --------------------------------------------------------------
+FEM3DFun::FEM3DFun() : gridSize(3, 0) { femh_default_opts(&solverOpts); }
+
+FEM3DFun::~FEM3DFun() {
+  if (ctx_) femh_destroy(ctx_);
+}
+
+void FEM3DFun::initArrays() {
+  G = Mat6{};
+  G0 = Mat6{};
+  wG = Mat6{};
+}
+
+void FEM3DFun::ensureContext() {
+  if (ctx_) return;
+  if (gridSize.size() != 3 || gridSize[0] < 2 || gridSize[1] < 2 || gridSize[2] < 2)
+    throw HostError(FEMH_BAD_ARG, "FEM3DFun: gridSize must hold three sizes >= 2");
+  femh_opts o = solverOpts;
+  o.nx = gridSize[0];
+  o.ny = gridSize[1];
+  o.nz = gridSize[2];
+  o.E0 = 1.0;
+  o.nu = nu;
+  o.interp = FEMH_INTERP_FASTFEM;
+  int st = femh_create(&o, &ctx_);
+  if (st != FEMH_OK) {
+    femh_ctx *c = ctx_;
+    ctx_ = nullptr;
+    char buf[512] = {0};
+    if (c) {
+      femh_last_error(c, buf, sizeof(buf));
+      femh_destroy(c);
+    }
+    throw HostError(st, std::string("femh_create: ") + buf);
+  }
+}
+
+void FEM3DFun::init(const VectorXd &x0) {
+  param = x0;
+  ensureContext();
+}
+
+void FEM3DFun::setParam(const VectorXd &x0) {
+  if (!field) throw HostError(FEMH_BAD_ARG, "FEM3DFun: no field");
+  ensureContext();
+  param = x0;
+  for (size_t i = 0; i < field->param.size() && i < x0.size(); i++) field->setParam((int)i, x0[i]);
+  const int nx = gridSize[0], ny = gridSize[1], nz = gridSize[2];
+  distribution.assign((size_t)nx * ny * nz, 0.0);
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        const double x[3] = {(i + 0.5) / nx, (j + 0.5) / ny, (k + 0.5) / nz};
+        const double v = field->f(x);
+        distribution[((size_t)i * ny + j) * nz + k] = std::min(std::max(v, 1e-3), 1.0);
+      }
+  Mat6 C;
+  check(ctx_, femh_homogenize(ctx_, distribution.data(), C.a.data(), &stats_), "femh_homogenize");
+  // the reference accumulates G(i, j) for j >= i only (fast_fem/FEM3DFun.cpp:165-175)
+  G = Mat6{};
+  for (int a = 0; a < 6; a++)
+    for (int b = a; b < 6; b++) G(a, b) = C(a, b);
+  density = 0;
+  for (double d : distribution) density += d;
+  density /= (double)distribution.size();
+}
+
+double FEM3DFun::f() {
+  double sum = 0;
+  for (int q = 0; q < 36; q++) {
+    const double d = G.a[q] - G0.a[q];
+    sum += wG.a[q] * d * d;
+  }
+  return 0.5 * sum + 0.5 * mw * (density - m0) * (density - m0);
+}
+
+VectorXd FEM3DFun::df() {
+  ensureContext();
+  const size_t nE = distribution.size();
+  // dJ/dG(i, j) for every (i, j), as fast_fem/FEM3DFun.cpp:236-249 sums them
+  double dJdC[36];
+  for (int q = 0; q < 36; q++) dJdC[q] = wG.a[q] * (G.a[q] - G0.a[q]);
+  VectorXd dfddist(nE);
+  check(ctx_, femh_gradient(ctx_, dJdC, dfddist.data()), "femh_gradient");
+  const double dm = (mw / (double)nE) * (density - m0);
+  for (double &v : dfddist) v += dm;
+  VectorXd grad(param.size(), 0.0);
+  const int nx = gridSize[0], ny = gridSize[1], nz = gridSize[2];
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        const double x[3] = {(i + 0.5) / nx, (j + 0.5) / ny, (k + 0.5) / nz};
+        const size_t e = ((size_t)i * ny + j) * nz + k;
+        for (const auto &iv : field->df(x)) grad[iv.first] += dfddist[e] * iv.second;
+      }
+  return grad;
+}
+
+VectorXd FEM3DFun::fields() {
+  ensureContext();
+  VectorXd u((size_t)18 * gridSize[0] * gridSize[1] * gridSize[2]);
+  check(ctx_, femh_get_fields(ctx_, u.data()), "femh_get_fields");
+  return u;
+}
+
+void FEM3DFun::log(std::ostream &out) {
+  for (int i = 0; i < 6; i++) {
+    for (int j = 0; j < 6; j++) out << G(i, j) << (j < 5 ? " " : "\n");
+  }
+}
+
+// ---- optimisation helpers (fast_fem/run3d.cpp:238-300) ---------------------
+double infNorm(const VectorXd &a) {
+  double m = 0;
+  for (double v : a) m = std::max(m, std::fabs(v));
+  return m;
+}
+
+void clampVector(VectorXd &x, const VectorXd &lb, const VectorXd &ub) {
+  for (size_t i = 0; i < x.size(); i++) {
+    if (i < lb.size()) x[i] = std::max(x[i], lb[i]);
+    if (i < ub.size()) x[i] = std::min(x[i], ub[i]);
+  }
+}
+
+int lineSearch(RealFun *fun, VectorXd &x0, const VectorXd &dir, double &h) {
+  const double f0 = fun->f();
+  for (int step = 0; step < 10; step++) {
+    VectorXd x(x0.size());
+    for (size_t i = 0; i < x.size(); i++) x[i] = x0[i] - h * dir[i];
+    clampVector(x, fun->lowerBounds, fun->upperBounds);
+    fun->setParam(x);
+    if (fun->f() < f0) {
+      x0 = x;
+      return 0;
+    }
+    h /= 2;
+  }
+  return -1;
+}
+
+void gradientDescent(RealFun *fun, VectorXd &x0, int nSteps, double maxStep, int logInterval,
+                     std::ostream *logfile) {
+  VectorXd x = x0;
+  fun->setParam(x);
+  for (int ii = 0; ii < nSteps; ii++) {
+    const VectorXd grad = fun->df();
+    const double norm = infNorm(grad);
+    if (!(norm > 0)) break;
+    double h = maxStep / norm;
+    if (lineSearch(fun, x, grad, h) < 0) break;
+    x0 = x;
+    if (logfile && logInterval > 0 && ii % logInterval == 0) fun->log(*logfile);
+  }
+  if (logfile) fun->log(*logfile);
+}
+
+double check_df(RealFun *fun, const VectorXd &x0, double h) {
+  fun->setParam(x0);
+  const VectorXd ana = fun->df();
+  double max_diff = 0, max_val = 0;
+  for (size_t i = 0; i < x0.size(); i++) {
+    VectorXd x = x0;
+    x[i] = x0[i] + h;
+    fun->setParam(x);
+    const double fp = fun->f();
+    x[i] = x0[i] - h;
+    fun->setParam(x);
+    const double fm = fun->f();
+    const double num = (fp - fm) / (2 * h);
+    max_diff = std::max(max_diff, std::fabs(num - ana[i]));
+    max_val = std::max(max_val, std::fabs(ana[i]));
+  }
+  fun->setParam(x0);
+  return max_val > 0 ? max_diff / max_val : max_diff;
+}
+
+// ---- coarsening --------------------------------------------------------------
+void repMaterialAssignment(int nx, int ny, int nz, const std::vector<int> &in, int repX, int repY, int repZ,
+                           int sub, std::vector<int> &out) {
+  const int NX = repX * nx * sub, NY = repY * ny * sub, NZ = repZ * nz * sub;
+  out.assign((size_t)NX * NY * NZ, -1);
+  for (int X = 0; X < NX; X++)
+    for (int Y = 0; Y < NY; Y++)
+      for (int Z = 0; Z < NZ; Z++) {
+        const int m = (X / sub) % nx, n = (Y / sub) % ny, p = (Z / sub) % nz;
+        out[((size_t)X * NY + Y) * NZ + Z] = in[((size_t)m * ny + n) * nz + p];
+      }
+}
+
+double computeMaterialDensity(const std::vector<int> &a, const std::vector<double> &dens) {
+  double r = 0;
+  for (int m : a) r += dens.at(m);
+  return a.empty() ? 0.0 : r / (double)a.size();
+}
+
+void cubicParameters(const Mat6 &C, double &E, double &nu, double &G) {
+  G = C(3, 3);
+  const double a = C(0, 1) / C(0, 0);
+  nu = a / (1 + a);
+  E = C(0, 1) * (1 + nu) * (1 - 2 * nu) / nu;
+}
+
+NumericalCoarseningUsingMultigrid::NumericalCoarseningUsingMultigrid() { femh_default_opts(&m_opts); }
+
+NumericalCoarseningUsingMultigrid::~NumericalCoarseningUsingMultigrid() {
+  if (m_ctx) femh_destroy(m_ctx);
+}
+
+void NumericalCoarseningUsingMultigrid::init(int N[3], int rep, int sub, const std::vector<BaseMaterial> &mats,
+                                             StructureType) {
+  if (mats.size() < 2) throw HostError(FEMH_BAD_ARG, "need two base materials (void, solid)");
+  m_nbBlockRep = rep;
+  m_nbSubdiv = sub;
+  m_baseMaterials = mats;
+  for (int d = 0; d < 3; d++) m_n[d] = N[d] * rep * sub;
+  if (m_ctx) femh_destroy(m_ctx);
+  m_ctx = nullptr;
+  // the solid (material 1) is the base material; voxels of material 0 get
+  // the coefficient m_densityMin (NumericalCoarseningUsingMultiGrid.cpp:186-193, :239)
+  double E, nu;
+  mats[1].youngPoisson(E, nu);
+  femh_opts o = m_opts;
+  o.nx = m_n[0];
+  o.ny = m_n[1];
+  o.nz = m_n[2];
+  o.E0 = E;
+  o.nu = nu;
+  o.interp = FEMH_INTERP_MODULUS;
+  const int st = femh_create(&o, &m_ctx);
+  if (st != FEMH_OK) {
+    char buf[512] = {0};
+    if (m_ctx) {
+      femh_last_error(m_ctx, buf, sizeof(buf));
+      femh_destroy(m_ctx);
+    }
+    m_ctx = nullptr;
+    throw HostError(st, std::string("femh_create: ") + buf);
+  }
+}
+
+void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParameters3D(
+    const std::vector<int> &iMaterials, int N[3], const std::vector<double> &dens, StructureType iType,
+    std::vector<double> &ioTensorValues, std::vector<double> &ioParameters) {
+  if (!m_ctx) throw HostError(FEMH_BAD_ARG, "init() was not called");
+  std::vector<int> cells;
+  repMaterialAssignment(N[0], N[1], N[2], iMaterials, m_nbBlockRep, m_nbBlockRep, m_nbBlockRep, m_nbSubdiv,
+                        cells);
+  if ((int)cells.size() != m_n[0] * m_n[1] * m_n[2])
+    throw HostError(FEMH_BAD_ARG, "material grid does not match init()");
+  double E0, nu;
+  m_baseMaterials[1].youngPoisson(E0, nu);
+  std::vector<double> Emod(cells.size());
+  for (size_t e = 0; e < cells.size(); e++)
+    Emod[e] = E0 * (m_useVariableDensity ? (cells[e] == 1 ? 1.0 : m_densityMin) : (cells[e] == 1 ? 1.0 : 0.0));
+  femh_stats st;
+  check(m_ctx, femh_homogenize_modulus(m_ctx, Emod.data(), m_C.a.data(), &st), "femh_homogenize_modulus");
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j <= i; j++) ioTensorValues.push_back(m_referenceTriangle && j < i ? 0.0 : m_C(i, j));
+  if (iType == Cubic) {
+    ioParameters.push_back(computeMaterialDensity(iMaterials, dens));
+    double E, nuc, G;
+    cubicParameters(m_C, E, nuc, G);
+    ioParameters.push_back(E);
+    ioParameters.push_back(nuc);
+    ioParameters.push_back(G);
+  }
+}
+
+// ---- MaterialOptimizer (MaterialOptimizer.cpp:530-592, 621-652) ---------------
+Mat6 MaterialOptimizer::computeTargetStrains(int idim, float E, float nu, float Gs) {
+  if (idim != 3) throw HostError(FEMH_BAD_ARG, "only the 3D target is provided");
+  Mat6 S;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) S(i, j) = (i == j ? 1.0 : -(double)nu) / (double)E;
+  for (int i = 3; i < 6; i++) S(i, i) = 1.0 / (double)Gs;
+  return S;
+}
+
+int MaterialOptimizer::lineSearch(RealFun *fun, VectorXd &x0, const VectorXd &dir, double &h) {
+  const double f0 = fun->f();
+  double f1 = f0;
+  while (h > 1e-8) {
+    VectorXd x(x0.size());
+    for (size_t i = 0; i < x.size(); i++) x[i] = x0[i] - h * dir[i];
+    clampVector(x, fun->lowerBounds, fun->upperBounds);
+    fun->setParam(x);
+    f1 = fun->f();
+    if (f1 < f0) {
+      x0 = x;
+      h *= 2;
+      break;
+    }
+    h /= 2;
+  }
+  return std::fabs(f0 - f1) < 1e-8 ? -1 : 0;
+}
+
+void MaterialOptimizer::gradientDescent(FEM3DFun *fun, VectorXd &x0, int nSteps, std::vector<VectorXd> &oParameters) {
+  const double maxStep = 0.5;
+  VectorXd x = x0;
+  double h = 1;
+  fun->setParam(x);
+  for (int ii = 0; ii < nSteps; ii++) {
+    const VectorXd grad = fun->df();
+    const double norm = infNorm(grad);
+    if (!(norm > 0)) break;
+    if (ii == 0) h = maxStep / norm;
+    if (lineSearch(fun, x, grad, h) < 0) break;
+    x0 = x;
+    oParameters.push_back(fun->distribution);
+  }
+}
+
+bool MaterialOptimizer::run3D(int N[3], const std::vector<BaseMaterial> &mats, const std::vector<int> &assign,
+                              const std::vector<float> &current, const std::vector<float> &target,
+                              std::vector<std::vector<int>> &out, int nSteps) {
+  (void)current;
+  if (target.size() < 4) throw HostError(FEMH_BAD_ARG, "target parameters are (density, E, nu, G)");
+  const int nx = N[0], ny = N[1], nz = N[2];
+  if ((int)assign.size() != nx * ny * nz) throw HostError(FEMH_BAD_ARG, "assignment size");
+  FEM3DFun fem;
+  if (m_haveOpts) fem.solverOpts = m_opts;
+  double E, nu;
+  mats.at(1).youngPoisson(E, nu);
+  fem.nu = nu;
+  PiecewiseConstant3D plain;
+  PiecewiseConstantSym3D sym;
+  PiecewiseConstantCubic3D cubic;
+  RealField *field = &plain;
+  PiecewiseConstant3D *pc = &plain;
+  if (m_structureType == Cubic) pc = &cubic;
+  if (m_structureType == Orthotropic) pc = &sym;
+  field = pc;
+  // symmetric fields carry one octant (nx/2 per axis); the plain field the full grid
+  const bool half = m_structureType != General;
+  const int px = half ? nx / 2 : nx, py = half ? ny / 2 : ny, pz = half ? nz / 2 : nz;
+  pc->allocate(px, py, pz);
+  fem.field = field;
+  fem.gridSize = {nx, ny, nz};
+  fem.lowerBounds.assign(field->param.size(), 1e-3);
+  fem.upperBounds.assign(field->param.size(), 1.0);
+  // start from the shrunk 0/1 assignment (the octant for symmetric fields)
+  VectorXd x0(field->param.size());
+  for (int i = 0; i < px; i++)
+    for (int j = 0; j < py; j++)
+      for (int k = 0; k < pz; k++) {
+        const int m = assign[((size_t)i * ny + j) * nz + k] == 0 ? 0 : 1;
+        x0[((size_t)i * py + j) * pz + k] = 0.5 + (m - 0.5) * 0.3;
+      }
+  fem.initArrays();
+  fem.init(x0);
+  fem.setParam(x0);
+  // target: the cubic stiffness of (E, nu, G), matched on C00, C01, C33
+  const Mat6 Ct = inverse6(computeTargetStrains(3, target[1], target[2], target[3]));
+  fem.G0 = fem.G;
+  fem.G0(0, 0) = Ct(0, 0);
+  fem.G0(0, 1) = Ct(0, 1);
+  fem.G0(3, 3) = Ct(3, 3);
+  fem.wG(0, 0) = fem.wG(0, 1) = fem.wG(3, 3) = 1.0;
+  fem.m0 = target[0];
+  fem.mw = 0.1 * fem.G(0, 0) / std::max(fem.density, 1e-12);
+  std::vector<VectorXd> dists;
+  gradientDescent(&fem, x0, nSteps, dists);
+  for (const VectorXd &d : dists) {
+    std::vector<int> a(d.size());
+    for (size_t e = 0; e < d.size(); e++) a[e] = d[e] >= 0.5 ? 1 : 0;
+    out.push_back(a);
+  }
+  return !dists.empty();
+}
+
+}  // namespace femh

@@ -1,0 +1,237 @@
+// femh_host.hpp -- C++ host facade over the femh C ABI (include/femh.h) that
+// keeps desaic/fem's class names and call shapes for the homogenization path,
+// so that callers written against the reference switch by changing includes
+// and the vector type (std::vector<double> instead of Eigen::VectorXd):
+//
+//   RealFun                          MaterialOpt/include/RealFun.hpp:13-52
+//   RealField, PiecewiseConstant3D,  MaterialOpt/src/PiecewiseConstant3D.cpp,
+//   PiecewiseConstantSym3D,          PiecewiseConstantSym3D.cpp,
+//   PiecewiseConstantCubic3D         PiecewiseConstantCubic3D.cpp
+//   FEM3DFun : RealFun               fast_fem/FEM3DFun.hpp:13-92,
+//                                    fast_fem/FEM3DFun.cpp:128-268
+//   lineSearch, gradientDescent,     fast_fem/run3d.cpp:238-300
+//   check_df
+//   NumericalCoarseningUsingMultigrid SamplesGenerator/NumericalCoarseningUsingMultigrid.h:11-48,
+//                                    NumericalCoarseningUsingMultiGrid.cpp:134-340
+//   NumericalCoarsening              SamplesGenerator/NumericalCoarsening.h:13-86 (3D entry)
+//   MaterialOptimizer                SamplesGenerator/MaterialOptimizer.h:13-60,
+//                                    MaterialOptimizer.cpp:530-592
+//
+// Differences from the reference, on purpose:
+//   * errors raise femh::HostError (with the library's status and message)
+//     instead of asserts / printf;
+//   * the solve is the engine's strain-driven periodic homogenization (GPU
+//     multigrid-PCG) for both coarsening classes;
+//   * NumericalCoarseningUsingMultigrid writes the full lower triangle of the
+//     symmetric C_H; the reference fills C(i, j) for j >= i and then emits
+//     j <= i, i.e. only the diagonal (NumericalCoarseningUsingMultiGrid.cpp:
+//     297-325).  setEmitReferenceTriangle(true) reproduces that output.
+#pragma once
+#include <array>
+#include <ostream>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/femh.h"
+
+namespace femh {
+
+struct HostError : std::runtime_error {
+  int status;
+  HostError(int st, const std::string &m) : std::runtime_error(m), status(st) {}
+};
+
+using VectorXd = std::vector<double>;
+
+// 6x6 row-major matrix (Voigt xx, yy, zz, yz, xz, xy)
+struct Mat6 {
+  std::array<double, 36> a{};
+  double &operator()(int i, int j) { return a[6 * i + j]; }
+  double operator()(int i, int j) const { return a[6 * i + j]; }
+};
+
+// One base material of the reference's MaterialQuad list, by its Lame
+// parameters (MaterialQuad::e[0]->param = {mu, lambda}).
+struct BaseMaterial {
+  double mu, lambda;
+  void youngPoisson(double &E, double &nu) const;  // cfgMaterialUtilities.cpp:2935-2944
+};
+
+// ---- optimisation interface ---------------------------------------------
+class RealFun {
+ public:
+  VectorXd param;
+  // may be empty: unbounded
+  VectorXd lowerBounds, upperBounds;
+  virtual void init(const VectorXd &x0) { param = x0; }
+  virtual void setParam(const VectorXd &x0) { param = x0; }
+  virtual double f() = 0;
+  virtual VectorXd df();
+  virtual void log(std::ostream &) {}
+  virtual ~RealFun() {}
+};
+
+// material distribution over the unit cube from a parameter vector; df
+// returns the (index, weight) entries of d f(x) / d param
+class RealField {
+ public:
+  VectorXd param;
+  virtual double f(const double x[3]) = 0;
+  virtual std::vector<std::pair<int, double>> df(const double x[3]) = 0;
+  void setParam(int i, double v) { param[i] = v; }
+  virtual ~RealField() {}
+};
+
+class PiecewiseConstant3D : public RealField {
+ public:
+  int gridSize[3] = {0, 0, 0};
+  void allocate(int nx, int ny, int nz);
+  double f(const double x[3]) override;
+  std::vector<std::pair<int, double>> df(const double x[3]) override;
+
+ protected:
+  virtual void gridIdx(const double x[3], int idx[3]) const;
+};
+
+// mirror-symmetric in each axis about the cell centre
+class PiecewiseConstantSym3D : public PiecewiseConstant3D {
+ protected:
+  void gridIdx(const double x[3], int idx[3]) const override;
+};
+
+// mirror-symmetric and invariant under axis permutations
+class PiecewiseConstantCubic3D : public PiecewiseConstant3D {
+ protected:
+  void gridIdx(const double x[3], int idx[3]) const override;
+};
+
+// Objective 0.5 sum wG (G - G0)^2 + 0.5 mw (density - m0)^2 over the
+// homogenized tensor G of the distribution clamp(field(x), 1e-3, 1), with the
+// fast_fem interpolation E = rho / (3 - 2 rho) of a unit base material.
+class FEM3DFun : public RealFun {
+ public:
+  FEM3DFun();
+  ~FEM3DFun() override;
+  FEM3DFun(const FEM3DFun &) = delete;
+  FEM3DFun &operator=(const FEM3DFun &) = delete;
+
+  // G(i, j) for j >= i (the upper triangle the reference computes), 0 below
+  Mat6 G, G0, wG;
+  double density = 0, m0 = 0, mw = 0;
+  std::vector<int> gridSize;  // voxels per axis
+  double nu = 0.45;           // base material (E = 1), as fast_fem/run3d.cpp:156-157
+  femh_opts solverOpts;       // knobs of the engine (grid/material fields are set here)
+  RealField *field = nullptr;  // not owned
+  VectorXd distribution;
+
+  void initArrays();
+  void init(const VectorXd &x0) override;
+  void setParam(const VectorXd &x0) override;
+  double f() override;
+  VectorXd df() override;
+  void log(std::ostream &out) override;
+  // periodic fluctuation fields of the last setParam: [6][3][V]
+  VectorXd fields();
+  const femh_stats &lastStats() const { return stats_; }
+
+ private:
+  femh_ctx *ctx_ = nullptr;
+  femh_stats stats_{};
+  void ensureContext();
+};
+
+int lineSearch(RealFun *fun, VectorXd &x0, const VectorXd &dir, double &h);
+void gradientDescent(RealFun *fun, VectorXd &x0, int nSteps, double maxStep, int logInterval,
+                     std::ostream *logfile = nullptr);
+// max |analytic - central difference| / max |analytic|
+double check_df(RealFun *fun, const VectorXd &x0, double h);
+double infNorm(const VectorXd &a);
+void clampVector(VectorXd &x, const VectorXd &lb, const VectorXd &ub);
+
+// ---- coarsening ------------------------------------------------------------
+class NumericalCoarseningUsingMultigrid {
+ public:
+  enum StructureType { General, Cubic, Orthotropic };
+  NumericalCoarseningUsingMultigrid();
+  ~NumericalCoarseningUsingMultigrid();
+  NumericalCoarseningUsingMultigrid(const NumericalCoarseningUsingMultigrid &) = delete;
+  NumericalCoarseningUsingMultigrid &operator=(const NumericalCoarseningUsingMultigrid &) = delete;
+
+  void setUseVariableDensity(bool v) { m_useVariableDensity = v; }
+  void setEmitReferenceTriangle(bool v) { m_referenceTriangle = v; }
+  void setSolverOptions(const femh_opts &o) { m_opts = o; }
+
+  void init(int N[3], int iNbBlockRep, int iNbSubdiv, const std::vector<BaseMaterial> &iBaseMaterials,
+            StructureType iType);
+  // appends 21 values (row i, j <= i) to ioTensorValues and, for Cubic,
+  // (density, E, nu, G) to ioParameters
+  void computeCoarsenedElasticityTensorAndParameters3D(const std::vector<int> &iMaterials, int N[3],
+                                                       const std::vector<double> &iBaseMaterialDensities,
+                                                       StructureType iType,
+                                                       std::vector<double> &ioTensorValues,
+                                                       std::vector<double> &ioParameters);
+  // the full tensor of the last call
+  const Mat6 &lastTensor() const { return m_C; }
+
+ protected:
+  femh_ctx *m_ctx = nullptr;
+  femh_opts m_opts;
+  int m_n[3] = {0, 0, 0};
+  int m_nbBlockRep = 1, m_nbSubdiv = 1;
+  std::vector<BaseMaterial> m_baseMaterials;
+  bool m_useVariableDensity = true;
+  bool m_referenceTriangle = false;
+  double m_densityMin = 1e-4;
+  Mat6 m_C;
+};
+
+// Same entry point as the reference's PARDISO-based coarsening; it runs the
+// engine's periodic strain-driven solve (no face tractions, no Lagrange rows).
+class NumericalCoarsening : public NumericalCoarseningUsingMultigrid {
+ public:
+  void init(int N[3], int iNbBlockRep, int iNbSubdiv, const std::vector<BaseMaterial> &iBaseMaterials) {
+    NumericalCoarseningUsingMultigrid::init(N, iNbBlockRep, iNbSubdiv, iBaseMaterials, General);
+  }
+};
+
+// replicate a coarse assignment rep times per axis with each cell split into
+// subdiv^3 (cfgMaterialUtilities.cpp:920-960)
+void repMaterialAssignment(int nx, int ny, int nz, const std::vector<int> &iMaterialCombIndices, int repX,
+                           int repY, int repZ, int iNbSubdivisions, std::vector<int> &oMaterials);
+double computeMaterialDensity(const std::vector<int> &iMaterialAssignment,
+                              const std::vector<double> &iBaseMaterialDensities);
+// cubic (E, nu, G) from C (NumericalCoarseningUsingMultiGrid.cpp:327-340)
+void cubicParameters(const Mat6 &C, double &E, double &nu, double &G);
+
+// ---- optimiser ---------------------------------------------------------------
+class MaterialOptimizer {
+ public:
+  enum StructureType { General, Cubic, Orthotropic };
+  MaterialOptimizer() = default;
+  void setStructureType(StructureType t) { m_structureType = t; }
+  void setSolverOptions(const femh_opts &o) { m_opts = o; m_haveOpts = true; }
+
+  // Projected gradient descent towards the cubic tensor of (E, nu, G) =
+  // iTargetParams[1..3] from the 0/1 assignment iMaterialAssignments (N^3);
+  // appends the thresholded assignment after every accepted step.
+  bool run3D(int N[3], const std::vector<BaseMaterial> &iBaseMaterials,
+             const std::vector<int> &iMaterialAssignments, const std::vector<float> &iCurrentParams,
+             const std::vector<float> &iTargetParams, std::vector<std::vector<int>> &oNewMaterialAssignments,
+             int nSteps = 30);
+
+  int lineSearch(RealFun *fun, VectorXd &x0, const VectorXd &dir, double &h);
+  void gradientDescent(FEM3DFun *fun, VectorXd &x0, int nSteps, std::vector<VectorXd> &oParameters);
+  double check_df(RealFun *fun, const VectorXd &x0, double h) { return femh::check_df(fun, x0, h); }
+  double infNorm(const VectorXd &a) { return femh::infNorm(a); }
+  // isotropic/cubic stiffness in Voigt order with shear modulus G
+  static Mat6 computeTargetStrains(int idim, float iYoungModulus, float iPoissonRatio, float iShearModulus);
+
+ private:
+  StructureType m_structureType = General;
+  femh_opts m_opts{};
+  bool m_haveOpts = false;
+};
+
+}  // namespace femh

@@ -78,7 +78,8 @@ typedef struct femh_opts {
   int levels;           /* multigrid levels incl. fine (0 = as deep as the grid allows) */
   int nu1, nu2;         /* Chebyshev smoothing degree before / after coarse correction */
   double cheby_ratio;   /* smoothing interval [lmax/ratio, lmax] of D^-1 A */
-  int coarse_iters;     /* inner Jacobi-PCG iterations when the coarsest level is large */
+  int coarse_iters;     /* Chebyshev steps of the coarsest solve when that level is too
+                           large for its dense inverse (> 648 dofs) */
   double tol;           /* ||r||/||b|| target (FP64) / outer refinement target (mixed) */
   int max_iter;         /* outer iteration cap per load case */
   int precision;        /* femh_precision */

@@ -53,3 +53,16 @@ def test_host_element_table_is_symmetric_psd():
     assert np.abs(K - K.T).max() < 1e-15
     assert np.linalg.eigvalsh(K).min() > -1e-13
     assert femh.lib().femh_element_stiffness(1.0, 0.7, K.ctypes.data) == femh.BAD_ARG
+
+
+def test_host_facade_library_exports_reference_classes():
+    """The C++ facade is built and exports the reference's class names."""
+    import subprocess
+    lib = os.path.join(ROOT, "fem_amd", "libfemh_host.so")
+    assert os.path.exists(lib)
+    syms = subprocess.run(["nm", "-DC", "--defined-only", lib], capture_output=True, text=True).stdout
+    for name in ("femh::FEM3DFun::setParam", "femh::FEM3DFun::df",
+                 "femh::NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParameters3D",
+                 "femh::MaterialOptimizer::run3D", "femh::gradientDescent", "femh::lineSearch",
+                 "femh::check_df", "femh::PiecewiseConstantCubic3D"):
+        assert name in syms, name

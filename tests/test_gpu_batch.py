@@ -33,6 +33,20 @@ def test_fp64_batch_equals_single_sample_solves(batch):
         assert stb[s]["iters"] == st1["iters"]
 
 
+def test_fp64_batch_64_uses_multi_rhs_coarse_kernels():
+    """At 64^3 with 4 samples the 32^3 level runs all load cases of a sample
+    per thread; the result must equal the one-sample solves (one RHS per
+    thread there) exactly up to rounding."""
+    rho = simp_set(4, 64, seed0=7, corr=4.0)
+    hb = Homogenizer((64, 64, 64), nu=NU, tol=1e-10, batch=4)
+    Cb, stb = hb.homogenize_many(rho)
+    h1 = Homogenizer((64, 64, 64), nu=NU, tol=1e-10)
+    for s in range(4):
+        C1, st1 = h1.homogenize(rho[s])
+        assert rel(Cb[s], C1) <= 1e-13
+        assert stb[s]["iters"] == st1["iters"]
+
+
 def test_batch_against_oracle():
     rho = simp_set(3, 12, seed0=40)
     h = Homogenizer((12, 12, 12), nu=NU, tol=1e-12, batch=3)

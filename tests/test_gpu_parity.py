@@ -25,8 +25,11 @@ def test_fine_apply_matches_oracle(shape):
     assert np.abs(y - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
 
 
-@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 8, 4)])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 8, 4), (32, 32, 32), (32, 48, 40)])
 def test_coarse_operators_match_oracle_galerkin(shape):
+    """Every coarse level's operator against the oracle's Galerkin hierarchy;
+    levels with 16x16 planes or more run the LDS-tiled kernel (full and
+    partial tiles)."""
     E = W.binary_modulus(W.binary_cube(shape, 3))
     h = Homogenizer(shape, nu=NU, precond="mg")
     h.prepare(E)

@@ -40,6 +40,21 @@ def test_simp64_multigrid_fp64_matches_golden(kernel):
     assert max(st["iters"]) <= 40
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_simp64_three_level_vcycle_matches_golden(precision):
+    """64^3 SIMP cell with a 3-level hierarchy (64, 32, 16): the 16^3
+    coarsest level is too large for the dense inverse and is solved by a
+    Chebyshev polynomial; V(2,2)-PCG to 1e-10."""
+    g = GOLD["simp64_s0"]
+    rho = W.simp_density(64, 0, 4.0)
+    h = Homogenizer((64, 64, 64), nu=NU, penal=3.0, tol=1e-10, levels=3, nu1=2, nu2=2,
+                    precision=precision, ch_tol=0.0)
+    C, st = h.homogenize(rho)
+    assert st["levels"] == 3
+    assert rel(C, g["C"]) <= 1e-10, st
+    assert max(st["iters"]) <= 200
+
+
 def test_tiled_and_dense_apply_agree_on_odd_grid():
     shape = (5, 7, 9)
     E = np.random.default_rng(3).uniform(1e-3, 1.0, shape)
