@@ -41,6 +41,15 @@ def parse():
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solves")
+    ap.add_argument("--nu1", type=int, default=2)
+    ap.add_argument("--nu2", type=int, default=2)
+    ap.add_argument("--workload", default="simp64", choices=["simp64", "uneven512", "opt96"],
+                    help="simp64: the headline 64^3 line; uneven512: 512 binary cells of "
+                         "16..48^3; opt96: 30 projected-gradient steps on 64 cells of 96^3")
+    ap.add_argument("--cells", type=int, default=0, help="cells of uneven512 / opt96 (0: 512 / 64)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="first run one 256 MiB device-to-device copy (known traffic) so that "
+                         "counter passes can scale their readings")
     return ap.parse_args()
 
 
@@ -74,6 +83,17 @@ def main():
     from fem_amd import Homogenizer
     from fem_amd import workloads as W
 
+    if a.copy_calib:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # 256 MiB read + 256 MiB written
+        torch.cuda.synchronize()
+        del src, dst
+    if a.workload == "uneven512":
+        return run_uneven(a)
+    if a.workload == "opt96":
+        return run_opt(a)
     n = a.n
     B = a.batch
     shape = (n, n, n)
@@ -87,7 +107,7 @@ def main():
     V = n ** 3
 
     mixed = Homogenizer(shape, nu=0.3, penal=3.0, precision="mixed", tol=1e-10,
-                        ch_tol=a.ch_tol, inner_tol=a.inner_tol, batch=B)
+                        ch_tol=a.ch_tol, inner_tol=a.inner_tol, batch=B, nu1=a.nu1, nu2=a.nu2)
 
     def step(k, stats=None):
         st = mixed.homogenize_many_device(B, d_rho[k * B].data_ptr(), d_C[k * B].data_ptr())
@@ -172,6 +192,100 @@ def main():
                        "GBps": k64["bytes"] / (k64["ms"] * 1e-3) / 1e9 if k64["ms"] > 0 else 0.0},
     }
     print(json.dumps(out))
+
+
+def run_uneven(a):
+    """512 binary cells (edge U{16,24,32,40,48}, fill U[0.2,0.8], E_void 1e-9),
+    mixed precision through femh_homogenize_batch (grouped by size, `batch`
+    cells per launch sequence), each checked against FP64."""
+    import numpy as np
+    from fem_amd import Homogenizer, homogenize_batch
+    from fem_amd import workloads as W
+    count = a.cells or 512
+    grids = W.batch_uneven(count, 3)
+    kw = dict(nu=0.3, precision="mixed", tol=1e-10, ch_tol=a.ch_tol, inner_tol=a.inner_tol,
+              max_iter=20000, batch=a.batch)
+    homogenize_batch(grids[:8], **kw)  # warm-up: builds and loads
+    t0 = time.perf_counter()
+    C, st = homogenize_batch(grids, **kw)
+    elapsed = time.perf_counter() - t0
+    errs = []
+    t64 = 0.0
+    if not a.no_fp64:
+        by = {}
+        for i, g in enumerate(grids):
+            by.setdefault(g.shape, []).append(i)
+        for shp, ids in by.items():
+            h = Homogenizer(shp, nu=0.3, tol=1e-10, max_iter=20000, batch=8, interp="modulus")
+            t = time.perf_counter()
+            C64, _ = h.homogenize_many(np.stack([grids[i] for i in ids]), modulus=True)
+            t64 += time.perf_counter() - t
+            for k, i in enumerate(ids):
+                errs.append(float(np.linalg.norm(C[i] - C64[k]) / np.linalg.norm(C64[k])))
+            h.close()
+    vox = sum(g.size for g in grids)
+    print(json.dumps({
+        "metric": "uneven batch homogenizations/sec (512 binary cells 16..48^3, mixed, C_H rel err "
+                  "<=1e-6 vs FP64)",
+        "value": count / elapsed, "unit": "homogenizations/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": 1e3 * elapsed, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {"workload": f"{count} binary cells, edge U{{16..48}}, fill U[0.2,0.8], "
+                               f"E_void 1e-9, nu 0.3", "batch": a.batch, "ch_tol": a.ch_tol,
+                   "voxels": vox},
+        "ch_rel_err_vs_fp64_max": max(errs) if errs else None,
+        "ch_err_ok": (max(errs) <= 1e-6) if errs else None,
+        "fp64_batched_seconds": t64 if errs else None,
+        "iters_max": int(st.get("iters", [0])[0]) if st else None}))
+
+
+def run_opt(a):
+    """64 cells of 96^3, each with a seeded target C_H; 30 projected-gradient
+    steps (forward + adjoint + halving line search), warm-started mixed MG-PCG.
+    Reports the C_H error of the final iterate against FP64 for 8 cells."""
+    import numpy as np
+    from fem_amd import Homogenizer
+    from fem_amd import workloads as W
+    from fem_amd.optimize import BatchedDescent
+    from oracle import oracle as O
+    n, count, steps = 96, a.cells or 64, 30
+    x = np.stack([W.simp_density(n, 100 + s, 6.0).reshape(-1) for s in range(count)])
+    Et = 0.05 + 0.1 * W.uniform(5, count)
+    C0 = np.zeros((count, 6, 6))
+    for s in range(count):
+        lam, mu = O.lame(float(Et[s]), 0.25)
+        C0[s, :3, :3] = lam
+        C0[s, range(3), range(3)] += 2 * mu
+        C0[s, 3, 3] = C0[s, 4, 4] = C0[s, 5, 5] = mu
+    opt = BatchedDescent((n, n, n), C0, m0=0.0, mw=0.0, batch=a.batch, precision="mixed",
+                         tol=1e-10, ch_tol=a.ch_tol, inner_tol=a.inner_tol)
+    opt.evaluate(x)  # warm-up (hierarchies, first fields)
+    t0 = time.perf_counter()
+    xf, hist = opt.run(x, steps, log=lambda h: print(f"step {h['step']}: accepted "
+                                                      f"{h['accepted']}, {h['seconds']:.2f} s",
+                                                      file=sys.stderr, flush=True))
+    elapsed = time.perf_counter() - t0
+    Cm, _ = opt.evaluate(xf)
+    errs = []
+    if not a.no_fp64:
+        h = Homogenizer((n, n, n), nu=0.3, tol=1e-10, batch=8)
+        C64, _ = h.homogenize_many(xf[:8].reshape(8, n, n, n))
+        errs = [float(np.linalg.norm(Cm[s] - C64[s]) / np.linalg.norm(C64[s])) for s in range(8)]
+    J0 = hist[0]["J0"]
+    Jn = hist[-1]["J"]
+    print(json.dumps({
+        "metric": "96^3 optimisation steps/sec (forward + adjoint + line search, 64 cells, "
+                  "warm-started mixed MG-PCG)",
+        "value": count * steps / elapsed, "unit": "cell-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic",
+        "config": {"workload": f"{count} SIMP cells of 96^3 (GRF corr 6), isotropic targets "
+                               "E~U[0.05,0.15] nu 0.25, 30 steps", "batch": a.batch,
+                   "ch_tol": a.ch_tol},
+        "solves": opt.solves, "objective_ratio_median": float(np.median(Jn / J0)),
+        "ch_err_est_max": float(max(opt.est)) if opt.est else None,
+        "ch_rel_err_vs_fp64_max_final": max(errs) if errs else None}))
+    opt.close()
 
 
 if __name__ == "__main__":

@@ -101,6 +101,7 @@ void femh_default_opts(femh_opts *o) {
   o->max_outer = 12;
   o->apply_kernel = 0;
   o->batch = 1;
+  o->warm_start = 0;
 }
 
 int femh_create(const femh_opts *o, femh_ctx **out) {

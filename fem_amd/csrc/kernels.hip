@@ -454,6 +454,15 @@ __global__ __launch_bounds__(kBlock) void k_mean_sub(int V, T *__restrict__ x,
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock) xc[t] -= m;
 }
 
+// zero the fields of the right-hand sides flagged in act
+__global__ __launch_bounds__(kBlock) void k_zero_rhs(int V, double *__restrict__ x,
+                                                     const int *__restrict__ act) {
+  const int r = blockIdx.y;
+  if (!act[r]) return;
+  const size_t off = (size_t)r * 3 * V;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock) x[off + t] = 0.0;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill_hash(size_t n, size_t period, T *__restrict__ x,
                                                       unsigned long long seed) {
@@ -725,15 +734,15 @@ void launch_cheb_first(int V, int R, const T *r, const T *D, T *d, T *x, double 
 template <typename T>
 void launch_pcg_update(int V, int R, T *x, T *r, const T *p, const T *q, const T *D, T *z,
                        int rzslot, int rz_out, const int *act, const Reduce &red, int np_pq,
-                       hipStream_t s) {
+                       int np_rz, hipStream_t s) {
   hipLaunchKernelGGL(k_pcg_update<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, x, r, p, q, D, z,
-                     rzslot, rz_out, act, red, np_pq, grid_for(3 * V));
+                     rzslot, rz_out, act, red, np_pq, np_rz);
 }
 template <typename T>
 void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int first,
-                     const int *act, const Reduce &red, hipStream_t s) {
+                     const int *act, const Reduce &red, int np_rz, hipStream_t s) {
   hipLaunchKernelGGL(k_p_update<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, p, z, rz_new, rz_old,
-                     first, act, red, grid_for(3 * V));
+                     first, act, red, np_rz);
 }
 template <typename T>
 void launch_dot(int V, int R, const T *a, const T *b, int slot, const int *act, const Reduce &red,
@@ -781,6 +790,9 @@ void launch_normalize(int V, int R, const T *y, T *x, const int *act, const Redu
                       hipStream_t s) {
   hipLaunchKernelGGL(k_normalize<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, y, x, act, red,
                      grid_for(3 * V));
+}
+void launch_zero_rhs(int V, int R, double *x, const int *act, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_rhs, grid2(3 * V, R), dim3(kBlock), 0, s, V, x, act);
 }
 void launch_material(size_t n, const double *rho, double *E, int interp, double E0, double Emin,
                      double p, hipStream_t s) {
@@ -842,9 +854,10 @@ void reduce_init(Reduce &red, hipStream_t s) {
   template void launch_cheb_first<T>(int, int, const T *, const T *, T *, T *, double, int,       \
                                      const int *, hipStream_t);                                   \
   template void launch_pcg_update<T>(int, int, T *, T *, const T *, const T *, const T *, T *,    \
-                                     int, int, const int *, const Reduce &, int, hipStream_t);    \
+                                     int, int, const int *, const Reduce &, int, int,             \
+                                     hipStream_t);                                                \
   template void launch_p_update<T>(int, int, T *, const T *, int, int, int, const int *,          \
-                                   const Reduce &, hipStream_t);                                  \
+                                   const Reduce &, int, hipStream_t);                             \
   template void launch_dot<T>(int, int, const T *, const T *, int, const int *, const Reduce &,   \
                               hipStream_t);                                                       \
   template void launch_remove_mean<T>(int, int, T *, const int *, const Reduce &, hipStream_t);   \

@@ -118,15 +118,16 @@ template <typename T>
 void launch_cheb_first(int V, int R, const T *r, const T *D, T *d, T *x, double inv_theta,
                        int x_zero, const int *act, hipStream_t s);
 // PCG: alpha = rz[rzslot]/pq ; x += alpha p ; r -= alpha q ; rr -> S_RR
-//      jacobi (D != null): z = r / D and rz -> slot rz_out
+//      jacobi (D != null): z = r / D and rz -> slot rz_out (vec_parts(V) partials)
+// np_pq / np_rz: partial counts of the slots read (set by their producers)
 template <typename T>
 void launch_pcg_update(int V, int R, T *x, T *r, const T *p, const T *q, const T *D, T *z,
                        int rzslot, int rz_out, const int *act, const Reduce &red, int np_pq,
-                       hipStream_t s);
+                       int np_rz, hipStream_t s);
 // p = z + beta p, beta = rz[new]/rz[old] (first: p = z)
 template <typename T>
 void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int first,
-                     const int *act, const Reduce &red, hipStream_t s);
+                     const int *act, const Reduce &red, int np_rz, hipStream_t s);
 // dot products a.b -> slot (per RHS)
 template <typename T>
 void launch_dot(int V, int R, const T *a, const T *b, int slot, const int *act, const Reduce &red,
@@ -135,6 +136,8 @@ template <typename Ts, typename Td>
 void launch_convert(size_t n, const Ts *src, Td *dst, hipStream_t s);
 template <typename T>
 void launch_invert(size_t n, const T *d, T *di, hipStream_t s);
+// x[q] = 0 for the right-hand sides q flagged in act
+void launch_zero_rhs(int V, int R, double *x, const int *act, hipStream_t s);
 // x64 += (double) d32 ; both [R][3][V]
 void launch_add_f32_to_f64(int V, int R, const float *d, double *x, const int *act, hipStream_t s);
 // subtract each RHS/component mean (slot S_AUX)

@@ -4,6 +4,8 @@
 // by s = q / 6.
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "device_common.hpp"
 #include "ops.hpp"
 
@@ -38,7 +40,7 @@ constexpr int kEOH = 13;         // LDS stride of one voxel's 12 plane forces (o
 //   e2 = xprev (0 without one)
 template <typename T>
 struct EpiIn {
-  T e0[3], e1[3], e2[3];
+  T e0[3], e1[3], e2[3], e3[3];  // e3: EP_STEP with a dot: b (for b . x)
 };
 
 template <typename T, int EP>
@@ -51,6 +53,7 @@ __device__ __forceinline__ void epi_load(const OpArgs<T> &a, const T *__restrict
     if (EP == EP_STEP) p.e0[c] = a.rprev[q];
     if (EP == EP_CHEB0 || EP == EP_STEP) p.e1[c] = Ds[(size_t)c * V + n];
     if (EP == EP_STEP) p.e2[c] = a.xprev ? a.xprev[q] : (T)0;
+    if (EP == EP_STEP) p.e3[c] = a.dot ? a.b[q] : (T)0;
   }
 }
 
@@ -70,16 +73,20 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, 
     } else if (EP == EP_CHEB0) {
       const T rv = p.e0[c] - Au[c];
       a.r[q] = rv;
-      a.dout[q] = rv * ti * p.e1[c];
+      if (!a.last) a.dout[q] = rv * ti * p.e1[c];  // `last`: no smoothing step follows
       a.x[q] = uo[c];
     } else {  // EP_STEP
       const T rv = p.e0[c] - Au[c];
-      a.r[q] = rv;
       const T dn = c1 * uo[c] + c2 * rv * p.e1[c];
-      a.dout[q] = dn;
       T xv = p.e2[c] + uo[c];
-      if (a.last) xv += dn;
+      if (a.last) {
+        xv += dn;  // the last step's r and d are not read again
+      } else {
+        a.r[q] = rv;
+        a.dout[q] = dn;
+      }
       a.x[q] = xv;
+      if (a.dot) dv += (double)p.e3[c] * (double)xv;
     }
   }
 }
@@ -306,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
         for (int c = 0; c < 3; c++) acc[c] += src[c];
       }
   }
-  if (EP == EP_AX && a.dot) {
+  if ((EP == EP_AX || EP == EP_STEP) && a.dot) {
     const double s = block_sum(dv);
     block_partial(a.red, a.slot, r, blockIdx.y * gridDim.x + blockIdx.x, s);
   }
@@ -341,7 +348,9 @@ void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCo
   GO(ST_PLAIN, EP_STEP)
   GO(ST_PROLONG, EP_CHEB0)
   GO(ST_PLAIN, EP_CHEB0)
+  GO(ST_DSCALE, EP_CHEB0)
 #undef GO
+  throw std::runtime_error("no operator kernel for this stage/epilogue pair");
 }
 
 // ---------------------------------------------------------------------------
@@ -561,8 +570,10 @@ void launch_stencil(int st, int ep, const Dims &g, int R, const T *S, const OpAr
   // (all load cases of a sample per workgroup in FP32, three in FP64)
   constexpr int NG = sizeof(T) == 4 ? kRHS : kRHS / 2;
   const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
-  const bool tiled = g.ny >= kTile && g.nz >= kTile && g.nx >= 3 && R % kRHS == 0;
   const dim3 tgrid(tiles_j * tiles_k, (g.nx + kStTI - 1) / kStTI, nsmp * (kRHS / NG));
+  // ... when that still gives >= 2 workgroups per CU
+  const bool tiled = g.ny >= kTile && g.nz >= kTile && g.nx >= 3 && R % kRHS == 0 &&
+                     (long)tgrid.x * tgrid.y * tgrid.z >= 512;
   // all load cases per thread once that still leaves >= 2 workgroups per CU
   const bool multi = (long)blocks * nsmp >= 512 && R % kRHS == 0;
 #define GO(S_, E_)                                                                               \
@@ -583,7 +594,9 @@ void launch_stencil(int st, int ep, const Dims &g, int R, const T *S, const OpAr
   GO(ST_DSCALE, EP_STEP)
   GO(ST_PLAIN, EP_STEP)
   GO(ST_PLAIN, EP_CHEB0)
+  GO(ST_DSCALE, EP_CHEB0)
 #undef GO
+  throw std::runtime_error("no operator kernel for this stage/epilogue pair");
 }
 
 template <typename T>

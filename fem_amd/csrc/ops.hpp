@@ -11,9 +11,11 @@
 // staged value there:
 //   EP_AX       y = Au            (optional FP64 partial of u . Au)
 //   EP_RES      y = b - Au
-//   EP_CHEB0    r = b - Au ; dout = r theta_inv / D ; x = u_own
+//   EP_CHEB0    r = b - Au ; dout = r theta_inv / D (not with `last`) ; x = u_own
 //   EP_STEP     r = rprev - Au ; dn = c1 u_own + c2 r / D ; dout = dn ;
-//               x = xprev + u_own (+ dn when `last`)
+//               (with `dot`, fine level: the FP64 partial of b . x_new)
+//               x = xprev + u_own ; with `last`: x += dn, and r and dout are
+//               not written (nothing reads them after the last step)
 // A kernel never writes the vector it stages (neighbours read it), so x/xn,
 // d/d2 are ping-ponged by the caller.  Chebyshev coefficients are per sample
 // (each sample's smoother has its own spectrum estimate).

@@ -86,6 +86,7 @@ Context::Context(const femh_opts &opts) : o(opts) {
   for (DBuf *bf : {&b, &x, &r, &z, &p, &q}) bf->alloc(field_bytes(g, B, sizeof(double)));
   if (o.precision == FEMH_MIXED)
     for (DBuf *bf : {&b32, &r32, &x32, &z32, &p32, &q32}) bf->alloc(field_bytes(g, B, sizeof(float)));
+  if (o.warm_start && o.precision != FEMH_MIXED) xw.alloc(field_bytes(g, B, sizeof(double)));
   red_part.alloc(sizeof(double) * kNumSlots * kSlotWidth * kMaxPartials);
   red_ctr.alloc(sizeof(unsigned) * kNumSlots * kSlotWidth);
   red_scal.alloc(sizeof(double) * kNumSlots * kSlotWidth);
@@ -459,7 +460,7 @@ void Context::cheb_run(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *ac
     a.r = L.r.template as<T>();
     a.x = xx;
     a.dout = L.d.template as<T>();
-    a.last = 0;
+    a.last = 1;  // c1 = c2 = 0: x = staged value
     set_cheb(h, l, a, 0, ratio);
     op<T>(h, l, ST_DSCALE, EP_STEP, a);
     return;
@@ -493,8 +494,10 @@ void Context::pre_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *
   cheb_run<T>(h, l, bb, xx, act_, o.nu1, o.cheby_ratio);
 }
 
+// Returns true when the last launch also left the partials of b . x in
+// dot_slot (fine level, degree >= 2).
 template <typename T>
-void Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_) {
+bool Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_, int dot_slot) {
   Level<T> &L = h.lv[l];
   Level<T> &C = h.lv[l + 1];
   const int k = o.nu2;
@@ -519,7 +522,7 @@ void Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int 
   }
   if (k == 1) {
     launch_add2<T>(L.g.V, R, L.xn.template as<T>(), L.d2.template as<T>(), xx, act_, stream);
-    return;
+    return false;
   }
   T *dcur = L.d2.template as<T>(), *dnxt = L.d.template as<T>();
   OpArgs<T> s1;
@@ -527,45 +530,83 @@ void Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int 
   s1.r = L.r.template as<T>();
   s1.rprev = L.r.template as<T>();
   s1.x = xx;
+  const bool fuse = l == 0 && dot_slot >= 0;
   for (int s = 1; s < k; s++) {
     s1.u = dcur;
     s1.xprev = (s == 1) ? L.xn.template as<T>() : xx;
     s1.dout = dnxt;
     s1.last = (s == k - 1);
+    if (s1.last && fuse) {
+      s1.b = bb;
+      s1.red = red;
+      s1.slot = dot_slot;
+      s1.dot = 1;
+    }
     set_cheb(h, l, s1, s, o.cheby_ratio);
     op<T>(h, l, ST_PLAIN, EP_STEP, s1);
     std::swap(dcur, dnxt);
   }
+  return fuse;
 }
 
 template <typename T>
-void Context::vcycle(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_) {
+bool Context::vcycle(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_, int dot_slot) {
   const int last = (int)h.lv.size() - 1;
   if (l == last) {
     if (h.ndense)
       launch_dense_solve<T>(h.ndense, R, h.Ainv.template as<T>(), bb, xx, act_, stream);
     else
       cheb_run<T>(h, l, bb, xx, act_, std::max(2, o.coarse_iters), kCoarseChebRatio);
-    return;
+    return false;
   }
   Level<T> &L = h.lv[l];
   Level<T> &C = h.lv[l + 1];
-  pre_smooth<T>(h, l, bb, xx, act_);
-  apply<T>(h, l, xx, L.r.template as<T>(), bb, 1, act_, nullptr, 0);
+  if (o.nu1 == 1) {
+    // degree-1 pre-smoothing x = b / (theta D) needs no operator apply: one
+    // launch stages it and leaves the residual b - A x for the restriction
+    OpArgs<T> a;
+    a.act = act_;
+    a.u = bb;
+    a.b = bb;
+    a.r = L.r.template as<T>();
+    a.dout = L.d.template as<T>();
+    a.x = xx;
+    a.last = 1;
+    set_cheb(h, l, a, 0, o.cheby_ratio);
+    op<T>(h, l, ST_DSCALE, EP_CHEB0, a);
+  } else {
+    pre_smooth<T>(h, l, bb, xx, act_);
+    apply<T>(h, l, xx, L.r.template as<T>(), bb, 1, act_, nullptr, 0);
+  }
   launch_restrict<T>(L.g, C.g, R, L.r.template as<T>(), C.b.template as<T>(), act_, stream);
-  vcycle<T>(h, l + 1, C.b.template as<T>(), C.x.template as<T>(), act_);
-  post_smooth<T>(h, l, bb, xx, act_);
+  vcycle<T>(h, l + 1, C.b.template as<T>(), C.x.template as<T>(), act_, -1);
+  return post_smooth<T>(h, l, bb, xx, act_, dot_slot);
 }
 
+// z = M^-1 r, and r . z into rz_slot
 template <typename T>
-void Context::precond(Hierarchy<T> &h, const T *rv, T *zv, const int *act_) {
+void Context::precond(Hierarchy<T> &h, const T *rv, T *zv, const int *act_, int rz_slot) {
+  bool fused = false;
   if (o.precond == FEMH_PRECOND_MG && h.lv.size() > 1) {
-    vcycle<T>(h, 0, rv, zv, act_);
+    fused = vcycle<T>(h, 0, rv, zv, act_, rz_slot);
   } else {
     // Jacobi: inside the PCG update kernel; this path serves the first step
     launch_cheb_first<T>(h.lv[0].g.V, R, rv, h.lv[0].Dinv.template as<T>(), zv, zv, 1.0, 1, act_,
                          stream);
   }
+  if (fused != fuses_rz(h)) throw Error(FEMH_DEVICE_ERROR, "inconsistent r.z reduction path");
+  if (!fused) launch_dot<T>(h.lv[0].g.V, R, rv, zv, rz_slot, act_, red, stream);
+}
+
+// the V-cycle's last fine-level smoothing launch also reduces r . z
+template <typename T>
+bool Context::fuses_rz(const Hierarchy<T> &h) const {
+  return o.precond == FEMH_PRECOND_MG && h.lv.size() > 1 && o.nu2 >= 2;
+}
+
+template <typename T>
+int Context::rz_parts(const Hierarchy<T> &h) const {
+  return fuses_rz(h) ? tiled_parts(g, R) : vec_parts(g.V);
 }
 
 // PCG over all right-hand sides flagged in `active` (updated in place as
@@ -588,19 +629,17 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
   if (!nact) return;
   set_act(A_PCG, active);
   const int *ap = actp(A_PCG);
-  if (!z_ready) {
-    // z_ready: the caller already left z = M^-1 b and r.z in S_RZ0
-    precond<T>(h, rv, zv, ap);
-    launch_dot<T>(V, R, rv, zv, S_RZ0, ap, red, stream);
-  }
-  launch_p_update<T>(V, R, pv, zv, S_RZ0, S_RZ0, 1, ap, red, stream);
+  const int np_rz = jac ? vec_parts(V) : rz_parts(h);
+  // z_ready: the caller already left z = M^-1 b and r.z in S_RZ0
+  if (!z_ready) precond<T>(h, rv, zv, ap, S_RZ0);
+  launch_p_update<T>(V, R, pv, zv, S_RZ0, S_RZ0, 1, ap, red, np_rz, stream);
   int cur = 0;
   std::vector<double> rr(R);
   for (int it = 1; it <= max_iter; it++) {
     const int rzc = cur ? S_RZ1 : S_RZ0, rzn = cur ? S_RZ0 : S_RZ1;
     apply<T>(h, 0, pv, qv, nullptr, 0, ap, &red, S_PQ);
     launch_pcg_update<T>(V, R, xv, rv, pv, qv, jac ? D0 : nullptr, zv, rzc, rzn, ap, red,
-                         apply_parts(), stream);
+                         apply_parts(), np_rz, stream);
     FEMH_HIP(hipGetLastError());
     if (it % o.check_every == 0 || it == max_iter) {
       read_scalars(S_RR, rr.data(), R, vec_parts(V));
@@ -620,11 +659,8 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
       if (!nact) break;
       if (changed) set_act(A_PCG, active);
     }
-    if (!jac) {
-      vcycle<T>(h, 0, rv, zv, ap);
-      launch_dot<T>(V, R, rv, zv, rzn, ap, red, stream);
-    }
-    launch_p_update<T>(V, R, pv, zv, rzn, rzc, 0, ap, red, stream);
+    if (!jac) precond<T>(h, rv, zv, ap, rzn);
+    launch_p_update<T>(V, R, pv, zv, rzn, rzc, 0, ap, red, np_rz, stream);
     cur ^= 1;
   }
   FEMH_HIP(hipGetLastError());
@@ -687,6 +723,10 @@ void Context::compute_ch(const double *fields, double *C) {
 
 void Context::homogenize(double *C, femh_stats *st) {
   if (nb < 1) throw Error(FEMH_BAD_ARG, "no input loaded");
+  // warm start: each slot begins from the fields of the previous call with
+  // the same sample count (valid fields are kept only after a finite result)
+  const bool warm = o.warm_start && prev_nb == nb;
+  prev_nb = 0;
   auto t0 = std::chrono::steady_clock::now();
   hipEvent_t ev[3];
   for (auto &e : ev) FEMH_HIP(hipEventCreate(&e));
@@ -725,13 +765,44 @@ void Context::homogenize(double *C, femh_stats *st) {
   std::vector<int> iters(R, 0), outer(R, 0);
   std::vector<double> relres(R, 0.0), ch_est(nb, 0.0);
   int status = FEMH_OK;
+  if (warm) {
+    // vanishing right-hand sides restart from zero
+    std::vector<int> zero(R);
+    bool any = false;
+    for (int t = 0; t < R; t++) any |= (zero[t] = !active[t]) != 0;
+    if (any) {
+      set_act(A_POW, zero);
+      launch_zero_rhs(g.V, R, x.as<double>(), actp(A_POW), stream);
+    }
+  }
   if (o.precision == FEMH_FP64) {
     std::vector<int> act_pcg = active;
+    if (warm) {
+      // solve K d = b - K x_prev and return x_prev + d
+      FEMH_HIP(hipMemcpyAsync(xw.p, x.p, field_bytes(g, nb, sizeof(double)), hipMemcpyDeviceToDevice,
+                              stream));
+      apply<double>(h64, 0, xw.as<double>(), b.as<double>(), b.as<double>(), 1, actp(A_ALL), nullptr, 0);
+      launch_dot<double>(g.V, R, b.as<double>(), b.as<double>(), S_RR, actp(A_ALL), red, stream);
+      std::vector<double> r0(R);
+      read_scalars(S_RR, r0.data(), R, vec_parts(g.V));
+      for (int t = 0; t < R; t++)
+        if (act_pcg[t]) {
+          relres[t] = std::sqrt(r0[t] / bb[t]);
+          if (relres[t] <= o.tol) act_pcg[t] = 0;
+        }
+    }
+    std::vector<int> act_run = act_pcg;
+    const std::vector<double> rel0 = relres;
     pcg<double>(h64, b.as<double>(), x.as<double>(), r.as<double>(), z.as<double>(), p.as<double>(),
-                q.as<double>(), act_pcg, o.tol, o.max_iter, bb.data(), iters.data(), relres.data());
+                q.as<double>(), act_run, o.tol, o.max_iter, bb.data(), iters.data(), relres.data());
+    if (warm) {
+      for (int t = 0; t < R; t++)
+        if (active[t] && !act_pcg[t]) relres[t] = rel0[t];  // converged before any iteration
+      launch_add2<double>(g.V, R, x.as<double>(), xw.as<double>(), x.as<double>(), actp(A_ALL), stream);
+    }
     for (int t = 0; t < R; t++) {
-      outer[t] = active[t];
-      if (act_pcg[t]) status = FEMH_NOT_CONVERGED;
+      outer[t] = act_pcg[t];
+      if (act_run[t]) status = FEMH_NOT_CONVERGED;
     }
   } else {
     // FP64 iterative refinement around FP32 multigrid-PCG corrections.
@@ -741,17 +812,32 @@ void Context::homogenize(double *C, femh_stats *st) {
     // ||dC||_F <= trace(dC) = sum_i ||e_i||_K^2 / V.  With C_ii estimated as
     // (a_ii sum E - b.x - x.r)/V, est = sqrt(6) sum_i rz_i / (V trace C) bounds
     // the relative Frobenius error of a sample when M^-1 ~ K^-1.
-    FEMH_HIP(hipMemsetAsync(x.p, 0, field_bytes(g, nb, sizeof(double)), stream));
-    FEMH_HIP(hipMemcpyAsync(r.p, b.p, field_bytes(g, nb, sizeof(double)), hipMemcpyDeviceToDevice,
-                            stream));
     std::vector<double> rr = bb;
+    std::vector<double> bx(R, 0.0), xr(R, 0.0), rz(R), rn(R), bxn(R), xrn(R);
+    if (warm) {
+      apply<double>(h64, 0, x.as<double>(), r.as<double>(), b.as<double>(), 1, actp(A_ALL), nullptr, 0);
+      launch_dot<double>(g.V, R, r.as<double>(), r.as<double>(), S_RR, actp(A_ALL), red, stream);
+      launch_dot<double>(g.V, R, b.as<double>(), x.as<double>(), S_BX, actp(A_ALL), red, stream);
+      launch_dot<double>(g.V, R, x.as<double>(), r.as<double>(), S_XR, actp(A_ALL), red, stream);
+      read_scalars(S_RR, rr.data(), R, vec_parts(g.V));
+      read_scalars(S_BX, bx.data(), R, vec_parts(g.V));
+      read_scalars(S_XR, xr.data(), R, vec_parts(g.V));
+      for (int t = 0; t < R; t++)
+        if (active[t]) {
+          relres[t] = std::sqrt(rr[t] / bb[t]);
+          if (relres[t] <= o.tol) active[t] = 0;
+        }
+    } else {
+      FEMH_HIP(hipMemsetAsync(x.p, 0, field_bytes(g, nb, sizeof(double)), stream));
+      FEMH_HIP(hipMemcpyAsync(r.p, b.p, field_bytes(g, nb, sizeof(double)), hipMemcpyDeviceToDevice,
+                              stream));
+    }
     double aii[kRHS];
     for (int k = 0; k < kRHS; k++) {
       double acc = 0;
       for (int t = 0; t < 24; t++) acc += tab.u_aff[k * 24 + t] * tab.g[k * 24 + t];
       aii[k] = acc;
     }
-    std::vector<double> bx(R, 0.0), xr(R, 0.0), rz(R), rn(R), bxn(R), xrn(R);
     std::vector<int> it_in(R);
     std::vector<double> rel_in(R);
     const int max_outer = o.max_outer > 0 ? o.max_outer : 12;
@@ -764,11 +850,10 @@ void Context::homogenize(double *C, femh_stats *st) {
       // z = M^-1 r and r.z for the estimate; reused as the first PCG step
       FEMH_HIP(hipMemcpyAsync(r32.p, b32.p, field_bytes(g, nb, sizeof(float)), hipMemcpyDeviceToDevice,
                               stream));
-      precond<float>(h32, r32.as<float>(), z32.as<float>(), ao);
-      launch_dot<float>(g.V, R, r32.as<float>(), z32.as<float>(), S_RZ0, ao, red, stream);
+      precond<float>(h32, r32.as<float>(), z32.as<float>(), ao, S_RZ0);
       FEMH_HIP(hipGetLastError());
-      if (o.ch_tol > 0 && sweep > 0) {
-        read_scalars(S_RZ0, rz.data(), R, vec_parts(g.V));
+      if (o.ch_tol > 0 && (sweep > 0 || warm)) {
+        read_scalars(S_RZ0, rz.data(), R, rz_parts(h32));
         for (int s = 0; s < nb; s++) {
           double trC = 0, srz = 0;
           bool any = false;
@@ -851,6 +936,7 @@ void Context::homogenize(double *C, femh_stats *st) {
   for (auto &e : ev) (void)hipEventDestroy(e);
   for (int k = 0; k < 36 * nb; k++)
     if (!std::isfinite(C[k])) throw Error(FEMH_NONFINITE, "C_H is not finite");
+  prev_nb = nb;
   if (status != FEMH_OK) throw Error(status, "PCG did not reach the tolerance within max_iter");
 }
 

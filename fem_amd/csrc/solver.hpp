@@ -105,6 +105,8 @@ struct Context {
   Hierarchy<double> h64;
   Hierarchy<float> h32;
   bool have_fields = false;
+  int prev_nb = 0;  // samples of the last successful call (warm start source)
+  DBuf xw;          // warm start: previous fields [R][3][V] (FP64)
   bool modulus_input = false;  // last load gave moduli (dE/drho = 1)
   int nlevels = 1;
 
@@ -150,11 +152,15 @@ struct Context {
   template <typename T>
   void pre_smooth(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_);
   template <typename T>
-  void post_smooth(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_);
+  bool post_smooth(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_, int dot_slot);
   template <typename T>
-  void vcycle(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_);
+  bool vcycle(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_, int dot_slot);
   template <typename T>
-  void precond(Hierarchy<T> &h, const T *r, T *z, const int *act_);
+  void precond(Hierarchy<T> &h, const T *r, T *z, const int *act_, int rz_slot);
+  template <typename T>
+  bool fuses_rz(const Hierarchy<T> &h) const;
+  template <typename T>
+  int rz_parts(const Hierarchy<T> &h) const;
   template <typename T>
   void pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T *qv,
            std::vector<int> &active, double tol, int max_iter, const double *bnorm2, int *iters,

@@ -1,0 +1,114 @@
+"""Batched projected-gradient material optimisation over the C ABI.
+
+Per cell this is the loop of fast_fem/run3d.cpp:238-289 (gradientDescent):
+  g = dJ/drho at x; h = max_step / ||g||_inf; line search x' = clamp(x - h g,
+  lb, ub), halving h up to 10 times until J(x') < J(x) (run3d.cpp:238-262),
+with the objective of fast_fem/FEM3DFun.cpp:198-220,
+  J = 0.5 sum_ij wG_ij (C_ij - C0_ij)^2 + 0.5 mw (mean(rho) - m0)^2,
+and its gradient dJ/drho_e = sum_ij wG_ij (C_ij - C0_ij) dC_ij/drho_e
++ mw (mean(rho) - m0) / V (fast_fem/FEM3DFun.cpp:222-268).  The density
+parameter is per voxel (the field is the identity).
+
+All cells advance together: every forward solve is one batched launch
+sequence of up to `batch` cells, warm-started from the cells' previous fields;
+cells whose line search already succeeded (or gave up) leave the batch.
+"""
+import time
+
+import numpy as np
+
+from .femh import Homogenizer
+
+
+def objective(C, C0, wG, rho_mean, m0, mw):
+    d = C - C0
+    return 0.5 * np.sum(wG * d * d, axis=(-2, -1)) + 0.5 * mw * (rho_mean - m0) ** 2
+
+
+class BatchedDescent:
+    """n cells of one grid; x[n, V] densities in [lb, ub]."""
+
+    def __init__(self, shape, C0, wG=None, m0=0.0, mw=0.0, nu=0.3, batch=8, lb=1e-3, ub=1.0,
+                 interp="simp", penal=3.0, **solver):
+        self.shape = tuple(shape)
+        self.C0 = np.asarray(C0, dtype=np.float64)
+        self.n = self.C0.shape[0]
+        self.wG = np.ones((self.n, 6, 6)) if wG is None else np.broadcast_to(wG, (self.n, 6, 6))
+        self.m0, self.mw, self.lb, self.ub = m0, mw, lb, ub
+        self.batch = batch
+        # one context per group of `batch` cells keeps each group's fields
+        # resident for the warm start of its next solve
+        self.groups = [list(range(g, min(g + batch, self.n))) for g in range(0, self.n, batch)]
+        self.ctx = [Homogenizer(self.shape, nu=nu, batch=len(g), interp=interp, penal=penal,
+                                warm_start=True, **solver) for g in self.groups]
+        self.solves = 0
+        self.est = []
+
+    def evaluate(self, x, cells=None, grad=False):
+        """C (and dJ/drho when grad) of the given cells at densities x[cells]."""
+        cells = list(range(self.n)) if cells is None else list(cells)
+        C = np.zeros((self.n, 6, 6))
+        G = np.zeros_like(x) if grad else None
+        for gi, grp in enumerate(self.groups):
+            sel = [c for c in grp if c in set(cells)]
+            if not sel:
+                continue
+            # the context solves its whole group (slots keep their cells, so
+            # the warm start follows each cell); unselected cells ride along
+            # at their current point
+            Cg, st = self.ctx[gi].homogenize_many(x[grp].reshape(len(grp), *self.shape))
+            self.solves += len(grp)
+            self.est.extend(s["ch_err_est"] for s in st)
+            C[grp] = Cg
+            if grad:
+                nb = len(grp)
+                rho_mean = x[grp].mean(axis=1)
+                dJdC = self.wG[grp] * (Cg - self.C0[grp])
+                g = self.ctx[gi].gradient(dJdC, nb=nb).reshape(nb, -1)
+                g += (self.mw * (rho_mean - self.m0) / x.shape[1])[:, None]
+                G[grp] = g
+        return C, G
+
+    def J(self, C, x):
+        return objective(C, self.C0, self.wG, x.mean(axis=1), self.m0, self.mw)
+
+    def step(self, x, max_step=0.5, max_halvings=10):
+        """One projected-gradient step for every cell; returns (x_new, J_new, accepted)."""
+        C, G = self.evaluate(x, grad=True)
+        J0 = self.J(C, x)
+        norm = np.abs(G).max(axis=1)
+        h = np.where(norm > 0, max_step / np.maximum(norm, 1e-300), 0.0)
+        xn = x.copy()
+        accepted = np.zeros(self.n, dtype=bool)
+        Jn = J0.copy()
+        pending = np.flatnonzero(norm > 0)
+        for _ in range(max_halvings):
+            if pending.size == 0:
+                break
+            trial = x.copy()
+            trial[pending] = np.clip(x[pending] - h[pending, None] * G[pending], self.lb, self.ub)
+            trial[accepted] = xn[accepted]
+            Ct, _ = self.evaluate(trial, cells=pending)
+            Jt = self.J(Ct, trial)
+            ok = pending[Jt[pending] < J0[pending]]
+            xn[ok] = trial[ok]
+            Jn[ok] = Jt[ok]
+            accepted[ok] = True
+            h[pending] *= 0.5
+            pending = np.setdiff1d(pending, ok)
+        return xn, Jn, accepted, J0
+
+    def run(self, x, steps, max_step=0.5, log=None):
+        hist = []
+        for k in range(steps):
+            t = time.perf_counter()
+            x, Jn, acc, J0 = self.step(x, max_step)
+            hist.append(dict(step=k, J0=J0.copy(), J=Jn.copy(), accepted=int(acc.sum()),
+                             seconds=time.perf_counter() - t))
+            if log:
+                log(hist[-1])
+        return x, hist
+
+    def close(self):
+        for h in self.ctx:
+            h.close()

@@ -91,6 +91,9 @@ typedef struct femh_opts {
   int max_outer;        /* mixed: refinement sweep cap */
   int apply_kernel;     /* fine-grid operator: 0 tiled cube-mode (default), 1 dense gather */
   int batch;            /* samples solved together per launch sequence (1..8) */
+  int warm_start;       /* 1: start each sample slot from the fields of the previous
+                           call with the same sample count (an optimisation loop's
+                           previous iterate); 0: start from zero */
 } femh_opts;
 
 typedef struct femh_stats {

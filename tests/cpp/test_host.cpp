@@ -69,7 +69,9 @@ int main() {
     const double c0 = C(0, 0);
     const double asym = std::fmax(std::fabs(C(1, 1) - c0), std::fabs(C(2, 2) - c0)) / c0 +
                         std::fabs(C(0, 1) - C(1, 2)) / c0 + std::fabs(C(3, 3) - C(5, 5)) / c0;
-    expect(asym < 1e-8 && par[0] > 0.2 && par[0] < 0.5, "coarsening: frame lattice is cubic", asym);
+    // 10 of the 64 cells have at least two zero coordinates
+    expect(asym < 1e-8 && std::fabs(par[0] - 10.0 / 64.0) < 1e-15, "coarsening: frame lattice is cubic",
+           asym);
     nc.setEmitReferenceTriangle(true);
     std::vector<double> tq, pq;
     nc.computeCoarsenedElasticityTensorAndParameters3D(m, N, {0.0, 1.0}, NumericalCoarseningUsingMultigrid::Cubic,

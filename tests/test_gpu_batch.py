@@ -141,3 +141,14 @@ def test_batch_errors():
     rho[1, 2, 3, 4] = 0.5
     C, _ = h.homogenize_many(rho)
     assert np.isfinite(C).all()
+
+
+def test_uneven_binary_batch_mixed_meets_1e6():
+    """Uneven batch of binary cells (E_void 1e-9, edges 16..48): mixed
+    precision per cell within 1e-6 of FP64, as in the 512-cell workload."""
+    grids = W.batch_uneven(24, 3)
+    kw = dict(nu=NU, tol=1e-10, max_iter=20000, batch=8)
+    Cm, _ = homogenize_batch(grids, precision="mixed", ch_tol=1e-8, **kw)
+    C64, _ = homogenize_batch(grids, **kw)
+    errs = [rel(a, b) for a, b in zip(Cm, C64)]
+    assert max(errs) <= 1e-6, errs

@@ -231,3 +231,25 @@ def test_not_converged_is_reported():
     assert max(st["iters"]) == 3
     with pytest.raises(FemhError):
         h.homogenize_modulus(E)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_warm_start_reuses_previous_fields(precision):
+    """An optimisation loop re-solves a slightly changed cell: warm starts
+    from the previous fields need fewer iterations and give the cold result."""
+    rho = W.simp_density(32, 3, 3.0)
+    rho2 = np.clip(rho + 0.01 * np.sin(np.arange(rho.size)).reshape(rho.shape), 1e-3, 1.0)
+    kw = dict(nu=NU, tol=1e-10, precision=precision, ch_tol=1e-9 if precision == "mixed" else 0.0)
+    cold = Homogenizer(rho.shape, **kw)
+    C1c, _ = cold.homogenize(rho)
+    C2c, st2c = cold.homogenize(rho2)
+    warm = Homogenizer(rho.shape, warm_start=True, **kw)
+    C1, st1 = warm.homogenize(rho)
+    assert rel(C1, C1c) <= 1e-12
+    # the same cell again: already converged, no iterations
+    C1b, st1b = warm.homogenize(rho)
+    assert max(st1b["iters"]) == 0, st1b
+    assert rel(C1b, C1c) <= (1e-12 if precision == "fp64" else 1e-8)
+    C2, st2 = warm.homogenize(rho2)
+    assert sum(st2["iters"]) < sum(st2c["iters"]), (st2["iters"], st2c["iters"])
+    assert rel(C2, C2c) <= (1e-10 if precision == "fp64" else 1e-7)
