@@ -36,9 +36,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--batch", type=int, default=8, help="cells per step (1..8)")
-    ap.add_argument("--ch-tol", type=float, default=1e-8,
-                    help="a-posteriori C_H error target of the mixed path")
-    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--ch-tol", type=float, default=1e-7,
+                    help="a-posteriori C_H error target of the mixed path (the metric allows 1e-6; "
+                         "the estimate has stayed within 3x of the true error)")
+    ap.add_argument("--inner-tol", type=float, default=3e-4,
+                    help="relative residual reduction of one FP32 PCG sweep")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solves")
     ap.add_argument("--nu1", type=int, default=2)

@@ -121,6 +121,11 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
   __shared__ T up[3][3][kTN * kTN];
   __shared__ T eo[kTE * kTE * kEOH];
   const int tid = threadIdx.x;
+  // staging slots and voxels (324 and 289 of them) go to threads in an order
+  // rotated by a wave per workgroup, so that the second round of the extra
+  // ones does not fall on the same wave -- hence the same SIMD -- in every
+  // workgroup; owned nodes (the epilogue) keep the plain thread order
+  const int vt = (tid + 64 * (int)((blockIdx.x + blockIdx.y + blockIdx.z) & 3)) & (kBlock - 1);
   const int j0 = (blockIdx.x / tiles_k) * kTile, k0 = (blockIdx.x % tiles_k) * kTile;
   const int i0 = blockIdx.y * TI;
   const int iend = min(i0 + TI, g.nx);
@@ -133,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
   float sjw[2], skw[2];
 #pragma unroll
   for (int q = 0; q < 2; q++) {
-    const int t = tid + q * kBlock;
+    const int t = vt + q * kBlock;
     soff[q] = -1;
     if (t < kTN * kTN) {
       const int nj = t / kTN, nk = t - nj * kTN;
@@ -153,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
   int eoff[2];
 #pragma unroll
   for (int q = 0; q < 2; q++) {
-    const int e = tid + q * kBlock;
+    const int e = vt + q * kBlock;
     eoff[q] = -1;
     if (e < kTE * kTE) {
       const int ej = e / kTE, ek = e - ej * kTE;
@@ -184,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       if (soff[q] < 0) continue;
-      const int t = tid + q * kBlock;
+      const int t = vt + q * kBlock;
 #pragma unroll
       for (int c = 0; c < 3; c++) up[buf][c][t] = (ST == ST_DSCALE) ? pf[q][c] * (ti * pd[q][c]) : pf[q][c];
     }
@@ -199,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       if (soff[q] < 0) continue;
-      const int t = tid + q * kBlock;
+      const int t = vt + q * kBlock;
       const int n = iw * plane + soff[q];
       T v[3];
 #pragma unroll
@@ -253,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       if (eoff[q] < 0) continue;
-      const int e = tid + q * kBlock;
+      const int e = vt + q * kBlock;
       const int ej = e / kTE, ek = e - ej * kTE;
       T uv[24], fv[24];
 #pragma unroll
@@ -297,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       if (eoff[q] < 0) continue;
-      T *dst = eo + (tid + q * kBlock) * kEOH;
+      T *dst = eo + (vt + q * kBlock) * kEOH;
 #pragma unroll
       for (int t = 0; t < 12; t++) dst[t] = keep[q][t];
     }
