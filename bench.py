@@ -55,6 +55,25 @@ def parse():
     return ap.parse_args()
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tiled_fine_operator_64cube_mixed_batch8_hbm_traffic.txt")
+
+
+def pmc_traffic():
+    """HBM bytes per fine-operator launch from the committed FETCH_SIZE /
+    WRITE_SIZE counter passes (copy-calibrated), with the algorithmic bytes
+    per launch of the same run; None when the summary is absent."""
+    try:
+        vals = {}
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                vals["traffic"] = float(line.split(":")[1])
+            if line.startswith("algorithmic_bytes_per_launch"):
+                vals["alg"] = float(line.split(":")[1])
+        return vals["traffic"], vals["alg"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(n):
     """The oracle (CPU FP64 restatement, multigrid-PCG to 1e-10) on one cell of
     the same workload, timed on this host's cores."""
@@ -162,6 +181,11 @@ def main():
             "bytes_per_launch": dom["bytes"] / max(dom["launches"], 1),
             "flops_frac_fp64_peak": (dom["flops"] / (dom["ms"] * 1e-3) / 1e12) / FP64_PEAK_TFLOPS
             if dom["ms"] > 0 else 0.0}
+    tr = pmc_traffic() if (n == 64 and B == 8) else None
+    if tr:
+        roof["traffic"] = tr[0]
+        roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+        roof["traffic_over_algorithmic"] = tr[0] / tr[1]
     cpu = None if a.no_cpu else cpu_baseline(n)
     iters = [max(s["iters"]) for s in stats]
     out = {

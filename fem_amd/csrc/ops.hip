@@ -109,12 +109,29 @@ int tiled_parts(const Dims &g, int R) {
   return tiles * ((g.nx + TI - 1) / TI);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b to XCD
+// b mod 8), each with its own L2.  xcd_order maps the dispatch index to a
+// logical index so that every XCD runs one contiguous range of logical
+// workgroups: consecutive logical workgroups -- which share halo rows and
+// per-cell data -- then meet in the same L2.
+__device__ __forceinline__ int xcd_order(int b, int n) {
+  const int x = b & 7, slot = b >> 3, per = n >> 3, rem = n & 7;
+  return x * per + (x < rem ? x : rem) + slot;
+}
+
+// grid: one dimension of tiles * chunks * R workgroups; logical order (cell,
+// chunk, tile, load case), load case fastest, so that the six load cases of a
+// tile (same moduli and diagonal) and then the neighbouring tiles follow each
+// other on one XCD
 template <typename T, int ST, int EP>
 __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ E, ModeT<T> mc,
-                                                  OpArgs<T> a, int TI, int tiles_k) {
-  const int r = blockIdx.z;
+                                                  OpArgs<T> a, int TI, int tiles_k, int tiles,
+                                                  int chunks) {
+  const int L = xcd_order(blockIdx.x, gridDim.x);
+  const int lc = L % kRHS, tile = (L / kRHS) % tiles, chunk = (L / (kRHS * tiles)) % chunks;
+  const int smp = L / (kRHS * tiles * chunks);
+  const int r = smp * kRHS + lc;
   if (!a.act[r]) return;
-  const int smp = r / kRHS;
   E += (size_t)smp * g.V;
   const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * g.V : nullptr;
   const T ti = a.theta_inv[smp], c1 = a.c1[smp], c2 = a.c2[smp];
@@ -125,9 +142,9 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
   // rotated by a wave per workgroup, so that the second round of the extra
   // ones does not fall on the same wave -- hence the same SIMD -- in every
   // workgroup; owned nodes (the epilogue) keep the plain thread order
-  const int vt = (tid + 64 * (int)((blockIdx.x + blockIdx.y + blockIdx.z) & 3)) & (kBlock - 1);
-  const int j0 = (blockIdx.x / tiles_k) * kTile, k0 = (blockIdx.x % tiles_k) * kTile;
-  const int i0 = blockIdx.y * TI;
+  const int vt = (tid + 64 * (L & 3)) & (kBlock - 1);
+  const int j0 = (tile / tiles_k) * kTile, k0 = (tile % tiles_k) * kTile;
+  const int i0 = chunk * TI;
   const int iend = min(i0 + TI, g.nx);
   const int V = g.V;
   const size_t off = (size_t)r * 3 * V;
@@ -320,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
   }
   if ((EP == EP_AX || EP == EP_STEP) && a.dot) {
     const double s = block_sum(dv);
-    block_partial(a.red, a.slot, r, blockIdx.y * gridDim.x + blockIdx.x, s);
+    block_partial(a.red, a.slot, r, chunk * tiles + tile, s);
   }
 }
 
@@ -329,8 +346,10 @@ static void tiled_go(const Dims &g, int R, const T *E, const ModeT<T> &mc, const
                      hipStream_t s) {
   const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
   const int TI = tiled_chunk(g, R);
-  dim3 grid(tiles_j * tiles_k, (g.nx + TI - 1) / TI, R);
-  hipLaunchKernelGGL((k_tiled<T, ST, EP>), grid, dim3(kBlock), 0, s, g, E, mc, a, TI, tiles_k);
+  const int tiles = tiles_j * tiles_k, chunks = (g.nx + TI - 1) / TI;
+  if (R % kRHS) throw std::runtime_error("right-hand sides must come in groups of six");
+  hipLaunchKernelGGL((k_tiled<T, ST, EP>), dim3(tiles * chunks * R), dim3(kBlock), 0, s, g, E, mc, a,
+                     TI, tiles_k, tiles, chunks);
 }
 
 template <typename T>
@@ -465,17 +484,20 @@ constexpr int kStTI = 2;  // planes per workgroup on the tiled coarse path
 
 template <typename T, int ST, int EP, int NG>
 __global__ __launch_bounds__(kBlock) void k_stencil_tiled(Dims g, const T *__restrict__ S, OpArgs<T> a,
-                                                          int tiles_k, int sym) {
+                                                          int tiles_k, int tiles, int chunks, int sym) {
   constexpr int G = kRHS / NG;  // load-case groups per sample
-  const int smp = blockIdx.z / G, lc0 = (blockIdx.z % G) * NG;
+  // logical order (cell, chunk, tile, group), XCD-contiguous (see xcd_order)
+  const int L = xcd_order(blockIdx.x, gridDim.x);
+  const int grp = L % G, tile = (L / G) % tiles, chunk = (L / (G * tiles)) % chunks;
+  const int smp = L / (G * tiles * chunks), lc0 = grp * NG;
   unsigned mask = 0;
 #pragma unroll
   for (int t = 0; t < NG; t++) mask |= a.act[smp * kRHS + lc0 + t] ? (1u << t) : 0u;
   if (!mask) return;
   __shared__ T us[3][NG][3][kTN * kTN];
   const int tid = threadIdx.x;
-  const int j0 = (blockIdx.x / tiles_k) * kTile, k0 = (blockIdx.x % tiles_k) * kTile;
-  const int i0 = blockIdx.y * kStTI, iend = min(i0 + kStTI, g.nx);
+  const int j0 = (tile / tiles_k) * kTile, k0 = (tile % tiles_k) * kTile;
+  const int i0 = chunk * kStTI, iend = min(i0 + kStTI, g.nx);
   const int V = g.V, plane = g.ny * g.nz;
   const T *__restrict__ Ss = S + (size_t)smp * 243 * V;
   const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * V : nullptr;
@@ -575,17 +597,17 @@ void launch_stencil(int st, int ep, const Dims &g, int R, const T *S, const OpAr
   // (all load cases of a sample per workgroup in FP32, three in FP64)
   constexpr int NG = sizeof(T) == 4 ? kRHS : kRHS / 2;
   const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
-  const dim3 tgrid(tiles_j * tiles_k, (g.nx + kStTI - 1) / kStTI, nsmp * (kRHS / NG));
+  const int st_tiles = tiles_j * tiles_k, st_chunks = (g.nx + kStTI - 1) / kStTI;
+  const long st_blocks = (long)st_tiles * st_chunks * nsmp * (kRHS / NG);
   // ... when that still gives >= 2 workgroups per CU
-  const bool tiled = g.ny >= kTile && g.nz >= kTile && g.nx >= 3 && R % kRHS == 0 &&
-                     (long)tgrid.x * tgrid.y * tgrid.z >= 512;
+  const bool tiled = g.ny >= kTile && g.nz >= kTile && g.nx >= 3 && R % kRHS == 0 && st_blocks >= 512;
   // all load cases per thread once that still leaves >= 2 workgroups per CU
   const bool multi = (long)blocks * nsmp >= 512 && R % kRHS == 0;
 #define GO(S_, E_)                                                                               \
   if (st == S_ && ep == E_) {                                                                    \
     if (tiled)                                                                                   \
-      hipLaunchKernelGGL((k_stencil_tiled<T, S_, E_, NG>), tgrid, dim3(kBlock), 0, s, g, S, a,   \
-                         tiles_k, sym);                                                          \
+      hipLaunchKernelGGL((k_stencil_tiled<T, S_, E_, NG>), dim3((unsigned)st_blocks), dim3(kBlock), \
+                         0, s, g, S, a, tiles_k, st_tiles, st_chunks, sym);                      \
     else if (multi)                                                                              \
       hipLaunchKernelGGL((k_stencil<T, S_, E_, kRHS>), dim3(blocks, nsmp), dim3(kBlock), 0, s, g, S, \
                          a, sym);                                                                \
