@@ -89,7 +89,7 @@ void femh_default_opts(femh_opts *o) {
   o->levels = 0;
   o->nu1 = 2;
   o->nu2 = 2;
-  o->cheby_ratio = 8.0;
+  o->cheby_ratio = 20.0;
   o->coarse_iters = 64;
   o->tol = 1e-10;
   o->max_iter = 5000;

@@ -764,6 +764,7 @@ void Context::homogenize(double *C, femh_stats *st) {
 
   std::vector<int> iters(R, 0), outer(R, 0);
   std::vector<double> relres(R, 0.0), ch_est(nb, 0.0);
+  fallback_.assign(nb, 0);
   int status = FEMH_OK;
   if (warm) {
     // vanishing right-hand sides restart from zero
@@ -879,7 +880,7 @@ void Context::homogenize(double *C, femh_stats *st) {
       std::vector<int> act_in = active;
       pcg<float>(h32, b32.as<float>(), x32.as<float>(), r32.as<float>(), z32.as<float>(),
                  p32.as<float>(), q32.as<float>(), act_in, o.inner_tol > 0 ? o.inner_tol : 1e-4,
-                 o.max_iter, rr.data(), it_in.data(), rel_in.data(), true);
+                 std::min(o.max_iter, kInnerMaxIter), rr.data(), it_in.data(), rel_in.data(), true);
       set_act(A_OUT, active);
       launch_add_f32_to_f64(g.V, R, x32.as<float>(), x.as<double>(), ao, stream);
       apply<double>(h64, 0, x.as<double>(), r.as<double>(), b.as<double>(), 1, ao, nullptr, 0);
@@ -890,18 +891,54 @@ void Context::homogenize(double *C, femh_stats *st) {
       read_scalars(S_RR, rn.data(), R, vec_parts(g.V));
       read_scalars(S_BX, bxn.data(), R, vec_parts(g.V));
       read_scalars(S_XR, xrn.data(), R, vec_parts(g.V));
+      // A sweep that did not reach its tolerance, or that failed to halve the
+      // FP64 residual, means the FP32 hierarchy does not precondition this
+      // cell (FP32 rounding of operators with a modulus contrast near its
+      // precision can make the coarse levels indefinite); the a-posteriori
+      // estimate, built on the same preconditioner, is then not trusted
+      // either.  Such right-hand sides are finished by FP64 MG-PCG from the
+      // current iterate.
+      std::vector<int> fail(R, 0);
+      bool any_fail = false;
       for (int t = 0; t < R; t++) {
         if (!active[t]) continue;
         if (!std::isfinite(rn[t])) throw Error(FEMH_NONFINITE, "residual became non-finite");
         iters[t] += it_in[t];
         outer[t]++;
+        if (act_in[t] || rn[t] > 0.25 * rr[t]) {
+          fail[t] = 1;
+          any_fail = true;
+        }
         rr[t] = rn[t];
         bx[t] = bxn[t];
         xr[t] = xrn[t];
         relres[t] = std::sqrt(rn[t] / bb[t]);
         if (relres[t] <= o.tol) {
           active[t] = 0;
+          fail[t] = 0;
           nact--;
+        }
+      }
+      if (any_fail) {
+        if (!dx.p) dx.alloc(field_bytes(g, B, sizeof(double)));
+        std::vector<int> it64(R, 0), act64 = fail;
+        std::vector<double> rel64(R, 0.0);
+        // K d = r (relative to the original right-hand side), x += d
+        if (!dq.p) dq.alloc(field_bytes(g, B, sizeof(double)));
+        pcg<double>(h64, r.as<double>(), dx.as<double>(), z.as<double>(), p.as<double>(),
+                    q.as<double>(), dq.as<double>(), act64, o.tol, o.max_iter, bb.data(), it64.data(),
+                    rel64.data());
+        set_act(A_POW, fail);
+        launch_add2<double>(g.V, R, x.as<double>(), dx.as<double>(), x.as<double>(), actp(A_POW), stream);
+        for (int t = 0; t < R; t++) {
+          if (!fail[t]) continue;
+          iters[t] += it64[t];
+          outer[t]++;
+          fallback_[t / kRHS] = 1;
+          relres[t] = rel64[t];
+          active[t] = 0;
+          nact--;
+          if (act64[t]) status = FEMH_NOT_CONVERGED;
         }
       }
     }
@@ -926,7 +963,8 @@ void Context::homogenize(double *C, femh_stats *st) {
         o_.outer[k] = outer[kRHS * s + k];
         o_.relres[k] = relres[kRHS * s + k];
       }
-      o_.ch_err_est = ch_est[s];
+      o_.ch_err_est = fallback_[s] ? 0.0 : ch_est[s];
+      o_.fp64_fallback = fallback_[s];
       o_.ms_setup = ms1;
       o_.ms_solve = ms2;
       o_.ms_total = ms_total;

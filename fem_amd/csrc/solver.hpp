@@ -77,6 +77,8 @@ struct Hierarchy {
 
 // Chebyshev interval [lmax / ratio, lmax] of the iterative coarsest solve
 constexpr double kCoarseChebRatio = 1000.0;
+// iteration cap of one FP32 correction sweep in mixed precision
+constexpr int kInnerMaxIter = 1000;
 
 // activity-array slots (device int[kMaxR] each)
 enum ActSlot : int { A_ALL = 0, A_PCG = 1, A_OUT = 2, A_POW = 3, kActSlots = 4 };
@@ -107,6 +109,8 @@ struct Context {
   bool have_fields = false;
   int prev_nb = 0;  // samples of the last successful call (warm start source)
   DBuf xw;          // warm start: previous fields [R][3][V] (FP64)
+  DBuf dx, dq;      // mixed precision: FP64 fallback correction and PCG scratch
+  std::vector<int> fallback_;  // per sample: finished in FP64 this call
   bool modulus_input = false;  // last load gave moduli (dE/drho = 1)
   int nlevels = 1;
 

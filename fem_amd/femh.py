@@ -55,12 +55,13 @@ class Stats(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int * 6), ("outer", ctypes.c_int * 6),
                 ("relres", ctypes.c_double * 6), ("ch_err_est", ctypes.c_double),
                 ("ms_setup", ctypes.c_double), ("ms_solve", ctypes.c_double),
-                ("ms_total", ctypes.c_double), ("levels", ctypes.c_int)]
+                ("ms_total", ctypes.c_double), ("levels", ctypes.c_int),
+                ("fp64_fallback", ctypes.c_int)]
 
     def as_dict(self):
         return dict(iters=list(self.iters), outer=list(self.outer), relres=list(self.relres),
                     ch_err_est=self.ch_err_est, ms_setup=self.ms_setup, ms_solve=self.ms_solve,
-                    ms_total=self.ms_total, levels=self.levels)
+                    ms_total=self.ms_total, levels=self.levels, fp64_fallback=self.fp64_fallback)
 
 
 def lib():
@@ -107,7 +108,7 @@ def _ptr(a):
 
 
 def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond="mg",
-              levels=0, nu1=2, nu2=2, cheby_ratio=8.0, coarse_iters=64, tol=1e-10,
+              levels=0, nu1=2, nu2=2, cheby_ratio=20.0, coarse_iters=64, tol=1e-10,
               max_iter=5000, precision="fp64", ch_tol=0.0, check_every=1, device=0,
               inner_tol=1e-4, max_outer=12, apply_kernel="tiled", batch=1, warm_start=False):
     o = Opts()

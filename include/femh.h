@@ -105,6 +105,9 @@ typedef struct femh_stats {
   double ms_solve;      /* device time: PCG */
   double ms_total;      /* wall time of the call */
   int levels;           /* multigrid levels used */
+  int fp64_fallback;    /* mixed precision: 1 when an FP32 correction sweep failed to
+                           converge for this sample and FP64 MG-PCG finished it (the
+                           error is then that of the FP64 tolerance; ch_err_est is 0) */
 } femh_stats;
 
 void femh_default_opts(femh_opts *o);

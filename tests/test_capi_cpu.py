@@ -48,6 +48,22 @@ def test_opts_struct_layout_matches_header():
     assert fields == [f[0] for f in femh.Opts._fields_]
 
 
+def test_stats_struct_layout_matches_header():
+    src = open(os.path.join(ROOT, "include", "femh.h")).read()
+    start = src.index("typedef struct")
+    start = src.index("typedef struct", src.index("} femh_opts;"))
+    body = src[start:src.index("} femh_stats;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for line in body.splitlines()[1:]:
+        line = line.strip().rstrip(";")
+        if not line:
+            continue
+        typ, rest = line.split(None, 1)
+        fields += [re.sub(r"\[.*\]", "", f).strip() for f in rest.split(",")]
+    assert fields == [f[0] for f in femh.Stats._fields_]
+
+
 def test_host_element_table_is_symmetric_psd():
     K = femh.element_stiffness(1.0, 0.3)
     assert np.abs(K - K.T).max() < 1e-15

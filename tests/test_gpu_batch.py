@@ -152,3 +152,18 @@ def test_uneven_binary_batch_mixed_meets_1e6():
     C64, _ = homogenize_batch(grids, **kw)
     errs = [rel(a, b) for a, b in zip(Cm, C64)]
     assert max(errs) <= 1e-6, errs
+
+
+def test_mixed_precision_falls_back_to_fp64_when_fp32_sweeps_fail():
+    """Cell 143 of the 512-cell uneven workload (40^3 binary, fill 0.64,
+    E_void 1e-9): its FP32 correction sweeps stall, and its a-posteriori
+    estimate is then wrong by orders of magnitude.  The mixed path must
+    notice, finish the cell in FP64 and still meet 1e-6."""
+    g = W.batch_uneven(512, 3)[143]
+    assert g.shape == (40, 40, 40)
+    kw = dict(nu=NU, max_iter=20000, interp="modulus")
+    Cm, st = Homogenizer(g.shape, precision="mixed", tol=1e-10, ch_tol=1e-7, inner_tol=3e-4,
+                         **kw).homogenize_modulus(g)
+    C64, _ = Homogenizer(g.shape, tol=1e-12, **kw).homogenize_modulus(g)
+    assert rel(Cm, C64) <= 1e-6, (rel(Cm, C64), st)
+    assert max(st["iters"]) < 20000, st
