@@ -34,6 +34,9 @@ constexpr int kTN = kTile + 2;   // staged nodes per row
 constexpr int kTE = kTile + 1;   // voxels per row
 constexpr int kEOH = 13;         // LDS stride of one voxel's 12 plane forces (odd: no bank conflicts)
 
+// periodic index for i in [-n, 2n) without a division
+__device__ __forceinline__ int wrap1(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
 // epilogue inputs of one owned node (loaded ahead of the operator work so
 // that their latency overlaps it):
 //   EP_RES: e0 = b ; EP_CHEB0: e0 = b, e1 = 1/D ; EP_STEP: e0 = rprev, e1 = 1/D,
@@ -57,19 +60,31 @@ __device__ __forceinline__ void epi_load(const OpArgs<T> &a, const T *__restrict
   }
 }
 
+// FP64 partial sums of the epilogue: EP_AX u.Au ; EP_STEP b.x_new ;
+// EP_RES r.r, b.x, x.r (x = the staged vector)
+struct DotAcc {
+  double a = 0, b = 0, c = 0;
+};
+
 // epilogue at one owned node (all three components)
 template <typename T, int EP>
 __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, T ti, T c1, T c2,
                                          size_t off, int V, int n, const T Au[3], const T uo[3],
-                                         double &dv) {
+                                         DotAcc &dv) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const size_t q = off + (size_t)c * V + n;
     if (EP == EP_AX) {
       a.y[q] = Au[c];
-      if (a.dot) dv += (double)uo[c] * (double)Au[c];
+      if (a.dot) dv.a += (double)uo[c] * (double)Au[c];
     } else if (EP == EP_RES) {
-      a.y[q] = p.e0[c] - Au[c];
+      const T rv = p.e0[c] - Au[c];
+      a.y[q] = rv;
+      if (a.dot) {
+        dv.a += (double)rv * (double)rv;
+        dv.b += (double)p.e0[c] * (double)uo[c];
+        dv.c += (double)uo[c] * (double)rv;
+      }
     } else if (EP == EP_CHEB0) {
       const T rv = p.e0[c] - Au[c];
       a.r[q] = rv;
@@ -86,7 +101,7 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, 
         a.dout[q] = dn;
       }
       a.x[q] = xv;
-      if (a.dot) dv += (double)p.e3[c] * (double)xv;
+      if (a.dot) dv.a += (double)p.e3[c] * (double)xv;
     }
   }
 }
@@ -159,6 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
     soff[q] = -1;
     if (t < kTN * kTN) {
       const int nj = t / kTN, nk = t - nj * kTN;
+      // slots of a tile wider than the grid wrap more than once
       const int jj = ((j0 - 1 + nj) % g.ny + g.ny) % g.ny, kk = ((k0 - 1 + nk) % g.nz + g.nz) % g.nz;
       soff[q] = jj * g.nz + kk;
       if (ST == ST_PROLONG) {
@@ -190,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
   // sample only) stages directly.
   T pf[2][3], pd[2][3];
   auto fetch = [&](int ip) {
-    const int iw = ((ip % g.nx) + g.nx) % g.nx;
+    const int iw = wrap1(ip, g.nx);
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       if (soff[q] < 0) continue;
@@ -217,7 +233,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
       put(buf);
       return;
     }
-    const int iw = ((ip % g.nx) + g.nx) % g.nx;
+    const int iw = wrap1(ip, g.nx);
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       if (soff[q] < 0) continue;
@@ -249,13 +265,17 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
   const bool owns = (gj < g.ny) && (gk < g.nz);
   const int ot = (nj + 1) * kTN + nk + 1;  // owned node in the staged plane
   T acc[3] = {0, 0, 0};
-  double dv = 0;
+  DotAcc dv;
 
+  // FP32: each voxel thread keeps its top corner values for the next slab
+  // (halves the LDS gathers; FP64 would spend too many registers)
+  constexpr bool kReuse = sizeof(T) == 4;
+  T utop[2][12];
   stage(0, i0 - 1);
   if (ST != ST_PROLONG) fetch(i0);
   for (int sl = i0 - 1, p = 0; sl < iend; sl++, p++) {
     const int cur = p % 3, nxt = (p + 1) % 3;
-    const int sw = ((sl % g.nx) + g.nx) % g.nx;
+    const int sw = wrap1(sl, g.nx);
     // loads in the order they are consumed: moduli of this slab, epilogue
     // inputs of plane sl, then plane sl + 2 for the next slab
     T Ev[2];
@@ -282,10 +302,21 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
       for (int v = 0; v < 8; v++) {
         const int bi = (v >> 2) & 1, bj = (v >> 1) & 1, bk = v & 1;
         const int t = (ej + bj) * kTN + ek + bk;
-        const int bb = bi ? nxt : cur;
-        uv[3 * v] = up[bb][0][t];
-        uv[3 * v + 1] = up[bb][1][t];
-        uv[3 * v + 2] = up[bb][2][t];
+        if (kReuse && bi == 0 && p > 0) {
+          // the bottom corners are the previous slab's top corners
+          uv[3 * v] = utop[q][3 * v];
+          uv[3 * v + 1] = utop[q][3 * v + 1];
+          uv[3 * v + 2] = utop[q][3 * v + 2];
+        } else {
+          const int bb = bi ? nxt : cur;
+          uv[3 * v] = up[bb][0][t];
+          uv[3 * v + 1] = up[bb][1][t];
+          uv[3 * v + 2] = up[bb][2][t];
+        }
+      }
+      if (kReuse) {
+#pragma unroll
+        for (int t = 0; t < 12; t++) utop[q][t] = uv[12 + t];
       }
       hadamard_voxel<T>(uv, Ev[q], mc, fv);
       T *dst = eo + e * kEOH;
@@ -335,9 +366,13 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
         for (int c = 0; c < 3; c++) acc[c] += src[c];
       }
   }
-  if ((EP == EP_AX || EP == EP_STEP) && a.dot) {
-    const double s = block_sum(dv);
-    block_partial(a.red, a.slot, r, chunk * tiles + tile, s);
+  if ((EP == EP_AX || EP == EP_STEP || EP == EP_RES) && a.dot) {
+    const int pidx = chunk * tiles + tile;
+    block_partial(a.red, a.slot, r, pidx, block_sum(dv.a));
+    if (EP == EP_RES) {
+      block_partial(a.red, a.slot2, r, pidx, block_sum(dv.b));
+      block_partial(a.red, a.slot3, r, pidx, block_sum(dv.c));
+    }
   }
 }
 
@@ -442,7 +477,7 @@ __device__ __forceinline__ void stencil_node(const Dims &g, const T *__restrict_
     T uo[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) uo[c] = stage_at<T, ST>(a, Ds, ti, off, V, c, n);
-    double dv = 0;
+    DotAcc dv;
     EpiIn<T> pin;
     epi_load<T, EP>(a, Ds, off, V, n, pin);
     epilogue<T, EP>(a, pin, ti, a.c1[smp], a.c2[smp], off, V, n, acc[t], uo, dv);
@@ -510,12 +545,12 @@ __global__ __launch_bounds__(kBlock) void k_stencil_tiled(Dims g, const T *__res
     soff[q] = -1;
     if (t < kTN * kTN) {
       const int nj = t / kTN, nk = t - nj * kTN;
-      const int jj = (j0 - 1 + nj + g.ny) % g.ny, kk = (k0 - 1 + nk + g.nz) % g.nz;
+      const int jj = ((j0 - 1 + nj) % g.ny + g.ny) % g.ny, kk = ((k0 - 1 + nk) % g.nz + g.nz) % g.nz;
       soff[q] = jj * g.nz + kk;
     }
   }
   auto stage = [&](int buf, int ip) {
-    const int iw = (ip % g.nx + g.nx) % g.nx;
+    const int iw = wrap1(ip, g.nx);
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       if (soff[q] < 0) continue;
@@ -532,8 +567,8 @@ __global__ __launch_bounds__(kBlock) void k_stencil_tiled(Dims g, const T *__res
   const int nj = tid / kTile, nk = tid % kTile;
   const int gj = j0 + nj, gk = k0 + nk;
   const bool owns = gj < g.ny && gk < g.nz;
-  const int Jn[3] = {(gj - 1 + g.ny) % g.ny, gj % g.ny, (gj + 1) % g.ny};
-  const int Kn[3] = {(gk - 1 + g.nz) % g.nz, gk % g.nz, (gk + 1) % g.nz};
+  const int Jn[3] = {wrap1(gj - 1, g.ny), wrap1(gj, g.ny), wrap1(gj + 1, g.ny)};
+  const int Kn[3] = {wrap1(gk - 1, g.nz), wrap1(gk, g.nz), wrap1(gk + 1, g.nz)};
   stage(0, i0 - 1);
   stage(1, i0);
   for (int i = i0, p = 0; i < iend; i++, p++) {
@@ -541,7 +576,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil_tiled(Dims g, const T *__res
     stage(bp, i + 1);
     __syncthreads();
     if (owns) {
-      const int In[3] = {(i - 1 + g.nx) % g.nx, i, (i + 1) % g.nx};
+      const int In[3] = {wrap1(i - 1, g.nx), i, wrap1(i + 1, g.nx)};
       const int n = (i * g.ny + gj) * g.nz + gk;
       T acc[NG][3];
 #pragma unroll
@@ -577,7 +612,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil_tiled(Dims g, const T *__res
         if (!((mask >> t) & 1u)) continue;
         const size_t off = off0 + (size_t)t * 3 * V;
         T uo[3] = {us[b0][t][0][own], us[b0][t][1][own], us[b0][t][2][own]};
-        double dv = 0;
+        DotAcc dv;
         EpiIn<T> pin;
         epi_load<T, EP>(a, Ds, off, V, n, pin);
         epilogue<T, EP>(a, pin, ti, a.c1[smp], a.c2[smp], off, V, n, acc[t], uo, dv);

@@ -10,7 +10,7 @@
 // Epilogue at each owned node, with Au the operator result and u_own the
 // staged value there:
 //   EP_AX       y = Au            (optional FP64 partial of u . Au)
-//   EP_RES      y = b - Au
+//   EP_RES      y = b - Au            (optional FP64 partials of r.r, b.u, u.r)
 //   EP_CHEB0    r = b - Au ; dout = r theta_inv / D (not with `last`) ; x = u_own
 //   EP_STEP     r = rprev - Au ; dn = c1 u_own + c2 r / D ; dout = dn ;
 //               (with `dot`, fine level: the FP64 partial of b . x_new)
@@ -45,7 +45,8 @@ struct OpArgs {
   const int *act = nullptr;  // [R] activity flags
   Reduce red{};
   int slot = 0;
-  int dot = 0;
+  int dot = 0;               // EP_RES: r.r -> slot, b.x -> slot2, x.r -> slot3
+  int slot2 = 0, slot3 = 0;
   T theta_inv[kMaxB] = {};   // per sample
   T c1[kMaxB] = {}, c2[kMaxB] = {};
 };

@@ -85,7 +85,7 @@ Context::Context(const femh_opts &opts) : o(opts) {
                           stream));
   for (DBuf *bf : {&b, &x, &r, &z, &p, &q}) bf->alloc(field_bytes(g, B, sizeof(double)));
   if (o.precision == FEMH_MIXED)
-    for (DBuf *bf : {&b32, &r32, &x32, &z32, &p32, &q32}) bf->alloc(field_bytes(g, B, sizeof(float)));
+    for (DBuf *bf : {&r32, &x32, &z32, &p32, &q32}) bf->alloc(field_bytes(g, B, sizeof(float)));
   if (o.warm_start && o.precision != FEMH_MIXED) xw.alloc(field_bytes(g, B, sizeof(double)));
   red_part.alloc(sizeof(double) * kNumSlots * kSlotWidth * kMaxPartials);
   red_ctr.alloc(sizeof(unsigned) * kNumSlots * kSlotWidth);
@@ -202,6 +202,31 @@ void Context::op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a) {
   } else {
     launch_stencil<T>(st, ep, L.g, R, L.S.template as<T>(), a, stream);
   }
+}
+
+// r = b - K x in FP64 with r.r, b.x and x.r of every flagged RHS into S_RR,
+// S_BX and S_XR (one fused launch on the tiled path); returns their partial
+// count
+int Context::residual_dots(const int *act_) {
+  if (o.apply_kernel == 0) {
+    OpArgs<double> a;
+    a.u = x.as<double>();
+    a.y = r.as<double>();
+    a.b = b.as<double>();
+    a.act = act_;
+    a.red = red;
+    a.slot = S_RR;
+    a.slot2 = S_BX;
+    a.slot3 = S_XR;
+    a.dot = 1;
+    op<double>(h64, 0, ST_PLAIN, EP_RES, a);
+    return tiled_parts(g, R);
+  }
+  apply<double>(h64, 0, x.as<double>(), r.as<double>(), b.as<double>(), 1, act_, nullptr, 0);
+  launch_dot<double>(g.V, R, r.as<double>(), r.as<double>(), S_RR, act_, red, stream);
+  launch_dot<double>(g.V, R, b.as<double>(), x.as<double>(), S_BX, act_, red, stream);
+  launch_dot<double>(g.V, R, x.as<double>(), r.as<double>(), S_XR, act_, red, stream);
+  return vec_parts(g.V);
 }
 
 int Context::apply_parts() const { return o.apply_kernel == 1 ? dense_parts(g) : tiled_parts(g, R); }
@@ -619,7 +644,9 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
   const bool jac = !(o.precond == FEMH_PRECOND_MG && h.lv.size() > 1);
   const T *D0 = h.lv[0].Dinv.template as<T>();
   FEMH_HIP(hipMemsetAsync(xv, 0, field_bytes(g, nb, sizeof(T)), stream));
-  FEMH_HIP(hipMemcpyAsync(rv, bvec, field_bytes(g, nb, sizeof(T)), hipMemcpyDeviceToDevice, stream));
+  // z_ready: the caller also left r = b in rv
+  if (!z_ready)
+    FEMH_HIP(hipMemcpyAsync(rv, bvec, field_bytes(g, nb, sizeof(T)), hipMemcpyDeviceToDevice, stream));
   for (int t = 0; t < R; t++) {
     iters[t] = 0;
     relres[t] = 0.0;
@@ -816,13 +843,10 @@ void Context::homogenize(double *C, femh_stats *st) {
     std::vector<double> rr = bb;
     std::vector<double> bx(R, 0.0), xr(R, 0.0), rz(R), rn(R), bxn(R), xrn(R);
     if (warm) {
-      apply<double>(h64, 0, x.as<double>(), r.as<double>(), b.as<double>(), 1, actp(A_ALL), nullptr, 0);
-      launch_dot<double>(g.V, R, r.as<double>(), r.as<double>(), S_RR, actp(A_ALL), red, stream);
-      launch_dot<double>(g.V, R, b.as<double>(), x.as<double>(), S_BX, actp(A_ALL), red, stream);
-      launch_dot<double>(g.V, R, x.as<double>(), r.as<double>(), S_XR, actp(A_ALL), red, stream);
-      read_scalars(S_RR, rr.data(), R, vec_parts(g.V));
-      read_scalars(S_BX, bx.data(), R, vec_parts(g.V));
-      read_scalars(S_XR, xr.data(), R, vec_parts(g.V));
+      const int npd = residual_dots(actp(A_ALL));
+      read_scalars(S_RR, rr.data(), R, npd);
+      read_scalars(S_BX, bx.data(), R, npd);
+      read_scalars(S_XR, xr.data(), R, npd);
       for (int t = 0; t < R; t++)
         if (active[t]) {
           relres[t] = std::sqrt(rr[t] / bb[t]);
@@ -847,10 +871,9 @@ void Context::homogenize(double *C, femh_stats *st) {
     for (int sweep = 0; sweep < max_outer && nact; sweep++) {
       set_act(A_OUT, active);
       const int *ao = actp(A_OUT);
-      launch_convert<double, float>((size_t)R * 3 * g.V, r.as<double>(), b32.as<float>(), stream);
-      // z = M^-1 r and r.z for the estimate; reused as the first PCG step
-      FEMH_HIP(hipMemcpyAsync(r32.p, b32.p, field_bytes(g, nb, sizeof(float)), hipMemcpyDeviceToDevice,
-                              stream));
+      // the FP32 sweep solves K d = r: its right-hand side is the residual
+      // itself; z = M^-1 r and r.z serve the estimate and the first PCG step
+      launch_convert<double, float>((size_t)R * 3 * g.V, r.as<double>(), r32.as<float>(), stream);
       precond<float>(h32, r32.as<float>(), z32.as<float>(), ao, S_RZ0);
       FEMH_HIP(hipGetLastError());
       if (o.ch_tol > 0 && (sweep > 0 || warm)) {
@@ -878,19 +901,16 @@ void Context::homogenize(double *C, femh_stats *st) {
         if (!nact) break;
       }
       std::vector<int> act_in = active;
-      pcg<float>(h32, b32.as<float>(), x32.as<float>(), r32.as<float>(), z32.as<float>(),
+      pcg<float>(h32, r32.as<float>(), x32.as<float>(), r32.as<float>(), z32.as<float>(),
                  p32.as<float>(), q32.as<float>(), act_in, o.inner_tol > 0 ? o.inner_tol : 1e-4,
                  std::min(o.max_iter, kInnerMaxIter), rr.data(), it_in.data(), rel_in.data(), true);
       set_act(A_OUT, active);
       launch_add_f32_to_f64(g.V, R, x32.as<float>(), x.as<double>(), ao, stream);
-      apply<double>(h64, 0, x.as<double>(), r.as<double>(), b.as<double>(), 1, ao, nullptr, 0);
-      launch_dot<double>(g.V, R, r.as<double>(), r.as<double>(), S_RR, ao, red, stream);
-      launch_dot<double>(g.V, R, b.as<double>(), x.as<double>(), S_BX, ao, red, stream);
-      launch_dot<double>(g.V, R, x.as<double>(), r.as<double>(), S_XR, ao, red, stream);
+      const int npd = residual_dots(ao);
       FEMH_HIP(hipGetLastError());
-      read_scalars(S_RR, rn.data(), R, vec_parts(g.V));
-      read_scalars(S_BX, bxn.data(), R, vec_parts(g.V));
-      read_scalars(S_XR, xrn.data(), R, vec_parts(g.V));
+      read_scalars(S_RR, rn.data(), R, npd);
+      read_scalars(S_BX, bxn.data(), R, npd);
+      read_scalars(S_XR, xrn.data(), R, npd);
       // A sweep that did not reach its tolerance, or that failed to halve the
       // FP64 residual, means the FP32 hierarchy does not precondition this
       // cell (FP32 rounding of operators with a modulus contrast near its
@@ -944,7 +964,9 @@ void Context::homogenize(double *C, femh_stats *st) {
     }
     if (nact) status = FEMH_NOT_CONVERGED;
   }
-  launch_remove_mean<double>(g.V, R, x.as<double>(), actp(A_ALL), red, stream);
+  // C_H and the adjoint do not see constant shifts of the fields (K0 maps
+  // them to zero); the mean is taken out when the fields are read out
+  fields_centered = false;
   FEMH_HIP(hipEventRecord(ev[2], stream));
   compute_ch(x.as<double>(), C);
   flush_profile();
@@ -1003,6 +1025,12 @@ void Context::gradient(const double *dJdC, double *out_host, bool energy_only) {
 
 void Context::get_fields(double *u) {
   if (!have_fields) throw Error(FEMH_NO_FIELDS, "no solved fields: call femh_homogenize first");
+  if (!fields_centered) {
+    std::vector<int> all(R, 1);
+    set_act(A_POW, all);
+    launch_remove_mean<double>(g.V, R, x.as<double>(), actp(A_POW), red, stream);
+    fields_centered = true;
+  }
   FEMH_HIP(hipMemcpyAsync(u, x.p, field_bytes(g, nb, sizeof(double)), hipMemcpyDeviceToHost, stream));
   FEMH_HIP(hipStreamSynchronize(stream));
 }

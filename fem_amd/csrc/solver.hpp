@@ -99,7 +99,7 @@ struct Context {
   // FP64 solver state, each [R][3][V]
   DBuf b, x, r, z, p, q;
   // FP32 inner solver state (mixed)
-  DBuf b32, r32, x32, z32, p32, q32;
+  DBuf r32, x32, z32, p32, q32;
   DBuf red_part, red_ctr, red_scal;
   Reduce red{};
   DBuf ch_part, ch_out;
@@ -107,6 +107,7 @@ struct Context {
   Hierarchy<double> h64;
   Hierarchy<float> h32;
   bool have_fields = false;
+  bool fields_centered = false;  // x has zero mean per RHS and component
   int prev_nb = 0;  // samples of the last successful call (warm start source)
   DBuf xw;          // warm start: previous fields [R][3][V] (FP64)
   DBuf dx, dq;      // mixed precision: FP64 fallback correction and PCG scratch
@@ -174,6 +175,7 @@ struct Context {
   template <typename T>
   void cheb_run(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_, int k, double ratio);
   int apply_parts() const;
+  int residual_dots(const int *act_);
   template <typename T>
   void estimate_lmax(Hierarchy<T> &h);
   void dense_coarse(Hierarchy<double> &h);
