@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--batch", type=int, default=8, help="cells per step (1..8)")
+    ap.add_argument("--batch", type=int, default=16, help="cells per step (1..16)")
     ap.add_argument("--ch-tol", type=float, default=1e-7,
                     help="a-posteriori C_H error target of the mixed path (the metric allows 1e-6; "
                          "the estimate has stayed within 3x of the true error)")
@@ -184,7 +184,7 @@ def main():
             "bytes_per_launch": dom["bytes"] / max(dom["launches"], 1),
             "flops_frac_fp64_peak": (dom["flops"] / (dom["ms"] * 1e-3) / 1e12) / FP64_PEAK_TFLOPS
             if dom["ms"] > 0 else 0.0}
-    tr = pmc_traffic() if (n == 64 and B == 8) else None
+    tr = pmc_traffic() if (n == 64 and B == 8) else None  # the configuration it was measured on
     if tr:
         roof["traffic"] = tr[0]
         roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)

@@ -26,7 +26,7 @@
 namespace femh {
 
 constexpr int kRHS = 6;             // load cases per sample
-constexpr int kMaxB = 8;            // samples per context
+constexpr int kMaxB = 16;           // samples per context
 constexpr int kMaxR = kRHS * kMaxB; // right-hand sides per context
 constexpr int kBlock = 256;
 constexpr int kMaxPartials = 1024;  // partial sums per RHS per reduction

@@ -90,7 +90,7 @@ typedef struct femh_opts {
   double inner_tol;     /* mixed: relative tolerance of each FP32 correction solve */
   int max_outer;        /* mixed: refinement sweep cap */
   int apply_kernel;     /* fine-grid operator: 0 tiled cube-mode (default), 1 dense gather */
-  int batch;            /* samples solved together per launch sequence (1..8) */
+  int batch;            /* samples solved together per launch sequence (1..16) */
   int warm_start;       /* 1: start each sample slot from the fields of the previous
                            call with the same sample count (an optimisation loop's
                            previous iterate); 0: start from zero */

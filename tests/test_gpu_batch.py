@@ -21,13 +21,13 @@ def simp_set(n, edge, seed0=20, corr=3.0):
     return np.stack([W.simp_density(edge, seed0 + s, corr) for s in range(n)])
 
 
-@pytest.mark.parametrize("batch", [2, 4, 8])
+@pytest.mark.parametrize("batch", [2, 4, 8, 16])
 def test_fp64_batch_equals_single_sample_solves(batch):
-    rho = simp_set(6, 16)
+    rho = simp_set(6 if batch < 16 else 16, 16)
     hb = Homogenizer((16, 16, 16), nu=NU, tol=1e-11, batch=batch)
     Cb, stb = hb.homogenize_many(rho)
     h1 = Homogenizer((16, 16, 16), nu=NU, tol=1e-11)
-    for s in range(6):
+    for s in range(len(rho)):
         C1, st1 = h1.homogenize(rho[s])
         assert rel(Cb[s], C1) <= 1e-13
         assert stb[s]["iters"] == st1["iters"]
@@ -130,7 +130,7 @@ def test_uneven_batch_uses_batched_groups():
 
 def test_batch_errors():
     with pytest.raises(FemhError) as e:
-        Homogenizer((8, 8, 8), batch=9)
+        Homogenizer((8, 8, 8), batch=17)
     assert e.value.status == BAD_ARG
     h = Homogenizer((8, 8, 8), nu=NU, batch=2)
     rho = simp_set(2, 8)
