@@ -57,7 +57,7 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tiled_fine_operator_64cube_mixed_batch8_hbm_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tiled_fine_operator_64cube_mixed_batch16_hbm_traffic.txt")
 
 
 def pmc_traffic():
@@ -184,7 +184,7 @@ def main():
             "bytes_per_launch": dom["bytes"] / max(dom["launches"], 1),
             "flops_frac_fp64_peak": (dom["flops"] / (dom["ms"] * 1e-3) / 1e12) / FP64_PEAK_TFLOPS
             if dom["ms"] > 0 else 0.0}
-    tr = pmc_traffic() if (n == 64 and B == 8) else None  # the configuration it was measured on
+    tr = pmc_traffic() if (n == 64 and B == 16) else None  # the configuration it was measured on
     if tr:
         roof["traffic"] = tr[0]
         roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
