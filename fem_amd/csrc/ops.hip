@@ -629,8 +629,8 @@ void launch_stencil(int st, int ep, const Dims &g, int R, const T *S, const OpAr
   const int nsmp = (R + kRHS - 1) / kRHS;
   const int sym = (g.nx > 2 && g.ny > 2 && g.nz > 2) ? 1 : 0;
   // planes of at least one full 16x16 tile: the LDS-tiled marching kernel
-  // (all load cases of a sample per workgroup in FP32, three in FP64)
-  constexpr int NG = sizeof(T) == 4 ? kRHS : kRHS / 2;
+  // (three load cases of a sample per workgroup: 35 KB of LDS in FP32)
+  constexpr int NG = kRHS / 2;
   const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
   const int st_tiles = tiles_j * tiles_k, st_chunks = (g.nx + kStTI - 1) / kStTI;
   const long st_blocks = (long)st_tiles * st_chunks * nsmp * (kRHS / NG);
