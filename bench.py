@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solves")
     ap.add_argument("--nu1", type=int, default=2)
     ap.add_argument("--nu2", type=int, default=2)
+    ap.add_argument("--coarse-nu", type=int, default=3,
+                    help="Chebyshev degree on the coarse levels (0: --nu1/--nu2)")
     ap.add_argument("--cheby-ratio", type=float, default=20.0,
                     help="Chebyshev smoothing interval [lmax/ratio, lmax]")
     ap.add_argument("--workload", default="simp64", choices=["simp64", "uneven512", "opt96"],
@@ -131,7 +133,7 @@ def main():
 
     mixed = Homogenizer(shape, nu=0.3, penal=3.0, precision="mixed", tol=1e-10,
                         ch_tol=a.ch_tol, inner_tol=a.inner_tol, batch=B, nu1=a.nu1, nu2=a.nu2,
-                        cheby_ratio=a.cheby_ratio)
+                        cheby_ratio=a.cheby_ratio, coarse_nu=a.coarse_nu)
 
     def step(k, stats=None):
         st = mixed.homogenize_many_device(B, d_rho[k * B].data_ptr(), d_C[k * B].data_ptr())

@@ -102,6 +102,7 @@ void femh_default_opts(femh_opts *o) {
   o->apply_kernel = 0;
   o->batch = 1;
   o->warm_start = 0;
+  o->coarse_nu = 3;
 }
 
 int femh_create(const femh_opts *o, femh_ctx **out) {

@@ -54,7 +54,8 @@ Context::Context(const femh_opts &opts) : o(opts) {
     throw Error(FEMH_BAD_ARG, "unknown preconditioner");
   if (o.precision != FEMH_FP64 && o.precision != FEMH_MIXED)
     throw Error(FEMH_BAD_ARG, "unknown precision");
-  if (o.nu1 < 1 || o.nu2 < 1 || o.nu1 > kMaxCheb || o.nu2 > kMaxCheb)
+  if (o.nu1 < 1 || o.nu2 < 1 || o.nu1 > kMaxCheb || o.nu2 > kMaxCheb || o.coarse_nu < 0 ||
+      o.coarse_nu > kMaxCheb)
     throw Error(FEMH_BAD_ARG, "smoothing degrees must lie in [1,16]");
   if (!(o.cheby_ratio > 1.0)) throw Error(FEMH_BAD_ARG, "cheby_ratio must exceed 1");
   if (!(o.tol > 0) || o.max_iter < 1) throw Error(FEMH_BAD_ARG, "bad tolerance / iteration cap");
@@ -516,7 +517,7 @@ void Context::cheb_run(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *ac
 
 template <typename T>
 void Context::pre_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_) {
-  cheb_run<T>(h, l, bb, xx, act_, o.nu1, o.cheby_ratio);
+  cheb_run<T>(h, l, bb, xx, act_, nu_pre(l), o.cheby_ratio);
 }
 
 // Returns true when the last launch also left the partials of b . x in
@@ -525,7 +526,7 @@ template <typename T>
 bool Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_, int dot_slot) {
   Level<T> &L = h.lv[l];
   Level<T> &C = h.lv[l + 1];
-  const int k = o.nu2;
+  const int k = nu_post(l);
   OpArgs<T> a;
   a.act = act_;
   a.u = xx;
@@ -586,7 +587,7 @@ bool Context::vcycle(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_
   }
   Level<T> &L = h.lv[l];
   Level<T> &C = h.lv[l + 1];
-  if (o.nu1 == 1) {
+  if (nu_pre(l) == 1) {
     // degree-1 pre-smoothing x = b / (theta D) needs no operator apply: one
     // launch stages it and leaves the residual b - A x for the restriction
     OpArgs<T> a;

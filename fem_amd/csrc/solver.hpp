@@ -175,6 +175,9 @@ struct Context {
   template <typename T>
   void cheb_run(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_, int k, double ratio);
   int apply_parts() const;
+  // Chebyshev degrees of pre- and post-smoothing on level l
+  int nu_pre(int l) const { return (l > 0 && o.coarse_nu > 0) ? o.coarse_nu : o.nu1; }
+  int nu_post(int l) const { return (l > 0 && o.coarse_nu > 0) ? o.coarse_nu : o.nu2; }
   int residual_dots(const int *act_);
   template <typename T>
   void estimate_lmax(Hierarchy<T> &h);

@@ -91,6 +91,8 @@ typedef struct femh_opts {
   int max_outer;        /* mixed: refinement sweep cap */
   int apply_kernel;     /* fine-grid operator: 0 tiled cube-mode (default), 1 dense gather */
   int batch;            /* samples solved together per launch sequence (1..16) */
+  int coarse_nu;        /* smoothing degree (pre and post) on the coarse levels; 0: nu1/nu2
+                           (default 3: coarse sweeps are cheap and save fine ones) */
   int warm_start;       /* 1: start each sample slot from the fields of the previous
                            call with the same sample count (an optimisation loop's
                            previous iterate); 0: start from zero */
