@@ -508,33 +508,47 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Dims g, const T *__restrict_
     stencil_node<T, ST, EP, NR>(g, S, a, n, smp, lc0, mask, sym != 0);
 }
 
-// Coarse levels whose planes hold at least a 16x16 tile: LDS-tiled marching
-// form of the stencil kernel.  A workgroup owns a 16x16 (j,k) node tile of
-// one sample and marches over TI planes in i; node plane i+1 of NG load cases
-// (staged as stage_at does) goes with a one-node halo into a three-plane LDS
-// ring, so each staged value is read from memory about (TI+2)/TI * 324/256
-// times instead of once per neighbour, and each node's stencil row is
-// streamed once for the NG load cases.
-constexpr int kStTI = 2;  // planes per workgroup on the tiled coarse path
+// One-plane form of the tiled stencil kernel, for levels with planes of at
+// least one 16x16 tile.  A workgroup owns one node plane i of a 16x16 (j,k)
+// tile and NG load cases of one sample.  The staged neighbour planes i-1, i,
+// i+1 (with the one-node j/k halo, as stage_at stages them) pass through a
+// two-buffer LDS ring in three phases -- plane i+1 is loaded into registers
+// while plane i-1 is used, and written over it after a barrier -- and each
+// phase applies the nine 3x3 blocks of the node's stencil row that couple
+// to that plane.  Against a three-plane ring this needs 2/3 of the LDS, so
+// more workgroups share a CU and hide the coefficient loads' latency.
 
-template <typename T, int ST, int EP, int NG>
-__global__ __launch_bounds__(kBlock) void k_stencil_tiled(Dims g, const T *__restrict__ S, OpArgs<T> a,
-                                                          int tiles_k, int tiles, int chunks, int sym) {
+// Stencil coefficients through a raw buffer resource: a scalar plane offset
+// plus one 32-bit lane offset per load (no vector address arithmetic), and a
+// load past the sample's stencil returns 0 instead of faulting.
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float *) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double *) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+constexpr int kStBatch = 3;  // stencil blocks whose coefficient loads are issued together
+
+template <typename T, int ST, int EP, int NG, int SYM>
+__global__ __launch_bounds__(kBlock) void k_stencil_plane(Dims g, const T *__restrict__ S, OpArgs<T> a,
+                                                          int tiles_k, int tiles) {
   constexpr int G = kRHS / NG;  // load-case groups per sample
-  // logical order (cell, chunk, tile, group), XCD-contiguous (see xcd_order)
+  // logical order (cell, plane, tile, group), XCD-contiguous (see xcd_order)
   const int L = xcd_order(blockIdx.x, gridDim.x);
-  const int grp = L % G, tile = (L / G) % tiles, chunk = (L / (G * tiles)) % chunks;
-  const int smp = L / (G * tiles * chunks), lc0 = grp * NG;
+  const int grp = L % G, tile = (L / G) % tiles, i = (L / (G * tiles)) % g.nx;
+  const int smp = L / (G * tiles * g.nx), lc0 = grp * NG;
   unsigned mask = 0;
 #pragma unroll
   for (int t = 0; t < NG; t++) mask |= a.act[smp * kRHS + lc0 + t] ? (1u << t) : 0u;
   if (!mask) return;
-  __shared__ T us[3][NG][3][kTN * kTN];
+  __shared__ T us[2][NG][3][kTN * kTN];
   const int tid = threadIdx.x;
   const int j0 = (tile / tiles_k) * kTile, k0 = (tile % tiles_k) * kTile;
-  const int i0 = chunk * kStTI, iend = min(i0 + kStTI, g.nx);
   const int V = g.V, plane = g.ny * g.nz;
-  const T *__restrict__ Ss = S + (size_t)smp * 243 * V;
+  const unsigned vb = (unsigned)V * (unsigned)sizeof(T);  // bytes per coefficient plane
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(S + (size_t)smp * 243 * V), (short)0, (int)(243u * vb), 0x00020000);
   const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * V : nullptr;
   const T ti = a.theta_inv[smp];
   const size_t off0 = (size_t)(smp * kRHS + lc0) * 3 * V;
@@ -545,23 +559,33 @@ __global__ __launch_bounds__(kBlock) void k_stencil_tiled(Dims g, const T *__res
     soff[q] = -1;
     if (t < kTN * kTN) {
       const int nj = t / kTN, nk = t - nj * kTN;
-      const int jj = ((j0 - 1 + nj) % g.ny + g.ny) % g.ny, kk = ((k0 - 1 + nk) % g.nz + g.nz) % g.nz;
-      soff[q] = jj * g.nz + kk;
+      soff[q] = wrap1(j0 - 1 + nj, g.ny) * g.nz + wrap1(k0 - 1 + nk, g.nz);
     }
   }
-  auto stage = [&](int buf, int ip) {
+  T stg[2][NG][3];
+  auto fetch = [&](int ip) {
     const int iw = wrap1(ip, g.nx);
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       if (soff[q] < 0) continue;
-      const int t = tid + q * kBlock;
       const int n = iw * plane + soff[q];
 #pragma unroll
-      for (int r = 0; r < NG; r++) {
-        if (!((mask >> r) & 1u)) continue;
+      for (int r = 0; r < NG; r++)
+        // inactive load cases stage zeros: the phases then run branch-free
 #pragma unroll
-        for (int c = 0; c < 3; c++) us[buf][r][c][t] = stage_at<T, ST>(a, Ds, ti, off0 + (size_t)r * 3 * V, V, c, n);
-      }
+        for (int c = 0; c < 3; c++)
+          stg[q][r][c] = ((mask >> r) & 1u) ? stage_at<T, ST>(a, Ds, ti, off0 + (size_t)r * 3 * V, V, c, n) : (T)0;
+    }
+  };
+  auto put = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      if (soff[q] < 0) continue;
+      const int t = tid + q * kBlock;
+#pragma unroll
+      for (int r = 0; r < NG; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) us[buf][r][c][t] = stg[q][r][c];
     }
   };
   const int nj = tid / kTile, nk = tid % kTile;
@@ -569,56 +593,85 @@ __global__ __launch_bounds__(kBlock) void k_stencil_tiled(Dims g, const T *__res
   const bool owns = gj < g.ny && gk < g.nz;
   const int Jn[3] = {wrap1(gj - 1, g.ny), wrap1(gj, g.ny), wrap1(gj + 1, g.ny)};
   const int Kn[3] = {wrap1(gk - 1, g.nz), wrap1(gk, g.nz), wrap1(gk + 1, g.nz)};
-  stage(0, i0 - 1);
-  stage(1, i0);
-  for (int i = i0, p = 0; i < iend; i++, p++) {
-    const int bm = p % 3, b0 = (p + 1) % 3, bp = (p + 2) % 3;
-    stage(bp, i + 1);
-    __syncthreads();
-    if (owns) {
-      const int In[3] = {wrap1(i - 1, g.nx), i, wrap1(i + 1, g.nx)};
-      const int n = (i * g.ny + gj) * g.nz + gk;
-      T acc[NG][3];
+  const int n = i * plane + gj * g.nz + gk;
+  T acc[NG][3];
 #pragma unroll
-      for (int t = 0; t < NG; t++) acc[t][0] = acc[t][1] = acc[t][2] = 0;
-#pragma unroll 3
-      for (int o = 0; o < 27; o++) {
-        const int di = o / 9, dj = (o / 3) % 3, dk = o % 3;
-        T sv[9];
-        if (sym && o < 13) {
-          const int nb = (In[di] * g.ny + Jn[dj]) * g.nz + Kn[dk];
+  for (int t = 0; t < NG; t++) acc[t][0] = acc[t][1] = acc[t][2] = 0;
+  // the nine blocks of offset plane di (0: i-1, 1: i, 2: i+1) against the
+  // staged plane in buffer `buf`
+  auto phase = [&](int di, int buf) {
+    const int pw = wrap1(i - 1 + di, g.nx);
 #pragma unroll
-          for (int c = 0; c < 3; c++)
+    for (int pos = 0; pos < 9; pos++) {
+      const int dj = pos / 3, dk = pos % 3;
+      const int o = di * 9 + pos;
+      const int slot = (nj + dj) * kTN + nk + dk;
+      T sv[9];
+      if (SYM && o < 13) {
+        // S(n, o) = S(n + o, 26 - o)^T
+        const unsigned nb = (unsigned)(pw * plane + Jn[dj] * g.nz + Kn[dk]) * (unsigned)sizeof(T);
 #pragma unroll
-            for (int d = 0; d < 3; d++) sv[3 * c + d] = Ss[(size_t)((26 - o) * 9 + 3 * d + c) * V + nb];
-        } else {
+        for (int c = 0; c < 3; c++)
 #pragma unroll
-          for (int q = 0; q < 9; q++) sv[q] = Ss[(size_t)(o * 9 + q) * V + n];
-        }
-        const int buf = di == 0 ? bm : (di == 1 ? b0 : bp);
-        const int slot = (nj + dj) * kTN + nk + dk;
+          for (int d = 0; d < 3; d++)
+            sv[3 * c + d] = bload(rs, nb, (unsigned)((26 - o) * 9 + 3 * d + c) * vb, (T *)nullptr);
+      } else {
+        const unsigned no = (unsigned)n * (unsigned)sizeof(T);
 #pragma unroll
-        for (int t = 0; t < NG; t++) {
-          if (!((mask >> t) & 1u)) continue;
-          const T u0 = us[buf][t][0][slot], u1 = us[buf][t][1][slot], u2 = us[buf][t][2][slot];
-          acc[t][0] += sv[0] * u0 + sv[1] * u1 + sv[2] * u2;
-          acc[t][1] += sv[3] * u0 + sv[4] * u1 + sv[5] * u2;
-          acc[t][2] += sv[6] * u0 + sv[7] * u1 + sv[8] * u2;
-        }
+        for (int q = 0; q < 9; q++) sv[q] = bload(rs, no, (unsigned)(o * 9 + q) * vb, (T *)nullptr);
       }
-      const int own = (nj + 1) * kTN + nk + 1;
 #pragma unroll
       for (int t = 0; t < NG; t++) {
-        if (!((mask >> t) & 1u)) continue;
-        const size_t off = off0 + (size_t)t * 3 * V;
-        T uo[3] = {us[b0][t][0][own], us[b0][t][1][own], us[b0][t][2][own]};
-        DotAcc dv;
-        EpiIn<T> pin;
-        epi_load<T, EP>(a, Ds, off, V, n, pin);
-        epilogue<T, EP>(a, pin, ti, a.c1[smp], a.c2[smp], off, V, n, acc[t], uo, dv);
+        const T u0 = us[buf][t][0][slot], u1 = us[buf][t][1][slot], u2 = us[buf][t][2][slot];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          T v = acc[t][c];
+          v = fma(sv[3 * c], u0, v);
+          v = fma(sv[3 * c + 1], u1, v);
+          acc[t][c] = fma(sv[3 * c + 2], u2, v);
+        }
+      }
+      // coefficient loads are issued kStBatch blocks at a time: the sums are
+      // pinned here, so the compiler neither sinks the FMAs into the
+      // epilogue nor hoists all 81 loads of the phase ahead of them (which
+      // costs occupancy)
+      if (pos % kStBatch == kStBatch - 1) {
+#pragma unroll
+        for (int t = 0; t < NG; t++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) asm volatile("" : "+v"(acc[t][c]));
       }
     }
-    __syncthreads();
+  };
+  fetch(i - 1);
+  put(0);
+  fetch(i);
+  put(1);
+  fetch(i + 1);  // in flight during the first phase
+  __syncthreads();
+  if (owns) phase(0, 0);
+  __syncthreads();
+  put(0);
+  T uo[NG][3];
+  if (owns) {
+    phase(1, 1);
+#pragma unroll
+    for (int t = 0; t < NG; t++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) uo[t][c] = us[1][t][c][(nj + 1) * kTN + nk + 1];
+  }
+  __syncthreads();
+  if (owns) {
+    phase(2, 0);
+#pragma unroll
+    for (int t = 0; t < NG; t++) {
+      if (!((mask >> t) & 1u)) continue;
+      const size_t off = off0 + (size_t)t * 3 * V;
+      DotAcc dv;
+      EpiIn<T> pin;
+      epi_load<T, EP>(a, Ds, off, V, n, pin);
+      epilogue<T, EP>(a, pin, ti, a.c1[smp], a.c2[smp], off, V, n, acc[t], uo[t], dv);
+    }
   }
 }
 
@@ -628,21 +681,25 @@ void launch_stencil(int st, int ep, const Dims &g, int R, const T *S, const OpAr
   const int blocks = (g.V + kBlock - 1) / kBlock;
   const int nsmp = (R + kRHS - 1) / kRHS;
   const int sym = (g.nx > 2 && g.ny > 2 && g.nz > 2) ? 1 : 0;
-  // planes of at least one full 16x16 tile: the LDS-tiled marching kernel
-  // (three load cases of a sample per workgroup: 35 KB of LDS in FP32)
+  // planes of at least one full 16x16 tile: the one-plane LDS kernel (three
+  // load cases of a sample per workgroup, 23 KB of LDS in FP32), once that
+  // gives a workgroup per CU
   constexpr int NG = kRHS / 2;
   const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
-  const int st_tiles = tiles_j * tiles_k, st_chunks = (g.nx + kStTI - 1) / kStTI;
-  const long st_blocks = (long)st_tiles * st_chunks * nsmp * (kRHS / NG);
-  // ... when that still gives >= 2 workgroups per CU
-  const bool tiled = g.ny >= kTile && g.nz >= kTile && g.nx >= 3 && R % kRHS == 0 && st_blocks >= 512;
+  const long pl_blocks = (long)tiles_j * tiles_k * g.nx * nsmp * (kRHS / NG);
+  // (and a sample's stencil addressable by a buffer resource: < 2 GiB)
+  const bool plane = g.ny >= kTile && g.nz >= kTile && g.nx >= 3 && R % kRHS == 0 && pl_blocks >= 256 &&
+                     (size_t)243 * g.V * sizeof(T) < ((size_t)1 << 31);
   // all load cases per thread once that still leaves >= 2 workgroups per CU
   const bool multi = (long)blocks * nsmp >= 512 && R % kRHS == 0;
 #define GO(S_, E_)                                                                               \
   if (st == S_ && ep == E_) {                                                                    \
-    if (tiled)                                                                                   \
-      hipLaunchKernelGGL((k_stencil_tiled<T, S_, E_, NG>), dim3((unsigned)st_blocks), dim3(kBlock), \
-                         0, s, g, S, a, tiles_k, st_tiles, st_chunks, sym);                      \
+    if (plane && sym)                                                                            \
+      hipLaunchKernelGGL((k_stencil_plane<T, S_, E_, NG, 1>), dim3((unsigned)pl_blocks), dim3(kBlock), \
+                         0, s, g, S, a, tiles_k, tiles_j * tiles_k);                             \
+    else if (plane)                                                                              \
+      hipLaunchKernelGGL((k_stencil_plane<T, S_, E_, NG, 0>), dim3((unsigned)pl_blocks), dim3(kBlock), \
+                         0, s, g, S, a, tiles_k, tiles_j * tiles_k);                             \
     else if (multi)                                                                              \
       hipLaunchKernelGGL((k_stencil<T, S_, E_, kRHS>), dim3(blocks, nsmp), dim3(kBlock), 0, s, g, S, \
                          a, sym);                                                                \
