@@ -69,6 +69,29 @@ struct ModeT {
   T cross[3][3][8];
 };
 
+// mode coupling z = E Lambda w (mode 0 is a rigid translation: no force);
+// modes sm = 4 si + 2 sj + sk per component
+template <typename T>
+__device__ __forceinline__ void mode_couple(const T (&w)[3][8], T E, const ModeT<T> &m, T (&z)[3][8]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    z[c][0] = 0;
+#pragma unroll
+    for (int sm = 1; sm < 8; sm++) {
+      T acc = m.diag[c][sm] * w[c][sm];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        if (d == c) continue;
+        const int bc = (sm >> (2 - c)) & 1, bd = (sm >> (2 - d)) & 1;
+        if (bc == bd) continue;
+        const int t = sm ^ (1 << (2 - c)) ^ (1 << (2 - d));
+        acc += m.cross[c][d][sm] * w[d][t];
+      }
+      z[c][sm] = E * acc;
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void hadamard_voxel(const T *__restrict__ u, T E, const ModeT<T> &m,
                                                T *__restrict__ f) {
@@ -89,25 +112,8 @@ __device__ __forceinline__ void hadamard_voxel(const T *__restrict__ u, T E, con
           w[c][v] = a + b;
           w[c][v | mb] = b - a;
         }
-  // mode coupling (mode 0 is a rigid translation: no force)
   T z[3][8];
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    z[c][0] = 0;
-#pragma unroll
-    for (int sm = 1; sm < 8; sm++) {
-      T acc = m.diag[c][sm] * w[c][sm];
-#pragma unroll
-      for (int d = 0; d < 3; d++) {
-        if (d == c) continue;
-        const int bc = (sm >> (2 - c)) & 1, bd = (sm >> (2 - d)) & 1;
-        if (bc == bd) continue;
-        const int t = sm ^ (1 << (2 - c)) ^ (1 << (2 - d));
-        acc += m.cross[c][d][sm] * w[d][t];
-      }
-      z[c][sm] = E * acc;
-    }
-  }
+  mode_couple<T>(w, E, m, z);
   // inverse: per axis (a, b) -> (a - b, a + b)
 #pragma unroll
   for (int c = 0; c < 3; c++)

@@ -400,6 +400,20 @@ __global__ __launch_bounds__(kBlock) void k_p_update(int V, T *__restrict__ p,
     p[off + t] = first ? z[off + t] : z[off + t] + bt * p[off + t];
 }
 
+__global__ __launch_bounds__(kBlock) void k_beta(int rz_new, int rz_old, int first,
+                                                 const int *__restrict__ act, Reduce red, int np,
+                                                 double *__restrict__ beta) {
+  const int rr = blockIdx.x;
+  if (!act[rr]) return;
+  double b = 0;
+  if (!first) {
+    const double den = sum_parts(red, rz_old, rr, np);
+    const double num = sum_parts(red, rz_new, rr, np);
+    b = den != 0.0 ? num / den : 0.0;
+  }
+  if (threadIdx.x == 0) beta[rr] = b;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(int V, const T *__restrict__ a,
                                                 const T *__restrict__ b, int slot,
@@ -743,6 +757,10 @@ void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int
                      const int *act, const Reduce &red, int np_rz, hipStream_t s) {
   hipLaunchKernelGGL(k_p_update<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, p, z, rz_new, rz_old,
                      first, act, red, np_rz);
+}
+void launch_beta(int R, int rz_new, int rz_old, int first, const int *act, const Reduce &red, int np_rz,
+                 double *beta, hipStream_t s) {
+  hipLaunchKernelGGL(k_beta, dim3(R), dim3(kBlock), 0, s, rz_new, rz_old, first, act, red, np_rz, beta);
 }
 template <typename T>
 void launch_dot(int V, int R, const T *a, const T *b, int slot, const int *act, const Reduce &red,

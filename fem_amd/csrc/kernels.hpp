@@ -128,6 +128,10 @@ void launch_pcg_update(int V, int R, T *x, T *r, const T *p, const T *q, const T
 template <typename T>
 void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int first,
                      const int *act, const Reduce &red, int np_rz, hipStream_t s);
+// beta[r] = rz[new]/rz[old] per active RHS (first: 0), for the fused
+// direction update of the fine operator (ST_PUPD)
+void launch_beta(int R, int rz_new, int rz_old, int first, const int *act, const Reduce &red, int np_rz,
+                 double *beta, hipStream_t s);
 // dot products a.b -> slot (per RHS)
 template <typename T>
 void launch_dot(int V, int R, const T *a, const T *b, int slot, const int *act, const Reduce &red,

@@ -76,6 +76,7 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, 
     const size_t q = off + (size_t)c * V + n;
     if (EP == EP_AX) {
       a.y[q] = Au[c];
+      if (a.pout) a.pout[q] = uo[c];
       if (a.dot) dv.a += (double)uo[c] * (double)Au[c];
     } else if (EP == EP_RES) {
       const T rv = p.e0[c] - Au[c];
@@ -139,7 +140,7 @@ __device__ __forceinline__ int xcd_order(int b, int n) {
 // tile (same moduli and diagonal) and then the neighbouring tiles follow each
 // other on one XCD
 template <typename T, int ST, int EP>
-__global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ E, ModeT<T> mc,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g, const T *__restrict__ E, ModeT<T> mc,
                                                   OpArgs<T> a, int TI, int tiles_k, int tiles,
                                                   int chunks) {
   const int L = xcd_order(blockIdx.x, gridDim.x);
@@ -150,6 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
   E += (size_t)smp * g.V;
   const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * g.V : nullptr;
   const T ti = a.theta_inv[smp], c1 = a.c1[smp], c2 = a.c2[smp];
+  const T bt = ST == ST_PUPD ? (T)a.beta[r] : (T)0;
   __shared__ T up[3][3][kTN * kTN];
   __shared__ T eo[kTE * kTE * kEOH];
   const int tid = threadIdx.x;
@@ -215,6 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
       for (int c = 0; c < 3; c++) {
         pf[q][c] = a.u[off + (size_t)c * V + n];
         if (ST == ST_DSCALE) pd[q][c] = Ds[(size_t)c * V + n];
+        if (ST == ST_PUPD) pd[q][c] = a.u2[off + (size_t)c * V + n];
       }
     }
   };
@@ -224,7 +227,10 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
       if (soff[q] < 0) continue;
       const int t = vt + q * kBlock;
 #pragma unroll
-      for (int c = 0; c < 3; c++) up[buf][c][t] = (ST == ST_DSCALE) ? pf[q][c] * (ti * pd[q][c]) : pf[q][c];
+      for (int c = 0; c < 3; c++)
+        up[buf][c][t] = (ST == ST_DSCALE) ? pf[q][c] * (ti * pd[q][c])
+                        : (ST == ST_PUPD) ? pf[q][c] + bt * pd[q][c]
+                                          : pf[q][c];
     }
   };
   auto stage = [&](int buf, int ip) {
@@ -264,13 +270,57 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
   const int gj = j0 + nj, gk = k0 + nk;
   const bool owns = (gj < g.ny) && (gk < g.nz);
   const int ot = (nj + 1) * kTN + nk + 1;  // owned node in the staged plane
-  T acc[3] = {0, 0, 0};
   DotAcc dv;
 
-  // FP32: each voxel thread keeps its top corner values for the next slab
-  // (halves the LDS gathers; FP64 would spend too many registers)
+  // Per voxel (three components, corners v = 4 bi + 2 bj + bk) the cube-mode
+  // transform is separable: the (j,k) transform of each face, then the i
+  // stage.  A voxel's top face is the bottom face of the voxel above, so
+  // its transformed values are carried to the next slab (FP32; FP64 would
+  // spend too many registers and transforms the bottom face again).  On the
+  // way back, the i stage splits the modal forces into a bottom and a top
+  // face; the top face of slab s-1 and the bottom face of slab s lie on the
+  // same node plane s, so they are summed before one inverse face transform
+  // -- plane s is then complete after a single LDS exchange.
   constexpr bool kReuse = sizeof(T) == 4;
-  T utop[2][12];
+  T ftop[2][3][4], tmod[2][3][4];
+  auto face_load = [&](int buf, int ej, int ek, int bi_off, T (&f)[3][4]) {
+    (void)bi_off;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int t = (ej + (v >> 1)) * kTN + ek + (v & 1);
+#pragma unroll
+      for (int c = 0; c < 3; c++) f[c][v] = up[buf][c][t];
+    }
+  };
+  // forward (j,k) transform per axis: (a, b) -> (a + b, b - a)
+  auto face_fwd = [](T (&f)[3][4]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int mb = 2; mb >= 1; mb >>= 1)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          if (!(v & mb)) {
+            const T a0 = f[c][v], b0 = f[c][v | mb];
+            f[c][v] = a0 + b0;
+            f[c][v | mb] = b0 - a0;
+          }
+  };
+  // inverse (j,k) transform per axis: (a, b) -> (a - b, a + b)
+  auto face_inv = [](T (&f)[3][4]) {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int mb = 2; mb >= 1; mb >>= 1)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          if (!(v & mb)) {
+            const T a0 = f[c][v], b0 = f[c][v | mb];
+            f[c][v] = a0 - b0;
+            f[c][v | mb] = a0 + b0;
+          }
+  };
+
   stage(0, i0 - 1);
   if (ST != ST_PROLONG) fetch(i0);
   for (int sl = i0 - 1, p = 0; sl < iend; sl++, p++) {
@@ -290,81 +340,81 @@ __global__ __launch_bounds__(kBlock) void k_tiled(Dims g, const T *__restrict__ 
     } else {
       stage(nxt, sl + 1);
     }
+    // (the previous slab's gather of eo and reads of this ring slot are
+    // behind the barrier of the previous slab's face exchange)
     __syncthreads();
-    T keep[2][12];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       if (eoff[q] < 0) continue;
       const int e = vt + q * kBlock;
       const int ej = e / kTE, ek = e - ej * kTE;
-      T uv[24], fv[24];
+      T fb[3][4], ft[3][4];
+      face_load(nxt, ej, ek, 1, ft);
+      face_fwd(ft);
+      if (kReuse && p > 0) {
 #pragma unroll
-      for (int v = 0; v < 8; v++) {
-        const int bi = (v >> 2) & 1, bj = (v >> 1) & 1, bk = v & 1;
-        const int t = (ej + bj) * kTN + ek + bk;
-        if (kReuse && bi == 0 && p > 0) {
-          // the bottom corners are the previous slab's top corners
-          uv[3 * v] = utop[q][3 * v];
-          uv[3 * v + 1] = utop[q][3 * v + 1];
-          uv[3 * v + 2] = utop[q][3 * v + 2];
-        } else {
-          const int bb = bi ? nxt : cur;
-          uv[3 * v] = up[bb][0][t];
-          uv[3 * v + 1] = up[bb][1][t];
-          uv[3 * v + 2] = up[bb][2][t];
-        }
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) fb[c][v] = ftop[q][c][v];
+      } else {
+        face_load(cur, ej, ek, 0, fb);
+        face_fwd(fb);
       }
       if (kReuse) {
 #pragma unroll
-        for (int t = 0; t < 12; t++) utop[q][t] = uv[12 + t];
-      }
-      hadamard_voxel<T>(uv, Ev[q], mc, fv);
-      T *dst = eo + e * kEOH;
+        for (int c = 0; c < 3; c++)
 #pragma unroll
-      for (int t = 0; t < 12; t++) {
-        dst[t] = fv[t];        // corners with bi = 0 (plane s)
-        keep[q][t] = fv[12 + t];  // corners with bi = 1 (plane s+1)
+          for (int v = 0; v < 4; v++) ftop[q][c][v] = ft[c][v];
+      }
+      // i stage: modes sm = 4 si + s_jk
+      T w[3][8];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          w[c][v] = fb[c][v] + ft[c][v];
+          w[c][4 + v] = ft[c][v] - fb[c][v];
+        }
+      T z[3][8];
+      mode_couple<T>(w, Ev[q], mc, z);
+      // inverse i stage: bottom-face (plane sl) and top-face (plane sl+1)
+      // modal forces
+      T bm[3][4];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          bm[c][v] = z[c][v] - z[c][4 + v];
+          if (p > 0) bm[c][v] += tmod[q][c][v];
+          tmod[q][c][v] = z[c][v] + z[c][4 + v];
+        }
+      if (p > 0) {
+        face_inv(bm);
+        T *dst = eo + e * kEOH;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) dst[3 * v + c] = bm[c][v];
       }
     }
+    if (p == 0) continue;  // plane i0 - 1 is not owned (uniform branch)
     __syncthreads();
-    // plane s: voxels (nj+dj, nk+dk) hold this node as corner (0, 1-dj, 1-dk)
-    T f[3] = {0, 0, 0};
-#pragma unroll
-    for (int dj = 0; dj < 2; dj++)
-#pragma unroll
-      for (int dk = 0; dk < 2; dk++) {
-        const T *src = eo + ((nj + dj) * kTE + nk + dk) * kEOH + 3 * (2 * (1 - dj) + (1 - dk));
-#pragma unroll
-        for (int c = 0; c < 3; c++) f[c] += src[c];
-      }
+    // plane sl: voxel columns (nj+dj, nk+dk) hold this node as face corner
+    // (1-dj, 1-dk)
     if (sl >= i0 && owns) {
-      T Au[3], uo[3];
+      T Au[3] = {0, 0, 0}, uo[3];
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        Au[c] = acc[c] + f[c];
-        uo[c] = up[cur][c][ot];
-      }
+      for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+        for (int dk = 0; dk < 2; dk++) {
+          const T *src = eo + ((nj + dj) * kTE + nk + dk) * kEOH + 3 * (2 * (1 - dj) + (1 - dk));
+#pragma unroll
+          for (int c = 0; c < 3; c++) Au[c] += src[c];
+        }
+#pragma unroll
+      for (int c = 0; c < 3; c++) uo[c] = up[cur][c][ot];
       epilogue<T, EP>(a, pin, ti, c1, c2, off, V, (int)nown, Au, uo, dv);
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-      if (eoff[q] < 0) continue;
-      T *dst = eo + (vt + q * kBlock) * kEOH;
-#pragma unroll
-      for (int t = 0; t < 12; t++) dst[t] = keep[q][t];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 3; c++) acc[c] = 0;
-#pragma unroll
-    for (int dj = 0; dj < 2; dj++)
-#pragma unroll
-      for (int dk = 0; dk < 2; dk++) {
-        const T *src = eo + ((nj + dj) * kTE + nk + dk) * kEOH + 3 * (2 * (1 - dj) + (1 - dk));
-#pragma unroll
-        for (int c = 0; c < 3; c++) acc[c] += src[c];
-      }
   }
   if ((EP == EP_AX || EP == EP_STEP || EP == EP_RES) && a.dot) {
     const int pidx = chunk * tiles + tile;
@@ -402,6 +452,7 @@ void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCo
     return;                                        \
   }
   GO(ST_PLAIN, EP_AX)
+  GO(ST_PUPD, EP_AX)
   GO(ST_PLAIN, EP_RES)
   GO(ST_DSCALE, EP_STEP)
   GO(ST_PLAIN, EP_STEP)

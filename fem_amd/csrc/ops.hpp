@@ -6,10 +6,13 @@
 //   ST_PLAIN    u
 //   ST_DSCALE   u * theta_inv / D              (first Chebyshev step from x = 0)
 //   ST_PROLONG  u + P xc                       (coarse correction added on the fly)
+//   ST_PUPD     u + beta u2                    (PCG direction p = z + beta p_old,
+//                                               beta per RHS from a device array)
 // (D enters as its precomputed inverse: no divisions in the kernels)
 // Epilogue at each owned node, with Au the operator result and u_own the
 // staged value there:
-//   EP_AX       y = Au            (optional FP64 partial of u . Au)
+//   EP_AX       y = Au            (optional FP64 partial of u . Au; with pout,
+//                                  pout = u_own: the new PCG direction)
 //   EP_RES      y = b - Au            (optional FP64 partials of r.r, b.u, u.r)
 //   EP_CHEB0    r = b - Au ; dout = r theta_inv / D (not with `last`) ; x = u_own
 //   EP_STEP     r = rprev - Au ; dn = c1 u_own + c2 r / D ; dout = dn ;
@@ -24,13 +27,16 @@
 
 namespace femh {
 
-enum Stage : int { ST_PLAIN = 0, ST_DSCALE = 1, ST_PROLONG = 2 };
+enum Stage : int { ST_PLAIN = 0, ST_DSCALE = 1, ST_PROLONG = 2, ST_PUPD = 3 };
 enum Epi : int { EP_AX = 0, EP_RES = 1, EP_CHEB0 = 2, EP_STEP = 3 };
 constexpr int kMaxCheb = 16;
 
 template <typename T>
 struct OpArgs {
   const T *u = nullptr;      // staged vector [R][3][V]
+  const T *u2 = nullptr;     // ST_PUPD: previous direction [R][3][V]
+  const double *beta = nullptr;  // ST_PUPD: per-RHS beta [R]
+  T *pout = nullptr;         // EP_AX: staged value written back (new direction)
   const T *xc = nullptr;     // ST_PROLONG: coarse vector [R][3][Vc]
   Dims gc{};                 // ST_PROLONG: coarse dims
   const T *Dinv = nullptr;   // inverse diagonal [B][3][V]

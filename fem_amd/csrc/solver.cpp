@@ -189,6 +189,10 @@ void Context::op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a) {
         vec_out = 3;
       }
       if (st == ST_PROLONG) vec_in += 0.125;
+      if (st == ST_PUPD) {
+        vec_in += 1;                 // previous direction
+        if (a.pout) vec_out += 1;    // new direction
+      }
       const bool usesD = st == ST_DSCALE || ep == EP_CHEB0 || ep == EP_STEP;
       const double samples = std::ceil(Rn / kRHS);
       pe.bytes = V * (3.0 * sz * Rn * (vec_in + vec_out) + samples * sz * (usesD ? 4.0 : 1.0));
@@ -660,13 +664,42 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
   const int np_rz = jac ? vec_parts(V) : rz_parts(h);
   // z_ready: the caller already left z = M^-1 b and r.z in S_RZ0
   if (!z_ready) precond<T>(h, rv, zv, ap, S_RZ0);
-  launch_p_update<T>(V, R, pv, zv, S_RZ0, S_RZ0, 1, ap, red, np_rz, stream);
+  // Fine-level tiled operator: the direction update p = z + beta p runs
+  // inside the A p launch (staged on the fly, written back by the owner),
+  // ping-ponging p between pv and a second buffer since neighbours still
+  // read the old direction.  beta comes from a one-block-per-RHS kernel.
+  const bool fuse_p = !jac && o.apply_kernel != 1;
+  T *pcur = pv, *pnxt = nullptr;
+  if (fuse_p) {
+    DBuf &alt = sizeof(T) == 8 ? palt : palt32;
+    if (!alt.p) alt.alloc(field_bytes(g, B, sizeof(T)));
+    if (!beta_d.p) beta_d.alloc(sizeof(double) * kMaxR);
+    pnxt = alt.template as<T>();
+    launch_beta(R, S_RZ0, S_RZ0, 1, ap, red, np_rz, beta_d.as<double>(), stream);
+  } else {
+    launch_p_update<T>(V, R, pv, zv, S_RZ0, S_RZ0, 1, ap, red, np_rz, stream);
+  }
   int cur = 0;
   std::vector<double> rr(R);
   for (int it = 1; it <= max_iter; it++) {
     const int rzc = cur ? S_RZ1 : S_RZ0, rzn = cur ? S_RZ0 : S_RZ1;
-    apply<T>(h, 0, pv, qv, nullptr, 0, ap, &red, S_PQ);
-    launch_pcg_update<T>(V, R, xv, rv, pv, qv, jac ? D0 : nullptr, zv, rzc, rzn, ap, red,
+    if (fuse_p) {
+      OpArgs<T> a;
+      a.u = zv;
+      a.u2 = it == 1 ? zv : pcur;  // beta = 0 on the first step: p = z
+      a.beta = beta_d.as<double>();
+      a.pout = pnxt;
+      a.y = qv;
+      a.act = ap;
+      a.red = red;
+      a.slot = S_PQ;
+      a.dot = 1;
+      op<T>(h, 0, ST_PUPD, EP_AX, a);
+      std::swap(pcur, pnxt);
+    } else {
+      apply<T>(h, 0, pv, qv, nullptr, 0, ap, &red, S_PQ);
+    }
+    launch_pcg_update<T>(V, R, xv, rv, pcur, qv, jac ? D0 : nullptr, zv, rzc, rzn, ap, red,
                          apply_parts(), np_rz, stream);
     FEMH_HIP(hipGetLastError());
     if (it % o.check_every == 0 || it == max_iter) {
@@ -688,7 +721,10 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
       if (changed) set_act(A_PCG, active);
     }
     if (!jac) precond<T>(h, rv, zv, ap, rzn);
-    launch_p_update<T>(V, R, pv, zv, rzn, rzc, 0, ap, red, np_rz, stream);
+    if (fuse_p)
+      launch_beta(R, rzn, rzc, 0, ap, red, np_rz, beta_d.as<double>(), stream);
+    else
+      launch_p_update<T>(V, R, pv, zv, rzn, rzc, 0, ap, red, np_rz, stream);
     cur ^= 1;
   }
   FEMH_HIP(hipGetLastError());

@@ -111,6 +111,8 @@ struct Context {
   int prev_nb = 0;  // samples of the last successful call (warm start source)
   DBuf xw;          // warm start: previous fields [R][3][V] (FP64)
   DBuf dx, dq;      // mixed precision: FP64 fallback correction and PCG scratch
+  DBuf palt, palt32;  // PCG: second direction buffer of the fused direction update
+  DBuf beta_d;        // PCG: per-RHS beta [kMaxR] (double)
   std::vector<int> fallback_;  // per sample: finished in FP64 this call
   bool modulus_input = false;  // last load gave moduli (dE/drho = 1)
   int nlevels = 1;
