@@ -568,24 +568,53 @@ __global__ __launch_bounds__(kBlock) void k_material(size_t n, const double *__r
     E[t] = interp_E(rho[t], interp, E0, Emin, p);
 }
 
-// chi of voxel (i,j,k) for load case lc of one sample: affine local field +
-// gathered periodic fluctuation (us = that sample's 6 fields)
-__device__ __forceinline__ void gather_chi(const Dims &g, const int I[2], const int J[2],
-                                           const int K[2], const double *__restrict__ us,
-                                           const double *__restrict__ uaff, int lc, double *chi) {
+// W_p(e) = chi_a^T K0 chi_b for the 21 pairs a <= b (row order, p = 0..20; not
+// divided by V), handed to sink(p, W_p) one at a time (p is a compile-time
+// constant after unrolling).  In cube modes, with w = H chi and z = Lambda w
+// (K0 = H^T Lambda H, Lambda symmetric), W_ab = z_a . w_b.  The load cases go
+// in three passes of two: a pass keeps z of its two cases and streams the
+// later cases' w past them, so the 24 corner values of a case are gathered
+// 12 times in all (not once per pair) and each case is coupled once.
+__device__ __forceinline__ void chi_modes(const Dims &g, const int (&n)[8], const double *__restrict__ us,
+                                          const double *__restrict__ uaff, int lc, double (&w)[3][8]) {
   const double *ul = us + (size_t)lc * 3 * g.V;
+  // the corner indices pass through an empty asm: this gather's loads are
+  // not hoisted above the previous case's work (which would hold all 288
+  // loads of the element in registers at once)
+  int m[8];
 #pragma unroll
   for (int v = 0; v < 8; v++) {
-    const int n = (I[(v >> 2) & 1] * g.ny + J[(v >> 1) & 1]) * g.nz + K[v & 1];
-#pragma unroll
-    for (int c = 0; c < 3; c++) chi[3 * v + c] = uaff[lc * 24 + 3 * v + c] + ul[(size_t)c * g.V + n];
+    m[v] = n[v];
+    asm volatile("" : "+v"(m[v]));
   }
+#pragma unroll
+  for (int v = 0; v < 8; v++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) w[c][v] = uaff[lc * 24 + 3 * v + c] + ul[(size_t)c * g.V + m[v]];
+  // forward: per axis (a, b) -> (a + b, b - a)
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int mb = 4; mb >= 1; mb >>= 1)
+#pragma unroll
+      for (int v = 0; v < 8; v++)
+        if (!(v & mb)) {
+          const double x0 = w[c][v], x1 = w[c][v | mb];
+          w[c][v] = x0 + x1;
+          w[c][v | mb] = x1 - x0;
+        }
 }
 
-// W_p(e) = chi_a^T K0 chi_b for the 21 pairs a <= b (row order, p = 0..20; not
-// divided by V), handed to sink(p, W_p) one at a time.  K0 chi_a goes through
-// the cube-mode factorization.  The pair loops stay rolled (register
-// pressure), so sinks index memory with the runtime p.
+__device__ __forceinline__ double mode_dot(const double (&z)[3][8], const double (&w)[3][8]) {
+  double s = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int sm = 1; sm < 8; sm++) s = fma(z[c][sm], w[c][sm], s);  // z[c][0] = 0
+  asm volatile("" : "+v"(s));  // ordered before the next gather
+  return s;
+}
+
 template <typename F>
 __device__ __forceinline__ void element_energy(const Dims &g, int e, const ModeT<double> &mc,
                                                const double *__restrict__ uaff,
@@ -595,24 +624,34 @@ __device__ __forceinline__ void element_energy(const Dims &g, int e, const ModeT
   const int I[2] = {i, i == g.nx - 1 ? 0 : i + 1};
   const int J[2] = {j, j == g.ny - 1 ? 0 : j + 1};
   const int K[2] = {k, k == g.nz - 1 ? 0 : k + 1};
-#pragma unroll 1
-  for (int a = 0; a < 6; a++) {
-    double chi[24], w[24];
-    gather_chi(g, I, J, K, us, uaff, a, chi);
-    hadamard_voxel<double>(chi, 1.0, mc, w);
-#pragma unroll 1
-    for (int b = a; b < 6; b++) {
-      double chib[24];
-      gather_chi(g, I, J, K, us, uaff, b, chib);
-      double s = 0;
+  int n[8];
 #pragma unroll
-      for (int q = 0; q < 24; q++) s += w[q] * chib[q];
-      sink(a * 6 - a * (a - 1) / 2 + (b - a), s);
+  for (int v = 0; v < 8; v++) n[v] = (I[(v >> 2) & 1] * g.ny + J[(v >> 1) & 1]) * g.nz + K[v & 1];
+  auto pair = [](int a, int b) { return a * 6 - a * (a - 1) / 2 + (b - a); };
+#pragma unroll
+  for (int g0 = 0; g0 < 6; g0 += 2) {
+    double z0[3][8], z1[3][8];
+    {
+      double w[3][8];
+      chi_modes(g, n, us, uaff, g0, w);
+      mode_couple<double>(w, 1.0, mc, z0);
+      sink(pair(g0, g0), mode_dot(z0, w));
+      chi_modes(g, n, us, uaff, g0 + 1, w);
+      sink(pair(g0, g0 + 1), mode_dot(z0, w));
+      mode_couple<double>(w, 1.0, mc, z1);
+      sink(pair(g0 + 1, g0 + 1), mode_dot(z1, w));
+    }
+#pragma unroll
+    for (int b = g0 + 2; b < 6; b++) {
+      double w[3][8];
+      chi_modes(g, n, us, uaff, b, w);
+      sink(pair(g0, b), mode_dot(z0, w));
+      sink(pair(g0 + 1, b), mode_dot(z1, w));
     }
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_ch_energy(Dims g, const double *__restrict__ E,
+__global__ __launch_bounds__(kBlock, 2) void k_ch_energy(Dims g, const double *__restrict__ E,
                                                       ModeT<double> mc,
                                                       const double *__restrict__ uaff,
                                                       const double *__restrict__ u,
@@ -642,7 +681,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restric
   if (threadIdx.x == 0) out[p] = s;
 }
 
-__global__ __launch_bounds__(kBlock) void k_gradient(Dims g, const double *__restrict__ rho,
+__global__ __launch_bounds__(kBlock, 2) void k_gradient(Dims g, const double *__restrict__ rho,
                                                      ModeT<double> mc,
                                                      const double *__restrict__ uaff,
                                                      const double *__restrict__ u,

@@ -18,15 +18,17 @@ namespace femh {
 // node columns (j,k) and marches over a chunk of TI planes in i.  Per voxel
 // slab (planes s, s+1):
 //   1. node plane s+1 of the tile plus a one-node halo on the low j/k sides
-//      (18x18 nodes, periodic wrap) is staged into LDS (three plane buffers,
+//      (18x18 nodes, periodic wrap) is staged into LDS (a three-plane ring,
 //      so the owned node's staged value stays readable for the epilogue);
-//   2. each of the 17x17 voxels touching the tile transforms its 24 corner
-//      values to cube modes (three butterfly stages per component), applies
-//      the 45-term mode coupling scaled by its modulus and transforms back;
-//   3. the voxels' plane-s corner forces go through LDS to the 4 tile nodes
-//      that share each of them: plane s is then complete and passes through
-//      the epilogue; the plane s+1 forces follow the same way and are carried
-//      in registers into the next slab.
+//   2. each of the 17x17 voxels touching the tile transforms its top face
+//      ((j,k) butterflies; the bottom face is the top face of the slab
+//      below, carried in registers), forms the 8 cube modes per component
+//      with the i stage, applies the 45-term mode coupling scaled by its
+//      modulus and splits the result back into bottom- and top-face modes;
+//   3. the bottom-face modes plus the top-face modes of the voxel below (the
+//      same node plane s) go through one inverse face transform into LDS,
+//      and the 4 tile nodes sharing each corner gather them: plane s is
+//      complete and passes through the epilogue.
 // The j/k halo voxels are recomputed by neighbouring tiles (17^2/16^2 of the
 // voxel work) and the i halo slab once per chunk ((TI+1)/TI).
 constexpr int kTile = 16;
@@ -109,10 +111,10 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, 
 
 static inline int tiled_chunk(const Dims &g, int R) {
   const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
-  // planes per chunk: about 8 (halo slab overhead 1/8), but enough chunks to
+  // planes per chunk: about 16 (halo slab overhead 1/16), but enough chunks to
   // put ~2 workgroups per CU across the right-hand sides, and at most
   // kMaxPartials workgroups per right-hand side
-  int TI = 8;
+  int TI = 16;
   while (TI > 2 && (long)tiles * ((g.nx + TI - 1) / TI) * R < 512) TI /= 2;
   if (TI > g.nx) TI = g.nx;
   while ((long)tiles * ((g.nx + TI - 1) / TI) > kMaxPartials) TI++;
@@ -283,8 +285,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
   // -- plane s is then complete after a single LDS exchange.
   constexpr bool kReuse = sizeof(T) == 4;
   T ftop[2][3][4], tmod[2][3][4];
-  auto face_load = [&](int buf, int ej, int ek, int bi_off, T (&f)[3][4]) {
-    (void)bi_off;
+  auto face_load = [&](int buf, int ej, int ek, T (&f)[3][4]) {
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int t = (ej + (v >> 1)) * kTN + ek + (v & 1);
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
       const int e = vt + q * kBlock;
       const int ej = e / kTE, ek = e - ej * kTE;
       T fb[3][4], ft[3][4];
-      face_load(nxt, ej, ek, 1, ft);
+      face_load(nxt, ej, ek, ft);
       face_fwd(ft);
       if (kReuse && p > 0) {
 #pragma unroll
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
 #pragma unroll
           for (int v = 0; v < 4; v++) fb[c][v] = ftop[q][c][v];
       } else {
-        face_load(cur, ej, ek, 0, fb);
+        face_load(cur, ej, ek, fb);
         face_fwd(fb);
       }
       if (kReuse) {
