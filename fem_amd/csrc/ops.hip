@@ -733,10 +733,11 @@ void launch_stencil(int st, int ep, const Dims &g, int R, const T *S, const OpAr
   const int blocks = (g.V + kBlock - 1) / kBlock;
   const int nsmp = (R + kRHS - 1) / kRHS;
   const int sym = (g.nx > 2 && g.ny > 2 && g.nz > 2) ? 1 : 0;
-  // planes of at least one full 16x16 tile: the one-plane LDS kernel (three
-  // load cases of a sample per workgroup, 23 KB of LDS in FP32), once that
-  // gives a workgroup per CU
-  constexpr int NG = kRHS / 2;
+  // planes of at least one full 16x16 tile: the one-plane LDS kernel (all six
+  // load cases of a sample per workgroup in FP32, 47 KB of LDS: the stencil
+  // row is read once per node; three in FP64), once that gives a workgroup
+  // per CU
+  constexpr int NG = sizeof(T) == 4 ? kRHS : kRHS / 2;
   const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
   const long pl_blocks = (long)tiles_j * tiles_k * g.nx * nsmp * (kRHS / NG);
   // (and a sample's stencil addressable by a buffer resource: < 2 GiB)
