@@ -240,72 +240,110 @@ __global__ __launch_bounds__(kBlock) void k_stencil_diag(StencilOp<T> op, T *__r
   }
 }
 
+// Grid transfers between a level and the next coarser one (dims exactly
+// halved, periodic).  A thread owns one (j,k) column of its output level and
+// a chunk of CI coarse planes: the (j,k)-weighted sum of an input plane is
+// formed once and shared by the two output planes it contributes to.
+// CI: up to 4 planes, fewer when that is needed for ~64K threads.
+static inline int xfer_chunk(const Dims &cg, int cols, int R) {
+  int ci = 4;
+  while (ci > 1 && (long)cols * R * ((cg.nx + ci - 1) / ci) < 65536) ci /= 2;
+  return ci;
+}
+
+// rc = P^T rf, full weighting (1/2, 1, 1/2) per axis
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_restrict(Dims fg, Dims cg, const T *__restrict__ rf,
-                                                     T *__restrict__ rc, const int *__restrict__ act) {
+                                                     T *__restrict__ rc, const int *__restrict__ act,
+                                                     int CI) {
   const int r = blockIdx.y;
   if (!act[r]) return;
+  const int cols = cg.ny * cg.nz;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int chunk = t / cols, col = t - chunk * cols;
+  const int I0 = chunk * CI;
+  if (I0 >= cg.nx) return;
+  const int I1 = min(I0 + CI, cg.nx);
+  const int J = col / cg.nz, K = col - J * cg.nz;
+  const int fj[3] = {J == 0 ? fg.ny - 1 : 2 * J - 1, 2 * J, 2 * J + 1};
+  const int fk[3] = {K == 0 ? fg.nz - 1 : 2 * K - 1, 2 * K, 2 * K + 1};
   const T *f = rf + (size_t)r * 3 * fg.V;
   T *c = rc + (size_t)r * 3 * cg.V;
-  for (int N = blockIdx.x * kBlock + threadIdx.x; N < cg.V; N += gridDim.x * kBlock) {
-    int I, J, K;
-    node_ijk(cg, N, I, J, K);
-    int fi[3], fj[3], fk[3];
+  auto plane2d = [&](int i, double (&sum)[3]) {
 #pragma unroll
-    for (int o = 0; o < 3; o++) {
-      fi[o] = (2 * I + o - 1 + fg.nx) % fg.nx;
-      fj[o] = (2 * J + o - 1 + fg.ny) % fg.ny;
-      fk[o] = (2 * K + o - 1 + fg.nz) % fg.nz;
+    for (int q = 0; q < 3; q++) {
+      double acc = 0;
+#pragma unroll
+      for (int oj = 0; oj < 3; oj++) {
+        const T *row = f + (size_t)q * fg.V + ((size_t)i * fg.ny + fj[oj]) * fg.nz;
+        const double rs = 0.5 * (double)row[fk[0]] + (double)row[fk[1]] + 0.5 * (double)row[fk[2]];
+        acc += (oj == 1 ? 1.0 : 0.5) * rs;
+      }
+      sum[q] = acc;
     }
-    double acc[3] = {0, 0, 0};
+  };
+  double prev[3];
+  plane2d(I0 == 0 ? fg.nx - 1 : 2 * I0 - 1, prev);
+  for (int I = I0; I < I1; I++) {
+    double cur[3], nxt[3];
+    plane2d(2 * I, cur);
+    plane2d(2 * I + 1, nxt);
 #pragma unroll
-    for (int on = 0; on < 27; on++) {
-      const int oi = on / 9, oj = (on / 3) % 3, ok = on % 3;
-      const double w = (oi != 1 ? 0.5 : 1.0) * (oj != 1 ? 0.5 : 1.0) * (ok != 1 ? 0.5 : 1.0);
-      const int n = (fi[oi] * fg.ny + fj[oj]) * fg.nz + fk[ok];
-#pragma unroll
-      for (int q = 0; q < 3; q++) acc[q] += w * (double)f[(size_t)q * fg.V + n];
+    for (int q = 0; q < 3; q++) {
+      c[(size_t)q * cg.V + ((size_t)I * cg.ny + J) * cg.nz + K] = (T)(0.5 * prev[q] + cur[q] + 0.5 * nxt[q]);
+      prev[q] = nxt[q];
     }
-#pragma unroll
-    for (int q = 0; q < 3; q++) c[(size_t)q * cg.V + N] = (T)acc[q];
   }
 }
 
 // ef += P ec (trilinear interpolation).  Along each axis a fine node 2I takes
 // coarse node I with weight 1, a fine node 2I+1 takes I and I+1 with 1/2
-// each; every lane reads the same 8 coarse nodes with per-axis weights
-// (w0, w1) = (1, 0) or (1/2, 1/2), so the loop does not diverge on parity.
+// each.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(Dims fg, Dims cg, const T *__restrict__ ec,
                                                         T *__restrict__ ef,
-                                                        const int *__restrict__ act) {
+                                                        const int *__restrict__ act, int CI) {
   const int r = blockIdx.y;
   if (!act[r]) return;
+  const int cols = fg.ny * fg.nz;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int chunk = t / cols, col = t - chunk * cols;
+  const int I0 = chunk * CI;
+  if (I0 >= cg.nx) return;
+  const int I1 = min(I0 + CI, cg.nx);
+  const int j = col / fg.nz, k = col - j * fg.nz;
+  // every lane reads two coarse rows / columns with weights (1, 0) or
+  // (1/2, 1/2): no divergence on parity
+  const int cj[2] = {j >> 1, (j + 1) >> 1 == cg.ny ? 0 : (j + 1) >> 1};
+  const int ck[2] = {k >> 1, (k + 1) >> 1 == cg.nz ? 0 : (k + 1) >> 1};
+  const T wj[2] = {(j & 1) ? (T)0.5 : (T)1, (j & 1) ? (T)0.5 : (T)0};
+  const T wk[2] = {(k & 1) ? (T)0.5 : (T)1, (k & 1) ? (T)0.5 : (T)0};
   const T *c = ec + (size_t)r * 3 * cg.V;
   T *f = ef + (size_t)r * 3 * fg.V;
-  for (int n = blockIdx.x * kBlock + threadIdx.x; n < fg.V; n += gridDim.x * kBlock) {
-    int i, j, k;
-    node_ijk(fg, n, i, j, k);
-    const int ci[2] = {i >> 1, ((i + 1) >> 1) % cg.nx};
-    const int cj[2] = {j >> 1, ((j + 1) >> 1) % cg.ny};
-    const int ck[2] = {k >> 1, ((k + 1) >> 1) % cg.nz};
-    const T wi[2] = {(i & 1) ? (T)0.5 : (T)1, (i & 1) ? (T)0.5 : (T)0};
-    const T wj[2] = {(j & 1) ? (T)0.5 : (T)1, (j & 1) ? (T)0.5 : (T)0};
-    const T wk[2] = {(k & 1) ? (T)0.5 : (T)1, (k & 1) ? (T)0.5 : (T)0};
-    T acc[3] = {0, 0, 0};
+  auto plane2d = [&](int I, T (&sum)[3]) {
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int q = 0; q < 3; q++) {
+      T acc = 0;
 #pragma unroll
-      for (int b2 = 0; b2 < 2; b2++)
+      for (int b2 = 0; b2 < 2; b2++) {
+        const T *row = c + (size_t)q * cg.V + ((size_t)I * cg.ny + cj[b2]) * cg.nz;
+        acc += wj[b2] * (wk[0] * row[ck[0]] + wk[1] * row[ck[1]]);
+      }
+      sum[q] = acc;
+    }
+  };
+  T a[3];
+  plane2d(I0, a);
+  for (int I = I0; I < I1; I++) {
+    T b[3];
+    plane2d(I + 1 == cg.nx ? 0 : I + 1, b);
 #pragma unroll
-        for (int d = 0; d < 2; d++) {
-          const int N = (ci[a] * cg.ny + cj[b2]) * cg.nz + ck[d];
-          const T w = wi[a] * wj[b2] * wk[d];
-#pragma unroll
-          for (int q = 0; q < 3; q++) acc[q] += w * c[(size_t)q * cg.V + N];
-        }
-#pragma unroll
-    for (int q = 0; q < 3; q++) f[(size_t)q * fg.V + n] += acc[q];
+    for (int q = 0; q < 3; q++) {
+      T *fp = f + (size_t)q * fg.V + (size_t)(2 * I) * fg.ny * fg.nz + col;
+      fp[0] += a[q];
+      fp[(size_t)fg.ny * fg.nz] += (T)0.5 * (a[q] + b[q]);
+      a[q] = b[q];
+    }
   }
 }
 
@@ -764,14 +802,18 @@ void launch_stencil_diag(const StencilOp<T> &op, int B, T *d, hipStream_t s) {
 template <typename T>
 void launch_restrict(Dims fine, Dims coarse, int R, const T *rf, T *rc, const int *act,
                      hipStream_t s) {
-  hipLaunchKernelGGL(k_restrict<T>, grid2(coarse.V, R), dim3(kBlock), 0, s, fine, coarse, rf, rc,
-                     act);
+  const int cols = coarse.ny * coarse.nz, ci = xfer_chunk(coarse, cols, R);
+  const long n = (long)cols * ((coarse.nx + ci - 1) / ci);
+  hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((n + kBlock - 1) / kBlock), R), dim3(kBlock), 0, s,
+                     fine, coarse, rf, rc, act, ci);
 }
 template <typename T>
 void launch_prolong_add(Dims fine, Dims coarse, int R, const T *ec, T *ef, const int *act,
                         hipStream_t s) {
-  hipLaunchKernelGGL(k_prolong_add<T>, grid2(fine.V, R), dim3(kBlock), 0, s, fine, coarse, ec, ef,
-                     act);
+  const int cols = fine.ny * fine.nz, ci = xfer_chunk(coarse, cols, R);
+  const long n = (long)cols * ((coarse.nx + ci - 1) / ci);
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)((n + kBlock - 1) / kBlock), R), dim3(kBlock), 0, s,
+                     fine, coarse, ec, ef, act, ci);
 }
 template <typename T>
 void launch_dense_solve(int n, int R, const T *Ainv, const T *b, T *x, const int *act,
