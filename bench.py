@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--ch-tol", type=float, default=1e-7,
                     help="a-posteriori C_H error target of the mixed path (the metric allows 1e-6; "
                          "the estimate has stayed within 3x of the true error)")
-    ap.add_argument("--inner-tol", type=float, default=3e-4,
+    ap.add_argument("--inner-tol", type=float, default=4e-4,
                     help="relative residual reduction of one FP32 PCG sweep")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solves")
