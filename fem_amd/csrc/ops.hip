@@ -621,12 +621,18 @@ __global__ __launch_bounds__(kBlock) void k_stencil_plane(Dims g, const T *__res
     for (int q = 0; q < 2; q++) {
       if (soff[q] < 0) continue;
       const int n = iw * plane + soff[q];
+      // the sample's inverse diagonal is shared by its load cases
+      T sc[3] = {1, 1, 1};
+      if (ST == ST_DSCALE) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) sc[c] = ti * Ds[(size_t)c * V + n];
+      }
 #pragma unroll
       for (int r = 0; r < NG; r++)
         // inactive load cases stage zeros: the phases then run branch-free
 #pragma unroll
         for (int c = 0; c < 3; c++)
-          stg[q][r][c] = ((mask >> r) & 1u) ? stage_at<T, ST>(a, Ds, ti, off0 + (size_t)r * 3 * V, V, c, n) : (T)0;
+          stg[q][r][c] = ((mask >> r) & 1u) ? a.u[off0 + (size_t)r * 3 * V + (size_t)c * V + n] * sc[c] : (T)0;
     }
   };
   auto put = [&](int buf) {
