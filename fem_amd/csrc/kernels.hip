@@ -103,11 +103,13 @@ __global__ __launch_bounds__(kBlock) void k_fine_diag(FineOp<T> op, T *__restric
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fine_rhs(FineOp<T> op, const double *__restrict__ gtab,
-                                                     T *__restrict__ b) {
+                                                     T *__restrict__ b, float *__restrict__ b32,
+                                                     Reduce red, int slot) {
   const Dims g = op.g;
   const int q = blockIdx.y, lc = q % kRHS;
   const T *Es = op.E + (size_t)(q / kRHS) * g.V;
   T *br = b + (size_t)q * 3 * g.V;
+  double ss = 0;
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
     int i, j, k;
     node_ijk(g, n, i, j, k);
@@ -120,10 +122,15 @@ __global__ __launch_bounds__(kBlock) void k_fine_rhs(FineOp<T> op, const double 
 #pragma unroll
       for (int c = 0; c < 3; c++) acc[c] -= Ee * gtab[lc * 24 + 3 * a + c];
     }
-    br[n] = (T)acc[0];
-    br[g.V + n] = (T)acc[1];
-    br[2 * g.V + n] = (T)acc[2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const T v = (T)acc[c];
+      br[(size_t)c * g.V + n] = v;
+      if (b32) b32[(size_t)q * 3 * g.V + (size_t)c * g.V + n] = (float)v;
+      ss += (double)v * (double)v;
+    }
   }
+  block_partial(red, slot, q, blockIdx.x, block_sum(ss));
 }
 
 // ------------------------------------------------------------------------
@@ -775,8 +782,12 @@ void launch_fine_diag(const FineOp<T> &op, int B, T *d, hipStream_t s) {
   hipLaunchKernelGGL(k_fine_diag<T>, grid2(op.g.V, B), dim3(kBlock), 0, s, op, d);
 }
 template <typename T>
-void launch_fine_rhs(const FineOp<T> &op, int R, const double *g, T *b, hipStream_t s) {
-  hipLaunchKernelGGL(k_fine_rhs<T>, grid2(op.g.V, R), dim3(kBlock), 0, s, op, g, b);
+void launch_fine_rhs(const FineOp<T> &op, int R, const double *g, T *b, float *b32, const Reduce &red,
+                     int slot, hipStream_t s) {
+  // vec_parts(V) blocks, the partial count the b.b reader expects (blocks
+  // past the last node contribute 0)
+  hipLaunchKernelGGL(k_fine_rhs<T>, dim3(vec_parts(op.g.V), R), dim3(kBlock), 0, s, op, g, b, b32, red,
+                     slot);
 }
 template <typename T>
 void launch_build_level1(Dims fine, Dims coarse, int B, const T *E, const double *M, T *S,
@@ -941,7 +952,8 @@ void reduce_init(Reduce &red, hipStream_t s) {
   template void launch_dense_apply<T>(const FineOp<T> &, int, const T *, T *, const T *, int,     \
                                       const int *, const Reduce *, int, hipStream_t);             \
   template void launch_fine_diag<T>(const FineOp<T> &, int, T *, hipStream_t);                    \
-  template void launch_fine_rhs<T>(const FineOp<T> &, int, const double *, T *, hipStream_t);     \
+  template void launch_fine_rhs<T>(const FineOp<T> &, int, const double *, T *, float *,          \
+                                   const Reduce &, int, hipStream_t);                             \
   template void launch_build_level1<T>(Dims, Dims, int, const T *, const double *, T *,           \
                                        hipStream_t);                                              \
   template void launch_galerkin<T>(Dims, Dims, int, const T *, T *, T *, T *, hipStream_t);       \

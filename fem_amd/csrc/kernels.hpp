@@ -86,10 +86,12 @@ void launch_dense_apply(const FineOp<T> &op, int R, const T *u, T *y, const T *b
                         const int *act, const Reduce *red, int slot, hipStream_t s);
 template <typename T>
 void launch_fine_diag(const FineOp<T> &op, int B, T *d, hipStream_t s);
-// b[q] = -sum_e E_e g_lc (element load of the affine field)
+// b[q] = -sum_e E_e g_lc (element load of the affine field), with b.b of
+// every RHS into slot (vec_parts(V) partials) and, when b32 is given, an FP32
+// copy of b
 template <typename T>
-void launch_fine_rhs(const FineOp<T> &op, int R, const double *g /*device 6x24*/, T *b,
-                     hipStream_t s);
+void launch_fine_rhs(const FineOp<T> &op, int R, const double *g /*device 6x24*/, T *b, float *b32,
+                     const Reduce &red, int slot, hipStream_t s);
 
 // ---- coarse levels --------------------------------------------------------
 template <typename T>

@@ -811,8 +811,11 @@ void Context::homogenize(double *C, femh_stats *st) {
   set_act(A_ALL, all);
   build_hierarchy();
   FineOp<double> f0{g, E64.as<double>(), Ke64.as<double>()};
-  launch_fine_rhs<double>(f0, R, gtab.as<double>(), b.as<double>(), stream);
-  launch_dot<double>(g.V, R, b.as<double>(), b.as<double>(), S_BB, actp(A_ALL), red, stream);
+  // a cold mixed-precision solve starts its first FP32 sweep from r = b:
+  // the right-hand side kernel also leaves b in FP32 there
+  const bool cold_mixed = o.precision == FEMH_MIXED && !warm;
+  launch_fine_rhs<double>(f0, R, gtab.as<double>(), b.as<double>(), cold_mixed ? r32.as<float>() : nullptr,
+                          red, S_BB, stream);
   FEMH_HIP(hipGetLastError());
   std::vector<double> bb(R);
   read_scalars(S_BB, bb.data(), R, vec_parts(g.V));
@@ -889,11 +892,10 @@ void Context::homogenize(double *C, femh_stats *st) {
           relres[t] = std::sqrt(rr[t] / bb[t]);
           if (relres[t] <= o.tol) active[t] = 0;
         }
-    } else {
-      FEMH_HIP(hipMemsetAsync(x.p, 0, field_bytes(g, nb, sizeof(double)), stream));
-      FEMH_HIP(hipMemcpyAsync(r.p, b.p, field_bytes(g, nb, sizeof(double)), hipMemcpyDeviceToDevice,
-                              stream));
     }
+    // cold start: x = 0 and r = b are implicit in the first sweep (r32 = b
+    // came from the right-hand side kernel; x is written as x32 after it)
+    bool x_set = warm;
     double aii[kRHS];
     for (int k = 0; k < kRHS; k++) {
       double acc = 0;
@@ -910,7 +912,7 @@ void Context::homogenize(double *C, femh_stats *st) {
       const int *ao = actp(A_OUT);
       // the FP32 sweep solves K d = r: its right-hand side is the residual
       // itself; z = M^-1 r and r.z serve the estimate and the first PCG step
-      launch_convert<double, float>((size_t)R * 3 * g.V, r.as<double>(), r32.as<float>(), stream);
+      if (x_set) launch_convert<double, float>((size_t)R * 3 * g.V, r.as<double>(), r32.as<float>(), stream);
       precond<float>(h32, r32.as<float>(), z32.as<float>(), ao, S_RZ0);
       FEMH_HIP(hipGetLastError());
       if (o.ch_tol > 0 && (sweep > 0 || warm)) {
@@ -942,7 +944,13 @@ void Context::homogenize(double *C, femh_stats *st) {
                  p32.as<float>(), q32.as<float>(), act_in, o.inner_tol > 0 ? o.inner_tol : 1e-4,
                  std::min(o.max_iter, kInnerMaxIter), rr.data(), it_in.data(), rel_in.data(), true);
       set_act(A_OUT, active);
-      launch_add_f32_to_f64(g.V, R, x32.as<float>(), x.as<double>(), ao, stream);
+      if (x_set) {
+        launch_add_f32_to_f64(g.V, R, x32.as<float>(), x.as<double>(), ao, stream);
+      } else {
+        // the sweep zeroed x32 for every RHS first: inactive ones stay 0
+        launch_convert<float, double>((size_t)R * 3 * g.V, x32.as<float>(), x.as<double>(), stream);
+        x_set = true;
+      }
       const int npd = residual_dots(ao);
       FEMH_HIP(hipGetLastError());
       read_scalars(S_RR, rn.data(), R, npd);
@@ -1000,6 +1008,8 @@ void Context::homogenize(double *C, femh_stats *st) {
       }
     }
     if (nact) status = FEMH_NOT_CONVERGED;
+    // every right-hand side vanished before a sweep: u = 0
+    if (!x_set) FEMH_HIP(hipMemsetAsync(x.p, 0, field_bytes(g, nb, sizeof(double)), stream));
   }
   // C_H and the adjoint do not see constant shifts of the fields (K0 maps
   // them to zero); the mean is taken out when the fields are read out
