@@ -385,62 +385,82 @@ void Context::dense_coarse(Hierarchy<double> &h) {
   h.ndense = n;
 }
 
+// level operators of h from its fine moduli h.E (built in h's precision; the
+// Galerkin kernels accumulate in FP64): fine diagonal, level-1 stencil from
+// the moduli, Galerkin stencils below, diagonals and their inverses
+template <typename T>
+void Context::build_ops(Hierarchy<T> &h) {
+  FineOp<T> f0{g, h.E.template as<T>(), h.Ke.template as<T>()};
+  launch_fine_diag<T>(f0, nb, h.lv[0].D.template as<T>(), stream);
+  for (int l = 1; l < nlevels; l++) {
+    Level<T> &L = h.lv[l];
+    if (l == 1)
+      launch_build_level1<T>(g, L.g, nb, h.E.template as<T>(), Mchild.as<double>(), L.S.template as<T>(),
+                             stream);
+    else
+      launch_galerkin<T>(h.lv[l - 1].g, L.g, nb, h.lv[l - 1].S.template as<T>(), L.S.template as<T>(),
+                         h.gt1.template as<T>(), h.gt2.template as<T>(), stream);
+    StencilOp<T> so{L.g, L.S.template as<T>()};
+    launch_stencil_diag<T>(so, nb, L.D.template as<T>(), stream);
+  }
+  for (int l = 0; l < nlevels; l++)
+    launch_invert<T>((size_t)3 * nb * h.lv[l].g.V, h.lv[l].D.template as<T>(), h.lv[l].Dinv.template as<T>(),
+                     stream);
+  FEMH_HIP(hipGetLastError());
+}
+
 void Context::build_hierarchy() {
   build_levels<double>(h64);
   nlevels = (int)h64.lv.size();
   FEMH_HIP(hipMemcpyAsync(h64.E.p, E64.p, sizeof(double) * nb * g.V, hipMemcpyDeviceToDevice, stream));
   FEMH_HIP(hipMemcpyAsync(h64.Ke.p, Ke64.p, sizeof(double) * 576, hipMemcpyDeviceToDevice, stream));
-  FineOp<double> f0{g, h64.E.as<double>(), h64.Ke.as<double>()};
-  launch_fine_diag<double>(f0, nb, h64.lv[0].D.as<double>(), stream);
-  for (int l = 1; l < nlevels; l++) {
-    Level<double> &L = h64.lv[l];
-    if (l == 1)
-      launch_build_level1<double>(g, L.g, nb, E64.as<double>(), Mchild.as<double>(), L.S.as<double>(),
-                                  stream);
-    else
-      launch_galerkin<double>(h64.lv[l - 1].g, L.g, nb, h64.lv[l - 1].S.as<double>(), L.S.as<double>(),
-                              h64.gt1.as<double>(), h64.gt2.as<double>(), stream);
-    StencilOp<double> so{L.g, L.S.as<double>()};
-    launch_stencil_diag<double>(so, nb, L.D.as<double>(), stream);
-  }
-  for (int l = 0; l < nlevels; l++)
-    launch_invert<double>((size_t)3 * nb * h64.lv[l].g.V, h64.lv[l].D.as<double>(),
-                          h64.lv[l].Dinv.as<double>(), stream);
-  FEMH_HIP(hipGetLastError());
-  if (o.precond == FEMH_PRECOND_MG) {
-    // the smoother's spectrum bound: from the operators that will smooth
-    // (the FP32 copies in mixed precision, estimated below)
-    if (o.precision != FEMH_MIXED) estimate_lmax(h64);
-    // the coarsest level is inverted densely when it is small enough, else
-    // solved approximately by a long Chebyshev polynomial (cheb_run)
-    if (nlevels > 1) dense_coarse(h64);
-  }
-  h64.ready = true;
-  if (o.precision == FEMH_MIXED) {
+  h64_ops = false;
+  if (o.precision != FEMH_MIXED) {
+    build_ops<double>(h64);
+    h64_ops = true;
+    if (o.precond == FEMH_PRECOND_MG) {
+      estimate_lmax(h64);
+      // the coarsest level is inverted densely when it is small enough, else
+      // solved approximately by a long Chebyshev polynomial (cheb_run)
+      if (nlevels > 1) dense_coarse(h64);
+    }
+  } else {
+    // mixed precision: the smoothing hierarchy is built in FP32 directly;
+    // the FP64 fine operator (residuals) needs only h64.E / h64.Ke
     build_levels<float>(h32);
     launch_convert<double, float>((size_t)nb * g.V, E64.as<double>(), h32.E.as<float>(), stream);
     launch_convert<double, float>(576, Ke64.as<double>(), h32.Ke.as<float>(), stream);
-    for (int l = 0; l < nlevels; l++) {
-      const size_t nd = (size_t)3 * nb * h64.lv[l].g.V;
-      launch_convert<double, float>(nd, h64.lv[l].D.as<double>(), h32.lv[l].D.as<float>(), stream);
-      launch_convert<double, float>(nd, h64.lv[l].Dinv.as<double>(), h32.lv[l].Dinv.as<float>(), stream);
-      if (l > 0)
-        launch_convert<double, float>((size_t)243 * nb * h64.lv[l].g.V, h64.lv[l].S.as<double>(),
-                                      h32.lv[l].S.as<float>(), stream);
-    }
-    if (h64.ndense) {
-      h32.Ainv.alloc(sizeof(float) * (size_t)h64.ndense * h64.ndense * B);
-      launch_convert<double, float>((size_t)h64.ndense * h64.ndense * nb, h64.Ainv.as<double>(),
-                                    h32.Ainv.as<float>(), stream);
-      h32.ndense = h64.ndense;
-    }
-    FEMH_HIP(hipGetLastError());
+    build_ops<float>(h32);
     if (o.precond == FEMH_PRECOND_MG) {
+      if (nlevels > 1) {
+        // dense coarsest inverse of the FP32 coarsest operator (host Cholesky
+        // in FP64 on its widened copy)
+        const Level<float> &Lc = h32.lv.back();
+        launch_convert<float, double>((size_t)243 * nb * Lc.g.V, Lc.S.as<float>(), h64.lv.back().S.as<double>(),
+                                      stream);
+        dense_coarse(h64);
+        if (h64.ndense) {
+          h32.Ainv.alloc(sizeof(float) * (size_t)h64.ndense * h64.ndense * B);
+          launch_convert<double, float>((size_t)h64.ndense * h64.ndense * nb, h64.Ainv.as<double>(),
+                                        h32.Ainv.as<float>(), stream);
+          h32.ndense = h64.ndense;
+        }
+      }
       estimate_lmax(h32);
       for (int l = 0; l < nlevels; l++) h64.lv[l].lmax = h32.lv[l].lmax;
     }
+    FEMH_HIP(hipGetLastError());
     h32.ready = true;
   }
+  h64.ready = true;
+}
+
+void Context::ensure_h64_ops() {
+  if (h64_ops) return;
+  build_ops<double>(h64);
+  // (spectrum bounds stay those of the FP32 operators)
+  if (o.precond == FEMH_PRECOND_MG && nlevels > 1) dense_coarse(h64);
+  h64_ops = true;
 }
 
 // ---------------------------------------------------------------------------
@@ -990,6 +1010,7 @@ void Context::homogenize(double *C, femh_stats *st) {
         std::vector<double> rel64(R, 0.0);
         // K d = r (relative to the original right-hand side), x += d
         if (!dq.p) dq.alloc(field_bytes(g, B, sizeof(double)));
+        ensure_h64_ops();
         pcg<double>(h64, r.as<double>(), dx.as<double>(), z.as<double>(), p.as<double>(),
                     q.as<double>(), dq.as<double>(), act64, o.tol, o.max_iter, bb.data(), it64.data(),
                     rel64.data());
@@ -1084,6 +1105,7 @@ void Context::get_fields(double *u) {
 
 void Context::apply_level(int level, const double *u, double *y) {
   if (!h64.ready) throw Error(FEMH_NO_FIELDS, "no hierarchy: call femh_prepare first");
+  ensure_h64_ops();
   if (level < 0 || level >= nlevels) throw Error(FEMH_BAD_ARG, "level out of range");
   Level<double> &L = h64.lv[level];
   const size_t n = (size_t)3 * L.g.V;

@@ -141,6 +141,10 @@ struct Context {
   // inputs: n samples (<= B) of density (interp applied) or modulus
   void load(const double *src, int n, bool device_src, bool modulus);
   void build_hierarchy();
+  // FP64 level operators of a mixed-precision context, built on first use
+  // (the FP64 fallback, apply_level); mixed solves smooth with FP32 ones
+  void ensure_h64_ops();
+  bool h64_ops = false;
   void homogenize(double *C, femh_stats *st);  // C[36 nb], st[nb] (may be null)
   void gradient(const double *dJdC, double *out_host, bool energy_only);
   void get_fields(double *u);
@@ -151,6 +155,8 @@ struct Context {
   void set_act(int slot, const std::vector<int> &v);
   template <typename T>
   void build_levels(Hierarchy<T> &h);
+  template <typename T>
+  void build_ops(Hierarchy<T> &h);
   template <typename T>
   void op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a);
   template <typename T>
