@@ -56,7 +56,7 @@ __device__ __forceinline__ void epi_load(const OpArgs<T> &a, const T *__restrict
     const size_t q = off + (size_t)c * V + n;
     if (EP == EP_RES || EP == EP_CHEB0) p.e0[c] = a.b[q];
     if (EP == EP_STEP) p.e0[c] = a.rprev[q];
-    if (EP == EP_CHEB0 || EP == EP_STEP) p.e1[c] = Ds[(size_t)c * V + n];
+    if (EP == EP_CHEB0 || EP == EP_STEP || (EP == EP_AX && a.ydinv)) p.e1[c] = Ds[(size_t)c * V + n];
     if (EP == EP_STEP) p.e2[c] = a.xprev ? a.xprev[q] : (T)0;
     if (EP == EP_STEP) p.e3[c] = a.dot ? a.b[q] : (T)0;
   }
@@ -77,7 +77,7 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, 
   for (int c = 0; c < 3; c++) {
     const size_t q = off + (size_t)c * V + n;
     if (EP == EP_AX) {
-      a.y[q] = Au[c];
+      a.y[q] = a.ydinv ? Au[c] * p.e1[c] : Au[c];
       if (a.pout) a.pout[q] = uo[c];
       if (a.dot) dv.a += (double)uo[c] * (double)Au[c];
     } else if (EP == EP_RES) {
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
     for (int q = 0; q < 2; q++) Ev[q] = eoff[q] >= 0 ? E[sw * plane + eoff[q]] : (T)0;
     const size_t nown = (size_t)sl * plane + gj * g.nz + gk;
     EpiIn<T> pin;
-    if (EP != EP_AX && sl >= i0 && owns) epi_load<T, EP>(a, Ds, off, V, (int)nown, pin);
+    if ((EP != EP_AX || a.ydinv) && sl >= i0 && owns) epi_load<T, EP>(a, Ds, off, V, (int)nown, pin);
     if (ST != ST_PROLONG) {
       put(nxt);
       if (sl + 2 <= iend) fetch(sl + 2);

@@ -12,7 +12,8 @@
 // Epilogue at each owned node, with Au the operator result and u_own the
 // staged value there:
 //   EP_AX       y = Au            (optional FP64 partial of u . Au; with pout,
-//                                  pout = u_own: the new PCG direction)
+//                                  pout = u_own: the new PCG direction; with
+//                                  ydinv, y = Au / D: a power-iteration step)
 //   EP_RES      y = b - Au            (optional FP64 partials of r.r, b.u, u.r)
 //   EP_CHEB0    r = b - Au ; dout = r theta_inv / D (not with `last`) ; x = u_own
 //   EP_STEP     r = rprev - Au ; dn = c1 u_own + c2 r / D ; dout = dn ;
@@ -37,6 +38,7 @@ struct OpArgs {
   const T *u2 = nullptr;     // ST_PUPD: previous direction [R][3][V]
   const double *beta = nullptr;  // ST_PUPD: per-RHS beta [R]
   T *pout = nullptr;         // EP_AX: staged value written back (new direction)
+  int ydinv = 0;             // EP_AX: y = Dinv * Au
   const T *xc = nullptr;     // ST_PROLONG: coarse vector [R][3][Vc]
   Dims gc{};                 // ST_PROLONG: coarse dims
   const T *Dinv = nullptr;   // inverse diagonal [B][3][V]

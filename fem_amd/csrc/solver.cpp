@@ -158,6 +158,7 @@ void Context::op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a) {
         FineOp<T> f{g0, h.E.template as<T>(), h.Ke.template as<T>()};
         launch_dense_apply<T>(f, R, a.u, a.y, a.b, ep == EP_RES ? 1 : 0, a.act,
                               a.dot ? &a.red : nullptr, a.slot, stream);
+        if (a.ydinv) launch_scale_by_dinv_norm<T>(g0.V, R, a.y, a.Dinv, a.act, red, stream);
       } else {
         launch_tiled<T>(st, ep, g0, R, h.E.template as<T>(), modecoef, a, stream);
       }
@@ -298,16 +299,27 @@ void Context::estimate_lmax(Hierarchy<T> &h) {
     T *xv = L.x.template as<T>(), *yv = L.Ad.template as<T>();
     // every sample starts from the same vector (batched == one-sample results)
     launch_fill_hash<T>((size_t)R * 3 * V, (size_t)kRHS * 3 * V, xv, 0x51ED27ull + l, stream);
+    // y_k = (D^-1 A)^k x_0 without normalising (ten steps grow it by at most
+    // lambda^10 <= 1e10 or so), one fused launch per step; the estimate
+    // ||y_10|| / ||y_9|| is that of the normalised iteration
     for (int it = 0; it < 10; it++) {
-      apply<T>(h, (int)l, xv, yv, nullptr, 0, ap, nullptr, 0);
-      launch_scale_by_dinv_norm<T>(V, R, yv, L.Dinv.template as<T>(), ap, red, stream);
-      launch_normalize<T>(V, R, yv, xv, ap, red, stream);
+      OpArgs<T> a;
+      a.u = xv;
+      a.y = yv;
+      a.act = ap;
+      a.ydinv = 1;
+      op<T>(h, (int)l, ST_PLAIN, EP_AX, a);
+      std::swap(xv, yv);
     }
+    launch_dot<T>(V, R, yv, yv, S_AUX, ap, red, stream);  // y_9
+    launch_dot<T>(V, R, xv, xv, S_RR, ap, red, stream);   // y_10
     FEMH_HIP(hipGetLastError());
-    std::vector<double> n2(R);
-    read_scalars(S_AUX, n2.data(), R, vec_parts(V));
+    std::vector<double> n9(R), n10(R);
+    read_scalars(S_AUX, n9.data(), R, vec_parts(V));
+    read_scalars(S_RR, n10.data(), R, vec_parts(V));
     for (int s = 0; s < nb; s++) {
-      L.lmax[s] = 1.1 * std::sqrt(std::max(n2[kRHS * s], 0.0));
+      const double a9 = n9[kRHS * s], a10 = n10[kRHS * s];
+      L.lmax[s] = 1.1 * std::sqrt(a9 > 0 ? std::max(a10, 0.0) / a9 : 0.0);
       if (!(L.lmax[s] > 0) || !std::isfinite(L.lmax[s]))
         throw Error(FEMH_NONFINITE, "operator is not finite");
     }
