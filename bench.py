@@ -43,11 +43,11 @@ def parse():
                     help="relative residual reduction of one FP32 PCG sweep")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solves")
-    ap.add_argument("--nu1", type=int, default=2)
-    ap.add_argument("--nu2", type=int, default=2)
+    ap.add_argument("--nu1", type=int, default=1)
+    ap.add_argument("--nu2", type=int, default=1)
     ap.add_argument("--coarse-nu", type=int, default=3,
                     help="Chebyshev degree on the coarse levels (0: --nu1/--nu2)")
-    ap.add_argument("--cheby-ratio", type=float, default=20.0,
+    ap.add_argument("--cheby-ratio", type=float, default=14.0,
                     help="Chebyshev smoothing interval [lmax/ratio, lmax]")
     ap.add_argument("--workload", default="simp64", choices=["simp64", "uneven512", "opt96"],
                     help="simp64: the headline 64^3 line; uneven512: 512 binary cells of "
