@@ -308,8 +308,8 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Dims fg, Dims cg, const T *
 // each.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(Dims fg, Dims cg, const T *__restrict__ ec,
-                                                        T *__restrict__ ef,
-                                                        const int *__restrict__ act, int CI) {
+                                                        T *ef, const int *__restrict__ act, int CI,
+                                                        T *out) {
   const int r = blockIdx.y;
   if (!act[r]) return;
   const int cols = fg.ny * fg.nz;
@@ -326,7 +326,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Dims fg, Dims cg, const 
   const T wj[2] = {(j & 1) ? (T)0.5 : (T)1, (j & 1) ? (T)0.5 : (T)0};
   const T wk[2] = {(k & 1) ? (T)0.5 : (T)1, (k & 1) ? (T)0.5 : (T)0};
   const T *c = ec + (size_t)r * 3 * cg.V;
-  T *f = ef + (size_t)r * 3 * fg.V;
+  const T *f = ef + (size_t)r * 3 * fg.V;
+  T *fo = (out ? out : ef) + (size_t)r * 3 * fg.V;
   auto plane2d = [&](int I, T (&sum)[3]) {
 #pragma unroll
     for (int q = 0; q < 3; q++) {
@@ -346,9 +347,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Dims fg, Dims cg, const 
     plane2d(I + 1 == cg.nx ? 0 : I + 1, b);
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-      T *fp = f + (size_t)q * fg.V + (size_t)(2 * I) * fg.ny * fg.nz + col;
-      fp[0] += a[q];
-      fp[(size_t)fg.ny * fg.nz] += (T)0.5 * (a[q] + b[q]);
+      const size_t e = (size_t)q * fg.V + (size_t)(2 * I) * fg.ny * fg.nz + col, e1 = e + (size_t)fg.ny * fg.nz;
+      fo[e] = f[e] + a[q];
+      fo[e1] = f[e1] + (T)0.5 * (a[q] + b[q]);
       a[q] = b[q];
     }
   }
@@ -820,11 +821,11 @@ void launch_restrict(Dims fine, Dims coarse, int R, const T *rf, T *rc, const in
 }
 template <typename T>
 void launch_prolong_add(Dims fine, Dims coarse, int R, const T *ec, T *ef, const int *act,
-                        hipStream_t s) {
+                        hipStream_t s, T *out) {
   const int cols = fine.ny * fine.nz, ci = xfer_chunk(coarse, cols, R);
   const long n = (long)cols * ((coarse.nx + ci - 1) / ci);
   hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)((n + kBlock - 1) / kBlock), R), dim3(kBlock), 0, s,
-                     fine, coarse, ec, ef, act, ci);
+                     fine, coarse, ec, ef, act, ci, out);
 }
 template <typename T>
 void launch_dense_solve(int n, int R, const T *Ainv, const T *b, T *x, const int *act,
@@ -959,7 +960,7 @@ void reduce_init(Reduce &red, hipStream_t s) {
   template void launch_galerkin<T>(Dims, Dims, int, const T *, T *, T *, T *, hipStream_t);       \
   template void launch_stencil_diag<T>(const StencilOp<T> &, int, T *, hipStream_t);              \
   template void launch_restrict<T>(Dims, Dims, int, const T *, T *, const int *, hipStream_t);    \
-  template void launch_prolong_add<T>(Dims, Dims, int, const T *, T *, const int *, hipStream_t); \
+  template void launch_prolong_add<T>(Dims, Dims, int, const T *, T *, const int *, hipStream_t, T *); \
   template void launch_dense_solve<T>(int, int, const T *, const T *, T *, const int *,           \
                                       hipStream_t);                                               \
   template void launch_cheb_first<T>(int, int, const T *, const T *, T *, T *, double, int,       \

@@ -106,9 +106,10 @@ void launch_stencil_diag(const StencilOp<T> &op, int B, T *d, hipStream_t s);
 template <typename T>
 void launch_restrict(Dims fine, Dims coarse, int R, const T *rf, T *rc, const int *act,
                      hipStream_t s);
+// ef += P ec; with out, out = ef + P ec (ef is then only read)
 template <typename T>
 void launch_prolong_add(Dims fine, Dims coarse, int R, const T *ec, T *ef, const int *act,
-                        hipStream_t s);
+                        hipStream_t s, T *out = nullptr);
 // dense coarsest solve: x = Ainv[s] b for each RHS (n = 3*V_coarsest)
 template <typename T>
 void launch_dense_solve(int n, int R, const T *Ainv, const T *b, T *x, const int *act,

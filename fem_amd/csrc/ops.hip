@@ -90,9 +90,15 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, 
       }
     } else if (EP == EP_CHEB0) {
       const T rv = p.e0[c] - Au[c];
-      a.r[q] = rv;
-      if (!a.last) a.dout[q] = rv * ti * p.e1[c];  // `last`: no smoothing step follows
-      a.x[q] = uo[c];
+      if (a.fin1) {
+        const T xv = uo[c] + rv * ti * p.e1[c];
+        a.x[q] = xv;
+        if (a.dot) dv.a += (double)p.e0[c] * (double)xv;
+      } else {
+        a.r[q] = rv;
+        if (!a.last) a.dout[q] = rv * ti * p.e1[c];  // `last`: no smoothing step follows
+        a.x[q] = uo[c];
+      }
     } else {  // EP_STEP
       const T rv = p.e0[c] - Au[c];
       const T dn = c1 * uo[c] + c2 * rv * p.e1[c];
@@ -417,7 +423,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
       epilogue<T, EP>(a, pin, ti, c1, c2, off, V, (int)nown, Au, uo, dv);
     }
   }
-  if ((EP == EP_AX || EP == EP_STEP || EP == EP_RES) && a.dot) {
+  if ((EP == EP_AX || EP == EP_STEP || EP == EP_RES || EP == EP_CHEB0) && a.dot) {
     const int pidx = chunk * tiles + tile;
     block_partial(a.red, a.slot, r, pidx, block_sum(dv.a));
     if (EP == EP_RES) {

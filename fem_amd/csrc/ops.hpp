@@ -16,6 +16,9 @@
 //                                  ydinv, y = Au / D: a power-iteration step)
 //   EP_RES      y = b - Au            (optional FP64 partials of r.r, b.u, u.r)
 //   EP_CHEB0    r = b - Au ; dout = r theta_inv / D (not with `last`) ; x = u_own
+//               with fin1 (a degree-1 smoothing that ends the V-cycle): only
+//               x = u_own + r theta_inv / D, and with `dot` the FP64 partial
+//               of b . x
 //   EP_STEP     r = rprev - Au ; dn = c1 u_own + c2 r / D ; dout = dn ;
 //               (with `dot`, fine level: the FP64 partial of b . x_new)
 //               x = xprev + u_own ; with `last`: x += dn, and r and dout are
@@ -39,6 +42,7 @@ struct OpArgs {
   const double *beta = nullptr;  // ST_PUPD: per-RHS beta [R]
   T *pout = nullptr;         // EP_AX: staged value written back (new direction)
   int ydinv = 0;             // EP_AX: y = Dinv * Au
+  int fin1 = 0;              // EP_CHEB0: write only the smoothed x (see above)
   const T *xc = nullptr;     // ST_PROLONG: coarse vector [R][3][Vc]
   Dims gc{};                 // ST_PROLONG: coarse dims
   const T *Dinv = nullptr;   // inverse diagonal [B][3][V]

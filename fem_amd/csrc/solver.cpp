@@ -571,6 +571,21 @@ bool Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int 
   a.dout = L.d2.template as<T>();
   a.x = L.xn.template as<T>();
   set_cheb(h, l, a, 0, o.cheby_ratio);
+  if (k == 1 && l == 0 && R > kRHS) {
+    // degree 1 at the fine level: xn = x + P xc out of place, then one launch
+    // leaves the smoothed x (and, fused, the partials of b . x = r . z)
+    launch_prolong_add<T>(L.g, C.g, R, C.x.template as<T>(), xx, act_, stream, L.xn.template as<T>());
+    a.u = L.xn.template as<T>();
+    a.x = xx;
+    a.fin1 = 1;
+    if (dot_slot >= 0) {
+      a.red = red;
+      a.slot = dot_slot;
+      a.dot = 1;
+    }
+    op<T>(h, l, ST_PLAIN, EP_CHEB0, a);
+    return dot_slot >= 0;
+  }
   if (l == 0 && R <= kRHS) {
     // one sample: staging x + P xc inside the operator saves a launch; with
     // more right-hand sides the recomputed interpolation costs more than the
@@ -661,9 +676,10 @@ void Context::precond(Hierarchy<T> &h, const T *rv, T *zv, const int *act_, int 
 }
 
 // the V-cycle's last fine-level smoothing launch also reduces r . z
+// (degree >= 2, or degree 1 on the multi-sample path)
 template <typename T>
 bool Context::fuses_rz(const Hierarchy<T> &h) const {
-  return o.precond == FEMH_PRECOND_MG && h.lv.size() > 1 && o.nu2 >= 2;
+  return o.precond == FEMH_PRECOND_MG && h.lv.size() > 1 && (o.nu2 >= 2 || (o.nu2 == 1 && R > kRHS));
 }
 
 template <typename T>
