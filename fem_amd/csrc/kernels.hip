@@ -697,7 +697,7 @@ __device__ __forceinline__ void element_energy(const Dims &g, int e, const ModeT
   }
 }
 
-__global__ __launch_bounds__(kBlock, 2) void k_ch_energy(Dims g, const double *__restrict__ E,
+__global__ __launch_bounds__(kBlock, 3) void k_ch_energy(Dims g, const double *__restrict__ E,
                                                       ModeT<double> mc,
                                                       const double *__restrict__ uaff,
                                                       const double *__restrict__ u,
