@@ -3,7 +3,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <fstream>
 
 namespace femh {
 
@@ -313,6 +315,7 @@ NumericalCoarseningUsingMultigrid::NumericalCoarseningUsingMultigrid() { femh_de
 
 NumericalCoarseningUsingMultigrid::~NumericalCoarseningUsingMultigrid() {
   if (m_ctx) femh_destroy(m_ctx);
+  if (m_batchCtx) femh_destroy(m_batchCtx);
 }
 
 void NumericalCoarseningUsingMultigrid::init(int N[3], int rep, int sub, const std::vector<BaseMaterial> &mats,
@@ -347,10 +350,9 @@ void NumericalCoarseningUsingMultigrid::init(int N[3], int rep, int sub, const s
   }
 }
 
-void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParameters3D(
-    const std::vector<int> &iMaterials, int N[3], const std::vector<double> &dens, StructureType iType,
-    std::vector<double> &ioTensorValues, std::vector<double> &ioParameters) {
-  if (!m_ctx) throw HostError(FEMH_BAD_ARG, "init() was not called");
+// moduli of one assignment on the engine grid (material 1 solid, others at
+// m_densityMin, or void without variable density)
+void NumericalCoarseningUsingMultigrid::moduli(const std::vector<int> &iMaterials, int N[3], double *oE) const {
   std::vector<int> cells;
   repMaterialAssignment(N[0], N[1], N[2], iMaterials, m_nbBlockRep, m_nbBlockRep, m_nbBlockRep, m_nbSubdiv,
                         cells);
@@ -358,9 +360,16 @@ void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParam
     throw HostError(FEMH_BAD_ARG, "material grid does not match init()");
   double E0, nu;
   m_baseMaterials[1].youngPoisson(E0, nu);
-  std::vector<double> Emod(cells.size());
   for (size_t e = 0; e < cells.size(); e++)
-    Emod[e] = E0 * (m_useVariableDensity ? (cells[e] == 1 ? 1.0 : m_densityMin) : (cells[e] == 1 ? 1.0 : 0.0));
+    oE[e] = E0 * (m_useVariableDensity ? (cells[e] == 1 ? 1.0 : m_densityMin) : (cells[e] == 1 ? 1.0 : 0.0));
+}
+
+void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParameters3D(
+    const std::vector<int> &iMaterials, int N[3], const std::vector<double> &dens, StructureType iType,
+    std::vector<double> &ioTensorValues, std::vector<double> &ioParameters) {
+  if (!m_ctx) throw HostError(FEMH_BAD_ARG, "init() was not called");
+  std::vector<double> Emod((size_t)m_n[0] * m_n[1] * m_n[2]);
+  moduli(iMaterials, N, Emod.data());
   femh_stats st;
   check(m_ctx, femh_homogenize_modulus(m_ctx, Emod.data(), m_C.a.data(), &st), "femh_homogenize_modulus");
   for (int i = 0; i < 6; i++)
@@ -373,6 +382,147 @@ void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParam
     ioParameters.push_back(nuc);
     ioParameters.push_back(G);
   }
+}
+
+void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParameters3D(
+    const std::vector<std::vector<int>> &assign, int N[3], const std::vector<double> &dens, StructureType iType,
+    std::vector<float> &oTensorValues, std::vector<float> &oParameters, int batch) {
+  if (!m_ctx) throw HostError(FEMH_BAD_ARG, "init() was not called");
+  if (batch < 1 || batch > 16) throw HostError(FEMH_BAD_ARG, "batch must be 1..16");
+  if (!m_batchCtx || m_batch != batch) {
+    if (m_batchCtx) femh_destroy(m_batchCtx);
+    m_batchCtx = nullptr;
+    double E, nu;
+    m_baseMaterials[1].youngPoisson(E, nu);
+    femh_opts o = m_opts;
+    o.nx = m_n[0];
+    o.ny = m_n[1];
+    o.nz = m_n[2];
+    o.E0 = E;
+    o.nu = nu;
+    o.interp = FEMH_INTERP_MODULUS;
+    o.batch = batch;
+    const int st = femh_create(&o, &m_batchCtx);
+    if (st != FEMH_OK) {
+      char buf[512] = {0};
+      if (m_batchCtx) {
+        femh_last_error(m_batchCtx, buf, sizeof(buf));
+        femh_destroy(m_batchCtx);
+      }
+      m_batchCtx = nullptr;
+      throw HostError(st, std::string("femh_create: ") + buf);
+    }
+    m_batch = batch;
+  }
+  const size_t V = (size_t)m_n[0] * m_n[1] * m_n[2];
+  std::vector<double> Emod(V * batch), C(36 * (size_t)batch);
+  std::vector<femh_stats> st(batch);
+  for (size_t c0 = 0; c0 < assign.size(); c0 += batch) {
+    const int nb = (int)std::min<size_t>(batch, assign.size() - c0);
+    for (int k = 0; k < nb; k++) moduli(assign[c0 + k], N, Emod.data() + V * k);
+    check(m_batchCtx, femh_homogenize_many_modulus(m_batchCtx, nb, Emod.data(), C.data(), st.data()),
+          "femh_homogenize_many_modulus");
+    for (int k = 0; k < nb; k++) {
+      for (int i = 0; i < 6; i++)
+        for (int j = 0; j <= i; j++)
+          oTensorValues.push_back((float)(m_referenceTriangle && j < i ? 0.0 : C[36 * k + 6 * i + j]));
+      oParameters.push_back((float)computeMaterialDensity(assign[c0 + k], dens));
+      if (iType == Cubic) {
+        for (int q = 0; q < 36; q++) m_C.a[q] = C[36 * k + q];
+        double E, nuc, G;
+        cubicParameters(m_C, E, nuc, G);
+        oParameters.push_back((float)E);
+        oParameters.push_back((float)nuc);
+        oParameters.push_back((float)G);
+      }
+    }
+  }
+}
+
+// ---- caller pipeline files ------------------------------------------------------
+namespace cfgbin {
+namespace {
+template <class T>
+bool read_vec(std::ifstream &f, std::vector<T> &out) {
+  int32_t n = 0;
+  if (!f.read(reinterpret_cast<char *>(&n), sizeof(n)) || n < 0) return false;
+  out.resize((size_t)n);
+  if (n > 0 && !f.read(reinterpret_cast<char *>(out.data()), (std::streamsize)(sizeof(T) * (size_t)n)))
+    return false;
+  return true;
+}
+template <class T>
+bool write_vec(std::ofstream &f, const std::vector<T> &v) {
+  const int32_t n = (int32_t)v.size();
+  f.write(reinterpret_cast<const char *>(&n), sizeof(n));
+  if (n > 0) f.write(reinterpret_cast<const char *>(v.data()), (std::streamsize)(sizeof(T) * v.size()));
+  return (bool)f;
+}
+template <class T>
+bool read1(const std::string &file, std::vector<T> &out) {
+  std::ifstream f(file, std::ios::binary);
+  out.clear();
+  return f.is_open() && read_vec(f, out);
+}
+template <class T>
+bool read2(const std::string &file, std::vector<std::vector<T>> &out) {
+  std::ifstream f(file, std::ios::binary);
+  out.clear();
+  int32_t nvec = 0;
+  if (!f.is_open() || !f.read(reinterpret_cast<char *>(&nvec), sizeof(nvec)) || nvec < 0) return false;
+  out.resize((size_t)nvec);
+  for (auto &v : out)
+    if (!read_vec(f, v)) return false;
+  return true;
+}
+template <class T>
+bool write1(const std::string &file, const std::vector<T> &v) {
+  std::ofstream f(file, std::ios::binary | std::ios::trunc);
+  return f.is_open() && write_vec(f, v);
+}
+template <class T>
+bool write2(const std::string &file, const std::vector<std::vector<T>> &v) {
+  std::ofstream f(file, std::ios::binary | std::ios::trunc);
+  if (!f.is_open()) return false;
+  const int32_t nvec = (int32_t)v.size();
+  f.write(reinterpret_cast<const char *>(&nvec), sizeof(nvec));
+  for (const auto &x : v)
+    if (!write_vec(f, x)) return false;
+  return (bool)f;
+}
+}  // namespace
+bool readBinary(const std::string &file, std::vector<float> &out) { return read1(file, out); }
+bool readBinary(const std::string &file, std::vector<int> &out) { return read1(file, out); }
+bool readBinary(const std::string &file, std::vector<std::vector<int>> &out) { return read2(file, out); }
+bool readBinary(const std::string &file, std::vector<std::vector<float>> &out) { return read2(file, out); }
+bool writeBinary(const std::string &file, const std::vector<float> &v) { return write1(file, v); }
+bool writeBinary(const std::string &file, const std::vector<int> &v) { return write1(file, v); }
+bool writeBinary(const std::string &file, const std::vector<std::vector<int>> &v) { return write2(file, v); }
+bool writeBinary(const std::string &file, const std::vector<std::vector<float>> &v) { return write2(file, v); }
+}  // namespace cfgbin
+
+int runTensorHomogenization(const std::string &dir, int level, int nbBlockRep, int nbSubdiv,
+                            const std::vector<BaseMaterial> &mats, const std::vector<double> &dens,
+                            bool cubicOnly, const femh_opts *solverOptions, int batch) {
+  const std::string root = dir + (dir.empty() || dir.back() == '/' ? "" : "/") + "level" + std::to_string(level) + "_";
+  std::vector<std::vector<int>> assign, baseMat;
+  std::vector<float> params;
+  if (!cfgbin::readBinary(root + "params.bin", params) || !cfgbin::readBinary(root + "baseMat.bin", baseMat) ||
+      !cfgbin::readBinary(root + "matAssignments.bin", assign))
+    throw HostError(FEMH_BAD_ARG, "cannot read " + root + "{params,baseMat,matAssignments}.bin");
+  const NumericalCoarseningUsingMultigrid::StructureType type =
+      cubicOnly ? NumericalCoarseningUsingMultigrid::Cubic : NumericalCoarseningUsingMultigrid::General;
+  NumericalCoarseningUsingMultigrid nc;
+  if (solverOptions) nc.setSolverOptions(*solverOptions);
+  int n[3] = {level, level, level};
+  nc.init(n, nbBlockRep, nbSubdiv, mats, type);
+  std::vector<float> tensors, newParams;
+  nc.computeCoarsenedElasticityTensorAndParameters3D(assign, n, dens, type, tensors, newParams, batch);
+  if (!cfgbin::writeBinary(root + "params.bin", newParams) || !cfgbin::writeBinary(root + "baseMat.bin", baseMat) ||
+      !cfgbin::writeBinary(root + "matAssignments.bin", assign) ||
+      !cfgbin::writeBinary(root + "elasticityTensors.bin", tensors))
+    throw HostError(FEMH_BAD_ARG, "cannot write " + root + "*.bin");
+  return (int)assign.size();
 }
 
 // ---- MaterialOptimizer (MaterialOptimizer.cpp:530-592, 621-652) ---------------

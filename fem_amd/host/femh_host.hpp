@@ -175,6 +175,17 @@ class NumericalCoarseningUsingMultigrid {
   // the full tensor of the last call
   const Mat6 &lastTensor() const { return m_C; }
 
+  // All assignments at once, `batch` cells per launch sequence (one extra
+  // engine context of that batch size): per assignment, in order, the same
+  // 21 values and (Cubic) 4 parameters as the one-cell call, as float (the
+  // caller pipeline's file type).  General appends the density only (the
+  // reference's MG path has no General parameters: NumericalCoarseningUsing-
+  // MultiGrid.cpp:342-345).
+  void computeCoarsenedElasticityTensorAndParameters3D(const std::vector<std::vector<int>> &iMaterialAssignments,
+                                                       int N[3], const std::vector<double> &iBaseMaterialDensities,
+                                                       StructureType iType, std::vector<float> &oTensorValues,
+                                                       std::vector<float> &oParameters, int batch = 16);
+
  protected:
   femh_ctx *m_ctx = nullptr;
   femh_opts m_opts;
@@ -185,6 +196,9 @@ class NumericalCoarseningUsingMultigrid {
   bool m_referenceTriangle = false;
   double m_densityMin = 1e-4;
   Mat6 m_C;
+  femh_ctx *m_batchCtx = nullptr;
+  int m_batch = 0;
+  void moduli(const std::vector<int> &iMaterials, int N[3], double *oE) const;
 };
 
 // Same entry point as the reference's PARDISO-based coarsening; it runs the
@@ -204,6 +218,34 @@ double computeMaterialDensity(const std::vector<int> &iMaterialAssignment,
                               const std::vector<double> &iBaseMaterialDensities);
 // cubic (E, nu, G) from C (NumericalCoarseningUsingMultiGrid.cpp:327-340)
 void cubicParameters(const Mat6 &C, double &E, double &nu, double &G);
+
+// ---- caller pipeline files -----------------------------------------------------
+// The sample generator's binary layout (Utilities/include/cfgUtilities.h:
+// 803-870), host byte order: a vector is an int32 count then the raw
+// elements; a vector of vectors is an int32 count then each vector as above.
+// read* return false on a missing or truncated file.
+namespace cfgbin {
+bool readBinary(const std::string &file, std::vector<float> &out);
+bool readBinary(const std::string &file, std::vector<int> &out);
+bool readBinary(const std::string &file, std::vector<std::vector<int>> &out);
+bool readBinary(const std::string &file, std::vector<std::vector<float>> &out);
+bool writeBinary(const std::string &file, const std::vector<float> &v);
+bool writeBinary(const std::string &file, const std::vector<int> &v);
+bool writeBinary(const std::string &file, const std::vector<std::vector<int>> &v);
+bool writeBinary(const std::string &file, const std::vector<std::vector<float>> &v);
+}  // namespace cfgbin
+
+// SamplesGeneratorImpl::runTensorHomogenization (SamplesGeneratorImpl.cpp:
+// 4613-4631, readFiles :3854-3923 binary branch, writeFiles :3826-3838) as a
+// GPU batch job: reads dir/level{L}_{params,baseMat,matAssignments}.bin,
+// homogenizes every assignment of the L^3 grid (replicated nbBlockRep times
+// per axis, each cell split into nbSubdiv^3 voxels), and writes
+// dir/level{L}_{params,baseMat,matAssignments,elasticityTensors}.bin.
+// Returns the number of assignments; throws HostError on a missing input.
+int runTensorHomogenization(const std::string &dir, int level, int nbBlockRep, int nbSubdiv,
+                            const std::vector<BaseMaterial> &baseMaterials,
+                            const std::vector<double> &baseMaterialDensities, bool cubicOnly,
+                            const femh_opts *solverOptions = nullptr, int batch = 16);
 
 // ---- optimiser ---------------------------------------------------------------
 class MaterialOptimizer {

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "../../fem_amd/host/femh_host.hpp"
@@ -136,6 +137,49 @@ int main() {
       threw = e.status == FEMH_BAD_ARG;
     }
     expect(threw, "bad grid raises HostError(BAD_ARG)", 0.0);
+  }
+  {  // the tensor-homogenization stage as a batch job over the caller's files
+    const char *tmp = std::getenv("TMPDIR");
+    const std::string dir = std::string(tmp && *tmp ? tmp : "/tmp");
+    const int level = 4, nsamp = 6;
+    std::vector<std::vector<int>> assign(nsamp, std::vector<int>(level * level * level, 0));
+    unsigned seed = 12345u;
+    for (auto &a : assign)
+      for (auto &m : a) {
+        seed = seed * 1664525u + 1013904223u;
+        m = (seed >> 16) % 3 != 0 ? 1 : 0;
+      }
+    const std::string root = dir + "/level4_";
+    bool ok = cfgbin::writeBinary(root + "matAssignments.bin", assign) &&
+              cfgbin::writeBinary(root + "baseMat.bin", std::vector<std::vector<int>>{{0}, {1}}) &&
+              cfgbin::writeBinary(root + "params.bin", std::vector<float>(nsamp * 4, 0.0f));
+    femh_opts o;
+    femh_default_opts(&o);
+    o.tol = 1e-12;
+    const int n = ok ? runTensorHomogenization(dir, level, 1, 2, mats, {0.0, 1.0}, true, &o, 4) : 0;
+    std::vector<float> tens, par;
+    std::vector<std::vector<int>> back;
+    ok = ok && n == nsamp && cfgbin::readBinary(root + "elasticityTensors.bin", tens) &&
+         cfgbin::readBinary(root + "params.bin", par) && cfgbin::readBinary(root + "matAssignments.bin", back);
+    expect(ok && tens.size() == 21u * nsamp && par.size() == 4u * nsamp && back == assign,
+           "runTensorHomogenization: files read and written in the caller's layout", (double)tens.size());
+    // against one-cell calls of the same facade
+    NumericalCoarseningUsingMultigrid nc;
+    nc.setSolverOptions(o);
+    int N[3] = {level, level, level};
+    nc.init(N, 1, 2, mats, NumericalCoarseningUsingMultigrid::Cubic);
+    double err = 0, perr = 0;
+    for (int s = 0; s < nsamp && ok; s++) {
+      std::vector<double> tv, pv;
+      nc.computeCoarsenedElasticityTensorAndParameters3D(assign[s], N, {0.0, 1.0},
+                                                         NumericalCoarseningUsingMultigrid::Cubic, tv, pv);
+      for (int q = 0; q < 21; q++) err = std::fmax(err, std::fabs(tens[21 * s + q] - tv[q]) / std::fabs(tv[0]));
+      for (int q = 0; q < 4; q++) perr = std::fmax(perr, std::fabs(par[4 * s + q] - pv[q]) / (std::fabs(pv[q]) + 1e-30));
+    }
+    expect(ok && err < 1e-6, "runTensorHomogenization: batched tensors = one-cell tensors (float)", err);
+    expect(ok && perr < 1e-5, "runTensorHomogenization: cubic parameters = one-cell parameters", perr);
+    for (const char *f : {"matAssignments", "baseMat", "params", "elasticityTensors"})
+      std::remove((root + f + ".bin").c_str());
   }
   std::printf("%s\n", fails ? "FAILED" : "ALL PASSED");
   return fails ? 1 : 0;
