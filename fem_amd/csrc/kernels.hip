@@ -554,19 +554,6 @@ __global__ __launch_bounds__(kBlock) void k_scale_dinv_norm(int V, T *__restrict
   block_partial(red, S_AUX, rr, blockIdx.x, s);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_normalize(int V, const T *__restrict__ y,
-                                                      T *__restrict__ x, const int *__restrict__ act,
-                                                      Reduce red, int np) {
-  const int rr = blockIdx.y;
-  if (!act[rr]) return;
-  const double nrm = sqrt(sum_parts(red, S_AUX, rr, np));
-  const double inv = nrm > 0 ? 1.0 / nrm : 0.0;
-  const size_t off = (size_t)rr * 3 * V;
-  for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
-    x[off + t] = (T)((double)y[off + t] * inv);
-}
-
 // ------------------------------------------------------------------------
 // material, C_H, adjoint
 __device__ __forceinline__ double interp_E(double rho, int interp, double E0, double Emin, double p) {
@@ -896,12 +883,6 @@ void launch_scale_by_dinv_norm(int V, int R, T *y, const T *Dinv, const int *act
   hipLaunchKernelGGL(k_scale_dinv_norm<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, y, Dinv, act,
                      red);
 }
-template <typename T>
-void launch_normalize(int V, int R, const T *y, T *x, const int *act, const Reduce &red,
-                      hipStream_t s) {
-  hipLaunchKernelGGL(k_normalize<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, y, x, act, red,
-                     grid_for(3 * V));
-}
 void launch_zero_rhs(int V, int R, double *x, const int *act, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_rhs, grid2(3 * V, R), dim3(kBlock), 0, s, V, x, act);
 }
@@ -976,9 +957,7 @@ void reduce_init(Reduce &red, hipStream_t s) {
   template void launch_fill_hash<T>(size_t, size_t, T *, unsigned long long, hipStream_t);                \
   template void launch_invert<T>(size_t, const T *, T *, hipStream_t);                            \
   template void launch_scale_by_dinv_norm<T>(int, int, T *, const T *, const int *,               \
-                                             const Reduce &, hipStream_t);                        \
-  template void launch_normalize<T>(int, int, const T *, T *, const int *, const Reduce &,        \
-                                    hipStream_t);
+                                             const Reduce &, hipStream_t);
 
 FEMH_INST(double)
 FEMH_INST(float)

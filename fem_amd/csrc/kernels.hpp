@@ -153,13 +153,11 @@ void launch_remove_mean(int V, int R, T *x, const int *act, const Reduce &red, h
 // x[t] = hash(seed, t mod period) in [-1/2, 1/2)
 template <typename T>
 void launch_fill_hash(size_t n, size_t period, T *x, unsigned long long seed, hipStream_t s);
-// power iteration: y *= Dinv and ||y||^2 -> S_AUX[q] ; x = y / ||y||
+// y *= Dinv and ||y||^2 -> S_AUX[q] (the D^-1 of a power-iteration step on
+// the dense fine-apply path)
 template <typename T>
 void launch_scale_by_dinv_norm(int V, int R, T *y, const T *Dinv, const int *act,
                                const Reduce &red, hipStream_t s);
-template <typename T>
-void launch_normalize(int V, int R, const T *y, T *x, const int *act, const Reduce &red,
-                      hipStream_t s);
 
 // ---- material / C_H / adjoint -------------------------------------------
 // E = Emin + (E0-Emin) rho^p (interp 0) | E0 rho/(3-2rho) (1) | rho (2)
