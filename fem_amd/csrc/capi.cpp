@@ -102,6 +102,7 @@ void femh_default_opts(femh_opts *o) {
   o->apply_kernel = 0;
   o->batch = 1;
   o->warm_start = 0;
+  o->structure = FEMH_STRUCT_GENERAL;
   o->coarse_nu = 3;
 }
 

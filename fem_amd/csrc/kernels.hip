@@ -446,6 +446,23 @@ __global__ __launch_bounds__(kBlock) void k_p_update(int V, T *__restrict__ p,
     p[off + t] = first ? z[off + t] : z[off + t] + bt * p[off + t];
 }
 
+// u_dst(i,j,k) = S u_src(sigma(i,j,k)), sigma and S the same axis swap
+__global__ __launch_bounds__(kBlock) void k_permute_case(int n, double *__restrict__ x, int src, int dst,
+                                                         int swap) {
+  const int smp = blockIdx.y;
+  const size_t V = (size_t)n * n * n;
+  const double *us = x + ((size_t)smp * kRHS + src) * 3 * V;
+  double *ud = x + ((size_t)smp * kRHS + dst) * 3 * V;
+  for (size_t t = blockIdx.x * (size_t)kBlock + threadIdx.x; t < V; t += (size_t)gridDim.x * kBlock) {
+    const int k = (int)(t % n), j = (int)((t / n) % n), i = (int)(t / ((size_t)n * n));
+    const size_t m = swap == 0 ? ((size_t)j * n + i) * n + k : ((size_t)k * n + j) * n + i;
+    const int cx = swap == 0 ? 1 : 2, cy = swap == 0 ? 0 : 1, cz = swap == 0 ? 2 : 0;
+    ud[t] = us[(size_t)cx * V + m];
+    ud[V + t] = us[(size_t)cy * V + m];
+    ud[2 * V + t] = us[(size_t)cz * V + m];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_beta(int rz_new, int rz_old, int first,
                                                  const int *__restrict__ act, Reduce red, int np,
                                                  double *__restrict__ beta) {
@@ -837,6 +854,9 @@ void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int
                      const int *act, const Reduce &red, int np_rz, hipStream_t s) {
   hipLaunchKernelGGL(k_p_update<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, p, z, rz_new, rz_old,
                      first, act, red, np_rz);
+}
+void launch_permute_case(int n, int nb, double *x, int src, int dst, int swap, hipStream_t s) {
+  hipLaunchKernelGGL(k_permute_case, dim3(grid_for(n * n * n), nb), dim3(kBlock), 0, s, n, x, src, dst, swap);
 }
 void launch_beta(int R, int rz_new, int rz_old, int first, const int *act, const Reduce &red, int np_rz,
                  double *beta, hipStream_t s) {

@@ -135,6 +135,10 @@ void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int
 // direction update of the fine operator (ST_PUPD)
 void launch_beta(int R, int rz_new, int rz_old, int first, const int *act, const Reduce &red, int np_rz,
                  double *beta, hipStream_t s);
+// cubic structure: load case dst of every sample from load case src by an
+// axis swap of the cube grid (swap 0: x <-> y, 1: x <-> z), points and
+// displacement components alike
+void launch_permute_case(int n, int nb, double *x, int src, int dst, int swap, hipStream_t s);
 // dot products a.b -> slot (per RHS)
 template <typename T>
 void launch_dot(int V, int R, const T *a, const T *b, int slot, const int *act, const Reduce &red,

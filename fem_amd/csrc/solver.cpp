@@ -60,6 +60,10 @@ Context::Context(const femh_opts &opts) : o(opts) {
   if (!(o.cheby_ratio > 1.0)) throw Error(FEMH_BAD_ARG, "cheby_ratio must exceed 1");
   if (!(o.tol > 0) || o.max_iter < 1) throw Error(FEMH_BAD_ARG, "bad tolerance / iteration cap");
   if (o.apply_kernel < 0 || o.apply_kernel > 1) throw Error(FEMH_BAD_ARG, "unknown apply kernel");
+  if (o.structure != FEMH_STRUCT_GENERAL && o.structure != FEMH_STRUCT_CUBIC)
+    throw Error(FEMH_BAD_ARG, "unknown structure");
+  if (o.structure == FEMH_STRUCT_CUBIC && !(o.nx == o.ny && o.ny == o.nz))
+    throw Error(FEMH_BAD_ARG, "a cubic structure needs a cube grid");
   if (o.batch < 0 || o.batch > kMaxB)
     throw Error(FEMH_BAD_ARG, "batch must lie in [1," + std::to_string(kMaxB) + "]");
   if (o.check_every < 1) o.check_every = 1;
@@ -168,12 +172,17 @@ void Context::op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a) {
       pe.a = take_event();
       pe.b = take_event();
       pe.which = sizeof(T) == 8 ? 0 : 1;
-      int na = 0;
+      int na = 0, ns = 0;
       {
         int hact[kMaxR];
         FEMH_HIP(hipMemcpyAsync(hact, a.act, sizeof(int) * R, hipMemcpyDeviceToHost, stream));
         FEMH_HIP(hipStreamSynchronize(stream));
         for (int t = 0; t < R; t++) na += hact[t] != 0;
+        for (int s0 = 0; s0 < R; s0 += kRHS) {
+          bool any = false;
+          for (int t = s0; t < s0 + kRHS && t < R; t++) any = any || hact[t] != 0;
+          ns += any;
+        }
       }
       const double Rn = na, V = (double)g0.V, sz = sizeof(T);
       // algorithmic bytes: staged vector read + outputs written + extra
@@ -195,7 +204,7 @@ void Context::op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a) {
         if (a.pout) vec_out += 1;    // new direction
       }
       const bool usesD = st == ST_DSCALE || ep == EP_CHEB0 || ep == EP_STEP;
-      const double samples = std::ceil(Rn / kRHS);
+      const double samples = ns;  // samples with an active RHS read E (and D) once
       pe.bytes = V * (3.0 * sz * Rn * (vec_in + vec_out) + samples * sz * (usesD ? 4.0 : 1.0));
       pe.flops = V * Rn * 1152.0;
       FEMH_HIP(hipEventRecord(pe.a, stream));
@@ -874,6 +883,9 @@ void Context::homogenize(double *C, femh_stats *st) {
     const double sumE = mst[3 * (t / kRHS)];
     const double zero_thresh = 1e-13 * (sumE / g.V) * std::sqrt((double)g.V);
     active[t] = std::sqrt(bb[t]) > zero_thresh;
+    // cubic structure: only the xx and yz load cases are solved; the others
+    // keep u = 0 (their C_H entries follow from symmetry below)
+    if (o.structure == FEMH_STRUCT_CUBIC && t % kRHS != 0 && t % kRHS != 3) active[t] = 0;
   }
   FEMH_HIP(hipEventRecord(ev[1], stream));
 
@@ -970,7 +982,11 @@ void Context::homogenize(double *C, femh_stats *st) {
           bool any = false;
           for (int k = 0; k < kRHS; k++) {
             const int t = kRHS * s + k;
-            trC += (aii[k] * mst[3 * s] - bx[t] - xr[t]) / g.V;
+            // (cubic: trace C = 3 (C_xx,xx + C_yz,yz) from the two solved cases)
+            if (o.structure == FEMH_STRUCT_CUBIC)
+              trC += (k == 0 || k == 3) ? 3.0 * (aii[k] * mst[3 * s] - bx[t] - xr[t]) / g.V : 0.0;
+            else
+              trC += (aii[k] * mst[3 * s] - bx[t] - xr[t]) / g.V;
             if (active[t]) {
               srz += std::max(rz[t], 0.0);
               any = true;
@@ -1064,7 +1080,32 @@ void Context::homogenize(double *C, femh_stats *st) {
   // them to zero); the mean is taken out when the fields are read out
   fields_centered = false;
   FEMH_HIP(hipEventRecord(ev[2], stream));
+  if (o.structure == FEMH_STRUCT_CUBIC) {
+    // the cell is invariant under axis swaps, so the unsolved load cases are
+    // the solved ones with axes swapped (the reference's swapDispXY,
+    // NumericalCoarseningUsingMultiGrid.cpp:44-70): yy and xz from xx and yz
+    // by x <-> y, zz and xy by x <-> z.  C_H then comes from the energy form
+    // of all six fields (second order in the solve error) and is projected
+    // onto the cubic form.
+    const int n = g.nx;
+    launch_permute_case(n, nb, x.as<double>(), 0, 1, 0, stream);
+    launch_permute_case(n, nb, x.as<double>(), 0, 2, 1, stream);
+    launch_permute_case(n, nb, x.as<double>(), 3, 4, 0, stream);
+    launch_permute_case(n, nb, x.as<double>(), 3, 5, 1, stream);
+    FEMH_HIP(hipGetLastError());
+  }
   compute_ch(x.as<double>(), C);
+  if (o.structure == FEMH_STRUCT_CUBIC) {
+    for (int smp = 0; smp < nb; smp++) {
+      double *Cs = C + 36 * smp;
+      const double c11 = (Cs[0] + Cs[7] + Cs[14]) / 3.0, c12 = (Cs[1] + Cs[2] + Cs[8]) / 3.0,
+                   c44 = (Cs[21] + Cs[28] + Cs[35]) / 3.0;
+      for (int q = 0; q < 36; q++) Cs[q] = 0.0;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Cs[6 * i + j] = i == j ? c11 : c12;
+      for (int i = 3; i < 6; i++) Cs[6 * i + i] = c44;
+    }
+  }
   flush_profile();
   have_fields = true;
   float ms1 = 0, ms2 = 0;

@@ -10,6 +10,8 @@ Workloads (sizes and laws as stated for each):
                    random field (white noise filtered by a Gaussian kernel of
                    standard deviation `corr` voxels, normalised to mean 0 and
                    variance 1); E = E0 * rho**p
+  cubic_average    symmetrisation over the cubic group (48 axis permutations
+                   and reflections), for the cubic-structure mode
   batch_uneven     cubic grids with edges drawn from {16,24,32,40,48}, fill
                    fraction uniform in [0.2, 0.8], binary voxels
 """
@@ -67,6 +69,22 @@ def grf(n, seed, corr, stream=0):
 def simp_density(n, seed, corr=4.0, rho_min=1e-3, stream=0):
     g = grf(n, seed, corr, stream)
     return np.clip(0.5 + 0.5 * g, rho_min, 1.0)
+
+
+def cubic_average(a):
+    """Mean of `a` over the 48 axis permutations and reflections of its last
+    three (equal) axes: a cell invariant under the cubic group, the input the
+    engine's cubic-structure mode expects."""
+    import itertools
+    a = np.asarray(a, dtype=np.float64)
+    lead = tuple(range(a.ndim - 3))
+    acc = np.zeros(a.shape)
+    for perm in itertools.permutations(range(3)):
+        b = np.transpose(a, lead + tuple(len(lead) + q for q in perm))
+        for flips in itertools.product((False, True), repeat=3):
+            ax = tuple(len(lead) + q for q, f in enumerate(flips) if f)
+            acc += np.flip(b, axis=ax) if ax else b
+    return acc / 48.0
 
 
 def simp_modulus(rho, p=3.0, E0=1.0, Emin=0.0):

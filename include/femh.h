@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define FEMH_ABI_VERSION 1
+#define FEMH_ABI_VERSION 2
 
 enum femh_status {
   FEMH_OK = 0,
@@ -68,6 +68,9 @@ enum femh_precision { FEMH_FP64 = 0, FEMH_MIXED = 1 };
  *   FEMH_INTERP_FASTFEM  E = E0 * rho / (3 - 2 rho)  (fast_fem/FEM3DFun.cpp:20-27)
  *   FEMH_INTERP_MODULUS  rho already holds E per voxel (binary grids etc.) */
 enum femh_interp { FEMH_INTERP_SIMP = 0, FEMH_INTERP_FASTFEM = 1, FEMH_INTERP_MODULUS = 2 };
+
+/* load cases solved (femh_opts.structure) */
+enum femh_structure { FEMH_STRUCT_GENERAL = 0, FEMH_STRUCT_CUBIC = 1 };
 
 typedef struct femh_opts {
   int nx, ny, nz;       /* grid (voxels) */
@@ -96,6 +99,12 @@ typedef struct femh_opts {
   int warm_start;       /* 1: start each sample slot from the fields of the previous
                            call with the same sample count (an optimisation loop's
                            previous iterate); 0: start from zero */
+  int structure;        /* femh_structure: FEMH_STRUCT_CUBIC solves only the xx and yz
+                           load cases of a cube cell that the caller knows to be
+                           cubic (invariant under axis swaps), forms the other four
+                           fields by swapping axes, and returns C_H projected onto
+                           the cubic form, as the reference's CUBIC mode
+                           (NumericalCoarseningUsingMultiGrid.cpp:275-282) */
 } femh_opts;
 
 typedef struct femh_stats {

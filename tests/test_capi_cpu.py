@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_default_opts():
     L = fem_amd.lib()
-    assert L.femh_abi_version() == 1
+    assert L.femh_abi_version() == 2
     o = femh.make_opts((8, 8, 8))
     assert o.nu == 0.3 and o.tol == 1e-10 and o.precond == 1 and o.nu1 == 2
 

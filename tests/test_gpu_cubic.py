@@ -1,0 +1,60 @@
+"""GPU tests of the cubic-structure fast path (femh_opts.structure): on cells
+invariant under the 48 axis permutations and reflections, solving the xx and
+yz load cases gives the full C_H of the six-case solve."""
+import numpy as np
+import pytest
+
+from fem_amd import FemhError, Homogenizer
+from fem_amd import workloads as W
+from fem_amd.femh import BAD_ARG
+
+pytestmark = pytest.mark.gpu
+NU = 0.3
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def cells(n, seeds):
+    out = []
+    for s in seeds:
+        rho = W.cubic_average(W.simp_density(n, s, 3.0))
+        out.append(rho ** 3)  # SIMP moduli
+        b = W.cubic_average(W.binary_cube(n, s + 100).astype(float)) > 0.5
+        out.append(np.where(b, 1.0, 1e-4))  # binary with a soft phase
+    return out
+
+
+def test_cubic_fp64_equals_six_case_solve():
+    """C_H and the adjoint from two solves plus axis swaps equal those of the
+    six-case solve."""
+    hc = Homogenizer((16, 16, 16), nu=NU, tol=1e-12, structure="cubic")
+    hg = Homogenizer((16, 16, 16), nu=NU, tol=1e-12)
+    G = np.random.default_rng(4).standard_normal((6, 6))
+    for E in cells(16, [1, 2]):
+        Cc, st = hc.homogenize_modulus(E)
+        Cg, _ = hg.homogenize_modulus(E)
+        assert rel(Cc, Cg) <= 1e-9, rel(Cc, Cg)
+        assert [st["iters"][k] for k in (1, 2, 4, 5)] == [0, 0, 0, 0]
+        assert st["iters"][0] > 0 and st["iters"][3] > 0
+        gc, gg = hc.gradient(G), hg.gradient(G)
+        assert np.abs(gc - gg).max() <= 1e-8 * np.abs(gg).max()
+
+
+def test_cubic_needs_a_cube_grid():
+    with pytest.raises(FemhError) as e:
+        Homogenizer((16, 16, 8), nu=NU, structure="cubic")
+    assert e.value.status == BAD_ARG
+
+
+def test_cubic_mixed_batch_within_1e6():
+    Es = np.stack(cells(32, [5, 6]))
+    hm = Homogenizer((32, 32, 32), nu=NU, precision="mixed", ch_tol=1e-7, inner_tol=3e-4, batch=4,
+                     structure="cubic")
+    Cm, stm = hm.homogenize_many(Es, modulus=True)
+    hg = Homogenizer((32, 32, 32), nu=NU, tol=1e-12, batch=4)
+    Cg, _ = hg.homogenize_many(Es, modulus=True)
+    for s in range(len(Es)):
+        assert rel(Cm[s], Cg[s]) <= 1e-6, (s, rel(Cm[s], Cg[s]))
+        assert stm[s]["iters"][1] == 0 and stm[s]["iters"][5] == 0

@@ -305,6 +305,22 @@ def test_oracle_ch_scaling_symmetry_rotation_and_bounds():
     assert np.linalg.eigvalsh(C - Cr).min() > -1e-12
 
 
+def test_cubic_cell_has_cubic_tensor():
+    """A cell invariant under the cubic group has C_H of cubic form: the
+    property the engine's cubic-structure mode (two solves, axis swaps and a
+    projection onto c11, c12, c44) relies on."""
+    E = W.cubic_average(W.binary_modulus(W.binary_cube(6, 5), 1.0, 1e-2))
+    assert np.abs(E - np.swapaxes(E, 0, 1)).max() < 1e-14 and np.abs(E - E[::-1]).max() < 1e-14
+    C = O.homogenize(E, NU, precond="jacobi", tol=1e-13, max_iter=20000)["C"]
+    c11, c12, c44 = C[0, 0], C[0, 1], C[3, 3]
+    Cc = np.zeros((6, 6))
+    Cc[:3, :3] = c12
+    Cc[range(3), range(3)] = c11
+    Cc[range(3, 6), range(3, 6)] = c44
+    assert np.abs(C - Cc).max() < 1e-10 * c11
+    assert abs(c11 - c12 - 2 * c44) > 1e-3 * c11  # cubic, not isotropic
+
+
 def test_oracle_jacobi_and_multigrid_agree():
     E = W.simp_modulus(W.simp_density(16, 3, 4.0))
     a = O.homogenize(E, NU, precond="jacobi", tol=1e-12, max_iter=20000)
