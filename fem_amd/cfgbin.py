@@ -10,8 +10,9 @@ vectors), level{L}_params.bin (float) and level{L}_elasticityTensors.bin
 (SamplesGenerator/SamplesGeneratorImpl.cpp:3826-3838, :3854-3923).
 
 run_tensor_homogenization() is SamplesGeneratorImpl::runTensorHomogenization
-(:4613-4631) as a batched GPU job over the engine; the C++ facade has the same
-job (femh::runTensorHomogenization).
+(:4613-4631) and run_thermal_analysis() is runThermalAnalysis (:4654-4680), each
+as a batched GPU job over the engine; the C++ facade has the same jobs
+(femh::runTensorHomogenization, femh::runThermalAnalysis).
 """
 import os
 import struct
@@ -93,8 +94,10 @@ def run_tensor_homogenization(directory, level, E_solid, nu_solid, base_densitie
     homogenize every assignment (material 1 solid at E_solid, other materials
     at density_min * E_solid, as NumericalCoarseningUsingMultiGrid.cpp:186-193,
     :239 does), and write the four files back.  Parameters per sample: cubic:
-    (density, E, nu, G); otherwise the density.  Returns (tensors[n, 21],
-    params[n, k])."""
+    (density, E, nu, G); otherwise the density.  cubic also selects the
+    engine's cubic-structure mode (two load cases for the cells invariant under
+    the axis swaps, all six for the others), as the reference's Cubic type
+    selects its CUBIC solve (:211).  Returns (tensors[n, 21], params[n, k])."""
     from .femh import Homogenizer
     root = os.path.join(directory, "level%d_" % level)
     read_vector(root + "params.bin", np.float32)  # present, as the reference requires
@@ -102,6 +105,8 @@ def run_tensor_homogenization(directory, level, E_solid, nu_solid, base_densitie
     assign = read_vectors(root + "matAssignments.bin", np.int32)
     n = level * rep * subdiv
     dens = np.asarray(base_densities, dtype=np.float64)
+    if cubic:
+        solver.setdefault("structure", "cubic")
     h = Homogenizer((n, n, n), nu=nu_solid, E0=E_solid, interp="modulus", batch=batch, **solver)
     tril = np.tril_indices(6)
     tensors, params = [], []
@@ -126,3 +131,34 @@ def run_tensor_homogenization(directory, level, E_solid, nu_solid, base_densitie
     write_vectors(root + "matAssignments.bin", assign, np.int32)
     write_vector(root + "elasticityTensors.bin", tensors.ravel(), np.float32)
     return tensors, params
+
+
+def run_thermal_analysis(directory, level, E_solid, nu_solid, rep=1, subdiv=1, density_min=1e-4, batch=16,
+                         **solver):
+    """SamplesGeneratorImpl::runThermalAnalysis (SamplesGeneratorImpl.cpp:4654-4680)
+    as a batched GPU job: read level{L}_matAssignments.bin from `directory`,
+    compute each cell's effective thermal expansion (material 1 expanding by
+    1 per unit temperature rise, other materials at density_min * E_solid not
+    expanding, as ThermalAnalysis.cpp:404-420 loads only material 1), and write
+    level{L}_thermalExpansionCoefficients.bin with the mean normal expansion
+    (a_xx + a_yy + a_zz) / 3 per cell.  Returns (coefficients[n],
+    expansions[n, 6])."""
+    from .femh import Homogenizer
+    root = os.path.join(directory, "level%d_" % level)
+    assign = read_vectors(root + "matAssignments.bin", np.int32)
+    n = level * rep * subdiv
+    h = Homogenizer((n, n, n), nu=nu_solid, E0=E_solid, interp="modulus", batch=batch, **solver)
+    out = []
+    try:
+        for c0 in range(0, len(assign), batch):
+            chunk = assign[c0:c0 + batch]
+            solid = np.stack([replicate_assignment(a, level, rep, subdiv) == 1 for a in chunk])
+            h.homogenize_many(E_solid * np.where(solid, 1.0, density_min), modulus=True)
+            _, a = h.thermal_expansion(solid.astype(np.float64), nb=len(chunk))
+            out.extend(np.asarray(a).reshape(len(chunk), 6))
+    finally:
+        h.close()
+    alpha = np.asarray(out, dtype=np.float64).reshape(len(assign), 6)
+    coef = alpha[:, :3].mean(axis=1).astype(np.float32)
+    write_vector(root + "thermalExpansionCoefficients.bin", coef, np.float32)
+    return coef, alpha

@@ -184,6 +184,12 @@ int femh_energy_terms(femh_ctx *ctx, double *W) {
   });
 }
 
+int femh_thermal_expansion(femh_ctx *ctx, const double *alpha, double *beta, double *alpha_eff) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!alpha) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] { ctx->c->thermal(alpha, beta, alpha_eff); });
+}
+
 int femh_get_fields(femh_ctx *ctx, double *u) {
   if (int s = need_ctx(ctx)) return s;
   if (!u) return FEMH_BAD_ARG;

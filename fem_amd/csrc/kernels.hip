@@ -446,10 +446,12 @@ __global__ __launch_bounds__(kBlock) void k_p_update(int V, T *__restrict__ p,
     p[off + t] = first ? z[off + t] : z[off + t] + bt * p[off + t];
 }
 
-// u_dst(i,j,k) = S u_src(sigma(i,j,k)), sigma and S the same axis swap
+// u_dst(i,j,k) = S u_src(sigma(i,j,k)), sigma and S the same axis swap, for
+// the samples whose bit is set in mask
 __global__ __launch_bounds__(kBlock) void k_permute_case(int n, double *__restrict__ x, int src, int dst,
-                                                         int swap) {
+                                                         int swap, unsigned mask) {
   const int smp = blockIdx.y;
+  if (!((mask >> smp) & 1u)) return;
   const size_t V = (size_t)n * n * n;
   const double *us = x + ((size_t)smp * kRHS + src) * 3 * V;
   double *ud = x + ((size_t)smp * kRHS + dst) * 3 * V;
@@ -590,17 +592,26 @@ __device__ __forceinline__ double interp_dE(double rho, int interp, double E0, d
 }
 
 // partials into S_AUX: idx 3s sum E, 3s+1 non-finite count, 3s+2 negative count
+// cube_n > 0 (a cube grid of that edge): also counts the voxels whose modulus
+// differs from that of their image under the x <-> y or the x <-> z swap by
+// more than 1e-12 relative, into S_AUX[3 B + smp]
 __global__ __launch_bounds__(kBlock) void k_modulus_stats(int V, const double *__restrict__ E,
-                                                          Reduce red) {
+                                                          int cube_n, Reduce red) {
   const int smp = blockIdx.y;
   const double *Es = E + (size_t)smp * V;
-  double s = 0, nf = 0, ng = 0;
+  double s = 0, nf = 0, ng = 0, na = 0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < V; t += gridDim.x * kBlock) {
     const double e = Es[t];
     if (!isfinite(e)) nf += 1.0;
     else {
       s += e;
       if (e < 0) ng += 1.0;
+    }
+    if (cube_n > 0) {
+      const int n = cube_n, k = t % n, j = (t / n) % n, i = t / (n * n);
+      const double exy = Es[(j * n + i) * n + k], exz = Es[(k * n + j) * n + i];
+      const double tol = 1e-12 * fmax(fabs(e), fmax(fabs(exy), fabs(exz)));
+      if (!(fabs(e - exy) <= tol && fabs(e - exz) <= tol)) na += 1.0;
     }
   }
   double b = block_sum(s);
@@ -609,6 +620,10 @@ __global__ __launch_bounds__(kBlock) void k_modulus_stats(int V, const double *_
   block_partial(red, S_AUX, 3 * smp + 1, blockIdx.x, b);
   b = block_sum(ng);
   block_partial(red, S_AUX, 3 * smp + 2, blockIdx.x, b);
+  if (cube_n > 0) {
+    b = block_sum(na);
+    block_partial(red, S_AUX, 3 * gridDim.y + smp, blockIdx.x, b);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_material(size_t n, const double *__restrict__ rho,
@@ -719,6 +734,45 @@ __global__ __launch_bounds__(kBlock, 3) void k_ch_energy(Dims g, const double *_
   for (int p = 0; p < 21; p++) {
     double s = block_sum(acc[p][tid]);
     if (tid == 0) partials[((size_t)smp * 21 + p) * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// Thermal stress sums: S_q = sum_e alpha_e E_e (element mean of div chi_q),
+// chi_q the total field of load case q.  In the Hadamard modes of the corner
+// values the x-, y- and z-difference modes (4, 2, 1) sum the corner values
+// signed by their x, y and z side, and the element mean of the divergence is
+// (w_x[4] + w_y[2] + w_z[1]) / 4.  Per block partials [B][6][nblk].
+__global__ __launch_bounds__(kBlock) void k_thermal(Dims g, const double *__restrict__ E,
+                                                    const double *__restrict__ alpha,
+                                                    const double *__restrict__ uaff,
+                                                    const double *__restrict__ u,
+                                                    double *__restrict__ partials) {
+  const int smp = blockIdx.y;
+  const double *Es = E + (size_t)smp * g.V;
+  const double *as = alpha + (size_t)smp * g.V;
+  const double *us = u + (size_t)smp * kRHS * 3 * g.V;
+  double acc[kRHS] = {0, 0, 0, 0, 0, 0};
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < g.V; e += gridDim.x * kBlock) {
+    const double we = Es[e] * as[e];
+    if (we == 0.0) continue;
+    int i, j, k;
+    node_ijk(g, e, i, j, k);
+    const int I[2] = {i, i == g.nx - 1 ? 0 : i + 1};
+    const int J[2] = {j, j == g.ny - 1 ? 0 : j + 1};
+    const int K[2] = {k, k == g.nz - 1 ? 0 : k + 1};
+    int n[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) n[v] = (I[(v >> 2) & 1] * g.ny + J[(v >> 1) & 1]) * g.nz + K[v & 1];
+#pragma unroll
+    for (int lc = 0; lc < kRHS; lc++) {
+      double w[3][8];
+      chi_modes(g, n, us, uaff, lc, w);
+      acc[lc] += we * 0.25 * (w[0][4] + w[1][2] + w[2][1]);
+    }
+  }
+  for (int q = 0; q < kRHS; q++) {
+    const double s = block_sum(acc[q]);
+    if (threadIdx.x == 0) partials[((size_t)smp * kRHS + q) * gridDim.x + blockIdx.x] = s;
   }
 }
 
@@ -855,8 +909,9 @@ void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int
   hipLaunchKernelGGL(k_p_update<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, p, z, rz_new, rz_old,
                      first, act, red, np_rz);
 }
-void launch_permute_case(int n, int nb, double *x, int src, int dst, int swap, hipStream_t s) {
-  hipLaunchKernelGGL(k_permute_case, dim3(grid_for(n * n * n), nb), dim3(kBlock), 0, s, n, x, src, dst, swap);
+void launch_permute_case(int n, int nb, double *x, int src, int dst, int swap, unsigned mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_permute_case, dim3(grid_for(n * n * n), nb), dim3(kBlock), 0, s, n, x, src, dst, swap,
+                     mask);
 }
 void launch_beta(int R, int rz_new, int rz_old, int first, const int *act, const Reduce &red, int np_rz,
                  double *beta, hipStream_t s) {
@@ -914,8 +969,8 @@ void launch_material(size_t n, const double *rho, double *E, int interp, double 
   hipLaunchKernelGGL(k_material, dim3((unsigned)b), dim3(kBlock), 0, s, n, rho, E, interp, E0, Emin,
                      p);
 }
-int launch_modulus_stats(int V, int B, const double *E, const Reduce &red, hipStream_t s) {
-  hipLaunchKernelGGL(k_modulus_stats, grid2(V, B), dim3(kBlock), 0, s, V, E, red);
+int launch_modulus_stats(int V, int B, const double *E, const Reduce &red, hipStream_t s, int cube_n) {
+  hipLaunchKernelGGL(k_modulus_stats, grid2(V, B), dim3(kBlock), 0, s, V, E, cube_n, red);
   return grid_for(V);
 }
 static ModeT<double> mode_table(const ModeCoef &m) {
@@ -932,6 +987,11 @@ void launch_ch_energy(Dims g, int B, const double *E, const ModeCoef &m, const d
   hipLaunchKernelGGL(k_ch_energy, dim3(nblk, B), dim3(kBlock), 0, s, g, E, mode_table(m), uaff, u,
                      partials);
   hipLaunchKernelGGL(k_sum_partials, dim3(21 * B), dim3(kBlock), 0, s, partials, nblk, out);
+}
+void launch_thermal(Dims g, int B, const double *E, const double *alpha, const double *uaff, const double *u,
+                    double *partials, int nblk, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_thermal, dim3(nblk, B), dim3(kBlock), 0, s, g, E, alpha, uaff, u, partials);
+  hipLaunchKernelGGL(k_sum_partials, dim3(kRHS * B), dim3(kBlock), 0, s, partials, nblk, out);
 }
 void launch_gradient(Dims g, int B, const double *rho, const ModeCoef &m, const double *uaff,
                      const double *u, const double *Gs, int interp, double E0, double Emin,

@@ -135,10 +135,10 @@ void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int
 // direction update of the fine operator (ST_PUPD)
 void launch_beta(int R, int rz_new, int rz_old, int first, const int *act, const Reduce &red, int np_rz,
                  double *beta, hipStream_t s);
-// cubic structure: load case dst of every sample from load case src by an
-// axis swap of the cube grid (swap 0: x <-> y, 1: x <-> z), points and
-// displacement components alike
-void launch_permute_case(int n, int nb, double *x, int src, int dst, int swap, hipStream_t s);
+// cubic structure: load case dst from load case src by an axis swap of the
+// cube grid (swap 0: x <-> y, 1: x <-> z), points and displacement components
+// alike, for the samples s with bit s of mask set
+void launch_permute_case(int n, int nb, double *x, int src, int dst, int swap, unsigned mask, hipStream_t s);
 // dot products a.b -> slot (per RHS)
 template <typename T>
 void launch_dot(int V, int R, const T *a, const T *b, int slot, const int *act, const Reduce &red,
@@ -167,12 +167,18 @@ void launch_scale_by_dinv_norm(int V, int R, T *y, const T *Dinv, const int *act
 // E = Emin + (E0-Emin) rho^p (interp 0) | E0 rho/(3-2rho) (1) | rho (2)
 void launch_material(size_t n, const double *rho, double *E, int interp, double E0, double Emin,
                      double p, hipStream_t s);
-// per sample: sum / non-finite count / negative count of E into S_AUX[3s..3s+2]
-int launch_modulus_stats(int V, int B, const double *E, const Reduce &red, hipStream_t s);
+// per sample: sum / non-finite count / negative count of E into S_AUX[3s..3s+2];
+// cube_n > 0 (cube grid of that edge): the count of voxels not invariant under
+// the x <-> y and x <-> z swaps into S_AUX[3B + s]
+int launch_modulus_stats(int V, int B, const double *E, const Reduce &red, hipStream_t s, int cube_n = 0);
 // Energy-form C_H per sample: 21 upper-triangle sums (not yet / V) into out[s][21]
 void launch_ch_energy(Dims g, int B, const double *E, const ModeCoef &m, const double *uaff,
                       const double *u, double *partials /*[B][21][nblk]*/, int nblk, double *out,
                       hipStream_t s);
+// thermal stress sums per sample: out[s][q] = sum_e alpha_e E_e mean div(chi_q)
+// over element e (not yet scaled by 1/((1 - 2 nu) V)); partials [B][6][nblk]
+void launch_thermal(Dims g, int B, const double *E, const double *alpha, const double *uaff, const double *u,
+                    double *partials, int nblk, double *out, hipStream_t s);
 // dJ/drho_e = dE/drho(rho_e) * sum_p Gs[s][p] W_p(e) / V ; energy_only: W (21/voxel)
 void launch_gradient(Dims g, int B, const double *rho, const ModeCoef &m, const double *uaff,
                      const double *u, const double *Gs /*device [B][21]*/, int interp, double E0,

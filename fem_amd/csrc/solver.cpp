@@ -854,10 +854,17 @@ void Context::homogenize(double *C, femh_stats *st) {
   FEMH_HIP(hipEventRecord(ev[0], stream));
   have_fields = false;
   // validate the moduli before any device work depends on them
-  std::vector<double> mst(3 * nb);
+  // cubic structure: a cell takes the two-solve path only if its moduli are
+  // invariant under the axis swaps (counted on the device); other cells of
+  // the batch are solved with all six load cases
+  const bool cubic = o.structure == FEMH_STRUCT_CUBIC;
+  std::vector<double> mst(4 * nb);
+  std::vector<char> cubic_ok(nb, 0);
   {
-    const int np = launch_modulus_stats(g.V, nb, E64.as<double>(), red, stream);
-    read_scalars(S_AUX, mst.data(), 3 * nb, np);
+    const int np = launch_modulus_stats(g.V, nb, E64.as<double>(), red, stream, cubic ? g.nx : 0);
+    read_scalars(S_AUX, mst.data(), (cubic ? 4 : 3) * nb, np);
+    if (cubic)
+      for (int s = 0; s < nb; s++) cubic_ok[s] = mst[3 * nb + s] == 0.0;
   }
   for (int s = 0; s < nb; s++) {
     if (mst[3 * s + 1] > 0) throw Error(FEMH_NONFINITE, "non-finite modulus");
@@ -884,8 +891,8 @@ void Context::homogenize(double *C, femh_stats *st) {
     const double zero_thresh = 1e-13 * (sumE / g.V) * std::sqrt((double)g.V);
     active[t] = std::sqrt(bb[t]) > zero_thresh;
     // cubic structure: only the xx and yz load cases are solved; the others
-    // keep u = 0 (their C_H entries follow from symmetry below)
-    if (o.structure == FEMH_STRUCT_CUBIC && t % kRHS != 0 && t % kRHS != 3) active[t] = 0;
+    // are formed from them after the solve
+    if (cubic_ok[t / kRHS] && t % kRHS != 0 && t % kRHS != 3) active[t] = 0;
   }
   FEMH_HIP(hipEventRecord(ev[1], stream));
 
@@ -983,7 +990,7 @@ void Context::homogenize(double *C, femh_stats *st) {
           for (int k = 0; k < kRHS; k++) {
             const int t = kRHS * s + k;
             // (cubic: trace C = 3 (C_xx,xx + C_yz,yz) from the two solved cases)
-            if (o.structure == FEMH_STRUCT_CUBIC)
+            if (cubic_ok[s])
               trC += (k == 0 || k == 3) ? 3.0 * (aii[k] * mst[3 * s] - bx[t] - xr[t]) / g.V : 0.0;
             else
               trC += (aii[k] * mst[3 * s] - bx[t] - xr[t]) / g.V;
@@ -1080,32 +1087,36 @@ void Context::homogenize(double *C, femh_stats *st) {
   // them to zero); the mean is taken out when the fields are read out
   fields_centered = false;
   FEMH_HIP(hipEventRecord(ev[2], stream));
-  if (o.structure == FEMH_STRUCT_CUBIC) {
+  unsigned cmask = 0;
+  for (int smp = 0; smp < nb; smp++) cmask |= cubic_ok[smp] ? 1u << smp : 0u;
+  if (cmask) {
     // the cell is invariant under axis swaps, so the unsolved load cases are
     // the solved ones with axes swapped (the reference's swapDispXY,
     // NumericalCoarseningUsingMultiGrid.cpp:44-70): yy and xz from xx and yz
     // by x <-> y, zz and xy by x <-> z.  C_H then comes from the energy form
-    // of all six fields (second order in the solve error) and is projected
-    // onto the cubic form.
+    // of all six fields (second order in the solve error).
     const int n = g.nx;
-    launch_permute_case(n, nb, x.as<double>(), 0, 1, 0, stream);
-    launch_permute_case(n, nb, x.as<double>(), 0, 2, 1, stream);
-    launch_permute_case(n, nb, x.as<double>(), 3, 4, 0, stream);
-    launch_permute_case(n, nb, x.as<double>(), 3, 5, 1, stream);
+    launch_permute_case(n, nb, x.as<double>(), 0, 1, 0, cmask, stream);
+    launch_permute_case(n, nb, x.as<double>(), 0, 2, 1, cmask, stream);
+    launch_permute_case(n, nb, x.as<double>(), 3, 4, 0, cmask, stream);
+    launch_permute_case(n, nb, x.as<double>(), 3, 5, 1, cmask, stream);
     FEMH_HIP(hipGetLastError());
   }
   compute_ch(x.as<double>(), C);
-  if (o.structure == FEMH_STRUCT_CUBIC) {
-    for (int smp = 0; smp < nb; smp++) {
-      double *Cs = C + 36 * smp;
-      const double c11 = (Cs[0] + Cs[7] + Cs[14]) / 3.0, c12 = (Cs[1] + Cs[2] + Cs[8]) / 3.0,
-                   c44 = (Cs[21] + Cs[28] + Cs[35]) / 3.0;
-      for (int q = 0; q < 36; q++) Cs[q] = 0.0;
-      for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) Cs[6 * i + j] = i == j ? c11 : c12;
-      for (int i = 3; i < 6; i++) Cs[6 * i + i] = c44;
-    }
+  // the exact C_H of such a cell is invariant under the axis permutations,
+  // which permute Voigt indices: average it over them, which removes the
+  // part of the solve error that breaks that symmetry
+  static const int vperm[6][6] = {{0, 1, 2, 3, 4, 5}, {1, 0, 2, 4, 3, 5}, {2, 1, 0, 5, 4, 3},
+                                  {0, 2, 1, 3, 5, 4}, {1, 2, 0, 4, 5, 3}, {2, 0, 1, 5, 3, 4}};
+  for (int smp = 0; smp < nb; smp++) {
+    if (!cubic_ok[smp]) continue;
+    double *Cs = C + 36 * smp, avg[36] = {0};
+    for (const auto &pm : vperm)
+      for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) avg[6 * pm[i] + pm[j]] += Cs[6 * i + j] / 6.0;
+    for (int q = 0; q < 36; q++) Cs[q] = avg[q];
   }
+  C_last.assign(C, C + 36 * (size_t)nb);
   flush_profile();
   have_fields = true;
   float ms1 = 0, ms2 = 0;
@@ -1158,6 +1169,56 @@ void Context::gradient(const double *dJdC, double *out_host, bool energy_only) {
   FEMH_HIP(hipGetLastError());
   FEMH_HIP(hipMemcpyAsync(out_host, out.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
   FEMH_HIP(hipStreamSynchronize(stream));
+}
+
+void Context::thermal(const double *alpha, double *beta, double *alpha_eff) {
+  if (!have_fields) throw Error(FEMH_NO_FIELDS, "no solved fields: call femh_homogenize first");
+  alpha_d.alloc(sizeof(double) * (size_t)nb * g.V);
+  FEMH_HIP(hipMemcpyAsync(alpha_d.p, alpha, sizeof(double) * (size_t)nb * g.V, hipMemcpyHostToDevice, stream));
+  const int nblk = 1024;  // ch_part holds 21 * 1024 partials per sample
+  launch_thermal(g, nb, E64.as<double>(), alpha_d.as<double>(), uaff.as<double>(), x.as<double>(),
+                 ch_part.as<double>(), nblk, ch_out.as<double>(), stream);
+  FEMH_HIP(hipGetLastError());
+  std::vector<double> sums(kRHS * (size_t)nb);
+  FEMH_HIP(hipMemcpyAsync(sums.data(), ch_out.p, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+  // I : C0 : eps = 3 K tr(eps) with 3 K = E / (1 - 2 nu) for the unit modulus
+  const double scale = 1.0 / ((1.0 - 2.0 * o.nu) * (double)g.V);
+  for (int smp = 0; smp < nb; smp++) {
+    double bq[kRHS];
+    for (int q = 0; q < kRHS; q++) {
+      bq[q] = sums[kRHS * smp + q] * scale;
+      if (!std::isfinite(bq[q])) throw Error(FEMH_NONFINITE, "non-finite thermal stress");
+    }
+    if (beta)
+      for (int q = 0; q < kRHS; q++) beta[kRHS * smp + q] = bq[q];
+    if (!alpha_eff) continue;
+    // C_H is symmetric positive definite: Cholesky, then two triangular solves
+    double L[6][6] = {{0}};
+    const double *C = C_last.data() + 36 * smp;
+    for (int i = 0; i < 6; i++)
+      for (int j = 0; j <= i; j++) {
+        double v = C[6 * i + j];
+        for (int k = 0; k < j; k++) v -= L[i][k] * L[j][k];
+        if (i == j) {
+          if (!(v > 0)) throw Error(FEMH_NONFINITE, "C_H is not positive definite");
+          L[i][i] = std::sqrt(v);
+        } else {
+          L[i][j] = v / L[j][j];
+        }
+      }
+    double y[6];
+    for (int i = 0; i < 6; i++) {
+      double v = bq[i];
+      for (int k = 0; k < i; k++) v -= L[i][k] * y[k];
+      y[i] = v / L[i][i];
+    }
+    for (int i = 5; i >= 0; i--) {
+      double v = y[i];
+      for (int k = i + 1; k < 6; k++) v -= L[k][i] * alpha_eff[kRHS * smp + k];
+      alpha_eff[kRHS * smp + i] = v / L[i][i];
+    }
+  }
 }
 
 void Context::get_fields(double *u) {

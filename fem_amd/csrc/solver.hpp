@@ -103,6 +103,8 @@ struct Context {
   DBuf red_part, red_ctr, red_scal;
   Reduce red{};
   DBuf ch_part, ch_out;
+  DBuf alpha_d;                  // per-voxel expansion coefficients (thermal)
+  std::vector<double> C_last;    // C_H of the last homogenize call, 36 per sample
   DBuf act;
   Hierarchy<double> h64;
   Hierarchy<float> h32;
@@ -148,6 +150,9 @@ struct Context {
   void homogenize(double *C, femh_stats *st);  // C[36 nb], st[nb] (may be null)
   void gradient(const double *dJdC, double *out_host, bool energy_only);
   void get_fields(double *u);
+  // effective thermal stress beta[6 nb] and expansion C_H^-1 beta (either
+  // may be null) for per-voxel expansion coefficients alpha[nb V] (host)
+  void thermal(const double *alpha, double *beta, double *alpha_eff);
   void apply_level(int level, const double *u, double *y);
 
  private:

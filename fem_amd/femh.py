@@ -26,7 +26,8 @@ PRECISION = {"fp64": 0, "mixed": 1}
 EXPORTS = ("femh_abi_version", "femh_default_opts", "femh_create", "femh_destroy",
            "femh_homogenize", "femh_homogenize_modulus", "femh_homogenize_device",
            "femh_homogenize_many", "femh_homogenize_many_modulus", "femh_homogenize_many_device",
-           "femh_gradient", "femh_energy_terms", "femh_get_fields", "femh_homogenize_batch",
+           "femh_gradient", "femh_energy_terms", "femh_get_fields", "femh_thermal_expansion",
+           "femh_homogenize_batch",
            "femh_prepare", "femh_apply_level", "femh_level_dims", "femh_set_profiling",
            "femh_kernel_stats", "femh_element_stiffness", "femh_last_error")
 
@@ -92,6 +93,7 @@ def lib():
         L.femh_gradient.argtypes = [vp, vp, vp]
         L.femh_energy_terms.argtypes = [vp, vp]
         L.femh_get_fields.argtypes = [vp, vp]
+        L.femh_thermal_expansion.argtypes = [vp, vp, vp, vp]
         L.femh_homogenize_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.femh_prepare.argtypes = [vp, vp]
         L.femh_apply_level.argtypes = [vp, ctypes.c_int, vp, vp]
@@ -235,6 +237,19 @@ class Homogenizer:
         u = np.zeros((nb * 6, 3, self.V))
         self._check(lib().femh_get_fields(self._h, _ptr(u)))
         return u if nb == 1 else u.reshape(nb, 6, 3, self.V)
+
+    def thermal_expansion(self, alpha, nb=1):
+        """Effective thermal stress beta and expansion C_H^-1 beta (Voigt,
+        engineering shear) of the last batch of nb cells, for the per-voxel
+        expansion coefficients alpha ([nb,] nx, ny, nz).  Returns (beta,
+        alpha_eff), each [6] (nb = 1) or [nb, 6]."""
+        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(nb * self.V)
+        beta = np.zeros(6 * nb)
+        aeff = np.zeros(6 * nb)
+        self._check(lib().femh_thermal_expansion(self._h, _ptr(a), _ptr(beta), _ptr(aeff)))
+        if nb == 1:
+            return beta, aeff
+        return beta.reshape(nb, 6), aeff.reshape(nb, 6)
 
     def set_profiling(self, on=True):
         self._check(lib().femh_set_profiling(self._h, int(bool(on))))

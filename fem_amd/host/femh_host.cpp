@@ -319,14 +319,21 @@ NumericalCoarseningUsingMultigrid::~NumericalCoarseningUsingMultigrid() {
 }
 
 void NumericalCoarseningUsingMultigrid::init(int N[3], int rep, int sub, const std::vector<BaseMaterial> &mats,
-                                             StructureType) {
+                                             StructureType iType) {
   if (mats.size() < 2) throw HostError(FEMH_BAD_ARG, "need two base materials (void, solid)");
   m_nbBlockRep = rep;
   m_nbSubdiv = sub;
   m_baseMaterials = mats;
   for (int d = 0; d < 3; d++) m_n[d] = N[d] * rep * sub;
+  // (NumericalCoarseningUsingMultiGrid.cpp:211: the structure type of init
+  // picks the solve)
+  const bool cube = m_n[0] == m_n[1] && m_n[1] == m_n[2];
+  m_structure = iType == Cubic && cube ? FEMH_STRUCT_CUBIC : m_opts.structure;
   if (m_ctx) femh_destroy(m_ctx);
   m_ctx = nullptr;
+  if (m_batchCtx) femh_destroy(m_batchCtx);
+  m_batchCtx = nullptr;
+  m_batch = 0;
   // the solid (material 1) is the base material; voxels of material 0 get
   // the coefficient m_densityMin (NumericalCoarseningUsingMultiGrid.cpp:186-193, :239)
   double E, nu;
@@ -338,6 +345,7 @@ void NumericalCoarseningUsingMultigrid::init(int N[3], int rep, int sub, const s
   o.E0 = E;
   o.nu = nu;
   o.interp = FEMH_INTERP_MODULUS;
+  o.structure = m_structure;
   const int st = femh_create(&o, &m_ctx);
   if (st != FEMH_OK) {
     char buf[512] = {0};
@@ -401,6 +409,7 @@ void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParam
     o.E0 = E;
     o.nu = nu;
     o.interp = FEMH_INTERP_MODULUS;
+    o.structure = m_structure;
     o.batch = batch;
     const int st = femh_create(&o, &m_batchCtx);
     if (st != FEMH_OK) {
@@ -522,6 +531,80 @@ int runTensorHomogenization(const std::string &dir, int level, int nbBlockRep, i
       !cfgbin::writeBinary(root + "matAssignments.bin", assign) ||
       !cfgbin::writeBinary(root + "elasticityTensors.bin", tensors))
     throw HostError(FEMH_BAD_ARG, "cannot write " + root + "*.bin");
+  return (int)assign.size();
+}
+
+// ---- ThermalAnalysis (ThermalAnalysis.cpp:210-274, :388-505) -------------------
+ThermalAnalysis::ThermalAnalysis() { femh_default_opts(&m_opts); }
+
+bool ThermalAnalysis::run3D(int N[3], const std::vector<BaseMaterial> &mats,
+                            const std::vector<std::vector<int>> &assign, const std::vector<float> &,
+                            const std::vector<float> &, int rep, int sub, std::vector<float> &oCoef, int batch) {
+  if (mats.size() < 2) throw HostError(FEMH_BAD_ARG, "need two base materials (void, solid)");
+  if (batch < 1 || batch > 16) throw HostError(FEMH_BAD_ARG, "batch must be 1..16");
+  oCoef.clear();
+  m_alpha.clear();
+  if (assign.empty()) return true;
+  const int n[3] = {N[0] * rep * sub, N[1] * rep * sub, N[2] * rep * sub};
+  double E1, nu1;
+  mats[1].youngPoisson(E1, nu1);
+  femh_opts o = m_opts;
+  o.nx = n[0];
+  o.ny = n[1];
+  o.nz = n[2];
+  o.E0 = E1;
+  o.nu = nu1;
+  o.interp = FEMH_INTERP_MODULUS;
+  o.batch = batch;
+  femh_ctx *ctx = nullptr;
+  const int st = femh_create(&o, &ctx);
+  if (st != FEMH_OK) {
+    if (ctx) femh_destroy(ctx);
+    fail(nullptr, st, "femh_create");
+  }
+  const size_t V = (size_t)n[0] * n[1] * n[2];
+  std::vector<double> Emod(V * batch), alpha(V * batch), C(36 * (size_t)batch), aeff(6 * (size_t)batch);
+  std::vector<int> cells;
+  try {
+    for (size_t c0 = 0; c0 < assign.size(); c0 += batch) {
+      const int nb = (int)std::min<size_t>(batch, assign.size() - c0);
+      for (int k = 0; k < nb; k++) {
+        repMaterialAssignment(N[0], N[1], N[2], assign[c0 + k], rep, rep, rep, sub, cells);
+        if (cells.size() != V) throw HostError(FEMH_BAD_ARG, "assignment does not match the grid");
+        for (size_t e = 0; e < V; e++) {
+          const bool solid = cells[e] == 1;
+          Emod[V * k + e] = E1 * (solid ? 1.0 : m_densityMin);
+          alpha[V * k + e] = solid ? 1.0 : 0.0;
+        }
+      }
+      check(ctx, femh_homogenize_many_modulus(ctx, nb, Emod.data(), C.data(), nullptr), "femh_homogenize_many_modulus");
+      check(ctx, femh_thermal_expansion(ctx, alpha.data(), nullptr, aeff.data()), "femh_thermal_expansion");
+      for (int k = 0; k < nb; k++) {
+        for (int q = 0; q < 6; q++) m_alpha.push_back(aeff[6 * k + q]);
+        oCoef.push_back((float)((aeff[6 * k] + aeff[6 * k + 1] + aeff[6 * k + 2]) / 3.0));
+      }
+    }
+  } catch (...) {
+    femh_destroy(ctx);
+    throw;
+  }
+  femh_destroy(ctx);
+  return true;
+}
+
+int runThermalAnalysis(const std::string &dir, int level, int nbBlockRep, int nbSubdiv,
+                       const std::vector<BaseMaterial> &mats, const femh_opts *solverOptions, int batch) {
+  const std::string root = dir + (dir.empty() || dir.back() == '/' ? "" : "/") + "level" + std::to_string(level) + "_";
+  std::vector<std::vector<int>> assign;
+  if (!cfgbin::readBinary(root + "matAssignments.bin", assign))
+    throw HostError(FEMH_BAD_ARG, "cannot read " + root + "matAssignments.bin");
+  ThermalAnalysis ta;
+  if (solverOptions) ta.setSolverOptions(*solverOptions);
+  int n[3] = {level, level, level};
+  std::vector<float> coef;
+  ta.run3D(n, mats, assign, {}, {}, nbBlockRep, nbSubdiv, coef, batch);
+  if (!cfgbin::writeBinary(root + "thermalExpansionCoefficients.bin", coef))
+    throw HostError(FEMH_BAD_ARG, "cannot write " + root + "thermalExpansionCoefficients.bin");
   return (int)assign.size();
 }
 

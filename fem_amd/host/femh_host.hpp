@@ -163,6 +163,9 @@ class NumericalCoarseningUsingMultigrid {
   void setEmitReferenceTriangle(bool v) { m_referenceTriangle = v; }
   void setSolverOptions(const femh_opts &o) { m_opts = o; }
 
+  // Cubic on a cube grid selects the engine's cubic-structure mode, as the
+  // reference's init selects its CUBIC solve (two load cases for cells
+  // invariant under the axis swaps; the engine solves other cells in full)
   void init(int N[3], int iNbBlockRep, int iNbSubdiv, const std::vector<BaseMaterial> &iBaseMaterials,
             StructureType iType);
   // appends 21 values (row i, j <= i) to ioTensorValues and, for Cubic,
@@ -198,6 +201,7 @@ class NumericalCoarseningUsingMultigrid {
   Mat6 m_C;
   femh_ctx *m_batchCtx = nullptr;
   int m_batch = 0;
+  int m_structure = FEMH_STRUCT_GENERAL;  // engine structure chosen by init()
   void moduli(const std::vector<int> &iMaterials, int N[3], double *oE) const;
 };
 
@@ -246,6 +250,45 @@ int runTensorHomogenization(const std::string &dir, int level, int nbBlockRep, i
                             const std::vector<BaseMaterial> &baseMaterials,
                             const std::vector<double> &baseMaterialDensities, bool cubicOnly,
                             const femh_opts *solverOptions = nullptr, int batch = 16);
+
+// ---- thermal expansion ---------------------------------------------------------
+// ThermalAnalysis (SamplesGenerator/ThermalAnalysis.h; run3D :210-274,
+// computeThermalCoefficient :388-505) on the engine: per assignment, C_H and
+// the effective thermal expansion of the periodic cell, with material 1
+// expanding by 1 per unit temperature rise and the other materials not at all
+// (the reference's heat forces come from material 1 only, :404-420).  The
+// other materials take m_densityMin * E_solid with the solid's Poisson ratio,
+// as the multigrid coarsening does.  The expansion comes from the six elastic
+// fields with no thermal solve, so iParameters and iTensors (inputs of the
+// reference's coarse-element estimate) are not needed.
+class ThermalAnalysis {
+ public:
+  ThermalAnalysis();
+  void setSolverOptions(const femh_opts &o) { m_opts = o; }
+  // appends, per assignment, the mean normal expansion (a_xx + a_yy + a_zz) / 3
+  // (for a cubic cell, its expansion coefficient); `batch` cells per launch
+  // sequence
+  bool run3D(int N[3], const std::vector<BaseMaterial> &iBaseMaterials,
+             const std::vector<std::vector<int>> &iMaterialAssignments, const std::vector<float> &iParameters,
+             const std::vector<float> &iTensors, int iNbBlockRep, int iNbSubdiv,
+             std::vector<float> &oThermalCoefficients, int batch = 16);
+  // the full expansions (Voigt, engineering shear) of the last run3D, 6 per
+  // assignment
+  const std::vector<double> &lastExpansions() const { return m_alpha; }
+
+ private:
+  femh_opts m_opts;
+  std::vector<double> m_alpha;
+  double m_densityMin = 1e-4;
+};
+
+// SamplesGeneratorImpl::runThermalAnalysis (SamplesGeneratorImpl.cpp:4654-4680)
+// as a GPU batch job: reads dir/level{L}_matAssignments.bin and writes
+// dir/level{L}_thermalExpansionCoefficients.bin (one float per assignment, as
+// ThermalAnalysis::run3D).  Returns the number of assignments.
+int runThermalAnalysis(const std::string &dir, int level, int nbBlockRep, int nbSubdiv,
+                       const std::vector<BaseMaterial> &baseMaterials, const femh_opts *solverOptions = nullptr,
+                       int batch = 16);
 
 // ---- optimiser ---------------------------------------------------------------
 class MaterialOptimizer {

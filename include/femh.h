@@ -22,6 +22,10 @@
  *       exposes the 6 solved periodic fields (the reference's
  *       Homogenization::unit_test_u consumed at NumericalCoarseningUsingMultiGrid
  *       .cpp:275-288, fast_fem/FEM3DFun.cpp:159-161).
+ *   femh_thermal_expansion
+ *       replaces ThermalAnalysis::computeThermalCoefficient / run3D
+ *       (SamplesGenerator/ThermalAnalysis.cpp:210-274, :388-505): effective
+ *       thermal stress and expansion from the solved fields.
  *   femh_homogenize_batch
  *       replaces the OpenMP sample loop of SamplesGeneratorImpl::
  *       computeParametersAndTensorValues (SamplesGenerator/SamplesGeneratorImpl
@@ -99,12 +103,16 @@ typedef struct femh_opts {
   int warm_start;       /* 1: start each sample slot from the fields of the previous
                            call with the same sample count (an optimisation loop's
                            previous iterate); 0: start from zero */
-  int structure;        /* femh_structure: FEMH_STRUCT_CUBIC solves only the xx and yz
-                           load cases of a cube cell that the caller knows to be
-                           cubic (invariant under axis swaps), forms the other four
-                           fields by swapping axes, and returns C_H projected onto
-                           the cubic form, as the reference's CUBIC mode
-                           (NumericalCoarseningUsingMultiGrid.cpp:275-282) */
+  int structure;        /* femh_structure.  FEMH_STRUCT_CUBIC (cube grids only): a cell
+                           whose moduli are invariant under the axis swaps (checked
+                           on the device, 1e-12 relative per voxel) is solved for the
+                           xx and yz load cases only; the other four fields are those
+                           two with axes swapped, and C_H is averaged over the axis
+                           permutations.  A cell that is not invariant is solved with
+                           all six cases (femh_stats.iters[1] > 0 tells which path
+                           ran).  The reference's CUBIC mode
+                           (NumericalCoarseningUsingMultiGrid.cpp:275-282) makes the
+                           same two solves */
 } femh_opts;
 
 typedef struct femh_stats {
@@ -164,6 +172,19 @@ int femh_energy_terms(struct femh_ctx *ctx, double *W);
 /* Copies the 6 solved periodic fluctuation fields (6*3*V doubles per sample
  * of the last batch). */
 int femh_get_fields(struct femh_ctx *ctx, double *u);
+
+/* Thermal expansion of the cells of the last homogenize call, with no further
+ * solve.  alpha holds the expansion coefficient of every voxel (nb*V doubles,
+ * host).  beta (6*nb, may be NULL) receives the effective thermal stress per
+ * unit temperature rise, beta_q = (1/V) sum_e alpha_e E_e int_e I : C0 : chi_q
+ * (Voigt), and alpha_eff (6*nb, may be NULL) the effective expansion
+ * C_H^-1 beta (engineering shear).  Replaces the per-cell force-driven solve
+ * of ThermalAnalysis::computeThermalCoefficient
+ * (SamplesGenerator/ThermalAnalysis.cpp:388-505), whose result run3D
+ * (:210-274) estimates; here the periodic value is exact for the
+ * discretisation (reciprocity with the six elastic fields).  Returns
+ * FEMH_NO_FIELDS when no solve has been done. */
+int femh_thermal_expansion(struct femh_ctx *ctx, const double *alpha, double *beta, double *alpha_eff);
 
 /* Batch: n samples, sample s has dims[3s..3s+2] and modulus grid E[s] (host).
  * C receives 36*n doubles.  Material nu/Emin from the context options.

@@ -20,6 +20,9 @@
  *   or_ch_energy     NumericalCoarseningUsingMultiGrid.cpp:290-313 and
  *                    fast_fem/FEM3DFun.cpp:159-180 (C_ij = 1/vol sum coef u_i^T K0 u_j)
  *   or_grad_E        fast_fem/FEM3DFun.cpp:222-268 (dG_ij/dx_e = coef'(x_e) u_i^T K0 u_j)
+ *   or_thermal_stress SamplesGenerator/ThermalAnalysis.cpp:210-274, :388-505 (the homogenized
+ *                    thermal-expansion coefficient; here by reciprocity from the
+ *                    six elastic fields, see the function)
  *   multigrid        the reference's multigrid lives in topoOptLib, which is not
  *                    vendored; this is a textbook Galerkin geometric multigrid
  *                    (trilinear prolongation, R = P^T, coarse element matrices
@@ -678,6 +681,59 @@ done:
   *relres_out = bn > 0 ? rn / bn : 0.0;
   free(r); free(z); free(p); free(q); free(zold);
   return conv ? 0 : 1;
+}
+
+/* Effective thermal stress per unit temperature rise: beta_q = (1/V) sum_e
+ * alpha_e E_e int_e I : C0 : eps(chi_q), chi_q = affine + fluctuation of load
+ * case q.  With the elastic fields this is the exact periodic thermal stress
+ * of the discretisation (reciprocity), so no thermal solve is needed; the
+ * reference estimates the same quantity by a force-driven solve per cell
+ * (SamplesGenerator/ThermalAnalysis.cpp:388-505, computeThermalCoefficient)
+ * and by Levin's formula (:263-266).  The strain is evaluated at the 8 Gauss
+ * points of or_ke_hex and the stress by the Lame form. */
+void or_thermal_stress(int nx, int ny, int nz, const double *E, double nu, const double *alpha,
+                       const double *u, double *beta) {
+  static const int sw[8][3] = {{-1, -1, -1}, {-1, -1, 1}, {-1, 1, -1}, {-1, 1, 1},
+                               {1, -1, -1},  {1, -1, 1},  {1, 1, -1},  {1, 1, 1}};
+  size_t V = (size_t)nx * ny * nz;
+  double lam, mu;
+  or_lame(1.0, nu, &lam, &mu);
+  const double g = 1.0 / sqrt(3.0);
+  double gN[8][8][3]; /* [gauss point][corner][axis] */
+  for (int q = 0; q < 8; q++)
+    for (int a = 0; a < 8; a++) {
+      double xi[3] = {sw[q][0] * g, sw[q][1] * g, sw[q][2] * g};
+      gN[q][a][0] = 0.25 * sw[a][0] * (1 + sw[a][1] * xi[1]) * (1 + sw[a][2] * xi[2]);
+      gN[q][a][1] = 0.25 * sw[a][1] * (1 + sw[a][0] * xi[0]) * (1 + sw[a][2] * xi[2]);
+      gN[q][a][2] = 0.25 * sw[a][2] * (1 + sw[a][0] * xi[0]) * (1 + sw[a][1] * xi[1]);
+    }
+  double u0[6][24];
+  for (int lc = 0; lc < 6; lc++) affine_local(lc, u0[lc]);
+  double acc[6] = {0};
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        size_t e = IDX(i, j, k, ny, nz);
+        double w = alpha[e] * E[e];
+        if (w == 0.0) continue;
+        for (int lc = 0; lc < 6; lc++) {
+          double chi[24];
+          gather_elem(nx, ny, nz, i, j, k, u + 3 * V * lc, chi);
+          for (int t = 0; t < 24; t++) chi[t] += u0[lc][t];
+          double s = 0;
+          for (int q = 0; q < 8; q++) {
+            double F[3][3] = {{0}};
+            for (int a = 0; a < 8; a++)
+              for (int c = 0; c < 3; c++)
+                for (int m = 0; m < 3; m++) F[c][m] += chi[3 * a + c] * gN[q][a][m];
+            double tr = F[0][0] + F[1][1] + F[2][2], trs = 0;
+            for (int c = 0; c < 3; c++) trs += lam * tr + 2.0 * mu * F[c][c]; /* I : sigma */
+            s += 0.125 * trs;
+          }
+          acc[lc] += w * s;
+        }
+      }
+  for (int lc = 0; lc < 6; lc++) beta[lc] = acc[lc] / (double)V;
 }
 
 void or_ch_energy(int nx, int ny, int nz, const double *E, double nu, const double *u, double *C) {

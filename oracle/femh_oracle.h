@@ -82,6 +82,10 @@ void or_energy_terms(int nx, int ny, int nz, double nu, const double *u,
                      double *W);
 
 /* dJ/dE_e = sum_ij G[6i+j] * chi_i^T Ke chi_j / V  (G = dJ/dC full 6x6). */
+/* beta[6] = (1/V) sum_e alpha_e E_e int_e I : C0 : eps(affine_q + u_q) (the
+ * effective thermal stress per unit temperature rise; u as or_homogenize) */
+void or_thermal_stress(int nx, int ny, int nz, const double *E, double nu, const double *alpha,
+                       const double *u, double *beta);
 void or_grad_E(int nx, int ny, int nz, double nu, const double *u,
                const double *G, double *dJdE);
 

@@ -118,6 +118,18 @@ def ch_energy(E, nu, u):
     return C.reshape(6, 6)
 
 
+def thermal_stress(E, nu, alpha, u):
+    """beta[6]: effective thermal stress per unit temperature rise from the six
+    elastic fields (Gauss-point strains, Lame stress)."""
+    E = _E(E)
+    a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(E.shape)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    beta = np.zeros(6)
+    lib().or_thermal_stress(*[ctypes.c_int(n) for n in E.shape], _dp(E), ctypes.c_double(nu), _dp(a),
+                            _dp(u), _dp(beta))
+    return beta
+
+
 def energy_terms(shape, nu, u):
     u = np.ascontiguousarray(u, dtype=np.float64)
     V = int(np.prod(shape))

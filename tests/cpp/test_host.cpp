@@ -28,6 +28,30 @@ static Mat6 c_iso(double E, double nu) {
   return C;
 }
 
+// x = C^-1 b by Gaussian elimination with partial pivoting
+static void solve6(const Mat6 &C, const double *b, double *x) {
+  double M[6][7];
+  for (int i = 0; i < 6; i++) {
+    for (int j = 0; j < 6; j++) M[i][j] = C(i, j);
+    M[i][6] = b[i];
+  }
+  for (int c = 0; c < 6; c++) {
+    int p = c;
+    for (int r = c + 1; r < 6; r++)
+      if (std::fabs(M[r][c]) > std::fabs(M[p][c])) p = r;
+    for (int j = 0; j < 7; j++) std::swap(M[c][j], M[p][j]);
+    for (int r = c + 1; r < 6; r++) {
+      const double f = M[r][c] / M[c][c];
+      for (int j = c; j < 7; j++) M[r][j] -= f * M[c][j];
+    }
+  }
+  for (int i = 5; i >= 0; i--) {
+    double v = M[i][6];
+    for (int j = i + 1; j < 6; j++) v -= M[i][j] * x[j];
+    x[i] = v / M[i][i];
+  }
+}
+
 int main() {
   // base materials as the sample generator builds them: mu = 100,
   // lambda = 1000, the void with E / 1000 (SamplesGeneratorImpl.cpp:4864-4893)
@@ -73,6 +97,16 @@ int main() {
     // 10 of the 64 cells have at least two zero coordinates
     expect(asym < 1e-8 && std::fabs(par[0] - 10.0 / 64.0) < 1e-15, "coarsening: frame lattice is cubic",
            asym);
+    // Cubic took the two-solve path on this swap-invariant cell; General
+    // solves all six cases
+    NumericalCoarseningUsingMultigrid ng;
+    ng.init(N, 2, 1, mats, NumericalCoarseningUsingMultigrid::General);
+    std::vector<double> tg, pg;
+    ng.computeCoarsenedElasticityTensorAndParameters3D(m, N, {0.0, 1.0}, NumericalCoarseningUsingMultigrid::General,
+                                                       tg, pg);
+    double dgc = 0;
+    for (int q = 0; q < 21; q++) dgc = std::fmax(dgc, std::fabs(tg[q] - tv[q]) / c0);
+    expect(dgc < 1e-8, "coarsening: Cubic (two solves + axis swaps) = General (six solves)", dgc);
     nc.setEmitReferenceTriangle(true);
     std::vector<double> tq, pq;
     nc.computeCoarsenedElasticityTensorAndParameters3D(m, N, {0.0, 1.0}, NumericalCoarseningUsingMultigrid::Cubic,
@@ -180,6 +214,59 @@ int main() {
     expect(ok && perr < 1e-5, "runTensorHomogenization: cubic parameters = one-cell parameters", perr);
     for (const char *f : {"matAssignments", "baseMat", "params", "elasticityTensors"})
       std::remove((root + f + ".bin").c_str());
+  }
+  {  // thermal expansion: a solid cell expands by 1; a two-phase cell obeys Levin's relation
+    femh_opts o;
+    femh_default_opts(&o);
+    o.tol = 1e-12;
+    int N[3] = {4, 4, 4};
+    std::vector<std::vector<int>> assign(2, std::vector<int>(64, 1));
+    unsigned seed = 777u;
+    for (auto &m : assign[1]) {
+      seed = seed * 1664525u + 1013904223u;
+      m = (seed >> 16) % 2;
+    }
+    ThermalAnalysis ta;
+    ta.setSolverOptions(o);
+    std::vector<float> coef;
+    ta.run3D(N, mats, assign, {}, {}, 1, 2, coef, 2);
+    const std::vector<double> &a = ta.lastExpansions();
+    expect(coef.size() == 2 && a.size() == 12 && std::fabs(a[0] - 1.0) < 1e-10 && std::fabs(a[3]) < 1e-10,
+           "thermal: a solid cell expands uniformly", std::fabs(a[0] - 1.0));
+    NumericalCoarseningUsingMultigrid nc;
+    nc.setSolverOptions(o);
+    nc.init(N, 1, 2, mats, NumericalCoarseningUsingMultigrid::General);
+    std::vector<double> tv, pv;
+    nc.computeCoarsenedElasticityTensorAndParameters3D(assign[1], N, {0.0, 1.0},
+                                                       NumericalCoarseningUsingMultigrid::General, tv, pv);
+    // Levin: alpha = phi I + (S I - <1/3K> I) / (1/3K1 - 1/3K2), K2 = 1e-4 K1
+    const double ones[6] = {1, 1, 1, 0, 0, 0};
+    double SI6[6];
+    solve6(nc.lastTensor(), ones, SI6);
+    const double k1 = E1 / (1 - 2 * nu1), k2 = 1e-4 * k1;  // 3K of each phase
+    double phi = 0;
+    for (int m : assign[1]) phi += m;
+    phi /= 64.0;
+    const double mc = phi / k1 + (1 - phi) / k2;
+    double lerr = 0;
+    for (int i = 0; i < 6; i++) {
+      const double SI = SI6[i], Ii = i < 3 ? 1.0 : 0.0;
+      const double lev = phi * Ii + (SI - mc * Ii) / (1 / k1 - 1 / k2);
+      lerr = std::fmax(lerr, std::fabs(a[6 + i] - lev));
+    }
+    expect(lerr < 1e-7, "thermal: two-phase cell obeys Levin's relation", lerr);
+    expect(std::fabs(coef[1] - (a[6] + a[7] + a[8]) / 3.0) < 1e-6, "thermal: coefficient is the mean normal expansion",
+           coef[1]);
+    // the stage as a batch job over the caller's files
+    const char *tmp = std::getenv("TMPDIR");
+    const std::string dir = std::string(tmp && *tmp ? tmp : "/tmp");
+    const std::string root = dir + "/level4_";
+    bool ok = cfgbin::writeBinary(root + "matAssignments.bin", assign);
+    const int cnt = ok ? runThermalAnalysis(dir, 4, 1, 2, mats, &o, 2) : 0;
+    std::vector<float> back;
+    ok = ok && cnt == 2 && cfgbin::readBinary(root + "thermalExpansionCoefficients.bin", back);
+    expect(ok && back == coef, "runThermalAnalysis: coefficients written in the caller's layout", (double)back.size());
+    for (const char *f : {"matAssignments", "thermalExpansionCoefficients"}) std::remove((root + f + ".bin").c_str());
   }
   std::printf("%s\n", fails ? "FAILED" : "ALL PASSED");
   return fails ? 1 : 0;

@@ -42,6 +42,22 @@ def test_cubic_fp64_equals_six_case_solve():
         assert np.abs(gc - gg).max() <= 1e-8 * np.abs(gg).max()
 
 
+def test_cubic_mode_solves_asymmetric_cells_in_full():
+    """In a cubic-mode batch, a cell that is not invariant under the axis swaps
+    takes all six solves; its neighbours keep the two-solve path."""
+    E = cells(16, [3])
+    asym = W.simp_modulus(W.simp_density(16, 8, 3.0))
+    Es = np.stack([E[0], asym, E[1]])
+    hc = Homogenizer((16, 16, 16), nu=NU, tol=1e-12, structure="cubic", batch=3)
+    hg = Homogenizer((16, 16, 16), nu=NU, tol=1e-12, batch=3)
+    Cc, stc = hc.homogenize_many(Es, modulus=True)
+    Cg, _ = hg.homogenize_many(Es, modulus=True)
+    for s in range(3):
+        assert rel(Cc[s], Cg[s]) <= 1e-9, (s, rel(Cc[s], Cg[s]))
+    assert [stc[s]["iters"][1] > 0 for s in range(3)] == [False, True, False]
+    assert abs(Cg[1][0, 3]) > 1e-6 * Cg[1][0, 0]  # an anisotropic coupling the average would erase
+
+
 def test_cubic_needs_a_cube_grid():
     with pytest.raises(FemhError) as e:
         Homogenizer((16, 16, 8), nu=NU, structure="cubic")

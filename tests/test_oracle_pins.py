@@ -321,6 +321,33 @@ def test_cubic_cell_has_cubic_tensor():
     assert abs(c11 - c12 - 2 * c44) > 1e-3 * c11  # cubic, not isotropic
 
 
+def levin(C, E1, E2, phi, nu):
+    """Levin's exact relation for two isotropic phases of equal Poisson ratio,
+    expansion 1 in phase 1 and 0 in phase 2: alpha_H = phi I + (S_H I -
+    <1/3K> I) / (1/3K1 - 1/3K2), for any microstructure (a uniform pressure
+    with a matched temperature strains both phases alike)."""
+    k = lambda e: e / (3 * (1 - 2 * nu))
+    I = np.array([1, 1, 1, 0, 0, 0.0])
+    mean_c = phi / (3 * k(E1)) + (1 - phi) / (3 * k(E2))
+    return phi * I + (np.linalg.solve(C, I) - mean_c * I) / (1 / (3 * k(E1)) - 1 / (3 * k(E2)))
+
+
+def test_oracle_thermal_stress_uniform_expansion_and_levin():
+    # a uniform expansion coefficient expands any cell uniformly
+    E = W.simp_modulus(W.simp_density(6, 2, 2.0))
+    r = O.homogenize(E, NU, precond="jacobi", tol=1e-13, max_iter=20000)
+    aH = np.linalg.solve(r["C"], O.thermal_stress(E, NU, np.full(E.shape, 0.7), r["u"]))
+    assert np.abs(aH - 0.7 * np.array([1, 1, 1, 0, 0, 0])).max() < 1e-12
+    # two phases, anisotropic cell: every component of Levin's relation
+    s = W.binary_cube(6, 4) > 0.5
+    E = np.where(s, 1.0, 0.05)
+    r = O.homogenize(E, NU, precond="jacobi", tol=1e-13, max_iter=20000)
+    aH = np.linalg.solve(r["C"], O.thermal_stress(E, NU, s.astype(float), r["u"]))
+    ref = levin(r["C"], 1.0, 0.05, s.mean(), NU)
+    assert np.abs(aH - ref).max() < 1e-11
+    assert np.abs(aH[3:]).max() > 1e-3  # the shear couplings are exercised
+
+
 def test_oracle_jacobi_and_multigrid_agree():
     E = W.simp_modulus(W.simp_density(16, 3, 4.0))
     a = O.homogenize(E, NU, precond="jacobi", tol=1e-12, max_iter=20000)
