@@ -58,6 +58,8 @@ def parse():
                     help="simp64 only: cubic symmetrises each cell over the cubic group and "
                          "solves it in the cubic-structure mode (2 of the 6 load cases); the FP64 "
                          "reference stays the six-case solve")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="1: launch shapes picked by the engine's autotuner (timed in the warm-up)")
     ap.add_argument("--cells", type=int, default=0, help="cells of uneven512 / opt96 (0: 512 / 64)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run one 256 MiB device-to-device copy (known traffic) so that "
@@ -144,7 +146,8 @@ def main():
 
     mixed = Homogenizer(shape, nu=0.3, penal=3.0, precision="mixed", tol=1e-10,
                         ch_tol=a.ch_tol, inner_tol=a.inner_tol, batch=B, nu1=a.nu1, nu2=a.nu2,
-                        cheby_ratio=a.cheby_ratio, coarse_nu=a.coarse_nu, structure=a.structure)
+                        cheby_ratio=a.cheby_ratio, coarse_nu=a.coarse_nu, structure=a.structure,
+                        autotune=bool(a.autotune))
 
     def step(k, stats=None):
         st = mixed.homogenize_many_device(B, d_rho[k * B].data_ptr(), d_C[k * B].data_ptr())
@@ -171,6 +174,7 @@ def main():
     k32 = mixed.kernel_stats(1)
     k64 = mixed.kernel_stats(0)
     mixed.set_profiling(False)
+    shapes = mixed.launch_shapes()
     mixed.close()
 
     errs = []
@@ -224,7 +228,7 @@ def main():
                                 f"nu=0.3; xx and yz load cases solved, 4 by axis swaps), ")
                                + f"mixed FP32/FP64 multigrid-PCG, {B} cells per step",
                    "n": n, "batch": B, "cells_timed": a.steps * B, "ch_tol": a.ch_tol,
-                   "inner_tol": a.inner_tol},
+                   "inner_tol": a.inner_tol, "autotune": a.autotune, "launch_shapes": shapes},
         "roofline": roof,
         "cpu_baseline": cpu,
         "ch_rel_err_vs_fp64_max": max(errs) if errs else None,
@@ -250,8 +254,13 @@ def run_uneven(a):
     count = a.cells or 512
     grids = W.batch_uneven(count, 3)
     kw = dict(nu=0.3, precision="mixed", tol=1e-10, ch_tol=a.ch_tol, inner_tol=a.inner_tol,
-              max_iter=20000, batch=a.batch)
-    homogenize_batch(grids[:8], **kw)  # warm-up: builds and loads
+              max_iter=20000, batch=a.batch, autotune=bool(a.autotune))
+    # warm-up: builds and loads, one cell of every size (and, with --autotune,
+    # times the launch shapes of every size)
+    first = {}
+    for g in grids:
+        first.setdefault(g.shape, g)
+    homogenize_batch(list(first.values()), **kw)
     t0 = time.perf_counter()
     C, st = homogenize_batch(grids, **kw)
     elapsed = time.perf_counter() - t0
@@ -278,7 +287,7 @@ def run_uneven(a):
         "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic",
         "config": {"workload": f"{count} binary cells, edge U{{16..48}}, fill U[0.2,0.8], "
                                f"E_void 1e-9, nu 0.3", "batch": a.batch, "ch_tol": a.ch_tol,
-                   "voxels": vox},
+                   "voxels": vox, "autotune": a.autotune},
         "ch_rel_err_vs_fp64_max": max(errs) if errs else None,
         "ch_err_ok": (max(errs) <= 1e-6) if errs else None,
         "fp64_batched_seconds": t64 if errs else None,

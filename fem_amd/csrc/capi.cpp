@@ -103,6 +103,7 @@ void femh_default_opts(femh_opts *o) {
   o->batch = 1;
   o->warm_start = 0;
   o->structure = FEMH_STRUCT_GENERAL;
+  o->autotune = 0;
   o->coarse_nu = 3;
 }
 
@@ -188,6 +189,15 @@ int femh_thermal_expansion(femh_ctx *ctx, const double *alpha, double *beta, dou
   if (int s = need_ctx(ctx)) return s;
   if (!alpha) return FEMH_BAD_ARG;
   return guarded(ctx, [&] { ctx->c->thermal(alpha, beta, alpha_eff); });
+}
+
+int femh_launch_shapes(femh_ctx *ctx, int *shapes, int n) {
+  if (int s = need_ctx(ctx)) return -s;
+  if (n < 0 || (n > 0 && !shapes)) return -FEMH_BAD_ARG;
+  const femh::Context &c = *ctx->c;
+  const int L = c.nlevels;
+  for (int i = 0; i < n && i < L; i++) shapes[i] = i == 0 ? c.shape_fine : c.xfer_shape(i - 1);
+  return L;
 }
 
 int femh_get_fields(femh_ctx *ctx, double *u) {

@@ -252,7 +252,10 @@ __global__ __launch_bounds__(kBlock) void k_stencil_diag(StencilOp<T> op, T *__r
 // a chunk of CI coarse planes: the (j,k)-weighted sum of an input plane is
 // formed once and shared by the two output planes it contributes to.
 // CI: up to 4 planes, fewer when that is needed for ~64K threads.
-static inline int xfer_chunk(const Dims &cg, int cols, int R) {
+// (want > 0: that many planes, an autotuned shape; the result is the same
+// for every CI, only the work split changes)
+static inline int xfer_chunk(const Dims &cg, int cols, int R, int want) {
+  if (want > 0) return want < cg.nx ? want : cg.nx;
   int ci = 4;
   while (ci > 1 && (long)cols * R * ((cg.nx + ci - 1) / ci) < 65536) ci /= 2;
   return ci;
@@ -871,16 +874,16 @@ void launch_stencil_diag(const StencilOp<T> &op, int B, T *d, hipStream_t s) {
 }
 template <typename T>
 void launch_restrict(Dims fine, Dims coarse, int R, const T *rf, T *rc, const int *act,
-                     hipStream_t s) {
-  const int cols = coarse.ny * coarse.nz, ci = xfer_chunk(coarse, cols, R);
+                     hipStream_t s, int planes) {
+  const int cols = coarse.ny * coarse.nz, ci = xfer_chunk(coarse, cols, R, planes);
   const long n = (long)cols * ((coarse.nx + ci - 1) / ci);
   hipLaunchKernelGGL(k_restrict<T>, dim3((unsigned)((n + kBlock - 1) / kBlock), R), dim3(kBlock), 0, s,
                      fine, coarse, rf, rc, act, ci);
 }
 template <typename T>
 void launch_prolong_add(Dims fine, Dims coarse, int R, const T *ec, T *ef, const int *act,
-                        hipStream_t s, T *out) {
-  const int cols = fine.ny * fine.nz, ci = xfer_chunk(coarse, cols, R);
+                        hipStream_t s, T *out, int planes) {
+  const int cols = fine.ny * fine.nz, ci = xfer_chunk(coarse, cols, R, planes);
   const long n = (long)cols * ((coarse.nx + ci - 1) / ci);
   hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)((n + kBlock - 1) / kBlock), R), dim3(kBlock), 0, s,
                      fine, coarse, ec, ef, act, ci, out);
@@ -1020,8 +1023,9 @@ void reduce_init(Reduce &red, hipStream_t s) {
                                        hipStream_t);                                              \
   template void launch_galerkin<T>(Dims, Dims, int, const T *, T *, T *, T *, hipStream_t);       \
   template void launch_stencil_diag<T>(const StencilOp<T> &, int, T *, hipStream_t);              \
-  template void launch_restrict<T>(Dims, Dims, int, const T *, T *, const int *, hipStream_t);    \
-  template void launch_prolong_add<T>(Dims, Dims, int, const T *, T *, const int *, hipStream_t, T *); \
+  template void launch_restrict<T>(Dims, Dims, int, const T *, T *, const int *, hipStream_t, int); \
+  template void launch_prolong_add<T>(Dims, Dims, int, const T *, T *, const int *, hipStream_t, T *, \
+                                      int);                                                       \
   template void launch_dense_solve<T>(int, int, const T *, const T *, T *, const int *,           \
                                       hipStream_t);                                               \
   template void launch_cheb_first<T>(int, int, const T *, const T *, T *, T *, double, int,       \

@@ -103,13 +103,15 @@ template <typename T>
 void launch_galerkin(Dims fine, Dims coarse, int B, const T *Sf, T *Sc, T *t1, T *t2, hipStream_t s);
 template <typename T>
 void launch_stencil_diag(const StencilOp<T> &op, int B, T *d, hipStream_t s);
+// planes: coarse planes per thread (0: the built-in choice; an autotuned
+// value otherwise -- the results do not depend on it)
 template <typename T>
 void launch_restrict(Dims fine, Dims coarse, int R, const T *rf, T *rc, const int *act,
-                     hipStream_t s);
+                     hipStream_t s, int planes = 0);
 // ef += P ec; with out, out = ef + P ec (ef is then only read)
 template <typename T>
 void launch_prolong_add(Dims fine, Dims coarse, int R, const T *ec, T *ef, const int *act,
-                        hipStream_t s, T *out = nullptr);
+                        hipStream_t s, T *out = nullptr, int planes = 0);
 // dense coarsest solve: x = Ainv[s] b for each RHS (n = 3*V_coarsest)
 template <typename T>
 void launch_dense_solve(int n, int R, const T *Ainv, const T *b, T *x, const int *act,

@@ -115,21 +115,27 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, 
   }
 }
 
-static inline int tiled_chunk(const Dims &g, int R) {
+static inline int tiled_chunk(const Dims &g, int R, int want) {
   const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
-  // planes per chunk: about 16 (halo slab overhead 1/16), but enough chunks to
-  // put ~2 workgroups per CU across the right-hand sides, and at most
-  // kMaxPartials workgroups per right-hand side
+  // planes per chunk: `want` when given (an autotuned shape), else about 16
+  // (halo slab overhead 1/16) but enough chunks to put ~2 workgroups per CU
+  // across the right-hand sides; at most kMaxPartials workgroups per
+  // right-hand side either way
   int TI = 16;
-  while (TI > 2 && (long)tiles * ((g.nx + TI - 1) / TI) * R < 512) TI /= 2;
+  if (want > 0)
+    TI = want;
+  else
+    while (TI > 2 && (long)tiles * ((g.nx + TI - 1) / TI) * R < 512) TI /= 2;
   if (TI > g.nx) TI = g.nx;
   while ((long)tiles * ((g.nx + TI - 1) / TI) > kMaxPartials) TI++;
   return TI;
 }
 
-int tiled_parts(const Dims &g, int R) {
+int tiled_slabs(const Dims &g, int R, int want) { return tiled_chunk(g, R, want); }
+
+int tiled_parts(const Dims &g, int R, int want) {
   const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
-  const int TI = tiled_chunk(g, R);
+  const int TI = tiled_chunk(g, R, want);
   return tiles * ((g.nx + TI - 1) / TI);
 }
 
@@ -435,9 +441,9 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
 
 template <typename T, int ST, int EP>
 static void tiled_go(const Dims &g, int R, const T *E, const ModeT<T> &mc, const OpArgs<T> &a,
-                     hipStream_t s) {
+                     hipStream_t s, int want) {
   const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
-  const int TI = tiled_chunk(g, R);
+  const int TI = tiled_chunk(g, R, want);
   const int tiles = tiles_j * tiles_k, chunks = (g.nx + TI - 1) / TI;
   if (R % kRHS) throw std::runtime_error("right-hand sides must come in groups of six");
   hipLaunchKernelGGL((k_tiled<T, ST, EP>), dim3(tiles * chunks * R), dim3(kBlock), 0, s, g, E, mc, a,
@@ -446,7 +452,7 @@ static void tiled_go(const Dims &g, int R, const T *E, const ModeT<T> &mc, const
 
 template <typename T>
 void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCoef &m,
-                  const OpArgs<T> &a, hipStream_t s) {
+                  const OpArgs<T> &a, hipStream_t s, int slabs) {
   ModeT<T> mc;
   for (int c = 0; c < 3; c++)
     for (int v = 0; v < 8; v++) {
@@ -455,7 +461,7 @@ void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCo
     }
 #define GO(S_, E_)                                 \
   if (st == S_ && ep == E_) {                      \
-    tiled_go<T, S_, E_>(g, R, E, mc, a, s);        \
+    tiled_go<T, S_, E_>(g, R, E, mc, a, s, slabs); \
     return;                                        \
   }
   GO(ST_PLAIN, EP_AX)
@@ -801,7 +807,7 @@ void launch_add2(int V, int R, const T *xn, const T *d, T *x, const int *act, hi
 
 #define FEMH_OPS_INST(T)                                                                        \
   template void launch_tiled<T>(int, int, const Dims &, int, const T *, const ModeCoef &,       \
-                                const OpArgs<T> &, hipStream_t);                                \
+                                const OpArgs<T> &, hipStream_t, int);                           \
   template void launch_stencil<T>(int, int, const Dims &, int, const T *, const OpArgs<T> &,    \
                                   hipStream_t);                                                 \
   template void launch_add2<T>(int, int, const T *, const T *, T *, const int *, hipStream_t);

@@ -63,11 +63,15 @@ struct OpArgs {
   T c1[kMaxB] = {}, c2[kMaxB] = {};
 };
 
-// fine level (tiled cube-mode voxel kernel); partial count of a dot launch
+// fine level (tiled cube-mode voxel kernel).  slabs: node planes per
+// workgroup (0: the built-in choice; an autotuned value otherwise).
+// tiled_parts: partial count of a dot launch with that shape; tiled_slabs:
+// the planes per workgroup actually used
 template <typename T>
 void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCoef &mc,
-                  const OpArgs<T> &a, hipStream_t s);
-int tiled_parts(const Dims &g, int R);
+                  const OpArgs<T> &a, hipStream_t s, int slabs = 0);
+int tiled_parts(const Dims &g, int R, int slabs = 0);
+int tiled_slabs(const Dims &g, int R, int slabs = 0);
 
 // coarse level (27-point block stencil, one RHS per thread)
 template <typename T>

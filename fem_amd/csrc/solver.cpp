@@ -14,9 +14,12 @@
 #include "solver.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace femh {
 
@@ -164,7 +167,7 @@ void Context::op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a) {
                               a.dot ? &a.red : nullptr, a.slot, stream);
         if (a.ydinv) launch_scale_by_dinv_norm<T>(g0.V, R, a.y, a.Dinv, a.act, red, stream);
       } else {
-        launch_tiled<T>(st, ep, g0, R, h.E.template as<T>(), modecoef, a, stream);
+        launch_tiled<T>(st, ep, g0, R, h.E.template as<T>(), modecoef, a, stream, shape_fine);
       }
     };
     if (prof) {
@@ -235,7 +238,7 @@ int Context::residual_dots(const int *act_) {
     a.slot3 = S_XR;
     a.dot = 1;
     op<double>(h64, 0, ST_PLAIN, EP_RES, a);
-    return tiled_parts(g, R);
+    return tiled_parts(g, R, shape_fine);
   }
   apply<double>(h64, 0, x.as<double>(), r.as<double>(), b.as<double>(), 1, act_, nullptr, 0);
   launch_dot<double>(g.V, R, r.as<double>(), r.as<double>(), S_RR, act_, red, stream);
@@ -244,7 +247,9 @@ int Context::residual_dots(const int *act_) {
   return vec_parts(g.V);
 }
 
-int Context::apply_parts() const { return o.apply_kernel == 1 ? dense_parts(g) : tiled_parts(g, R); }
+int Context::apply_parts() const {
+  return o.apply_kernel == 1 ? dense_parts(g) : tiled_parts(g, R, shape_fine);
+}
 
 template <typename T>
 void Context::apply(Hierarchy<T> &h, int l, const T *u, T *y, const T *bb, int mode,
@@ -430,6 +435,104 @@ void Context::build_ops(Hierarchy<T> &h) {
   FEMH_HIP(hipGetLastError());
 }
 
+// Launch-shape autotuner.  For each (grid, right-hand sides, precision,
+// levels) it times the fine operator (y = A u on every right-hand side) for a
+// few node-plane depths per workgroup, and the grid transfers (restriction
+// plus prolongation) of every level pair for a few planes per thread, on the
+// context's own buffers and stream, and keeps the fastest.  A candidate
+// within 2% of the built-in choice leaves the built-in choice in place.
+// Results are cached for the process, so later contexts of the same shape
+// take them without timing.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::array<int, 7>, std::vector<int>> g_tuned;
+}  // namespace
+
+template <typename T>
+void Context::autotune(Hierarchy<T> &h) {
+  const int L = (int)h.lv.size();
+  const std::array<int, 7> key{g.nx, g.ny, g.nz, R, (int)sizeof(T), L, o.apply_kernel};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      shape_fine = it->second[0];
+      shape_xfer.assign(it->second.begin() + 1, it->second.end());
+      return;
+    }
+  }
+  const bool was_prof = prof;
+  prof = false;
+  std::vector<int> all(R, 1);
+  set_act(A_POW, all);
+  hipEvent_t e0, e1;
+  FEMH_HIP(hipEventCreate(&e0));
+  FEMH_HIP(hipEventCreate(&e1));
+  auto time_ms = [&](auto &&body, int reps) {
+    body();  // warm
+    FEMH_HIP(hipEventRecord(e0, stream));
+    for (int k = 0; k < reps; k++) body();
+    FEMH_HIP(hipEventRecord(e1, stream));
+    FEMH_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    FEMH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return (double)ms / reps;
+  };
+  auto pick = [](const std::vector<int> &cand, const std::vector<double> &t) {
+    // cand[0] is the built-in choice (0)
+    size_t best = 0;
+    for (size_t i = 1; i < cand.size(); i++)
+      if (t[i] < t[best]) best = i;
+    return t[best] < 0.98 * t[0] ? cand[best] : 0;
+  };
+  for (int l = 0; l < L; l++) {
+    Level<T> &Lv = h.lv[l];
+    launch_fill_hash<T>((size_t)R * 3 * Lv.g.V, (size_t)kRHS * 3 * Lv.g.V, Lv.d.template as<T>(), 0x7A11ull + l,
+                        stream);
+  }
+  FEMH_HIP(hipGetLastError());
+  shape_fine = 0;
+  shape_xfer.assign(L > 1 ? L - 1 : 0, 0);
+  if (o.apply_kernel == 0) {
+    std::vector<int> cand{0};
+    const int base = tiled_slabs(g, R, 0);
+    for (int c : {2, 4, 8, 16, 32})
+      if (c <= g.nx && tiled_slabs(g, R, c) == c && c != base) cand.push_back(c);
+    std::vector<double> t;
+    for (int c : cand) {
+      shape_fine = c;
+      OpArgs<T> a;
+      a.u = h.lv[0].d.template as<T>();
+      a.y = h.lv[0].Ad.template as<T>();
+      a.act = actp(A_POW);
+      t.push_back(time_ms([&] { op<T>(h, 0, ST_PLAIN, EP_AX, a); }, 3));
+    }
+    shape_fine = pick(cand, t);
+  }
+  for (int l = 0; l + 1 < L; l++) {
+    Level<T> &F = h.lv[l], &C = h.lv[l + 1];
+    std::vector<int> cand{0, 1, 2, 4, 8};
+    std::vector<double> t;
+    for (int c : cand)
+      t.push_back(time_ms(
+          [&] {
+            launch_restrict<T>(F.g, C.g, R, F.d.template as<T>(), C.Ad.template as<T>(), actp(A_POW), stream, c);
+            launch_prolong_add<T>(F.g, C.g, R, C.d.template as<T>(), F.Ad.template as<T>(), actp(A_POW), stream,
+                                  nullptr, c);
+          },
+          3));
+    shape_xfer[l] = pick(cand, t);
+  }
+  FEMH_HIP(hipGetLastError());
+  FEMH_HIP(hipEventDestroy(e0));
+  FEMH_HIP(hipEventDestroy(e1));
+  prof = was_prof;
+  std::vector<int> val{shape_fine};
+  val.insert(val.end(), shape_xfer.begin(), shape_xfer.end());
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned.emplace(key, val);
+}
+
 void Context::build_hierarchy() {
   build_levels<double>(h64);
   nlevels = (int)h64.lv.size();
@@ -437,6 +540,7 @@ void Context::build_hierarchy() {
   FEMH_HIP(hipMemcpyAsync(h64.Ke.p, Ke64.p, sizeof(double) * 576, hipMemcpyDeviceToDevice, stream));
   h64_ops = false;
   if (o.precision != FEMH_MIXED) {
+    if (o.autotune) autotune<double>(h64);
     build_ops<double>(h64);
     h64_ops = true;
     if (o.precond == FEMH_PRECOND_MG) {
@@ -451,6 +555,7 @@ void Context::build_hierarchy() {
     build_levels<float>(h32);
     launch_convert<double, float>((size_t)nb * g.V, E64.as<double>(), h32.E.as<float>(), stream);
     launch_convert<double, float>(576, Ke64.as<double>(), h32.Ke.as<float>(), stream);
+    if (o.autotune) autotune<float>(h32);
     build_ops<float>(h32);
     if (o.precond == FEMH_PRECOND_MG) {
       if (nlevels > 1) {
@@ -583,7 +688,8 @@ bool Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int 
   if (k == 1 && l == 0 && R > kRHS) {
     // degree 1 at the fine level: xn = x + P xc out of place, then one launch
     // leaves the smoothed x (and, fused, the partials of b . x = r . z)
-    launch_prolong_add<T>(L.g, C.g, R, C.x.template as<T>(), xx, act_, stream, L.xn.template as<T>());
+    launch_prolong_add<T>(L.g, C.g, R, C.x.template as<T>(), xx, act_, stream, L.xn.template as<T>(),
+                          xfer_shape(l));
     a.u = L.xn.template as<T>();
     a.x = xx;
     a.fin1 = 1;
@@ -603,7 +709,7 @@ bool Context::post_smooth(Hierarchy<T> &h, int l, const T *bb, T *xx, const int 
     a.gc = C.g;
     op<T>(h, l, ST_PROLONG, EP_CHEB0, a);  // x3 = x + P xc staged, kept in xn
   } else {
-    launch_prolong_add<T>(L.g, C.g, R, C.x.template as<T>(), xx, act_, stream);
+    launch_prolong_add<T>(L.g, C.g, R, C.x.template as<T>(), xx, act_, stream, nullptr, xfer_shape(l));
     op<T>(h, l, ST_PLAIN, EP_CHEB0, a);
   }
   if (k == 1) {
@@ -664,7 +770,7 @@ bool Context::vcycle(Hierarchy<T> &h, int l, const T *bb, T *xx, const int *act_
     pre_smooth<T>(h, l, bb, xx, act_);
     apply<T>(h, l, xx, L.r.template as<T>(), bb, 1, act_, nullptr, 0);
   }
-  launch_restrict<T>(L.g, C.g, R, L.r.template as<T>(), C.b.template as<T>(), act_, stream);
+  launch_restrict<T>(L.g, C.g, R, L.r.template as<T>(), C.b.template as<T>(), act_, stream, xfer_shape(l));
   vcycle<T>(h, l + 1, C.b.template as<T>(), C.x.template as<T>(), act_, -1);
   return post_smooth<T>(h, l, bb, xx, act_, dot_slot);
 }
@@ -693,7 +799,7 @@ bool Context::fuses_rz(const Hierarchy<T> &h) const {
 
 template <typename T>
 int Context::rz_parts(const Hierarchy<T> &h) const {
-  return fuses_rz(h) ? tiled_parts(g, R) : vec_parts(g.V);
+  return fuses_rz(h) ? tiled_parts(g, R, shape_fine) : vec_parts(g.V);
 }
 
 // PCG over all right-hand sides flagged in `active` (updated in place as

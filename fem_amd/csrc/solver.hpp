@@ -118,6 +118,14 @@ struct Context {
   std::vector<int> fallback_;  // per sample: finished in FP64 this call
   bool modulus_input = false;  // last load gave moduli (dE/drho = 1)
   int nlevels = 1;
+  // launch shapes (0: the built-in choice): node planes per workgroup of the
+  // fine operator, coarse planes per thread of the transfers between level l
+  // and l + 1; set by the autotuner
+  int shape_fine = 0;
+  std::vector<int> shape_xfer;
+  int xfer_shape(int l) const { return l < (int)shape_xfer.size() ? shape_xfer[l] : 0; }
+  template <typename T>
+  void autotune(Hierarchy<T> &h);
 
   // kernel timing
   struct KStat {

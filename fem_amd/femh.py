@@ -27,7 +27,7 @@ EXPORTS = ("femh_abi_version", "femh_default_opts", "femh_create", "femh_destroy
            "femh_homogenize", "femh_homogenize_modulus", "femh_homogenize_device",
            "femh_homogenize_many", "femh_homogenize_many_modulus", "femh_homogenize_many_device",
            "femh_gradient", "femh_energy_terms", "femh_get_fields", "femh_thermal_expansion",
-           "femh_homogenize_batch",
+           "femh_launch_shapes", "femh_homogenize_batch",
            "femh_prepare", "femh_apply_level", "femh_level_dims", "femh_set_profiling",
            "femh_kernel_stats", "femh_element_stiffness", "femh_last_error")
 
@@ -50,7 +50,7 @@ class Opts(ctypes.Structure):
                 ("inner_tol", ctypes.c_double), ("max_outer", ctypes.c_int),
                 ("apply_kernel", ctypes.c_int), ("batch", ctypes.c_int),
                 ("coarse_nu", ctypes.c_int), ("warm_start", ctypes.c_int),
-                ("structure", ctypes.c_int)]
+                ("structure", ctypes.c_int), ("autotune", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -94,6 +94,7 @@ def lib():
         L.femh_energy_terms.argtypes = [vp, vp]
         L.femh_get_fields.argtypes = [vp, vp]
         L.femh_thermal_expansion.argtypes = [vp, vp, vp, vp]
+        L.femh_launch_shapes.argtypes = [vp, vp, ctypes.c_int]
         L.femh_homogenize_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.femh_prepare.argtypes = [vp, vp]
         L.femh_apply_level.argtypes = [vp, ctypes.c_int, vp, vp]
@@ -114,7 +115,7 @@ def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond
               levels=0, nu1=2, nu2=2, cheby_ratio=20.0, coarse_iters=64, tol=1e-10,
               max_iter=5000, precision="fp64", ch_tol=0.0, check_every=1, device=0,
               inner_tol=1e-4, max_outer=12, apply_kernel="tiled", batch=1, warm_start=False,
-              coarse_nu=3, structure="general"):
+              coarse_nu=3, structure="general", autotune=False):
     o = Opts()
     lib().femh_default_opts(ctypes.byref(o))
     o.nx, o.ny, o.nz = (int(s) for s in shape)
@@ -132,6 +133,7 @@ def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond
     o.warm_start = int(bool(warm_start))
     o.coarse_nu = coarse_nu
     o.structure = {"general": 0, "cubic": 1}[structure]
+    o.autotune = int(bool(autotune))
     return o
 
 
@@ -250,6 +252,15 @@ class Homogenizer:
         if nb == 1:
             return beta, aeff
         return beta.reshape(nb, 6), aeff.reshape(nb, 6)
+
+    def launch_shapes(self):
+        """Launch shapes in use: [fine-operator planes per workgroup, transfer
+        planes per thread of each level pair]; 0 = the built-in choice."""
+        buf = (ctypes.c_int * 16)()
+        n = lib().femh_launch_shapes(self._h, buf, 16)
+        if n < 0:
+            self._check(-n)
+        return [buf[i] for i in range(min(n, 16))]
 
     def set_profiling(self, on=True):
         self._check(lib().femh_set_profiling(self._h, int(bool(on))))

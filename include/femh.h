@@ -113,6 +113,13 @@ typedef struct femh_opts {
                            ran).  The reference's CUBIC mode
                            (NumericalCoarseningUsingMultiGrid.cpp:275-282) makes the
                            same two solves */
+  int autotune;         /* 1: before the first solve of each (grid, batch, precision)
+                           shape in the process, time the fine operator and the grid
+                           transfers for a few launch shapes (node planes per
+                           workgroup, planes per thread) and keep the fastest; the
+                           choice is cached for the process.  The fine-operator
+                           shape changes the order of the dot-product partial sums,
+                           so results move at rounding level.  0: built-in shapes */
 } femh_opts;
 
 typedef struct femh_stats {
@@ -185,6 +192,13 @@ int femh_get_fields(struct femh_ctx *ctx, double *u);
  * discretisation (reciprocity with the six elastic fields).  Returns
  * FEMH_NO_FIELDS when no solve has been done. */
 int femh_thermal_expansion(struct femh_ctx *ctx, const double *alpha, double *beta, double *alpha_eff);
+
+/* Launch shapes in use (after a solve; 0 = the built-in choice):
+ * shapes[0] = node planes per workgroup of the fine operator, shapes[1 + l] =
+ * coarse planes per thread of the transfers between levels l and l + 1.
+ * Writes at most n values; returns their count (levels) or a negative status.
+ * No counterpart in the reference, whose CPU loops have no launch shapes. */
+int femh_launch_shapes(struct femh_ctx *ctx, int *shapes, int n);
 
 /* Batch: n samples, sample s has dims[3s..3s+2] and modulus grid E[s] (host).
  * C receives 36*n doubles.  Material nu/Emin from the context options.
