@@ -58,6 +58,8 @@ def parse():
                     help="simp64 only: cubic symmetrises each cell over the cubic group and "
                          "solves it in the cubic-structure mode (2 of the 6 load cases); the FP64 "
                          "reference stays the six-case solve")
+    ap.add_argument("--check-every", type=int, default=1,
+                    help="PCG iterations between convergence checks (each check is a host sync)")
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: launch shapes picked by the engine's autotuner (timed in the warm-up)")
     ap.add_argument("--cells", type=int, default=0, help="cells of uneven512 / opt96 (0: 512 / 64)")
@@ -147,7 +149,7 @@ def main():
     mixed = Homogenizer(shape, nu=0.3, penal=3.0, precision="mixed", tol=1e-10,
                         ch_tol=a.ch_tol, inner_tol=a.inner_tol, batch=B, nu1=a.nu1, nu2=a.nu2,
                         cheby_ratio=a.cheby_ratio, coarse_nu=a.coarse_nu, structure=a.structure,
-                        autotune=bool(a.autotune))
+                        autotune=bool(a.autotune), check_every=a.check_every)
 
     def step(k, stats=None):
         st = mixed.homogenize_many_device(B, d_rho[k * B].data_ptr(), d_C[k * B].data_ptr())
@@ -254,7 +256,7 @@ def run_uneven(a):
     count = a.cells or 512
     grids = W.batch_uneven(count, 3)
     kw = dict(nu=0.3, precision="mixed", tol=1e-10, ch_tol=a.ch_tol, inner_tol=a.inner_tol,
-              max_iter=20000, batch=a.batch, autotune=bool(a.autotune))
+              max_iter=20000, batch=a.batch, autotune=bool(a.autotune), check_every=a.check_every)
     # warm-up: builds and loads, one cell of every size (and, with --autotune,
     # times the launch shapes of every size)
     first = {}
