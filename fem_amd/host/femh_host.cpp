@@ -448,6 +448,55 @@ void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParam
   }
 }
 
+void NumericalCoarsening::remember(int N[3], int rep, int sub, const std::vector<BaseMaterial> &mats) {
+  m_key[0] = N[0];
+  m_key[1] = N[1];
+  m_key[2] = N[2];
+  m_key[3] = rep;
+  m_key[4] = sub;
+  m_keyMats = mats;
+}
+
+void NumericalCoarsening::ensure(int N[3], int rep, int sub, const std::vector<BaseMaterial> &mats) {
+  bool same = m_ctx && m_key[0] == N[0] && m_key[1] == N[1] && m_key[2] == N[2] && m_key[3] == rep &&
+              m_key[4] == sub && m_keyMats.size() == mats.size();
+  for (size_t i = 0; same && i < mats.size(); i++)
+    same = m_keyMats[i].mu == mats[i].mu && m_keyMats[i].lambda == mats[i].lambda;
+  if (!same) init(N, rep, sub, mats);
+}
+
+Mat6 NumericalCoarsening::computeCoarsenedElasticityTensor(const std::vector<int> &iMaterials, int N[3], int rep,
+                                                          int sub, const std::vector<BaseMaterial> &mats) {
+  ensure(N, rep, sub, mats);
+  std::vector<double> tv, pv;
+  computeCoarsenedElasticityTensorAndParameters3D(iMaterials, N, {0.0, 1.0}, General, tv, pv);
+  return m_C;
+}
+
+void NumericalCoarsening::computeCoarsenedElasticityTensorAndParameters(
+    const std::vector<int> &iMaterials, int N[3], int rep, int sub, const std::vector<BaseMaterial> &mats,
+    StructureType iType, std::vector<double> &ioTensorValues, std::vector<double> &ioParameters) {
+  ensure(N, rep, sub, mats);
+  std::vector<double> pv;
+  computeCoarsenedElasticityTensorAndParameters3D(iMaterials, N, {0.0, 1.0}, General, ioTensorValues, pv);
+  // compliance columns: strains under a unit uniaxial stress along x, y, z
+  const Mat6 Sfull = inverse6(m_C);
+  double S[3][3];  // S[a][b]: strain b under a unit stress along a
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) S[a][b] = Sfull(b, a);
+  double ratio = 0;
+  for (int m : iMaterials) ratio += m == 1 ? 1.0 : 0.0;
+  ioParameters.push_back(iMaterials.empty() ? 0.0 : ratio / (double)iMaterials.size());
+  const int naxis = iType == Cubic ? 1 : 3;
+  for (int a = 0; a < naxis; a++) ioParameters.push_back(1.0 / S[a][a]);
+  ioParameters.push_back(-S[0][1] / S[0][0]);
+  if (iType != Cubic) {
+    ioParameters.push_back(-S[0][2] / S[0][0]);
+    ioParameters.push_back(-S[1][2] / S[1][1]);
+  }
+  for (int a = 0; a < naxis; a++) ioParameters.push_back(m_C(3 + a, 3 + a));
+}
+
 // ---- caller pipeline files ------------------------------------------------------
 namespace cfgbin {
 namespace {

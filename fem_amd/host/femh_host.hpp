@@ -211,7 +211,30 @@ class NumericalCoarsening : public NumericalCoarseningUsingMultigrid {
  public:
   void init(int N[3], int iNbBlockRep, int iNbSubdiv, const std::vector<BaseMaterial> &iBaseMaterials) {
     NumericalCoarseningUsingMultigrid::init(N, iNbBlockRep, iNbSubdiv, iBaseMaterials, General);
+    remember(N, iNbBlockRep, iNbSubdiv, iBaseMaterials);
   }
+  // C_H of one assignment (NumericalCoarsening.cpp:577-596); initialises for
+  // this grid and these materials when they differ from the last call
+  Mat6 computeCoarsenedElasticityTensor(const std::vector<int> &iMaterials, int N[3], int iNbBlockRep,
+                                        int iNbSubdiv, const std::vector<BaseMaterial> &iBaseMaterials);
+  // 21 values of C_H and the parameters of the reference's force-driven path
+  // (:486-552), read from the compliance S = C_H^-1 (its columns are the
+  // strains under a unit uniaxial stress): the material-1 ratio, Young's
+  // moduli 1/S_ii (x only for Cubic), nu_xy = -S_10/S_00 (and, unless Cubic,
+  // nu_xz = -S_20/S_00, nu_yz = -S_21/S_11), shear moduli C_33 (and C_44,
+  // C_55 unless Cubic)
+  void computeCoarsenedElasticityTensorAndParameters(const std::vector<int> &iMaterials, int N[3],
+                                                     int iNbBlockRep, int iNbSubdiv,
+                                                     const std::vector<BaseMaterial> &iBaseMaterials,
+                                                     StructureType iType, std::vector<double> &ioTensorValues,
+                                                     std::vector<double> &ioParameters);
+  using NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParameters3D;
+
+ private:
+  int m_key[5] = {0, 0, 0, 0, 0};
+  std::vector<BaseMaterial> m_keyMats;
+  void remember(int N[3], int rep, int sub, const std::vector<BaseMaterial> &mats);
+  void ensure(int N[3], int rep, int sub, const std::vector<BaseMaterial> &mats);
 };
 
 // replicate a coarse assignment rep times per axis with each cell split into

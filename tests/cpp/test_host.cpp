@@ -76,6 +76,26 @@ int main() {
     expect(par.size() == 4 && std::fabs(par[1] - E1) < 1e-9 * E1 && std::fabs(par[2] - nu1) < 1e-9,
            "coarsening: cubic parameters recover (E, nu)", std::fabs(par[1] - E1) / E1);
   }
+  {  // NumericalCoarsening's tensor-only and force-path parameter calls on a solid cell
+    NumericalCoarsening ncf;
+    int N[3] = {4, 4, 4};
+    std::vector<int> m(64, 1);
+    const Mat6 C = ncf.computeCoarsenedElasticityTensor(m, N, 1, 2, mats);
+    const Mat6 Ci = c_iso(E1, nu1);
+    double err = 0;
+    for (int q = 0; q < 36; q++) err = std::fmax(err, std::fabs(C.a[q] - Ci.a[q]) / Ci(0, 0));
+    expect(err < 1e-12, "coarsening: computeCoarsenedElasticityTensor of a solid cell is C_iso", err);
+    std::vector<double> tv, pv;
+    ncf.computeCoarsenedElasticityTensorAndParameters(m, N, 1, 2, mats, NumericalCoarsening::General, tv, pv);
+    const double mu1 = mats[1].mu;
+    // ratio, E x3, nu_xy, nu_xz, nu_yz, G x3
+    double perr = std::fabs(pv[0] - 1.0);
+    for (int a = 0; a < 3; a++) perr = std::fmax(perr, std::fabs(pv[1 + a] - E1) / E1);
+    for (int a = 0; a < 3; a++) perr = std::fmax(perr, std::fabs(pv[4 + a] - nu1));
+    for (int a = 0; a < 3; a++) perr = std::fmax(perr, std::fabs(pv[7 + a] - mu1) / mu1);
+    expect(tv.size() == 21 && pv.size() == 10 && perr < 1e-10,
+           "coarsening: force-path parameters (ratio, E, nu, G) of a solid cell", perr);
+  }
   {  // a cubic-symmetric pattern: symmetric, cubic invariants, reference triangle quirk
     NumericalCoarseningUsingMultigrid nc;
     int N[3] = {4, 4, 4};
