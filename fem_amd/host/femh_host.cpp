@@ -173,6 +173,35 @@ void FEM3DFun::setParam(const VectorXd &x0) {
   density = 0;
   for (double d : distribution) density += d;
   density /= (double)distribution.size();
+  const double dx = 1.0 / (double)std::max(nx, std::max(ny, nz));
+  if (Ke0.empty()) {
+    Ke0.assign(1, VectorXd(576));
+    check(nullptr, femh_element_stiffness(1.0, nu, Ke0[0].data()), "femh_element_stiffness");
+    for (double &v : Ke0[0]) v *= dx;  // 3D element stiffness scales with the cell size
+  }
+  if (!keepDisplacements) return;
+  // copyPeriodicU: the fluctuation repeats on the wrapped nodes and the
+  // affine part continues across them
+  const size_t V = (size_t)nx * ny * nz;
+  VectorXd chi(18 * V);
+  check(ctx_, femh_get_fields(ctx_, chi.data()), "femh_get_fields");
+  static const double eps[6][3][3] = {
+      {{1, 0, 0}, {0, 0, 0}, {0, 0, 0}},     {{0, 0, 0}, {0, 1, 0}, {0, 0, 0}},
+      {{0, 0, 0}, {0, 0, 0}, {0, 0, 1}},     {{0, 0, 0}, {0, 0, .5}, {0, .5, 0}},
+      {{0, 0, .5}, {0, 0, 0}, {.5, 0, 0}},   {{0, .5, 0}, {.5, 0, 0}, {0, 0, 0}}};
+  const int Nx = nx + 1, Ny = ny + 1, Nz = nz + 1;
+  u.assign(6, VectorXd((size_t)3 * Nx * Ny * Nz));
+  for (int lc = 0; lc < 6; lc++)
+    for (int i = 0; i < Nx; i++)
+      for (int j = 0; j < Ny; j++)
+        for (int k = 0; k < Nz; k++) {
+          const size_t m = ((size_t)(i % nx) * ny + (j % ny)) * nz + (k % nz);
+          const double X[3] = {(double)i, (double)j, (double)k};
+          for (int d = 0; d < 3; d++) {
+            const double aff = eps[lc][d][0] * X[0] + eps[lc][d][1] * X[1] + eps[lc][d][2] * X[2];
+            u[lc][3 * (((size_t)i * Ny + j) * Nz + k) + d] = dx * (aff + chi[(size_t)(3 * lc + d) * V + m]);
+          }
+        }
 }
 
 double FEM3DFun::f() {
@@ -728,6 +757,7 @@ bool MaterialOptimizer::run3D(int N[3], const std::vector<BaseMaterial> &mats, c
   pc->allocate(px, py, pz);
   fem.field = field;
   fem.gridSize = {nx, ny, nz};
+  fem.keepDisplacements = false;  // the loop reads G only
   fem.lowerBounds.assign(field->param.size(), 1e-3);
   fem.upperBounds.assign(field->param.size(), 1.0);
   // start from the shrunk 0/1 assignment (the octant for symmetric fields)

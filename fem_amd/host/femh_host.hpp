@@ -125,6 +125,16 @@ class FEM3DFun : public RealFun {
   femh_opts solverOpts;       // knobs of the engine (grid/material fields are set here)
   RealField *field = nullptr;  // not owned
   VectorXd distribution;
+  // total displacements of the 6 load cases after setParam, on the
+  // (n+1)^3 nodes of the unit cell, 3 * ((i (n+1) + j) (n+1) + k) + d, in
+  // cell coordinates X = dx (i, j, k) with dx = 1 / n: u = dx (eps X / dx +
+  // chi(i mod n, j mod n, k mod n)), the layout of copyPeriodicU
+  // (fast_fem/FEM3DFun.cpp:31-55, :157-159).  Filled when keepDisplacements.
+  std::vector<VectorXd> u;
+  bool keepDisplacements = true;
+  // base-material element stiffness for cells of size dx (E = 1): Ke0[0],
+  // 24 x 24 row-major, corner order z fastest, 3 dofs per corner
+  std::vector<VectorXd> Ke0;
 
   void initArrays();
   void init(const VectorXd &x0) override;

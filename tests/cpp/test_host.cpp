@@ -148,6 +148,32 @@ int main() {
     for (int i = 0; i < 64; i++) x0[i] = 0.3 + 0.6 * std::fabs(std::sin(1.7 * i + 0.3));
     fem.init(x0);
     fem.setParam(x0);
+    {  // the reference's own energy loop over u and Ke0 (fast_fem/FEM3DFun.cpp:160-177)
+      const int n = 8, N1 = n + 1;
+      double gerr = 0;
+      Mat6 Gr;
+      for (int c = 0; c < n * n * n; c++) {
+        const int i = c / (n * n), j = (c / n) % n, k = c % n;
+        const double dist = fem.distribution[c], coef = dist / (3 - 2 * dist);
+        double ue[6][24];
+        for (int lc = 0; lc < 6; lc++)
+          for (int v = 0; v < 8; v++)
+            for (int d = 0; d < 3; d++) {
+              const int vi = i + ((v >> 2) & 1), vj = j + ((v >> 1) & 1), vk = k + (v & 1);
+              ue[lc][3 * v + d] = fem.u[lc][3 * ((vi * N1 + vj) * N1 + vk) + d];
+            }
+        for (int a = 0; a < 6; a++)
+          for (int b = a; b < 6; b++) {
+            double e = 0;
+            for (int r = 0; r < 24; r++)
+              for (int q = 0; q < 24; q++) e += ue[a][r] * fem.Ke0[0][24 * r + q] * ue[b][q];
+            Gr(a, b) += coef * e;
+          }
+      }
+      for (int q = 0; q < 36; q++) gerr = std::fmax(gerr, std::fabs(Gr.a[q] - fem.G.a[q]) / fem.G(0, 0));
+      expect(fem.u.size() == 6 && fem.u[0].size() == 3u * N1 * N1 * N1 && gerr < 1e-10,
+             "FEM3DFun: u and Ke0 in the reference layout reproduce G", gerr);
+    }
     fem.G0 = fem.G;
     fem.G0(0, 0) *= 1.3;
     fem.G0(0, 1) *= 0.5;
