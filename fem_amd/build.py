@@ -49,7 +49,7 @@ def build(verbose=False):
             print(out)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     build_host(objdir)
     return LIB

@@ -1,9 +1,13 @@
 // capi.cpp -- extern "C" entry points declared in include/femh.h.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <memory>
 #include <new>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -213,11 +217,11 @@ int femh_homogenize_batch(femh_ctx *ctx, int n, const int *dims, const double *c
   return guarded(ctx, [&] {
     // group the samples by grid size; each group runs through a batched
     // sub-context, `batch` samples per launch sequence
+    const auto t0 = std::chrono::steady_clock::now();
     std::map<std::tuple<int, int, int>, std::vector<int>> groups;
     for (int s = 0; s < n; s++)
       groups[std::make_tuple(dims[3 * s], dims[3 * s + 1], dims[3 * s + 2])].push_back(s);
-    int maxit = 0;
-    double ms = 0;
+    std::vector<std::pair<femh::Context *, const std::vector<int> *>> work;
     for (auto &kv : groups) {
       auto it = ctx->sub.find(kv.first);
       if (it == ctx->sub.end()) {
@@ -229,29 +233,52 @@ int femh_homogenize_batch(femh_ctx *ctx, int n, const int *dims, const double *c
         if (o.batch < 1) o.batch = 1;
         it = ctx->sub.emplace(kv.first, std::unique_ptr<femh::Context>(new femh::Context(o))).first;
       }
-      femh::Context &c = *it->second;
-      const std::vector<int> &ids = kv.second;
-      const size_t V = (size_t)c.g.V;
-      std::vector<double> buf;
-      std::vector<double> Cg(36 * (size_t)c.B);
-      std::vector<femh_stats> st(c.B);
-      for (size_t s0 = 0; s0 < ids.size(); s0 += c.B) {
-        const int k = (int)std::min<size_t>(c.B, ids.size() - s0);
-        buf.resize((size_t)k * V);
-        for (int j = 0; j < k; j++) std::memcpy(buf.data() + j * V, E[ids[s0 + j]], sizeof(double) * V);
-        c.load(buf.data(), k, false, true);
-        c.homogenize(Cg.data(), st.data());
-        for (int j = 0; j < k; j++) {
-          std::memcpy(C + 36 * (size_t)ids[s0 + j], Cg.data() + 36 * j, sizeof(double) * 36);
-          for (int q = 0; q < 6; q++) maxit = std::max(maxit, st[j].iters[q]);
-        }
-        ms += st[0].ms_total;
-      }
+      work.emplace_back(it->second.get(), &kv.second);
     }
+    // The groups run concurrently, a few host threads pulling them in turn,
+    // each sub-context on its own stream: one group of small cells leaves
+    // most of the GPU idle, and a slow cell (an FP64 fallback) no longer holds
+    // the other groups back.  Every group writes its own rows of C.
+    std::vector<int> maxit(work.size(), 0);
+    std::vector<std::exception_ptr> errs(work.size());
+    std::atomic<size_t> next{0};
+    auto worker = [&] {
+      for (size_t w = next++; w < work.size(); w = next++) {
+        try {
+          femh::Context &c = *work[w].first;
+          const std::vector<int> &ids = *work[w].second;
+          FEMH_HIP(hipSetDevice(c.o.device));
+          const size_t V = (size_t)c.g.V;
+          std::vector<double> buf;
+          std::vector<double> Cg(36 * (size_t)c.B);
+          std::vector<femh_stats> st(c.B);
+          for (size_t s0 = 0; s0 < ids.size(); s0 += c.B) {
+            const int k = (int)std::min<size_t>(c.B, ids.size() - s0);
+            buf.resize((size_t)k * V);
+            for (int j = 0; j < k; j++) std::memcpy(buf.data() + j * V, E[ids[s0 + j]], sizeof(double) * V);
+            c.load(buf.data(), k, false, true);
+            c.homogenize(Cg.data(), st.data());
+            for (int j = 0; j < k; j++) {
+              std::memcpy(C + 36 * (size_t)ids[s0 + j], Cg.data() + 36 * j, sizeof(double) * 36);
+              for (int q = 0; q < 6; q++) maxit[w] = std::max(maxit[w], st[j].iters[q]);
+            }
+          }
+        } catch (...) {
+          errs[w] = std::current_exception();
+        }
+      }
+    };
+    const size_t nthreads = std::min<size_t>(work.size(), 8);
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nthreads; t++) pool.emplace_back(worker);
+    if (nthreads) worker();
+    for (auto &t : pool) t.join();
+    for (auto &e : errs)
+      if (e) std::rethrow_exception(e);  // the first group (by size) that failed
     if (stats) {
       std::memset(stats, 0, sizeof(*stats));
-      stats->iters[0] = maxit;
-      stats->ms_total = ms;
+      for (int m : maxit) stats->iters[0] = std::max(stats->iters[0], m);
+      stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
   });
 }
