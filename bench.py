@@ -60,6 +60,9 @@ def parse():
                          "reference stays the six-case solve")
     ap.add_argument("--check-every", type=int, default=1,
                     help="PCG iterations between convergence checks (each check is a host sync)")
+    ap.add_argument("--streams", type=int, default=3,
+                    help="simp64: batches solved at once, each on its own context and stream; a step "
+                         "is streams x batch cells")
     ap.add_argument("--autotune", type=int, default=0,
                     help="1: launch shapes picked by the engine's autotuner (timed in the warm-up)")
     ap.add_argument("--cells", type=int, default=0, help="cells of uneven512 / opt96 (0: 512 / 64)")
@@ -134,9 +137,11 @@ def main():
         return run_opt(a)
     n = a.n
     B = a.batch
+    S = max(1, a.streams)
+    cps = B * S  # cells per step
     shape = (n, n, n)
     nsteps = a.warmup + a.steps
-    nsamp = nsteps * B
+    nsamp = nsteps * cps
     rhos = np.stack([W.simp_density(n, seed, 4.0) for seed in range(nsamp)])
     if a.structure == "cubic":
         rhos = W.cubic_average(rhos)
@@ -149,10 +154,10 @@ def main():
     mixed = Homogenizer(shape, nu=0.3, penal=3.0, precision="mixed", tol=1e-10,
                         ch_tol=a.ch_tol, inner_tol=a.inner_tol, batch=B, nu1=a.nu1, nu2=a.nu2,
                         cheby_ratio=a.cheby_ratio, coarse_nu=a.coarse_nu, structure=a.structure,
-                        autotune=bool(a.autotune), check_every=a.check_every)
+                        autotune=bool(a.autotune), check_every=a.check_every, streams=S)
 
     def step(k, stats=None):
-        st = mixed.homogenize_many_device(B, d_rho[k * B].data_ptr(), d_C[k * B].data_ptr())
+        st = mixed.homogenize_many_device(cps, d_rho[k * cps].data_ptr(), d_C[k * cps].data_ptr())
         if stats is not None:
             stats.extend(st)
 
@@ -185,13 +190,13 @@ def main():
         ref = Homogenizer(shape, nu=0.3, penal=3.0, precision="fp64", tol=1e-10)
         ref.homogenize(rhos[0])
         t = time.perf_counter()
-        for s in range(a.warmup * B, nsamp):
+        for s in range(a.warmup * cps, nsamp):
             C64, _ = ref.homogenize(rhos[s])
             errs.append(float(np.linalg.norm(C_mixed[s] - C64) / np.linalg.norm(C64)))
-        fp64_s = (time.perf_counter() - t) / (nsamp - a.warmup * B)
+        fp64_s = (time.perf_counter() - t) / (nsamp - a.warmup * cps)
         ref.close()
 
-    value = a.steps * B / elapsed
+    value = a.steps * cps / elapsed
     dom = k32 if k32["launches"] else k64
     dom_name = ("k_tiled<float> (fine-level operator, all stage/epilogue variants)" if k32["launches"]
                 else "k_tiled<double> (fine-level operator, all stage/epilogue variants)")
@@ -228,8 +233,10 @@ def main():
                                 if a.structure == "general" else
                                 f"{n}^3 SIMP homogenization of cubic-symmetrised cells (GRF corr 4, p=3, "
                                 f"nu=0.3; xx and yz load cases solved, 4 by axis swaps), ")
-                               + f"mixed FP32/FP64 multigrid-PCG, {B} cells per step",
-                   "n": n, "batch": B, "cells_timed": a.steps * B, "ch_tol": a.ch_tol,
+                               + f"mixed FP32/FP64 multigrid-PCG, {cps} cells per step "
+                                 f"({S} concurrent batches of {B})",
+                   "n": n, "batch": B, "streams": S, "cells_per_step": cps,
+                   "cells_timed": a.steps * cps, "ch_tol": a.ch_tol,
                    "inner_tol": a.inner_tol, "autotune": a.autotune, "launch_shapes": shapes},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -21,6 +21,9 @@ struct femh_ctx {
   int last_status = FEMH_OK;
   // per-grid sub-contexts used by the uneven batch entry point
   std::map<std::tuple<int, int, int>, std::unique_ptr<femh::Context>> sub;
+  // peer contexts (opts.streams - 1 of them) for concurrent batches of a
+  // many-call
+  std::vector<std::unique_ptr<femh::Context>> peers;
 };
 
 namespace {
@@ -57,14 +60,40 @@ int need_ctx(femh_ctx *ctx) {
 }
 
 // n samples through a context in chunks of its batch capacity
-void run_many(femh::Context &c, int n, const double *src, bool device_src, bool modulus, double *C,
+// n cells in batches of B.  With opts.streams > 1 (and profiling off) the
+// batches are spread over that many contexts, each on its own stream and
+// host thread, so that one batch's host syncs and small coarse-level
+// launches overlap another's work.  Batch b goes to context
+// (nbatch - 1 - b) mod streams: the last batch runs on the primary context,
+// whose fields the gradient, field and thermal calls then read.
+void run_many(femh_ctx *ctx, int n, const double *src, bool device_src, bool modulus, double *C,
               femh_stats *stats) {
-  const size_t V = (size_t)c.g.V;
-  for (int s0 = 0; s0 < n; s0 += c.B) {
-    const int k = std::min(c.B, n - s0);
-    c.load(src + (size_t)s0 * V, k, device_src, modulus);
-    c.homogenize(C + 36 * (size_t)s0, stats ? stats + s0 : nullptr);
-  }
+  femh::Context &c0 = *ctx->c;
+  const size_t V = (size_t)c0.g.V;
+  const int B = c0.B, nbatch = (n + B - 1) / B;
+  const int S = c0.prof ? 1 : std::min(std::max(ctx->opts.streams, 1), nbatch);
+  while ((int)ctx->peers.size() < S - 1) ctx->peers.emplace_back(new femh::Context(ctx->opts));
+  std::vector<std::exception_ptr> errs(S);
+  auto work = [&](int w) {
+    try {
+      femh::Context &c = w == 0 ? c0 : *ctx->peers[w - 1];
+      if (w > 0) FEMH_HIP(hipSetDevice(c.o.device));
+      for (int b = 0; b < nbatch; b++) {
+        if ((nbatch - 1 - b) % S != w) continue;
+        const int s0 = b * B, k = std::min(B, n - s0);
+        c.load(src + (size_t)s0 * V, k, device_src, modulus);
+        c.homogenize(C + 36 * (size_t)s0, stats ? stats + s0 : nullptr);
+      }
+    } catch (...) {
+      errs[w] = std::current_exception();
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int w = 1; w < S; w++) pool.emplace_back(work, w);
+  work(0);
+  for (auto &t : pool) t.join();
+  for (auto &e : errs)
+    if (e) std::rethrow_exception(e);
 }
 
 }  // namespace
@@ -108,6 +137,7 @@ void femh_default_opts(femh_opts *o) {
   o->warm_start = 0;
   o->structure = FEMH_STRUCT_GENERAL;
   o->autotune = 0;
+  o->streams = 1;
   o->coarse_nu = 3;
 }
 
@@ -128,26 +158,26 @@ void femh_destroy(femh_ctx *ctx) { delete ctx; }
 int femh_homogenize(femh_ctx *ctx, const double *rho, double *C, femh_stats *stats) {
   if (int s = need_ctx(ctx)) return s;
   if (!rho || !C) return FEMH_BAD_ARG;
-  return guarded(ctx, [&] { run_many(*ctx->c, 1, rho, false, false, C, stats); });
+  return guarded(ctx, [&] { run_many(ctx, 1, rho, false, false, C, stats); });
 }
 
 int femh_homogenize_modulus(femh_ctx *ctx, const double *E, double *C, femh_stats *stats) {
   if (int s = need_ctx(ctx)) return s;
   if (!E || !C) return FEMH_BAD_ARG;
-  return guarded(ctx, [&] { run_many(*ctx->c, 1, E, false, true, C, stats); });
+  return guarded(ctx, [&] { run_many(ctx, 1, E, false, true, C, stats); });
 }
 
 int femh_homogenize_many(femh_ctx *ctx, int n, const double *rho, double *C, femh_stats *stats) {
   if (int s = need_ctx(ctx)) return s;
   if (n < 1 || !rho || !C) return FEMH_BAD_ARG;
-  return guarded(ctx, [&] { run_many(*ctx->c, n, rho, false, false, C, stats); });
+  return guarded(ctx, [&] { run_many(ctx, n, rho, false, false, C, stats); });
 }
 
 int femh_homogenize_many_modulus(femh_ctx *ctx, int n, const double *E, double *C,
                                  femh_stats *stats) {
   if (int s = need_ctx(ctx)) return s;
   if (n < 1 || !E || !C) return FEMH_BAD_ARG;
-  return guarded(ctx, [&] { run_many(*ctx->c, n, E, false, true, C, stats); });
+  return guarded(ctx, [&] { run_many(ctx, n, E, false, true, C, stats); });
 }
 
 static int many_device(femh_ctx *ctx, int n, const double *d_rho, double *d_C, void *stream,
@@ -157,7 +187,7 @@ static int many_device(femh_ctx *ctx, int n, const double *d_rho, double *d_C, v
   return guarded(ctx, [&] {
     FEMH_HIP(hipStreamSynchronize((hipStream_t)stream));
     std::vector<double> C(36 * (size_t)n);
-    run_many(*ctx->c, n, d_rho, true, false, C.data(), stats);
+    run_many(ctx, n, d_rho, true, false, C.data(), stats);
     FEMH_HIP(hipMemcpyAsync(d_C, C.data(), sizeof(double) * C.size(), hipMemcpyHostToDevice,
                             ctx->c->stream));
     FEMH_HIP(hipStreamSynchronize(ctx->c->stream));

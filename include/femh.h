@@ -120,6 +120,13 @@ typedef struct femh_opts {
                            choice is cached for the process.  The fine-operator
                            shape changes the order of the dot-product partial sums,
                            so results move at rounding level.  0: built-in shapes */
+  int streams;          /* many-calls with more than `batch` cells: batches run on
+                           this many contexts at once (each its own stream and host
+                           thread; streams - 1 extra contexts are created on first
+                           use).  Every cell's result is the same as with 1.  The
+                           last batch runs on the primary context, so femh_gradient
+                           and the field calls still refer to it.  Profiling runs
+                           the batches one at a time.  Default 1 */
 } femh_opts;
 
 typedef struct femh_stats {
