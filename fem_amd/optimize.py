@@ -14,6 +14,7 @@ sequence of up to `batch` cells, warm-started from the cells' previous fields;
 cells whose line search already succeeded (or gave up) leave the batch.
 """
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -43,29 +44,43 @@ class BatchedDescent:
                                 warm_start=True, **solver) for g in self.groups]
         self.solves = 0
         self.est = []
+        self._pool = ThreadPoolExecutor(max_workers=min(len(self.groups), 4))
 
     def evaluate(self, x, cells=None, grad=False):
         """C (and dJ/drho when grad) of the given cells at densities x[cells]."""
-        cells = list(range(self.n)) if cells is None else list(cells)
+        cells = set(range(self.n)) if cells is None else set(cells)
         C = np.zeros((self.n, 6, 6))
         G = np.zeros_like(x) if grad else None
-        for gi, grp in enumerate(self.groups):
-            sel = [c for c in grp if c in set(cells)]
-            if not sel:
-                continue
+
+        def solve(gi):
             # the context solves its whole group (slots keep their cells, so
             # the warm start follows each cell); unselected cells ride along
             # at their current point
+            grp = self.groups[gi]
             Cg, st = self.ctx[gi].homogenize_many(x[grp].reshape(len(grp), *self.shape))
-            self.solves += len(grp)
-            self.est.extend(s["ch_err_est"] for s in st)
-            C[grp] = Cg
+            g = None
             if grad:
                 nb = len(grp)
                 rho_mean = x[grp].mean(axis=1)
                 dJdC = self.wG[grp] * (Cg - self.C0[grp])
                 g = self.ctx[gi].gradient(dJdC, nb=nb).reshape(nb, -1)
                 g += (self.mw * (rho_mean - self.m0) / x.shape[1])[:, None]
+            return Cg, st, g
+
+        todo = [gi for gi, grp in enumerate(self.groups) if any(c in cells for c in grp)]
+        # the groups' contexts run at once (each on its own stream; the
+        # library call releases the interpreter lock), filling what one
+        # group leaves idle
+        if len(todo) > 1:
+            out = list(self._pool.map(solve, todo))
+        else:
+            out = [solve(gi) for gi in todo]
+        for gi, (Cg, st, g) in zip(todo, out):
+            grp = self.groups[gi]
+            self.solves += len(grp)
+            self.est.extend(s["ch_err_est"] for s in st)
+            C[grp] = Cg
+            if grad:
                 G[grp] = g
         return C, G
 
@@ -110,5 +125,6 @@ class BatchedDescent:
         return x, hist
 
     def close(self):
+        self._pool.shutdown()
         for h in self.ctx:
             h.close()
