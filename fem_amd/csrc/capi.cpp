@@ -6,6 +6,7 @@
 #include <exception>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <thread>
 #include <tuple>
@@ -20,7 +21,8 @@ struct femh_ctx {
   std::string err;
   int last_status = FEMH_OK;
   // per-grid sub-contexts used by the uneven batch entry point
-  std::map<std::tuple<int, int, int>, std::unique_ptr<femh::Context>> sub;
+  // (several per size: batches of one size can be in flight at once)
+  std::map<std::tuple<int, int, int>, std::vector<std::unique_ptr<femh::Context>>> sub;
   // peer contexts (opts.streams - 1 of them) for concurrent batches of a
   // many-call
   std::vector<std::unique_ptr<femh::Context>> peers;
@@ -251,60 +253,87 @@ int femh_homogenize_batch(femh_ctx *ctx, int n, const int *dims, const double *c
     std::map<std::tuple<int, int, int>, std::vector<int>> groups;
     for (int s = 0; s < n; s++)
       groups[std::make_tuple(dims[3 * s], dims[3 * s + 1], dims[3 * s + 2])].push_back(s);
-    std::vector<std::pair<femh::Context *, const std::vector<int> *>> work;
-    for (auto &kv : groups) {
-      auto it = ctx->sub.find(kv.first);
-      if (it == ctx->sub.end()) {
-        femh_opts o = ctx->opts;
-        o.nx = std::get<0>(kv.first);
-        o.ny = std::get<1>(kv.first);
-        o.nz = std::get<2>(kv.first);
-        o.interp = FEMH_INTERP_MODULUS;
-        if (o.batch < 1) o.batch = 1;
-        it = ctx->sub.emplace(kv.first, std::unique_ptr<femh::Context>(new femh::Context(o))).first;
-      }
-      work.emplace_back(it->second.get(), &kv.second);
-    }
-    // The groups run concurrently, a few host threads pulling them in turn,
-    // each sub-context on its own stream: one group of small cells leaves
-    // most of the GPU idle, and a slow cell (an FP64 fallback) no longer holds
-    // the other groups back.  Every group writes its own rows of C.
-    std::vector<int> maxit(work.size(), 0);
-    std::vector<std::exception_ptr> errs(work.size());
+    // Work items are batches of up to `batch` cells of one size, largest
+    // cells first.  A few host threads take them in turn, each on a free
+    // sub-context of that size (created on first need, each with its own
+    // stream), so batches of one size and of different sizes are in flight
+    // together: one batch of small cells leaves most of the GPU idle, and a
+    // slow cell (an FP64 fallback) no longer holds the others back.  Every
+    // batch writes its own rows of C; no sub-context warm-starts, so a
+    // cell's result does not depend on which sub-context solved it.
+    struct Item {
+      std::tuple<int, int, int> dims;
+      const std::vector<int> *ids;
+      size_t s0;
+      int k;
+    };
+    std::vector<Item> items;
+    int B = ctx->opts.batch < 1 ? 1 : ctx->opts.batch;
+    for (auto it = groups.rbegin(); it != groups.rend(); ++it)
+      for (size_t s0 = 0; s0 < it->second.size(); s0 += B)
+        items.push_back({it->first, &it->second, s0, (int)std::min<size_t>(B, it->second.size() - s0)});
+    std::mutex mu;
+    std::map<std::tuple<int, int, int>, std::vector<bool>> busy;
+    auto acquire = [&](const std::tuple<int, int, int> &d) -> std::pair<femh::Context *, size_t> {
+      std::lock_guard<std::mutex> lk(mu);
+      auto &pool = ctx->sub[d];
+      auto &b = busy[d];
+      b.resize(pool.size(), false);
+      for (size_t i = 0; i < pool.size(); i++)
+        if (!b[i]) {
+          b[i] = true;
+          return {pool[i].get(), i};
+        }
+      femh_opts o = ctx->opts;
+      o.nx = std::get<0>(d);
+      o.ny = std::get<1>(d);
+      o.nz = std::get<2>(d);
+      o.interp = FEMH_INTERP_MODULUS;
+      o.batch = B;
+      o.warm_start = 0;
+      pool.emplace_back(new femh::Context(o));
+      b.push_back(true);
+      return {pool.back().get(), pool.size() - 1};
+    };
+    auto release = [&](const std::tuple<int, int, int> &d, size_t i) {
+      std::lock_guard<std::mutex> lk(mu);
+      busy[d][i] = false;
+    };
+    std::vector<int> maxit(items.size(), 0);
+    std::vector<std::exception_ptr> errs(items.size());
     std::atomic<size_t> next{0};
     auto worker = [&] {
-      for (size_t w = next++; w < work.size(); w = next++) {
+      for (size_t w = next++; w < items.size(); w = next++) {
+        const Item &it = items[w];
+        std::pair<femh::Context *, size_t> slot{nullptr, 0};
         try {
-          femh::Context &c = *work[w].first;
-          const std::vector<int> &ids = *work[w].second;
-          FEMH_HIP(hipSetDevice(c.o.device));
+          FEMH_HIP(hipSetDevice(ctx->opts.device));
+          slot = acquire(it.dims);
+          femh::Context &c = *slot.first;
           const size_t V = (size_t)c.g.V;
-          std::vector<double> buf;
-          std::vector<double> Cg(36 * (size_t)c.B);
+          std::vector<double> buf((size_t)it.k * V), Cg(36 * (size_t)c.B);
           std::vector<femh_stats> st(c.B);
-          for (size_t s0 = 0; s0 < ids.size(); s0 += c.B) {
-            const int k = (int)std::min<size_t>(c.B, ids.size() - s0);
-            buf.resize((size_t)k * V);
-            for (int j = 0; j < k; j++) std::memcpy(buf.data() + j * V, E[ids[s0 + j]], sizeof(double) * V);
-            c.load(buf.data(), k, false, true);
-            c.homogenize(Cg.data(), st.data());
-            for (int j = 0; j < k; j++) {
-              std::memcpy(C + 36 * (size_t)ids[s0 + j], Cg.data() + 36 * j, sizeof(double) * 36);
-              for (int q = 0; q < 6; q++) maxit[w] = std::max(maxit[w], st[j].iters[q]);
-            }
+          for (int j = 0; j < it.k; j++)
+            std::memcpy(buf.data() + j * V, E[(*it.ids)[it.s0 + j]], sizeof(double) * V);
+          c.load(buf.data(), it.k, false, true);
+          c.homogenize(Cg.data(), st.data());
+          for (int j = 0; j < it.k; j++) {
+            std::memcpy(C + 36 * (size_t)(*it.ids)[it.s0 + j], Cg.data() + 36 * j, sizeof(double) * 36);
+            for (int q = 0; q < 6; q++) maxit[w] = std::max(maxit[w], st[j].iters[q]);
           }
         } catch (...) {
           errs[w] = std::current_exception();
         }
+        if (slot.first) release(it.dims, slot.second);
       }
     };
-    const size_t nthreads = std::min<size_t>(work.size(), 8);
+    const size_t nthreads = std::min<size_t>(items.size(), 8);
     std::vector<std::thread> pool;
     for (size_t t = 1; t < nthreads; t++) pool.emplace_back(worker);
     if (nthreads) worker();
     for (auto &t : pool) t.join();
     for (auto &e : errs)
-      if (e) std::rethrow_exception(e);  // the first group (by size) that failed
+      if (e) std::rethrow_exception(e);  // the first failed batch in work order
     if (stats) {
       std::memset(stats, 0, sizeof(*stats));
       for (int m : maxit) stats->iters[0] = std::max(stats->iters[0], m);
