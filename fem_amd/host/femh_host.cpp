@@ -174,11 +174,9 @@ void FEM3DFun::setParam(const VectorXd &x0) {
   for (double d : distribution) density += d;
   density /= (double)distribution.size();
   const double dx = 1.0 / (double)std::max(nx, std::max(ny, nz));
-  if (Ke0.empty()) {
-    Ke0.assign(1, VectorXd(576));
-    check(nullptr, femh_element_stiffness(1.0, nu, Ke0[0].data()), "femh_element_stiffness");
-    for (double &v : Ke0[0]) v *= dx;  // 3D element stiffness scales with the cell size
-  }
+  Ke0.assign(1, VectorXd(576));
+  check(nullptr, femh_element_stiffness(1.0, nu, Ke0[0].data()), "femh_element_stiffness");
+  for (double &v : Ke0[0]) v *= dx;  // 3D element stiffness scales with the cell size
   if (!keepDisplacements) return;
   // copyPeriodicU: the fluctuation repeats on the wrapped nodes and the
   // affine part continues across them
