@@ -107,12 +107,14 @@ def run_tensor_homogenization(directory, level, E_solid, nu_solid, base_densitie
     dens = np.asarray(base_densities, dtype=np.float64)
     if cubic:
         solver.setdefault("structure", "cubic")
+    solver.setdefault("streams", 3)  # batches in flight together
     h = Homogenizer((n, n, n), nu=nu_solid, E0=E_solid, interp="modulus", batch=batch, **solver)
     tril = np.tril_indices(6)
     tensors, params = [], []
     try:
-        for c0 in range(0, len(assign), batch):
-            chunk = assign[c0:c0 + batch]
+        step = batch * solver["streams"]
+        for c0 in range(0, len(assign), step):
+            chunk = assign[c0:c0 + step]
             E = np.stack([E_solid * np.where(replicate_assignment(a, level, rep, subdiv) == 1, 1.0, density_min)
                           for a in chunk])
             C, _ = h.homogenize_many(E, modulus=True)

@@ -421,10 +421,11 @@ void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParam
 
 void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParameters3D(
     const std::vector<std::vector<int>> &assign, int N[3], const std::vector<double> &dens, StructureType iType,
-    std::vector<float> &oTensorValues, std::vector<float> &oParameters, int batch) {
+    std::vector<float> &oTensorValues, std::vector<float> &oParameters, int batch, int streams) {
   if (!m_ctx) throw HostError(FEMH_BAD_ARG, "init() was not called");
   if (batch < 1 || batch > 16) throw HostError(FEMH_BAD_ARG, "batch must be 1..16");
-  if (!m_batchCtx || m_batch != batch) {
+  if (streams < 1) throw HostError(FEMH_BAD_ARG, "streams must be at least 1");
+  if (!m_batchCtx || m_batch != batch || m_streams != streams) {
     if (m_batchCtx) femh_destroy(m_batchCtx);
     m_batchCtx = nullptr;
     double E, nu;
@@ -438,6 +439,7 @@ void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParam
     o.interp = FEMH_INTERP_MODULUS;
     o.structure = m_structure;
     o.batch = batch;
+    o.streams = streams;
     const int st = femh_create(&o, &m_batchCtx);
     if (st != FEMH_OK) {
       char buf[512] = {0};
@@ -449,12 +451,15 @@ void NumericalCoarseningUsingMultigrid::computeCoarsenedElasticityTensorAndParam
       throw HostError(st, std::string("femh_create: ") + buf);
     }
     m_batch = batch;
+    m_streams = streams;
   }
   const size_t V = (size_t)m_n[0] * m_n[1] * m_n[2];
-  std::vector<double> Emod(V * batch), C(36 * (size_t)batch);
-  std::vector<femh_stats> st(batch);
-  for (size_t c0 = 0; c0 < assign.size(); c0 += batch) {
-    const int nb = (int)std::min<size_t>(batch, assign.size() - c0);
+  // one engine call per `streams` batches, so that they are in flight together
+  const size_t chunk = (size_t)batch * streams;
+  std::vector<double> Emod(V * chunk), C(36 * chunk);
+  std::vector<femh_stats> st(chunk);
+  for (size_t c0 = 0; c0 < assign.size(); c0 += chunk) {
+    const int nb = (int)std::min<size_t>(chunk, assign.size() - c0);
     for (int k = 0; k < nb; k++) moduli(assign[c0 + k], N, Emod.data() + V * k);
     check(m_batchCtx, femh_homogenize_many_modulus(m_batchCtx, nb, Emod.data(), C.data(), st.data()),
           "femh_homogenize_many_modulus");
@@ -588,7 +593,7 @@ bool writeBinary(const std::string &file, const std::vector<std::vector<float>> 
 
 int runTensorHomogenization(const std::string &dir, int level, int nbBlockRep, int nbSubdiv,
                             const std::vector<BaseMaterial> &mats, const std::vector<double> &dens,
-                            bool cubicOnly, const femh_opts *solverOptions, int batch) {
+                            bool cubicOnly, const femh_opts *solverOptions, int batch, int streams) {
   const std::string root = dir + (dir.empty() || dir.back() == '/' ? "" : "/") + "level" + std::to_string(level) + "_";
   std::vector<std::vector<int>> assign, baseMat;
   std::vector<float> params;
@@ -602,7 +607,7 @@ int runTensorHomogenization(const std::string &dir, int level, int nbBlockRep, i
   int n[3] = {level, level, level};
   nc.init(n, nbBlockRep, nbSubdiv, mats, type);
   std::vector<float> tensors, newParams;
-  nc.computeCoarsenedElasticityTensorAndParameters3D(assign, n, dens, type, tensors, newParams, batch);
+  nc.computeCoarsenedElasticityTensorAndParameters3D(assign, n, dens, type, tensors, newParams, batch, streams);
   if (!cfgbin::writeBinary(root + "params.bin", newParams) || !cfgbin::writeBinary(root + "baseMat.bin", baseMat) ||
       !cfgbin::writeBinary(root + "matAssignments.bin", assign) ||
       !cfgbin::writeBinary(root + "elasticityTensors.bin", tensors))

@@ -188,8 +188,8 @@ class NumericalCoarseningUsingMultigrid {
   // the full tensor of the last call
   const Mat6 &lastTensor() const { return m_C; }
 
-  // All assignments at once, `batch` cells per launch sequence (one extra
-  // engine context of that batch size): per assignment, in order, the same
+  // All assignments at once, `batch` cells per launch sequence, `streams`
+  // batches in flight (extra engine contexts of that batch size): per assignment, in order, the same
   // 21 values and (Cubic) 4 parameters as the one-cell call, as float (the
   // caller pipeline's file type).  General appends the density only (the
   // reference's MG path has no General parameters: NumericalCoarseningUsing-
@@ -197,7 +197,8 @@ class NumericalCoarseningUsingMultigrid {
   void computeCoarsenedElasticityTensorAndParameters3D(const std::vector<std::vector<int>> &iMaterialAssignments,
                                                        int N[3], const std::vector<double> &iBaseMaterialDensities,
                                                        StructureType iType, std::vector<float> &oTensorValues,
-                                                       std::vector<float> &oParameters, int batch = 16);
+                                                       std::vector<float> &oParameters, int batch = 16,
+                                                       int streams = 3);
 
  protected:
   femh_ctx *m_ctx = nullptr;
@@ -210,7 +211,7 @@ class NumericalCoarseningUsingMultigrid {
   double m_densityMin = 1e-4;
   Mat6 m_C;
   femh_ctx *m_batchCtx = nullptr;
-  int m_batch = 0;
+  int m_batch = 0, m_streams = 0;
   int m_structure = FEMH_STRUCT_GENERAL;  // engine structure chosen by init()
   void moduli(const std::vector<int> &iMaterials, int N[3], double *oE) const;
 };
@@ -282,7 +283,7 @@ bool writeBinary(const std::string &file, const std::vector<std::vector<float>> 
 int runTensorHomogenization(const std::string &dir, int level, int nbBlockRep, int nbSubdiv,
                             const std::vector<BaseMaterial> &baseMaterials,
                             const std::vector<double> &baseMaterialDensities, bool cubicOnly,
-                            const femh_opts *solverOptions = nullptr, int batch = 16);
+                            const femh_opts *solverOptions = nullptr, int batch = 16, int streams = 3);
 
 // ---- thermal expansion ---------------------------------------------------------
 // ThermalAnalysis (SamplesGenerator/ThermalAnalysis.h; run3D :210-274,
