@@ -26,6 +26,9 @@
  *       replaces ThermalAnalysis::computeThermalCoefficient / run3D
  *       (SamplesGenerator/ThermalAnalysis.cpp:210-274, :388-505): effective
  *       thermal stress and expansion from the solved fields.
+ *   femh_launch_shapes
+ *       reports the launch shapes in use (femh_opts.autotune); no counterpart
+ *       in the reference.
  *   femh_homogenize_batch
  *       replaces the OpenMP sample loop of SamplesGeneratorImpl::
  *       computeParametersAndTensorValues (SamplesGenerator/SamplesGeneratorImpl
