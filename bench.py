@@ -258,21 +258,20 @@ def run_uneven(a):
     mixed precision through femh_homogenize_batch (grouped by size, `batch`
     cells per launch sequence), each checked against FP64."""
     import numpy as np
-    from fem_amd import Homogenizer, homogenize_batch
+    from fem_amd import BatchHomogenizer, Homogenizer
     from fem_amd import workloads as W
     count = a.cells or 512
     grids = W.batch_uneven(count, 3)
     kw = dict(nu=0.3, precision="mixed", tol=1e-10, ch_tol=a.ch_tol, inner_tol=a.inner_tol,
               max_iter=20000, batch=a.batch, autotune=bool(a.autotune), check_every=a.check_every)
-    # warm-up: builds and loads, one cell of every size (and, with --autotune,
-    # times the launch shapes of every size)
-    first = {}
-    for g in grids:
-        first.setdefault(g.shape, g)
-    homogenize_batch(list(first.values()), **kw)
+    # warm-up: one untimed pass over the same cells builds the resident
+    # per-size sub-contexts (and, with --autotune, times the launch shapes)
+    bh = BatchHomogenizer(**kw)
+    bh.run(grids)
     t0 = time.perf_counter()
-    C, st = homogenize_batch(grids, **kw)
+    C, st = bh.run(grids)
     elapsed = time.perf_counter() - t0
+    bh.close()
     errs = []
     t64 = 0.0
     if not a.no_fp64:

@@ -304,17 +304,35 @@ def element_stiffness(E=1.0, nu=0.3):
     return K.reshape(24, 24)
 
 
+class BatchHomogenizer:
+    """Uneven-size batches over modulus grids (femh_homogenize_batch) through
+    one context whose per-size sub-contexts stay resident between calls, so
+    that repeated batches pay their set-up once."""
+
+    def __init__(self, **kw):
+        self._h = Homogenizer((8, 8, 8), interp="modulus", **kw)
+
+    def run(self, grids):
+        """Returns (C[n,6,6], stats)."""
+        grids = [np.ascontiguousarray(g, dtype=np.float64) for g in grids]
+        n = len(grids)
+        if n == 0:
+            return np.zeros((0, 6, 6)), {}
+        dims = np.array([g.shape for g in grids], dtype=np.int32).reshape(-1)
+        ptrs = (ctypes.c_void_p * n)(*[g.ctypes.data for g in grids])
+        C = np.zeros((n, 36))
+        s = Stats()
+        self._h._check(lib().femh_homogenize_batch(self._h._h, n, _ptr(dims), ptrs, _ptr(C), ctypes.byref(s)))
+        return C.reshape(n, 6, 6), s.as_dict()
+
+    def close(self):
+        self._h.close()
+
+
 def homogenize_batch(grids, **kw):
     """Batch over modulus grids of uneven shapes.  Returns (C[n,6,6], stats)."""
-    grids = [np.ascontiguousarray(g, dtype=np.float64) for g in grids]
-    n = len(grids)
-    if n == 0:
-        return np.zeros((0, 6, 6)), {}
-    dims = np.array([g.shape for g in grids], dtype=np.int32).reshape(-1)
-    ptrs = (ctypes.c_void_p * n)(*[g.ctypes.data for g in grids])
-    C = np.zeros((n, 36))
-    h = Homogenizer(grids[0].shape, interp="modulus", **kw)
-    s = Stats()
-    h._check(lib().femh_homogenize_batch(h._h, n, _ptr(dims), ptrs, _ptr(C), ctypes.byref(s)))
-    h.close()
-    return C.reshape(n, 6, 6), s.as_dict()
+    b = BatchHomogenizer(**kw)
+    try:
+        return b.run(grids)
+    finally:
+        b.close()

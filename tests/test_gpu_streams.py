@@ -27,3 +27,18 @@ def test_streams_give_identical_results_and_last_batch_fields(precision):
     assert np.array_equal(g1, g3)
     h1.close()
     h3.close()
+
+
+def test_resident_batch_object_repeats_exactly():
+    """BatchHomogenizer keeps its per-size sub-contexts between calls (several
+    per size when batches of one size are in flight together); a repeated
+    call and the one-shot helper give the same tensors."""
+    from fem_amd import BatchHomogenizer, homogenize_batch
+    grids = W.batch_uneven(40, 5, edges=(16, 24))
+    kw = dict(nu=0.3, precision="mixed", tol=1e-10, ch_tol=1e-7, inner_tol=3e-4, max_iter=20000, batch=4)
+    bh = BatchHomogenizer(**kw)
+    C1, _ = bh.run(grids)
+    C2, _ = bh.run(grids)
+    bh.close()
+    C3, _ = homogenize_batch(grids, **kw)
+    assert np.array_equal(C1, C2) and np.array_equal(C1, C3)
