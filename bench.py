@@ -115,7 +115,7 @@ def cpu_baseline(n):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+        print(f"bench.py: the engine runs on one GPU (asked for --gpus {a.gpus}); "
               "the batch of samples would shard across GPUs with no exchange but is not built here",
               file=sys.stderr)
         sys.exit(2)
