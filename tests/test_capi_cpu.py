@@ -1,9 +1,11 @@
 """CPU checks of the C ABI library: it loads, and exports every symbol
 include/femh.h declares (no device work is launched here)."""
+import ctypes
 import os
 import re
 
 import numpy as np
+import pytest
 
 import fem_amd
 from fem_amd import femh
@@ -31,6 +33,14 @@ def test_abi_version_and_default_opts():
     assert L.femh_abi_version() == 2
     o = femh.make_opts((8, 8, 8))
     assert o.nu == 0.3 and o.tol == 1e-10 and o.precond == 1 and o.nu1 == 2
+    # the library's own defaults for the fields added in version 2
+    d = femh.Opts()
+    L.femh_default_opts(ctypes.byref(d))
+    assert (d.structure, d.autotune, d.streams) == (0, 0, 1)
+    o = femh.make_opts((8, 8, 8), structure="cubic", autotune=True, streams=3)
+    assert (o.structure, o.autotune, o.streams) == (1, 1, 3)
+    with pytest.raises(KeyError):
+        femh.make_opts((8, 8, 8), structure="hexagonal")
 
 
 def test_opts_struct_layout_matches_header():
