@@ -126,7 +126,9 @@ typedef struct femh_opts {
   int streams;          /* many-calls with more than `batch` cells: batches run on
                            this many contexts at once (each its own stream and host
                            thread; streams - 1 extra contexts are created on first
-                           use).  Every cell's result is the same as with 1.  The
+                           use).  Without warm_start every cell's result is the same
+                           as with 1 (with it, each context starts from its own last
+                           batch).  The
                            last batch runs on the primary context, so femh_gradient
                            and the field calls still refer to it.  Profiling runs
                            the batches one at a time.  Default 1 */
