@@ -119,6 +119,11 @@ def main():
               "the batch of samples would shard across GPUs with no exchange but is not built here",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload == "uneven512":
+        # up to 8 sub-contexts' streams are busy at once; HIP gives a process
+        # 4 hardware queues unless told otherwise (set before the runtime starts)
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+            os.environ["GPU_MAX_HW_QUEUES"] = "8"
     import numpy as np
     import torch
     from fem_amd import Homogenizer
