@@ -44,6 +44,8 @@ def parse():
                          "simp64; 3e-4 for the binary uneven512 cells and opt96, whose C_H error "
                          "estimate is less tight)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-runs", type=int, default=5,
+                    help="timed CPU-baseline solves after one warm-up (the median is reported)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solves")
     ap.add_argument("--nu1", type=int, default=1)
     ap.add_argument("--nu2", type=int, default=1)
@@ -51,9 +53,14 @@ def parse():
                     help="Chebyshev degree on the coarse levels (0: --nu1/--nu2)")
     ap.add_argument("--cheby-ratio", type=float, default=14.0,
                     help="Chebyshev smoothing interval [lmax/ratio, lmax]")
-    ap.add_argument("--workload", default="simp64", choices=["simp64", "uneven512", "opt96"],
+    ap.add_argument("--workload", default="simp64",
+                    choices=["simp64", "uneven512", "opt96", "plain64"],
                     help="simp64: the headline 64^3 line; uneven512: 512 binary cells of "
-                         "16..48^3; opt96: 30 projected-gradient steps on 64 cells of 96^3")
+                         "16..48^3; opt96: 30 projected-gradient steps on 64 cells of 96^3; "
+                         "plain64: the plain FP64 path on 64^3 SIMP cells one at a time "
+                         "(3-level V(2,2)-PCG to 1e-10)")
+    ap.add_argument("--batch-threads", type=int, default=0,
+                    help="uneven512: host threads keeping batches in flight (0: the library's 8)")
     ap.add_argument("--structure", default="general", choices=["general", "cubic"],
                     help="simp64 only: cubic symmetrises each cell over the cubic group and "
                          "solves it in the cubic-structure mode (2 of the 6 load cases); the FP64 "
@@ -94,22 +101,66 @@ def pmc_traffic():
         return None
 
 
-def cpu_baseline(n):
-    """The oracle (CPU FP64 restatement, multigrid-PCG to 1e-10) on one cell of
-    the same workload, timed on this host's cores."""
-    import numpy as np  # noqa: F401
+def host_cpu():
+    """CPU model, logical CPU count, the CPUs this process may run on and
+    OMP_NUM_THREADS."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(n, runs):
+    """The oracle (CPU FP64 restatement of the reference method, multigrid-PCG
+    to 1e-10) on one cell of the same workload, timed on this host: one
+    warm-up solve, then the median of `runs` solves on all usable cores
+    (capped by OMP_NUM_THREADS where it is set); plus a one-thread figure from
+    a bounded sample (the set-up and the xx load case of the same cell on one
+    thread, with the load case counted six times: the six cases take the same
+    iteration count on this workload)."""
+    import numpy as np
     from fem_amd import workloads as W
     from oracle import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    threads = max(1, min(threads, os.cpu_count() or 1))
-    O.set_threads(threads)
+    host = host_cpu()
+    threads = host["usable_cpus"]
+    if host["OMP_NUM_THREADS"]:
+        threads = max(1, min(threads, int(host["OMP_NUM_THREADS"])))
     E = W.simp_modulus(W.simp_density(n, 0, 4.0), 3.0)
+    O.set_threads(threads)
+    r = O.homogenize_timed(E, 0.3, precond="mg", tol=1e-10)  # warm-up
+    times = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        r = O.homogenize_timed(E, 0.3, precond="mg", tol=1e-10)
+        times.append(time.perf_counter() - t)
+    med = float(np.median(times))
+    O.set_threads(1)
     t = time.perf_counter()
-    r = O.homogenize(E, 0.3, precond="mg", tol=1e-10)
-    dt = time.perf_counter() - t
-    return {"value": 1.0 / dt, "unit": "homogenizations/s", "cores": threads, "kind": "port",
+    r1 = O.homogenize_timed(E, 0.3, case_mask=1, precond="mg", tol=1e-10)
+    t1 = time.perf_counter() - t
+    setup1, case1 = float(r1["seconds"][0]), float(r1["seconds"][1])
+    one_cell_1t = setup1 + 6.0 * case1
+    O.set_threads(threads)
+    return {"value": 1.0 / med, "unit": "homogenizations/s", "cores": threads, "kind": "port",
             "sample": f"1 x {n}^3 SIMP cell (seed 0), FP64 multigrid-PCG to 1e-10, "
-                      f"{int(max(r['iters']))} iterations, {dt:.2f} s"}
+                      f"{int(max(r['iters']))} iterations per load case; 1 warm-up + median of "
+                      f"{runs} solves on {threads} threads",
+            "runs_s": [round(x, 3) for x in times], "median_s": med, **host,
+            "one_thread": {"value": 1.0 / one_cell_1t, "unit": "homogenizations/s", "cores": 1,
+                           "seconds_per_cell": one_cell_1t,
+                           "sample": f"set-up ({setup1:.2f} s) + xx load case ({case1:.2f} s, "
+                                     f"{int(r1['iters'][0])} iterations) of the same cell on one "
+                                     f"thread, the case counted 6x; {t1:.1f} s wall"}}
 
 
 def main():
@@ -140,6 +191,8 @@ def main():
         return run_uneven(a)
     if a.workload == "opt96":
         return run_opt(a)
+    if a.workload == "plain64":
+        return run_plain(a)
     n = a.n
     B = a.batch
     S = max(1, a.streams)
@@ -219,7 +272,7 @@ def main():
         roof["traffic"] = tr[0]
         roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
         roof["traffic_over_algorithmic"] = tr[0] / tr[1]
-    cpu = None if a.no_cpu else cpu_baseline(n)
+    cpu = None if a.no_cpu else cpu_baseline(n, a.cpu_runs)
     iters = [max(s["iters"]) for s in stats]
     out = {
         "metric": "64^3 homogenizations/sec at C_H rel err <=1e-6, mixed vs FP64; % HBM roofline",
@@ -268,7 +321,8 @@ def run_uneven(a):
     count = a.cells or 512
     grids = W.batch_uneven(count, 3)
     kw = dict(nu=0.3, precision="mixed", tol=1e-10, ch_tol=a.ch_tol, inner_tol=a.inner_tol,
-              max_iter=20000, batch=a.batch, autotune=bool(a.autotune), check_every=a.check_every)
+              max_iter=20000, batch=a.batch, autotune=bool(a.autotune), check_every=a.check_every,
+              batch_threads=a.batch_threads)
     # warm-up: one untimed pass over the same cells builds the resident
     # per-size sub-contexts (and, with --autotune, times the launch shapes)
     bh = BatchHomogenizer(**kw)
@@ -300,7 +354,7 @@ def run_uneven(a):
         "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic",
         "config": {"workload": f"{count} binary cells, edge U{{16..48}}, fill U[0.2,0.8], "
                                f"E_void 1e-9, nu 0.3", "batch": a.batch, "ch_tol": a.ch_tol,
-                   "voxels": vox, "autotune": a.autotune},
+                   "voxels": vox, "autotune": a.autotune, "batch_threads": a.batch_threads or 8},
         "ch_rel_err_vs_fp64_max": max(errs) if errs else None,
         "ch_err_ok": (max(errs) <= 1e-6) if errs else None,
         "fp64_batched_seconds": t64 if errs else None,
@@ -355,6 +409,69 @@ def run_opt(a):
         "ch_err_est_max": float(max(opt.est)) if opt.est else None,
         "ch_rel_err_vs_fp64_max_final": max(errs) if errs else None}))
     opt.close()
+
+
+def run_plain(a):
+    """The plain FP64 path of the 64^3 single-sample workload: SIMP cells
+    (GRF corr 4, p = 3), 3-level V(2,2) multigrid-PCG to ||r||/||b|| <= 1e-10,
+    one cell per call (femh_homogenize_device, densities resident in HBM).
+    The roofline figure is the FP64 fine operator (k_tiled<double>, six
+    right-hand sides per launch) from HIP events in an instrumented replica;
+    cell 0 (seed 0) is checked against the oracle's golden C_H."""
+    import numpy as np
+    import torch
+    from fem_amd import Homogenizer
+    from fem_amd import workloads as W
+    n = a.n
+    shape = (n, n, n)
+    nsteps = a.warmup + a.steps
+    rhos = np.stack([W.simp_density(n, seed, 4.0) for seed in range(nsteps)])
+    dev = torch.device("cuda:0")
+    d_rho = torch.from_numpy(rhos).to(dev)
+    d_C = torch.zeros((nsteps, 36), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    h = Homogenizer(shape, nu=0.3, penal=3.0, tol=1e-10, levels=3, nu1=2, nu2=2)
+    order = list(range(1, nsteps)) + [0]  # warm-up cells first; seed 0 is timed
+    for k in order[:a.warmup]:
+        h.homogenize_device(d_rho[k].data_ptr(), d_C[k].data_ptr())
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for k in order[a.warmup:]:
+        stats.append(h.homogenize_device(d_rho[k].data_ptr(), d_C[k].data_ptr()))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    C = d_C.cpu().numpy().reshape(nsteps, 6, 6)
+    h.set_profiling(True)
+    for k in order[a.warmup:]:
+        h.homogenize_device(d_rho[k].data_ptr(), d_C[k].data_ptr())
+    k64 = h.kernel_stats(0)
+    h.set_profiling(False)
+    h.close()
+    err0 = None
+    gold = os.path.join(ROOT, "tests", "golden", "ch_goldens.json")
+    if n == 64 and os.path.exists(gold):
+        g = np.array(json.load(open(gold))["simp64_s0"]["C"]).reshape(6, 6)
+        err0 = float(np.linalg.norm(C[0] - g) / np.linalg.norm(g))
+    ach = k64["bytes"] / (k64["ms"] * 1e-3) / 1e9 if k64["ms"] > 0 else 0.0
+    cpu = None if a.no_cpu else cpu_baseline(n, a.cpu_runs)
+    print(json.dumps({
+        "metric": "64^3 plain FP64 homogenizations/sec (one cell per call, 3-level V(2,2)-PCG to "
+                  "1e-10); % HBM roofline of the FP64 fine operator",
+        "value": a.steps / elapsed, "unit": "homogenizations/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+        "config": {"workload": f"{n}^3 SIMP homogenization (GRF corr 4, p=3, nu=0.3, 6 load cases), "
+                               "plain FP64 3-level V(2,2) multigrid-PCG, one cell per step", "n": n},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_tiled<double> (fine-level operator, all stage/epilogue variants)",
+                     "launches": k64["launches"],
+                     "avg_us": 1e3 * k64["ms"] / max(k64["launches"], 1),
+                     "bytes_per_launch": k64["bytes"] / max(k64["launches"], 1)},
+        "cpu_baseline": cpu,
+        "ch_rel_err_vs_golden_seed0": err0,
+        "iters_max": max(max(s["iters"]) for s in stats)}))
 
 
 if __name__ == "__main__":

@@ -75,19 +75,32 @@ void run_many(femh_ctx *ctx, int n, const double *src, bool device_src, bool mod
   const int B = c0.B, nbatch = (n + B - 1) / B;
   const int S = c0.prof ? 1 : std::min(std::max(ctx->opts.streams, 1), nbatch);
   while ((int)ctx->peers.size() < S - 1) ctx->peers.emplace_back(new femh::Context(ctx->opts));
-  std::vector<std::exception_ptr> errs(S);
+  // A batch that fails (a cell that does not converge, a non-finite
+  // modulus) does not stop the other batches: each batch's error is kept
+  // and the first one in batch order is raised after all batches ran, so the
+  // cells that are solved -- and which they are -- do not depend on
+  // `streams`.  (A batch that raises NOT_CONVERGED has still written its C
+  // and stats.)
+  std::vector<std::exception_ptr> errs(nbatch);
   auto work = [&](int w) {
+    femh::Context *cp = nullptr;
     try {
-      femh::Context &c = w == 0 ? c0 : *ctx->peers[w - 1];
-      if (w > 0) FEMH_HIP(hipSetDevice(c.o.device));
-      for (int b = 0; b < nbatch; b++) {
-        if ((nbatch - 1 - b) % S != w) continue;
-        const int s0 = b * B, k = std::min(B, n - s0);
-        c.load(src + (size_t)s0 * V, k, device_src, modulus);
-        c.homogenize(C + 36 * (size_t)s0, stats ? stats + s0 : nullptr);
-      }
+      cp = w == 0 ? &c0 : ctx->peers[w - 1].get();
+      if (w > 0) FEMH_HIP(hipSetDevice(cp->o.device));
     } catch (...) {
-      errs[w] = std::current_exception();
+      for (int b = 0; b < nbatch; b++)
+        if ((nbatch - 1 - b) % S == w) errs[b] = std::current_exception();
+      return;
+    }
+    for (int b = 0; b < nbatch; b++) {
+      if ((nbatch - 1 - b) % S != w) continue;
+      try {
+        const int s0 = b * B, k = std::min(B, n - s0);
+        cp->load(src + (size_t)s0 * V, k, device_src, modulus);
+        cp->homogenize(C + 36 * (size_t)s0, stats ? stats + s0 : nullptr);
+      } catch (...) {
+        errs[b] = std::current_exception();
+      }
     }
   };
   std::vector<std::thread> pool;
@@ -141,6 +154,7 @@ void femh_default_opts(femh_opts *o) {
   o->autotune = 0;
   o->streams = 1;
   o->coarse_nu = 3;
+  o->batch_threads = 0;
 }
 
 int femh_create(const femh_opts *o, femh_ctx **out) {
@@ -327,7 +341,8 @@ int femh_homogenize_batch(femh_ctx *ctx, int n, const int *dims, const double *c
         if (slot.first) release(it.dims, slot.second);
       }
     };
-    const size_t nthreads = std::min<size_t>(items.size(), 8);
+    const int want = ctx->opts.batch_threads > 0 ? ctx->opts.batch_threads : 8;
+    const size_t nthreads = std::min<size_t>(items.size(), (size_t)want);
     std::vector<std::thread> pool;
     for (size_t t = 1; t < nthreads; t++) pool.emplace_back(worker);
     if (nthreads) worker();

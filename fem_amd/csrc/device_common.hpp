@@ -92,6 +92,48 @@ __device__ __forceinline__ void mode_couple(const T (&w)[3][8], T E, const ModeT
   }
 }
 
+// Compact form of the same coupling.  By the symmetry of the cube under axis
+// permutations the 45 nonzero coefficients take 9 values: a diagonal
+// coefficient depends only on (bit of its own axis, number of other axes set)
+// -> d[5], a cross coefficient (c, d) only on (bit of c, bit of the third
+// axis k) -> x[4].  Passing 9 scalars instead of 96 keeps them in scalar
+// registers (the full table spills there).  Indices are compile-time
+// constants after unrolling.
+template <typename T>
+struct ModeC {
+  T d[5];
+  T x[4];
+};
+__host__ __device__ constexpr int mode_bit(int sm, int ax) { return (sm >> (2 - ax)) & 1; }
+__host__ __device__ constexpr int mode_diag_index(int c, int sm) {
+  // (b_c, n_other): (0,1) -> 0, (0,2) -> 1, (1,0) -> 2, (1,1) -> 3, (1,2) -> 4
+  return mode_bit(sm, c) ? 2 + mode_bit(sm, (c + 1) % 3) + mode_bit(sm, (c + 2) % 3)
+                         : mode_bit(sm, (c + 1) % 3) + mode_bit(sm, (c + 2) % 3) - 1;
+}
+__host__ __device__ constexpr int mode_cross_index(int c, int d, int sm) {
+  return 2 * mode_bit(sm, c) + mode_bit(sm, 3 - c - d);
+}
+
+template <typename T>
+__device__ __forceinline__ void mode_couple(const T (&w)[3][8], T E, const ModeC<T> &m, T (&z)[3][8]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    z[c][0] = 0;
+#pragma unroll
+    for (int sm = 1; sm < 8; sm++) {
+      T acc = m.d[mode_diag_index(c, sm)] * w[c][sm];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        if (d == c) continue;
+        if (mode_bit(sm, c) == mode_bit(sm, d)) continue;
+        const int t = sm ^ (1 << (2 - c)) ^ (1 << (2 - d));
+        acc += m.x[mode_cross_index(c, d, sm)] * w[d][t];
+      }
+      z[c][sm] = E * acc;
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void hadamard_voxel(const T *__restrict__ u, T E, const ModeT<T> &m,
                                                T *__restrict__ f) {

@@ -400,12 +400,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int V, T *__restrict__ x,
                                                        const T *__restrict__ D, T *__restrict__ z,
                                                        int rzslot, int rz_out,
                                                        const int *__restrict__ act, Reduce red,
-                                                       int np_pq, int np_rz) {
+                                                       int np_pq, int np_rz,
+                                                       double *__restrict__ alpha_out) {
   const int rr = blockIdx.y;
   if (!act[rr]) return;
   const double pq = sum_parts(red, S_PQ, rr, np_pq);
   const double rz = sum_parts(red, rzslot, rr, np_rz);
   const double alpha = pq != 0.0 ? rz / pq : 0.0;
+  if (alpha_out && blockIdx.x == 0 && threadIdx.x == 0) alpha_out[rr] = alpha;
   const T a = (T)alpha;
   const size_t off = (size_t)rr * 3 * V;
   const T *Ds = D ? D + (size_t)(rr / kRHS) * 3 * V : nullptr;
@@ -434,7 +436,7 @@ __global__ __launch_bounds__(kBlock) void k_p_update(int V, T *__restrict__ p,
                                                      const T *__restrict__ z, int rz_new,
                                                      int rz_old, int first,
                                                      const int *__restrict__ act, Reduce red,
-                                                     int np) {
+                                                     int np, double *__restrict__ beta_out) {
   const int rr = blockIdx.y;
   if (!act[rr]) return;
   double beta = 0;
@@ -443,6 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_p_update(int V, T *__restrict__ p,
     const double num = sum_parts(red, rz_new, rr, np);
     beta = den != 0.0 ? num / den : 0.0;
   }
+  if (beta_out && blockIdx.x == 0 && threadIdx.x == 0) beta_out[rr] = beta;
   const T bt = (T)beta;
   const size_t off = (size_t)rr * 3 * V;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
@@ -470,7 +473,8 @@ __global__ __launch_bounds__(kBlock) void k_permute_case(int n, double *__restri
 
 __global__ __launch_bounds__(kBlock) void k_beta(int rz_new, int rz_old, int first,
                                                  const int *__restrict__ act, Reduce red, int np,
-                                                 double *__restrict__ beta) {
+                                                 double *__restrict__ beta,
+                                                 double *__restrict__ beta_out) {
   const int rr = blockIdx.x;
   if (!act[rr]) return;
   double b = 0;
@@ -479,7 +483,10 @@ __global__ __launch_bounds__(kBlock) void k_beta(int rz_new, int rz_old, int fir
     const double num = sum_parts(red, rz_new, rr, np);
     b = den != 0.0 ? num / den : 0.0;
   }
-  if (threadIdx.x == 0) beta[rr] = b;
+  if (threadIdx.x == 0) {
+    beta[rr] = b;
+    if (beta_out) beta_out[rr] = b;
+  }
 }
 
 template <typename T>
@@ -555,6 +562,11 @@ __global__ __launch_bounds__(kBlock) void k_fill_hash(size_t n, size_t period, T
     z ^= z >> 31;
     x[t] = (T)((double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5);
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(size_t n, T *__restrict__ x, T v) {
+  for (size_t t = blockIdx.x * (size_t)kBlock + threadIdx.x; t < n; t += (size_t)gridDim.x * kBlock) x[t] = v;
 }
 
 template <typename T>
@@ -902,23 +914,24 @@ void launch_cheb_first(int V, int R, const T *r, const T *D, T *d, T *x, double 
 template <typename T>
 void launch_pcg_update(int V, int R, T *x, T *r, const T *p, const T *q, const T *D, T *z,
                        int rzslot, int rz_out, const int *act, const Reduce &red, int np_pq,
-                       int np_rz, hipStream_t s) {
+                       int np_rz, hipStream_t s, double *alpha_out) {
   hipLaunchKernelGGL(k_pcg_update<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, x, r, p, q, D, z,
-                     rzslot, rz_out, act, red, np_pq, np_rz);
+                     rzslot, rz_out, act, red, np_pq, np_rz, alpha_out);
 }
 template <typename T>
 void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int first,
-                     const int *act, const Reduce &red, int np_rz, hipStream_t s) {
+                     const int *act, const Reduce &red, int np_rz, hipStream_t s, double *beta_out) {
   hipLaunchKernelGGL(k_p_update<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, p, z, rz_new, rz_old,
-                     first, act, red, np_rz);
+                     first, act, red, np_rz, beta_out);
 }
 void launch_permute_case(int n, int nb, double *x, int src, int dst, int swap, unsigned mask, hipStream_t s) {
   hipLaunchKernelGGL(k_permute_case, dim3(grid_for(n * n * n), nb), dim3(kBlock), 0, s, n, x, src, dst, swap,
                      mask);
 }
 void launch_beta(int R, int rz_new, int rz_old, int first, const int *act, const Reduce &red, int np_rz,
-                 double *beta, hipStream_t s) {
-  hipLaunchKernelGGL(k_beta, dim3(R), dim3(kBlock), 0, s, rz_new, rz_old, first, act, red, np_rz, beta);
+                 double *beta, hipStream_t s, double *beta_out) {
+  hipLaunchKernelGGL(k_beta, dim3(R), dim3(kBlock), 0, s, rz_new, rz_old, first, act, red, np_rz, beta,
+                     beta_out);
 }
 template <typename T>
 void launch_dot(int V, int R, const T *a, const T *b, int slot, const int *act, const Reduce &red,
@@ -960,6 +973,13 @@ void launch_scale_by_dinv_norm(int V, int R, T *y, const T *Dinv, const int *act
                                const Reduce &red, hipStream_t s) {
   hipLaunchKernelGGL(k_scale_dinv_norm<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, y, Dinv, act,
                      red);
+}
+template <typename T>
+void launch_fill(size_t n, T *x, T v, hipStream_t s) {
+  size_t b = (n + kBlock - 1) / kBlock;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)b), dim3(kBlock), 0, s, n, x, v);
 }
 void launch_zero_rhs(int V, int R, double *x, const int *act, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_rhs, grid2(3 * V, R), dim3(kBlock), 0, s, V, x, act);
@@ -1032,14 +1052,15 @@ void reduce_init(Reduce &red, hipStream_t s) {
                                      const int *, hipStream_t);                                   \
   template void launch_pcg_update<T>(int, int, T *, T *, const T *, const T *, const T *, T *,    \
                                      int, int, const int *, const Reduce &, int, int,             \
-                                     hipStream_t);                                                \
+                                     hipStream_t, double *);                                      \
   template void launch_p_update<T>(int, int, T *, const T *, int, int, int, const int *,          \
-                                   const Reduce &, int, hipStream_t);                             \
+                                   const Reduce &, int, hipStream_t, double *);                   \
   template void launch_dot<T>(int, int, const T *, const T *, int, const int *, const Reduce &,   \
                               hipStream_t);                                                       \
   template void launch_remove_mean<T>(int, int, T *, const int *, const Reduce &, hipStream_t);   \
   template void launch_fill_hash<T>(size_t, size_t, T *, unsigned long long, hipStream_t);                \
   template void launch_invert<T>(size_t, const T *, T *, hipStream_t);                            \
+  template void launch_fill<T>(size_t, T *, T, hipStream_t);                                      \
   template void launch_scale_by_dinv_norm<T>(int, int, T *, const T *, const int *,               \
                                              const Reduce &, hipStream_t);
 

@@ -125,18 +125,20 @@ void launch_cheb_first(int V, int R, const T *r, const T *D, T *d, T *x, double 
 // PCG: alpha = rz[rzslot]/pq ; x += alpha p ; r -= alpha q ; rr -> S_RR
 //      jacobi (D != null): z = r / D and rz -> slot rz_out (vec_parts(V) partials)
 // np_pq / np_rz: partial counts of the slots read (set by their producers)
+// alpha_out (may be null): alpha of every active RHS (the Lanczos record)
 template <typename T>
 void launch_pcg_update(int V, int R, T *x, T *r, const T *p, const T *q, const T *D, T *z,
                        int rzslot, int rz_out, const int *act, const Reduce &red, int np_pq,
-                       int np_rz, hipStream_t s);
-// p = z + beta p, beta = rz[new]/rz[old] (first: p = z)
+                       int np_rz, hipStream_t s, double *alpha_out = nullptr);
+// p = z + beta p, beta = rz[new]/rz[old] (first: p = z); beta_out as alpha_out
 template <typename T>
 void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int first,
-                     const int *act, const Reduce &red, int np_rz, hipStream_t s);
+                     const int *act, const Reduce &red, int np_rz, hipStream_t s,
+                     double *beta_out = nullptr);
 // beta[r] = rz[new]/rz[old] per active RHS (first: 0), for the fused
-// direction update of the fine operator (ST_PUPD)
+// direction update of the fine operator (ST_PUPD); beta_out as alpha_out
 void launch_beta(int R, int rz_new, int rz_old, int first, const int *act, const Reduce &red, int np_rz,
-                 double *beta, hipStream_t s);
+                 double *beta, hipStream_t s, double *beta_out = nullptr);
 // cubic structure: load case dst from load case src by an axis swap of the
 // cube grid (swap 0: x <-> y, 1: x <-> z), points and displacement components
 // alike, for the samples s with bit s of mask set
@@ -149,6 +151,10 @@ template <typename Ts, typename Td>
 void launch_convert(size_t n, const Ts *src, Td *dst, hipStream_t s);
 template <typename T>
 void launch_invert(size_t n, const T *d, T *di, hipStream_t s);
+// x[t] = v for t < n (clears memory with a kernel, not a memset node, so that
+// the sequence stays valid under stream capture)
+template <typename T>
+void launch_fill(size_t n, T *x, T v, hipStream_t s);
 // x[q] = 0 for the right-hand sides q flagged in act
 void launch_zero_rhs(int V, int R, double *x, const int *act, hipStream_t s);
 // x64 += (double) d32 ; both [R][3][V]

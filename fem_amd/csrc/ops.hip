@@ -4,6 +4,7 @@
 // by s = q / 6.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <stdexcept>
 
 #include "device_common.hpp"
@@ -154,7 +155,7 @@ __device__ __forceinline__ int xcd_order(int b, int n) {
 // tile (same moduli and diagonal) and then the neighbouring tiles follow each
 // other on one XCD
 template <typename T, int ST, int EP>
-__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g, const T *__restrict__ E, ModeT<T> mc,
+__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g, const T *__restrict__ E, ModeC<T> mc,
                                                   OpArgs<T> a, int TI, int tiles_k, int tiles,
                                                   int chunks) {
   const int L = xcd_order(blockIdx.x, gridDim.x);
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
 }
 
 template <typename T, int ST, int EP>
-static void tiled_go(const Dims &g, int R, const T *E, const ModeT<T> &mc, const OpArgs<T> &a,
+static void tiled_go(const Dims &g, int R, const T *E, const ModeC<T> &mc, const OpArgs<T> &a,
                      hipStream_t s, int want) {
   const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
   const int TI = tiled_chunk(g, R, want);
@@ -453,12 +454,27 @@ static void tiled_go(const Dims &g, int R, const T *E, const ModeT<T> &mc, const
 template <typename T>
 void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCoef &m,
                   const OpArgs<T> &a, hipStream_t s, int slabs) {
-  ModeT<T> mc;
+  // the 9 distinct coefficients, checked against every entry of the table
+  ModeC<T> mc;
+  double dv[5] = {0, 0, 0, 0, 0}, xv[4] = {0, 0, 0, 0};
   for (int c = 0; c < 3; c++)
-    for (int v = 0; v < 8; v++) {
-      mc.diag[c][v] = (T)m.diag[c][v];
-      for (int d = 0; d < 3; d++) mc.cross[c][d][v] = (T)m.cross[c][d][v];
+    for (int sm = 1; sm < 8; sm++) {
+      dv[mode_diag_index(c, sm)] = m.diag[c][sm];
+      for (int d = 0; d < 3; d++)
+        if (d != c && mode_bit(sm, c) != mode_bit(sm, d)) xv[mode_cross_index(c, d, sm)] = m.cross[c][d][sm];
     }
+  for (int c = 0; c < 3; c++)
+    for (int sm = 1; sm < 8; sm++) {
+      double err = std::fabs(dv[mode_diag_index(c, sm)] - m.diag[c][sm]);
+      for (int d = 0; d < 3; d++)
+        if (d != c && mode_bit(sm, c) != mode_bit(sm, d))
+          err = std::fmax(err, std::fabs(xv[mode_cross_index(c, d, sm)] - m.cross[c][d][sm]));
+        else if (d != c && m.cross[c][d][sm] != 0.0)
+          err = 1.0;
+      if (err > 1e-15 * std::fabs(dv[2])) throw std::runtime_error("mode coupling is not cube-symmetric");
+    }
+  for (int q = 0; q < 5; q++) mc.d[q] = (T)dv[q];
+  for (int q = 0; q < 4; q++) mc.x[q] = (T)xv[q];
 #define GO(S_, E_)                                 \
   if (st == S_ && ep == E_) {                      \
     tiled_go<T, S_, E_>(g, R, E, mc, a, s, slabs); \

@@ -103,6 +103,10 @@ Context::Context(const femh_opts &opts) : o(opts) {
   ch_part.alloc(sizeof(double) * 21 * 1024 * B);
   ch_out.alloc(sizeof(double) * 21 * B);
   act.alloc(sizeof(int) * kActSlots * kMaxR);
+  // timing events of every homogenize call and of the autotuner, created
+  // once: no event is created or destroyed per call
+  for (auto &e : ev_call) FEMH_HIP(hipEventCreate(&e));
+  for (auto &e : ev_tune) FEMH_HIP(hipEventCreate(&e));
   FEMH_HIP(hipGetLastError());
   FEMH_HIP(hipStreamSynchronize(stream));
 }
@@ -113,6 +117,10 @@ Context::~Context() {
     (void)hipEventDestroy(pe.b);
   }
   for (auto e : ev_pool) (void)hipEventDestroy(e);
+  for (auto e : ev_call)
+    if (e) (void)hipEventDestroy(e);
+  for (auto e : ev_tune)
+    if (e) (void)hipEventDestroy(e);
   if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
@@ -152,6 +160,40 @@ void Context::load(const double *src, int n, bool device_src, bool modulus) {
 }
 
 // ---------------------------------------------------------------------------
+// Algorithmic HBM bytes of one fine-operator launch: every distinct per-RHS
+// vector the launch reads or writes, once per active right-hand side (3
+// components each), plus per active sample the moduli and, where the launch
+// uses it, the inverse diagonal (3 components).  A vector that is staged and
+// also read by the epilogue (a.u == a.b or a.u == a.rprev) counts once;
+// outputs count only where the variant writes them (ops.hpp).
+template <typename T>
+double tiled_alg_bytes(int st, int ep, const OpArgs<T> &a, double V, double sz, double Rn, double ns) {
+  const void *in[5] = {a.u, nullptr, nullptr, nullptr, nullptr};
+  int nin = 1;
+  auto add_in = [&](const void *p) {
+    if (!p) return;
+    for (int i = 0; i < nin; i++)
+      if (in[i] == p) return;
+    in[nin++] = p;
+  };
+  if (st == ST_PUPD) add_in(a.u2);
+  if (ep == EP_RES || ep == EP_CHEB0) add_in(a.b);
+  if (ep == EP_STEP) {
+    add_in(a.rprev);
+    add_in(a.xprev);
+    if (a.dot) add_in(a.b);
+  }
+  double vec_out = 0;
+  if (ep == EP_AX) vec_out = 1 + (a.pout ? 1 : 0);
+  if (ep == EP_RES) vec_out = 1;
+  if (ep == EP_CHEB0) vec_out = a.fin1 ? 1 : (a.last ? 2 : 3);
+  if (ep == EP_STEP) vec_out = a.last ? 1 : 3;
+  double vec = nin + vec_out;
+  if (st == ST_PROLONG) vec += 0.125;  // coarse correction, 1/8 of the nodes
+  const bool usesD = st == ST_DSCALE || ep == EP_CHEB0 || ep == EP_STEP || (ep == EP_AX && a.ydinv);
+  return V * (3.0 * sz * Rn * vec + ns * sz * (usesD ? 4.0 : 1.0));
+}
+
 // operator dispatch (with optional kernel timing of the fine level)
 template <typename T>
 void Context::op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a) {
@@ -188,27 +230,7 @@ void Context::op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a) {
         }
       }
       const double Rn = na, V = (double)g0.V, sz = sizeof(T);
-      // algorithmic bytes: staged vector read + outputs written + extra
-      // per-node inputs of the epilogue; per sample E read once and, where
-      // the smoother needs it, the inverse diagonal
-      double vec_in = 1, vec_out = 1;
-      if (ep == EP_RES) vec_in += 1;
-      if (ep == EP_CHEB0) {
-        vec_in += 1;
-        vec_out = 3;
-      }
-      if (ep == EP_STEP) {
-        vec_in += 1 + (a.xprev ? 1 : 0);
-        vec_out = 3;
-      }
-      if (st == ST_PROLONG) vec_in += 0.125;
-      if (st == ST_PUPD) {
-        vec_in += 1;                 // previous direction
-        if (a.pout) vec_out += 1;    // new direction
-      }
-      const bool usesD = st == ST_DSCALE || ep == EP_CHEB0 || ep == EP_STEP;
-      const double samples = ns;  // samples with an active RHS read E (and D) once
-      pe.bytes = V * (3.0 * sz * Rn * (vec_in + vec_out) + samples * sz * (usesD ? 4.0 : 1.0));
+      pe.bytes = tiled_alg_bytes(st, ep, a, V, sz, Rn, ns);
       pe.flops = V * Rn * 1152.0;
       FEMH_HIP(hipEventRecord(pe.a, stream));
       launch();
@@ -288,8 +310,11 @@ void Context::build_levels(Hierarchy<T> &h) {
     L.D.alloc(sizeof(T) * 3 * B * L.g.V);
     L.Dinv.alloc(sizeof(T) * 3 * B * L.g.V);
     L.lmax.assign(B, 0.0);
-    for (DBuf *bf : {&L.x, &L.b, &L.r, &L.d, &L.d2, &L.xn, &L.Ad})
-      bf->alloc(field_bytes(L.g, B, sizeof(T)));
+    // V-cycle work vectors (a one-level hierarchy -- Jacobi -- needs none;
+    // the autotuner and apply_level take theirs on first use)
+    if (dims.size() > 1)
+      for (DBuf *bf : {&L.x, &L.b, &L.r, &L.d, &L.d2, &L.xn, &L.Ad})
+        bf->alloc(field_bytes(L.g, B, sizeof(T)));
     if (l > 0) L.S.alloc(sizeof(T) * 243 * B * (size_t)L.g.V);
   }
   if (dims.size() > 2) {
@@ -465,9 +490,7 @@ void Context::autotune(Hierarchy<T> &h) {
   prof = false;
   std::vector<int> all(R, 1);
   set_act(A_POW, all);
-  hipEvent_t e0, e1;
-  FEMH_HIP(hipEventCreate(&e0));
-  FEMH_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = ev_tune[0], e1 = ev_tune[1];
   auto time_ms = [&](auto &&body, int reps) {
     body();  // warm
     FEMH_HIP(hipEventRecord(e0, stream));
@@ -487,6 +510,8 @@ void Context::autotune(Hierarchy<T> &h) {
   };
   for (int l = 0; l < L; l++) {
     Level<T> &Lv = h.lv[l];
+    Lv.d.alloc(field_bytes(Lv.g, B, sizeof(T)));
+    Lv.Ad.alloc(field_bytes(Lv.g, B, sizeof(T)));
     launch_fill_hash<T>((size_t)R * 3 * Lv.g.V, (size_t)kRHS * 3 * Lv.g.V, Lv.d.template as<T>(), 0x7A11ull + l,
                         stream);
   }
@@ -524,8 +549,6 @@ void Context::autotune(Hierarchy<T> &h) {
     shape_xfer[l] = pick(cand, t);
   }
   FEMH_HIP(hipGetLastError());
-  FEMH_HIP(hipEventDestroy(e0));
-  FEMH_HIP(hipEventDestroy(e1));
   prof = was_prof;
   std::vector<int> val{shape_fine};
   val.insert(val.end(), shape_xfer.begin(), shape_xfer.end());
@@ -811,7 +834,7 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
   const int V = g.V;
   const bool jac = !(o.precond == FEMH_PRECOND_MG && h.lv.size() > 1);
   const T *D0 = h.lv[0].Dinv.template as<T>();
-  FEMH_HIP(hipMemsetAsync(xv, 0, field_bytes(g, nb, sizeof(T)), stream));
+  launch_fill<T>(field_bytes(g, nb, sizeof(T)) / sizeof(T), xv, (T)0, stream);
   // z_ready: the caller also left r = b in rv
   if (!z_ready)
     FEMH_HIP(hipMemcpyAsync(rv, bvec, field_bytes(g, nb, sizeof(T)), hipMemcpyDeviceToDevice, stream));
@@ -862,8 +885,10 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
     } else {
       apply<T>(h, 0, pv, qv, nullptr, 0, ap, &red, S_PQ);
     }
+    double *lza = lz_alpha ? lz_alpha + (size_t)(it - 1) * kMaxR : nullptr;
+    double *lzb = lz_beta ? lz_beta + (size_t)(it - 1) * kMaxR : nullptr;
     launch_pcg_update<T>(V, R, xv, rv, pcur, qv, jac ? D0 : nullptr, zv, rzc, rzn, ap, red,
-                         apply_parts(), np_rz, stream);
+                         apply_parts(), np_rz, stream, lza);
     FEMH_HIP(hipGetLastError());
     if (it % o.check_every == 0 || it == max_iter) {
       read_scalars(S_RR, rr.data(), R, vec_parts(V));
@@ -885,9 +910,9 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
     }
     if (!jac) precond<T>(h, rv, zv, ap, rzn);
     if (fuse_p)
-      launch_beta(R, rzn, rzc, 0, ap, red, np_rz, beta_d.as<double>(), stream);
+      launch_beta(R, rzn, rzc, 0, ap, red, np_rz, beta_d.as<double>(), stream, lzb);
     else
-      launch_p_update<T>(V, R, pv, zv, rzn, rzc, 0, ap, red, np_rz, stream);
+      launch_p_update<T>(V, R, pv, zv, rzn, rzc, 0, ap, red, np_rz, stream, lzb);
     cur ^= 1;
   }
   FEMH_HIP(hipGetLastError());
@@ -948,6 +973,60 @@ void Context::compute_ch(const double *fields, double *C) {
   }
 }
 
+// Smallest eigenvalue of the k x k Lanczos matrix of a PCG run from its
+// coefficients a_j = alpha_j (j = 0..k-1) and b_j = beta_j (j = 0..k-2),
+// stored with stride `stride`:
+//   T(j,j) = 1/a_j + b_{j-1}/a_{j-1},  T(j,j+1) = sqrt(b_j)/a_j
+// (the Lanczos-CG relation, e.g. Saad, Iterative Methods for Sparse Linear
+// Systems, 2nd ed., section 6.7.3).  Its eigenvalues (Ritz values) lie in the
+// spectrum of the preconditioned operator M^-1 K, and the smallest one
+// approaches lambda_min(M^-1 K) from above as k grows.  Bisection on the
+// Sturm count of T - x I.  Returns NaN when the coefficients are not those of
+// a positive definite run (breakdown, rounding).
+static double lanczos_min(const double *a, const double *b, size_t stride, int k) {
+  if (k < 1) return NAN;
+  std::vector<double> d(k), e2(k > 1 ? k - 1 : 1, 0.0);
+  double hi = 0;
+  for (int j = 0; j < k; j++) {
+    const double aj = a[(size_t)j * stride];
+    if (!(aj > 0) || !std::isfinite(aj)) return NAN;
+    d[j] = 1.0 / aj;
+    if (j > 0) d[j] += b[(size_t)(j - 1) * stride] / a[(size_t)(j - 1) * stride];
+    if (j + 1 < k) {
+      const double bj = b[(size_t)j * stride];
+      if (!(bj >= 0) || !std::isfinite(bj)) return NAN;
+      e2[j] = bj / (aj * aj);
+    }
+  }
+  for (int j = 0; j < k; j++) {
+    double r = d[j];
+    if (j > 0) r += std::sqrt(e2[j - 1]);
+    if (j + 1 < k) r += std::sqrt(e2[j]);
+    hi = std::max(hi, r);
+  }
+  // eigenvalues of T - x I below 0: negative pivots of its LDL^T
+  auto below = [&](double x) {
+    int cnt = 0;
+    double piv = 1.0;
+    for (int j = 0; j < k; j++) {
+      piv = (d[j] - x) - (j > 0 ? e2[j - 1] / piv : 0.0);
+      if (piv == 0.0) piv = -1e-300;
+      cnt += piv < 0;
+    }
+    return cnt;
+  };
+  double lo = 0.0;
+  if (below(lo) > 0) return NAN;
+  for (int it = 0; it < 200 && hi - lo > 1e-12 * hi; it++) {
+    const double mid = 0.5 * (lo + hi);
+    if (below(mid) >= 1)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
 void Context::homogenize(double *C, femh_stats *st) {
   if (nb < 1) throw Error(FEMH_BAD_ARG, "no input loaded");
   // warm start: each slot begins from the fields of the previous call with
@@ -955,9 +1034,9 @@ void Context::homogenize(double *C, femh_stats *st) {
   const bool warm = o.warm_start && prev_nb == nb;
   prev_nb = 0;
   auto t0 = std::chrono::steady_clock::now();
-  hipEvent_t ev[3];
-  for (auto &e : ev) FEMH_HIP(hipEventCreate(&e));
-  FEMH_HIP(hipEventRecord(ev[0], stream));
+  hipEvent_t *ev = ev_call;
+  const bool timed = ev[0] != nullptr;
+  if (timed) FEMH_HIP(hipEventRecord(ev[0], stream));
   have_fields = false;
   // validate the moduli before any device work depends on them
   // cubic structure: a cell takes the two-solve path only if its moduli are
@@ -1000,7 +1079,7 @@ void Context::homogenize(double *C, femh_stats *st) {
     // are formed from them after the solve
     if (cubic_ok[t / kRHS] && t % kRHS != 0 && t % kRHS != 3) active[t] = 0;
   }
-  FEMH_HIP(hipEventRecord(ev[1], stream));
+  if (timed) FEMH_HIP(hipEventRecord(ev[1], stream));
 
   std::vector<int> iters(R, 0), outer(R, 0);
   std::vector<double> relres(R, 0.0), ch_est(nb, 0.0);
@@ -1078,6 +1157,18 @@ void Context::homogenize(double *C, femh_stats *st) {
     std::vector<int> it_in(R);
     std::vector<double> rel_in(R);
     const int max_outer = o.max_outer > 0 ? o.max_outer : 12;
+    // Energy-norm bound: r.K^-1 r <= r.M^-1 r / lambda_min(M^-1 K).  The
+    // correction sweeps' PCG coefficients give Lanczos matrices whose smallest
+    // Ritz value estimates lambda_min (from above, converging as the sweeps
+    // iterate); the smallest over all sweeps and load cases of a cell divides
+    // that cell's estimate.  A warm start begins with the value of the slot's
+    // previous call.
+    const bool bound = o.ch_tol > 0;
+    std::vector<double> theta(nb, INFINITY);
+    if (warm && (int)theta_prev.size() >= nb)
+      for (int s_ = 0; s_ < nb; s_++) theta[s_] = theta_prev[s_];
+    if (bound) lanczos.alloc(sizeof(double) * 2 * (size_t)kInnerMaxIter * kMaxR);
+    std::vector<double> lzh;
     int nact = 0;
     for (int t = 0; t < R; t++) nact += active[t];
     for (int sweep = 0; sweep < max_outer && nact; sweep++) {
@@ -1106,7 +1197,11 @@ void Context::homogenize(double *C, femh_stats *st) {
             }
           }
           if (!any) continue;
-          ch_est[s] = (trC > 0) ? std::sqrt(6.0) * srz / (g.V * trC) : 1.0;
+          // no Ritz value yet (a warm start's first sweep with no earlier
+          // measurement): no bound, so the sweep runs
+          ch_est[s] = (trC > 0 && std::isfinite(theta[s]) && theta[s] > 0)
+                          ? std::sqrt(6.0) * srz / (g.V * trC) / theta[s]
+                          : INFINITY;
           if (ch_est[s] <= o.ch_tol)
             for (int k = 0; k < kRHS; k++)
               if (active[kRHS * s + k]) {
@@ -1117,9 +1212,44 @@ void Context::homogenize(double *C, femh_stats *st) {
         if (!nact) break;
       }
       std::vector<int> act_in = active;
-      pcg<float>(h32, r32.as<float>(), x32.as<float>(), r32.as<float>(), z32.as<float>(),
-                 p32.as<float>(), q32.as<float>(), act_in, o.inner_tol > 0 ? o.inner_tol : 1e-4,
-                 std::min(o.max_iter, kInnerMaxIter), rr.data(), it_in.data(), rel_in.data(), true);
+      {
+        // the record covers this FP32 sweep only (at most kInnerMaxIter
+        // iterations), never the FP64 fallback
+        struct Record {
+          Context &c;
+          Record(Context &c_, bool on) : c(c_) {
+            if (on) {
+              c.lz_alpha = c.lanczos.as<double>();
+              c.lz_beta = c.lz_alpha + (size_t)kInnerMaxIter * kMaxR;
+            }
+          }
+          ~Record() { c.lz_alpha = c.lz_beta = nullptr; }
+        } rec(*this, bound);
+        pcg<float>(h32, r32.as<float>(), x32.as<float>(), r32.as<float>(), z32.as<float>(),
+                   p32.as<float>(), q32.as<float>(), act_in, o.inner_tol > 0 ? o.inner_tol : 1e-4,
+                   std::min(o.max_iter, kInnerMaxIter), rr.data(), it_in.data(), rel_in.data(), true);
+      }
+      if (bound) {
+        int kmax = 0;
+        for (int t = 0; t < R; t++)
+          if (active[t]) kmax = std::max(kmax, it_in[t]);
+        if (kmax > 0) {
+          // rows 0..kmax-1 of alpha and beta (beta rows stop one short)
+          lzh.resize((size_t)2 * kInnerMaxIter * kMaxR);
+          const double *la = lanczos.as<double>(), *lb = la + (size_t)kInnerMaxIter * kMaxR;
+          FEMH_HIP(hipMemcpyAsync(lzh.data(), la, sizeof(double) * (size_t)kmax * kMaxR,
+                                  hipMemcpyDeviceToHost, stream));
+          FEMH_HIP(hipMemcpyAsync(lzh.data() + (size_t)kInnerMaxIter * kMaxR, lb,
+                                  sizeof(double) * (size_t)kmax * kMaxR, hipMemcpyDeviceToHost, stream));
+          FEMH_HIP(hipStreamSynchronize(stream));
+          for (int t = 0; t < R; t++) {
+            if (!active[t] || it_in[t] < 1) continue;
+            const double th = lanczos_min(lzh.data() + t, lzh.data() + (size_t)kInnerMaxIter * kMaxR + t,
+                                          kMaxR, it_in[t]);
+            if (std::isfinite(th) && th > 0) theta[t / kRHS] = std::min(theta[t / kRHS], th);
+          }
+        }
+      }
       set_act(A_OUT, active);
       if (x_set) {
         launch_add_f32_to_f64(g.V, R, x32.as<float>(), x.as<double>(), ao, stream);
@@ -1186,13 +1316,14 @@ void Context::homogenize(double *C, femh_stats *st) {
       }
     }
     if (nact) status = FEMH_NOT_CONVERGED;
+    theta_prev = theta;
     // every right-hand side vanished before a sweep: u = 0
-    if (!x_set) FEMH_HIP(hipMemsetAsync(x.p, 0, field_bytes(g, nb, sizeof(double)), stream));
+    if (!x_set) launch_fill<double>(field_bytes(g, nb, sizeof(double)) / sizeof(double), x.as<double>(), 0.0, stream);
   }
   // C_H and the adjoint do not see constant shifts of the fields (K0 maps
   // them to zero); the mean is taken out when the fields are read out
   fields_centered = false;
-  FEMH_HIP(hipEventRecord(ev[2], stream));
+  if (timed) FEMH_HIP(hipEventRecord(ev[2], stream));
   unsigned cmask = 0;
   for (int smp = 0; smp < nb; smp++) cmask |= cubic_ok[smp] ? 1u << smp : 0u;
   if (cmask) {
@@ -1226,8 +1357,10 @@ void Context::homogenize(double *C, femh_stats *st) {
   flush_profile();
   have_fields = true;
   float ms1 = 0, ms2 = 0;
-  FEMH_HIP(hipEventElapsedTime(&ms1, ev[0], ev[1]));
-  FEMH_HIP(hipEventElapsedTime(&ms2, ev[1], ev[2]));
+  if (timed) {
+    FEMH_HIP(hipEventElapsedTime(&ms1, ev[0], ev[1]));
+    FEMH_HIP(hipEventElapsedTime(&ms2, ev[1], ev[2]));
+  }
   const double ms_total =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (st) {
@@ -1247,7 +1380,6 @@ void Context::homogenize(double *C, femh_stats *st) {
       o_.levels = nlevels;
     }
   }
-  for (auto &e : ev) (void)hipEventDestroy(e);
   for (int k = 0; k < 36 * nb; k++)
     if (!std::isfinite(C[k])) throw Error(FEMH_NONFINITE, "C_H is not finite");
   prev_nb = nb;
@@ -1345,6 +1477,8 @@ void Context::apply_level(int level, const double *u, double *y) {
   if (level < 0 || level >= nlevels) throw Error(FEMH_BAD_ARG, "level out of range");
   Level<double> &L = h64.lv[level];
   const size_t n = (size_t)3 * L.g.V;
+  L.x.alloc(field_bytes(L.g, B, sizeof(double)));
+  L.Ad.alloc(field_bytes(L.g, B, sizeof(double)));
   std::vector<int> one(R, 0);
   one[0] = 1;
   set_act(A_POW, one);

@@ -116,6 +116,12 @@ struct Context {
   DBuf palt, palt32;  // PCG: second direction buffer of the fused direction update
   DBuf beta_d;        // PCG: per-RHS beta [kMaxR] (double)
   std::vector<int> fallback_;  // per sample: finished in FP64 this call
+  // Lanczos record of the mixed path's correction sweeps: alpha and beta of
+  // every PCG iteration and right-hand side ([kInnerMaxIter][kMaxR] each);
+  // pcg writes them while lz_alpha / lz_beta are set
+  DBuf lanczos;
+  double *lz_alpha = nullptr, *lz_beta = nullptr;
+  std::vector<double> theta_prev;  // per slot: smallest Ritz value of the last call
   bool modulus_input = false;  // last load gave moduli (dE/drho = 1)
   int nlevels = 1;
   // launch shapes (0: the built-in choice): node planes per workgroup of the
@@ -141,6 +147,8 @@ struct Context {
   KStat kst[2];
   std::vector<Pending> pending;
   std::vector<hipEvent_t> ev_pool;
+  hipEvent_t ev_call[3] = {};  // homogenize: start, end of set-up, end of solve
+  hipEvent_t ev_tune[2] = {};  // autotuner timing
   void set_profiling(bool on);
   void flush_profile();
   hipEvent_t take_event();

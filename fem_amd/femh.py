@@ -51,7 +51,7 @@ class Opts(ctypes.Structure):
                 ("apply_kernel", ctypes.c_int), ("batch", ctypes.c_int),
                 ("coarse_nu", ctypes.c_int), ("warm_start", ctypes.c_int),
                 ("structure", ctypes.c_int), ("autotune", ctypes.c_int),
-                ("streams", ctypes.c_int)]
+                ("streams", ctypes.c_int), ("batch_threads", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -116,7 +116,7 @@ def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond
               levels=0, nu1=2, nu2=2, cheby_ratio=20.0, coarse_iters=64, tol=1e-10,
               max_iter=5000, precision="fp64", ch_tol=0.0, check_every=1, device=0,
               inner_tol=1e-4, max_outer=12, apply_kernel="tiled", batch=1, warm_start=False,
-              coarse_nu=3, structure="general", autotune=False, streams=1):
+              coarse_nu=3, structure="general", autotune=False, streams=1, batch_threads=0):
     o = Opts()
     lib().femh_default_opts(ctypes.byref(o))
     o.nx, o.ny, o.nz = (int(s) for s in shape)
@@ -136,6 +136,7 @@ def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond
     o.structure = {"general": 0, "cubic": 1}[structure]
     o.autotune = int(bool(autotune))
     o.streams = int(streams)
+    o.batch_threads = int(batch_threads)
     return o
 
 

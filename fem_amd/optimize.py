@@ -44,6 +44,7 @@ class BatchedDescent:
                                 warm_start=True, **solver) for g in self.groups]
         self.solves = 0
         self.est = []
+        self.last_C = None  # C_H at the start point of the last step (its forward solve)
         self._pool = ThreadPoolExecutor(max_workers=min(len(self.groups), 4))
 
     def evaluate(self, x, cells=None, grad=False):
@@ -90,6 +91,7 @@ class BatchedDescent:
     def step(self, x, max_step=0.5, max_halvings=10):
         """One projected-gradient step for every cell; returns (x_new, J_new, accepted)."""
         C, G = self.evaluate(x, grad=True)
+        self.last_C = C
         J0 = self.J(C, x)
         norm = np.abs(G).max(axis=1)
         h = np.where(norm > 0, max_step / np.maximum(norm, 1e-300), 0.0)

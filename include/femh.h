@@ -54,7 +54,7 @@
 extern "C" {
 #endif
 
-#define FEMH_ABI_VERSION 2
+#define FEMH_ABI_VERSION 3
 
 enum femh_status {
   FEMH_OK = 0,
@@ -132,6 +132,9 @@ typedef struct femh_opts {
                            last batch runs on the primary context, so femh_gradient
                            and the field calls still refer to it.  Profiling runs
                            the batches one at a time.  Default 1 */
+  int batch_threads;    /* femh_homogenize_batch: host threads that keep batches in flight,
+                           each on its own sub-context and stream (0 = 8).  1 runs the
+                           batches one after another on one thread */
 } femh_opts;
 
 typedef struct femh_stats {

@@ -70,6 +70,14 @@ int or_homogenize(int nx, int ny, int nz, const double *E, double nu,
                   const or_solver_opts *o, double *C, double *u,
                   int *iters, double *relres);
 
+/* Same, for the load cases set in case_mask (bit lc; the others return zero
+ * fields; C is written only when all six are set), with wall seconds of the
+ * set-up, of the solves and of C_H in seconds[3] (may be NULL) -- the CPU
+ * baseline times a bounded sample with it. */
+int or_homogenize_cases(int nx, int ny, int nz, const double *E, double nu,
+                        const or_solver_opts *o, int case_mask, double *C, double *u,
+                        int *iters, double *relres, double *seconds);
+
 /* Energy-form C_H from fluctuation fields (NumericalCoarseningUsingMultiGrid
  * .cpp:290-313, with the affine part re-added element-locally). */
 void or_ch_energy(int nx, int ny, int nz, const double *E, double nu,

@@ -110,6 +110,23 @@ def homogenize(E, nu, **kw):
     return dict(C=C.reshape(6, 6), u=u, iters=it, relres=rr, status=st)
 
 
+def homogenize_timed(E, nu, case_mask=63, **kw):
+    """homogenize() for the load cases in case_mask, with the wall seconds of
+    the set-up, of those solves and of C_H (dict key "seconds")."""
+    E = _E(E)
+    V = E.size
+    o = opts(**kw)
+    C = np.zeros(36)
+    u = np.zeros((6, 3, V))
+    it = np.zeros(6, dtype=np.int32)
+    rr = np.zeros(6)
+    sec = np.zeros(3)
+    st = lib().or_homogenize_cases(*[ctypes.c_int(n) for n in E.shape], _dp(E), ctypes.c_double(nu),
+                                   ctypes.byref(o), ctypes.c_int(case_mask), _dp(C), _dp(u), _dp(it),
+                                   _dp(rr), _dp(sec))
+    return dict(C=C.reshape(6, 6), u=u, iters=it, relres=rr, status=st, seconds=sec)
+
+
 def ch_energy(E, nu, u):
     E = _E(E)
     u = np.ascontiguousarray(u, dtype=np.float64)

@@ -6,7 +6,11 @@ the CPU FP64 oracle (run here, on the CPU; the GPU tests compare against it).
   simp64_s0         64^3 SIMP rho = clamp(0.5+0.5 g, 1e-3, 1), g periodic GRF
                     (corr 4 voxels), E = rho^3, nu=0.3, multigrid-PCG to 1e-12
   simp16_s{seed}    16^3 SIMP (corr 4), for gradient checks
+  uneven512_c{idx}  cell idx of the 512-cell uneven batch (batch_uneven(512, 3)):
+                    its first four 16^3 cells (binary, fill U[0.2,0.8], E_v=1e-9,
+                    nu=0.3), multigrid-PCG to 1e-12
 Inputs are regenerated from fem_amd.workloads with the stated seeds.
+Only entries missing from the file are computed (--all recomputes every one).
 """
 import json
 import os
@@ -20,7 +24,30 @@ from fem_amd import workloads as W  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 
+def uneven_cells(out):
+    grids = W.batch_uneven(512, 3)
+    idx = [i for i, g in enumerate(grids) if g.shape == (16, 16, 16)][:4]
+    for i in idx:
+        key = f"uneven512_c{i}"
+        if key in out:
+            continue
+        r = O.homogenize(grids[i], 0.3, precond="mg", tol=1e-12, max_iter=20000)
+        assert r["status"] == 0, (key, r["iters"])
+        out[key] = dict(n=16, cell=i, workload="batch_uneven(512, 3)", kind="binary",
+                        fill=float(grids[i].mean()), E_void=1e-9, nu=0.3,
+                        C=r["C"].ravel().tolist(), iters=r["iters"].tolist())
+        print(key, r["iters"], flush=True)
+
+
 def main():
+    path = os.path.join(HERE, "ch_goldens.json")
+    if "--all" not in sys.argv and os.path.exists(path):
+        with open(path) as f:
+            out = json.load(f)
+        uneven_cells(out)
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+        return
     out = {}
     for seed in (0, 1):
         E = W.binary_modulus(W.binary_cube(16, seed))
@@ -40,7 +67,8 @@ def main():
                             C=r["C"].ravel().tolist(), iters=r["iters"].tolist(),
                             cpu_seconds=time.time() - t)
     print("simp64", r["iters"], flush=True)
-    with open(os.path.join(HERE, "ch_goldens.json"), "w") as f:
+    uneven_cells(out)
+    with open(path, "w") as f:
         json.dump(out, f, indent=1)
 
 

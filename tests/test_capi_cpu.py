@@ -30,13 +30,14 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_default_opts():
     L = fem_amd.lib()
-    assert L.femh_abi_version() == 2
+    assert L.femh_abi_version() == 3
     o = femh.make_opts((8, 8, 8))
     assert o.nu == 0.3 and o.tol == 1e-10 and o.precond == 1 and o.nu1 == 2
     # the library's own defaults for the fields added in version 2
     d = femh.Opts()
     L.femh_default_opts(ctypes.byref(d))
     assert (d.structure, d.autotune, d.streams) == (0, 0, 1)
+    assert d.batch_threads == 0  # version 3: femh_homogenize_batch keeps its 8 host threads
     o = femh.make_opts((8, 8, 8), structure="cubic", autotune=True, streams=3)
     assert (o.structure, o.autotune, o.streams) == (1, 1, 3)
     with pytest.raises(KeyError):
@@ -92,3 +93,4 @@ def test_host_facade_library_exports_reference_classes():
                  "femh::MaterialOptimizer::run3D", "femh::gradientDescent", "femh::lineSearch",
                  "femh::check_df", "femh::PiecewiseConstantCubic3D"):
         assert name in syms, name
+

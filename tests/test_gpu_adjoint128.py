@@ -22,12 +22,16 @@ def iso(E, nu):
     return C
 
 
-def test_adjoint_128_matches_central_differences():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_adjoint_128_matches_central_differences(precision):
+    """precision "mixed": FP32 multigrid-PCG corrections inside FP64
+    iterative refinement to the same residual target, the adjoint formed in
+    FP64 from those fields."""
     n, hstep = 128, 1e-6
     rho = W.simp_density(n, 4, 8.0)
     Ct = iso(0.3, 0.25)
     h = Homogenizer((n, n, n), nu=NU, penal=3.0, tol=1e-12, max_iter=4000, batch=8,
-                    warm_start=True)
+                    warm_start=True, precision=precision)
     C0, st = h.homogenize(rho)
     g = h.gradient(2.0 * (C0 - Ct))
     idx = (W.uniform(12, 64) * rho.size).astype(int)
@@ -45,5 +49,5 @@ def test_adjoint_128_matches_central_differences():
     fd = (J[:, 0] - J[:, 1]) / (2 * hstep)
     ga = g.reshape(-1)[idx]
     err = np.abs(fd - ga).max() / np.abs(g).max()
-    print(f"max |fd - adjoint| / max |adjoint| = {err:.3e} over {len(idx)} voxels")
+    print(f"{precision}: max |fd - adjoint| / max |adjoint| = {err:.3e} over {len(idx)} voxels")
     assert err <= 1e-5, err

@@ -365,6 +365,24 @@ def test_goldens_reproduce_with_oracle():
     assert np.linalg.norm(r["C"].ravel() - g["C"]) / np.linalg.norm(g["C"]) < 1e-12
 
 
+def test_uneven_batch_goldens_regenerate_and_agree_across_solvers():
+    """The uneven-batch goldens (written with multigrid-PCG) are the cells the
+    generator makes today, and Jacobi-PCG -- a different preconditioner and
+    iteration path -- gives the same C_H."""
+    import json
+    G = json.load(open(os.path.join(HERE, "golden", "ch_goldens.json")))
+    keys = sorted(k for k in G if k.startswith("uneven512_c"))
+    assert len(keys) == 4
+    grids = W.batch_uneven(512, 3)
+    for k in keys[:2]:
+        g = G[k]
+        E = grids[g["cell"]]
+        assert E.shape == (16, 16, 16) and abs(E.mean() - g["fill"]) < 1e-15
+        r = O.homogenize(E, g["nu"], precond="jacobi", tol=1e-12, max_iter=40000)
+        assert r["status"] == 0
+        assert np.linalg.norm(r["C"].ravel() - g["C"]) / np.linalg.norm(g["C"]) < 1e-10, k
+
+
 # ---------------------------------------------------------------- adjoint
 def test_oracle_gradient_matches_central_differences():
     # single thread: the reduction order (hence the last bits of J) must not
