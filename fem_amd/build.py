@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libfemh.so")
 HOST_LIB = os.path.join(HERE, "libfemh_host.so")
 CXX = os.environ.get("CXX", "g++")
 
-SOURCES = ["kernels.hip", "ops.hip", "tables.cpp", "solver.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "ops.hip", "tables.cpp", "solver.cpp", "capi.cpp", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FEMH_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

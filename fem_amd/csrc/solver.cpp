@@ -30,18 +30,41 @@ void hip_check(hipError_t e, const char *what) {
   }
 }
 
+namespace {
+thread_local Arena *t_arena = nullptr;
+}
+
+Arena *current_arena() { return t_arena; }
+ArenaScope::ArenaScope(Arena *a) : prev(t_arena) { t_arena = a; }
+ArenaScope::~ArenaScope() { t_arena = prev; }
+
+void *Arena::take(size_t n) {
+  const size_t at = (used + 255) & ~(size_t)255;
+  if (at > cap || n > cap - at)
+    throw Error(FEMH_BAD_ARG, "the call needs more device scratch than the fixed region holds (" +
+                                  std::to_string(at + n) + " > " + std::to_string(cap) + " bytes)");
+  used = at + n;
+  return base + at;
+}
+
 void DBuf::alloc(size_t n) {
   if (n <= bytes && p) return;
   release();
   if (n == 0) return;
-  FEMH_HIP(hipMalloc(&p, n));
+  if (Arena *a = current_arena()) {
+    p = a->take(n);
+    borrowed = true;
+  } else {
+    FEMH_HIP(hipMalloc(&p, n));
+  }
   bytes = n;
 }
 
 void DBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p && !borrowed) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
+  borrowed = false;
 }
 
 // bytes of an [R][3][V] field for capacity B
@@ -73,8 +96,13 @@ Context::Context(const femh_opts &opts) : o(opts) {
   B = o.batch > 0 ? o.batch : 1;
   g = Dims{o.nx, o.ny, o.nz, o.nx * o.ny * o.nz};
   FEMH_HIP(hipSetDevice(o.device));
-  FEMH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  own_stream = true;
+  Arena *arena = current_arena();
+  if (arena) {
+    stream = arena->stream;
+  } else {
+    FEMH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    own_stream = true;
+  }
   tab = build_tables(o.nu);
   std::memcpy(modecoef.diag, tab.L.diag, sizeof(modecoef.diag));
   std::memcpy(modecoef.cross, tab.L.cross, sizeof(modecoef.cross));
@@ -104,9 +132,12 @@ Context::Context(const femh_opts &opts) : o(opts) {
   ch_out.alloc(sizeof(double) * 21 * B);
   act.alloc(sizeof(int) * kActSlots * kMaxR);
   // timing events of every homogenize call and of the autotuner, created
-  // once: no event is created or destroyed per call
-  for (auto &e : ev_call) FEMH_HIP(hipEventCreate(&e));
-  for (auto &e : ev_tune) FEMH_HIP(hipEventCreate(&e));
+  // once: no event is created or destroyed per call (none on an arena: the
+  // set-up and solve times are then not measured)
+  if (!arena) {
+    for (auto &e : ev_call) FEMH_HIP(hipEventCreate(&e));
+    for (auto &e : ev_tune) FEMH_HIP(hipEventCreate(&e));
+  }
   FEMH_HIP(hipGetLastError());
   FEMH_HIP(hipStreamSynchronize(stream));
 }

@@ -25,24 +25,48 @@ struct Error : std::runtime_error {
 void hip_check(hipError_t e, const char *what);
 #define FEMH_HIP(x) ::femh::hip_check((x), #x)
 
-// owning device allocation
+// Device memory arena: a caller-provided device region from which DBuf::alloc
+// carves buffers (bump allocation, 256-byte aligned) while an ArenaScope is
+// active on the calling thread.  Such buffers are never freed: the region
+// belongs to the caller.  A context built inside a scope also works on the
+// arena's stream and creates no stream or event of its own.
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  hipStream_t stream = nullptr;
+  void *take(size_t n);
+};
+struct ArenaScope {
+  explicit ArenaScope(Arena *a);
+  ~ArenaScope();
+  ArenaScope(const ArenaScope &) = delete;
+  ArenaScope &operator=(const ArenaScope &) = delete;
+  Arena *prev;
+};
+Arena *current_arena();
+
+// device allocation (owning, unless carved from an arena)
 struct DBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  bool borrowed = false;  // from an arena: not freed here
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
-  DBuf(DBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+  DBuf(DBuf &&o) noexcept : p(o.p), bytes(o.bytes), borrowed(o.borrowed) {
     o.p = nullptr;
     o.bytes = 0;
+    o.borrowed = false;
   }
   DBuf &operator=(DBuf &&o) noexcept {
     if (this != &o) {
       release();
       p = o.p;
       bytes = o.bytes;
+      borrowed = o.borrowed;
       o.p = nullptr;
       o.bytes = 0;
+      o.borrowed = false;
     }
     return *this;
   }

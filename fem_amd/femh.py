@@ -29,7 +29,9 @@ EXPORTS = ("femh_abi_version", "femh_default_opts", "femh_create", "femh_destroy
            "femh_gradient", "femh_energy_terms", "femh_get_fields", "femh_thermal_expansion",
            "femh_launch_shapes", "femh_homogenize_batch",
            "femh_prepare", "femh_apply_level", "femh_level_dims", "femh_set_profiling",
-           "femh_kernel_stats", "femh_element_stiffness", "femh_last_error")
+           "femh_kernel_stats", "femh_element_stiffness", "femh_last_error",
+           "femh_homogenize_device_noalloc")
+NOALLOC_MAX_VOXELS = 32768
 
 
 class FemhError(RuntimeError):
@@ -104,6 +106,8 @@ def lib():
         L.femh_set_profiling.argtypes = [vp, ctypes.c_int]
         L.femh_kernel_stats.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.femh_element_stiffness.argtypes = [ctypes.c_double, ctypes.c_double, vp]
+        L.femh_homogenize_device_noalloc.argtypes = [ctypes.POINTER(Opts), vp, vp, vp, vp,
+                                                     ctypes.c_char_p, ctypes.c_size_t]
         _LIB = L
     return _LIB
 
@@ -294,6 +298,22 @@ class Homogenizer:
         y = np.zeros_like(u)
         self._check(lib().femh_apply_level(self._h, level, _ptr(u), _ptr(y)))
         return y
+
+
+def homogenize_device_noalloc(shape, d_rho_ptr, d_C_ptr, stream_ptr=None, **kw):
+    """One cell through femh_homogenize_device_noalloc (Jacobi-PCG, FP64; device
+    pointers).  kw are make_opts() options; precond defaults to "jacobi".
+    Returns the stats dict; raises FemhError on failure."""
+    kw.setdefault("precond", "jacobi")
+    o = make_opts(shape, **kw)
+    s = Stats()
+    buf = ctypes.create_string_buffer(512)
+    st = lib().femh_homogenize_device_noalloc(ctypes.byref(o), ctypes.c_void_p(d_rho_ptr),
+                                              ctypes.c_void_p(d_C_ptr), ctypes.c_void_p(stream_ptr or 0),
+                                              ctypes.byref(s), buf, 512)
+    if st != OK:
+        raise FemhError(st, buf.value.decode(errors="replace"))
+    return s.as_dict()
 
 
 def element_stiffness(E=1.0, nu=0.3):

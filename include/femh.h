@@ -239,6 +239,24 @@ int femh_set_profiling(struct femh_ctx *ctx, int enable);
 int femh_kernel_stats(struct femh_ctx *ctx, int which, long long *launches, double *ms,
                       double *bytes, double *flops);
 
+/* Largest grid (voxels) femh_homogenize_device_noalloc accepts. */
+#define FEMH_NOALLOC_MAX_VOXELS 32768
+
+/* femh_homogenize_device without device memory allocation: no hipMalloc,
+ * hipFree, stream or event creation.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (serving loops and harnesses
+ * that forbid it).  The cell (o->nx*o->ny*o->nz <= FEMH_NOALLOC_MAX_VOXELS
+ * voxels, densities interpreted through o->interp) is solved by Jacobi-PCG in
+ * FP64 -- o->precond must be FEMH_PRECOND_JACOBI and o->precision FEMH_FP64,
+ * without warm start or autotuning -- giving the same C_H (device memory, 36
+ * doubles) as femh_homogenize_device with those options.  Work runs on
+ * `stream` (NULL = the default stream); the call returns when it is done.
+ * Every buffer comes from one 64 MiB device region compiled into the library
+ * (one per GPU, held until the process ends); calls on one GPU share it and
+ * are serialised.  err (may be NULL) receives the message of a failed call. */
+int femh_homogenize_device_noalloc(const femh_opts *o, const double *d_rho, double *d_C, void *stream,
+                                   femh_stats *stats, char *err, size_t err_len);
+
 /* Copies the last error message (NUL-terminated) into buf. Returns the
  * status of the last failed call on this context. */
 int femh_last_error(struct femh_ctx *ctx, char *buf, size_t len);

@@ -94,3 +94,20 @@ def test_host_facade_library_exports_reference_classes():
                  "femh::check_df", "femh::PiecewiseConstantCubic3D"):
         assert name in syms, name
 
+
+def test_noalloc_entry_point_rejects_bad_arguments_before_device_work():
+    """Argument checks of femh_homogenize_device_noalloc run on the host
+    before any device call: an oversized grid or a solver other than FP64
+    Jacobi-PCG returns BAD_ARG with a message (null pointers stand in for the
+    device buffers, which are never touched)."""
+    L = fem_amd.lib()
+    buf = ctypes.create_string_buffer(256)
+    fake = ctypes.c_void_p(16)
+    o = femh.make_opts((64, 64, 64), precond="jacobi")
+    st = L.femh_homogenize_device_noalloc(ctypes.byref(o), fake, fake, None, None, buf, 256)
+    assert st == femh.BAD_ARG and b"32768" in buf.value
+    o = femh.make_opts((8, 8, 8), precond="mg")
+    st = L.femh_homogenize_device_noalloc(ctypes.byref(o), fake, fake, None, None, buf, 256)
+    assert st == femh.BAD_ARG and b"Jacobi" in buf.value
+    st = L.femh_homogenize_device_noalloc(None, fake, fake, None, None, buf, 256)
+    assert st == femh.BAD_ARG
