@@ -155,6 +155,8 @@ void femh_default_opts(femh_opts *o) {
   o->streams = 1;
   o->coarse_nu = 3;
   o->batch_threads = 0;
+  o->graph = 1;
+  o->adapt_inner = 1;
 }
 
 int femh_create(const femh_opts *o, femh_ctx **out) {
@@ -235,6 +237,12 @@ int femh_energy_terms(femh_ctx *ctx, double *W) {
   });
 }
 
+int femh_coarse_strain_tensor(femh_ctx *ctx, double *C) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!C) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] { ctx->c->coarse_strain(C); });
+}
+
 int femh_thermal_expansion(femh_ctx *ctx, const double *alpha, double *beta, double *alpha_eff) {
   if (int s = need_ctx(ctx)) return s;
   if (!alpha) return FEMH_BAD_ARG;
@@ -248,6 +256,32 @@ int femh_launch_shapes(femh_ctx *ctx, int *shapes, int n) {
   const int L = c.nlevels;
   for (int i = 0; i < n && i < L; i++) shapes[i] = i == 0 ? c.shape_fine : c.xfer_shape(i - 1);
   return L;
+}
+
+int femh_set_launch_shapes(femh_ctx *ctx, const int *shapes, int n) {
+  if (int s = need_ctx(ctx)) return s;
+  if (n < 0 || (n > 0 && !shapes)) return FEMH_BAD_ARG;
+  femh::Context &c = *ctx->c;
+  if (n > 0) {
+    const int tile = (shapes[0] >> 8) & 0xff, planes = shapes[0] & 0xff;
+    if (shapes[0] < 0 || (shapes[0] >> 16) != 0 || tile >= femh::kNumTileShapes || planes > 64) {
+      ctx->err = "fine-operator shape: tile variant 0..3 in bits 8-15, at most 64 planes in bits 0-7";
+      ctx->last_status = FEMH_BAD_ARG;
+      return FEMH_BAD_ARG;
+    }
+  }
+  for (int i = 1; i < n; i++)
+    if (shapes[i] < 0 || shapes[i] > 8) {
+      ctx->err = "transfer shape: 0 (built-in) or 1..8 coarse planes per thread";
+      ctx->last_status = FEMH_BAD_ARG;
+      return FEMH_BAD_ARG;
+    }
+  if (n > 0) c.shape_fine = shapes[0];
+  if (n > 1) {
+    if ((int)c.shape_xfer.size() < n - 1) c.shape_xfer.resize(n - 1, 0);
+    for (int i = 1; i < n; i++) c.shape_xfer[i - 1] = shapes[i];
+  }
+  return FEMH_OK;
 }
 
 int femh_get_fields(femh_ctx *ctx, double *u) {

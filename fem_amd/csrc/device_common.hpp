@@ -29,6 +29,32 @@ static __device__ __forceinline__ double block_sum(double v) {
   return res;
 }
 
+// Sum over a block of NT threads (a multiple of 64); result valid in every
+// thread.
+template <int NT>
+static __device__ __forceinline__ double block_sum_n(double v) {
+  __shared__ double sh[NT / 64];
+  __shared__ double res;
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // pairwise: for 4 waves the order of block_sum
+    double t[NT / 64];
+#pragma unroll
+    for (int i = 0; i < NT / 64; i++) t[i] = sh[i];
+#pragma unroll
+    for (int w2 = 1; w2 < NT / 64; w2 *= 2)
+#pragma unroll
+      for (int i = 0; i + w2 < NT / 64; i += 2 * w2) t[i] += t[i + w2];
+    res = t[0];
+  }
+  __syncthreads();
+  return res;
+}
+
 // Block partial of a reduction: thread 0 stores the block's sum at
 // partials[slot][idx][pidx].  No atomics and no device-scope fences: the
 // partials are consumed by a later kernel (kernel boundaries order them).
@@ -130,6 +156,39 @@ __device__ __forceinline__ void mode_couple(const T (&w)[3][8], T E, const ModeC
         acc += m.x[mode_cross_index(c, d, sm)] * w[d][t];
       }
       z[c][sm] = E * acc;
+    }
+  }
+}
+
+// The coupling folded into the inverse i stage, one component at a time (so
+// the 24 modal forces are never all live): with z = E Lambda w,
+// bottom-face forces bm[c][v] = z[c][v] - z[c][4+v] (plus the carried
+// top-face forces tm of the voxel below when add_prev) and the new top-face
+// forces tm[c][v] = z[c][v] + z[c][4+v].
+template <typename T>
+__device__ __forceinline__ void couple_split(const T (&w)[3][8], T E, const ModeC<T> &m, T (&bm)[3][4],
+                                             T (&tm)[3][4], bool add_prev) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T zc[8];
+    zc[0] = 0;
+#pragma unroll
+    for (int sm = 1; sm < 8; sm++) {
+      T acc = m.d[mode_diag_index(c, sm)] * w[c][sm];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        if (d == c) continue;
+        if (mode_bit(sm, c) == mode_bit(sm, d)) continue;
+        const int t = sm ^ (1 << (2 - c)) ^ (1 << (2 - d));
+        acc += m.x[mode_cross_index(c, d, sm)] * w[d][t];
+      }
+      zc[sm] = E * acc;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const T lo = zc[v] - zc[4 + v], hi = zc[v] + zc[4 + v];
+      bm[c][v] = add_prev ? lo + tm[c][v] : lo;
+      tm[c][v] = hi;
     }
   }
 }

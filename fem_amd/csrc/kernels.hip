@@ -401,13 +401,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int V, T *__restrict__ x,
                                                        int rzslot, int rz_out,
                                                        const int *__restrict__ act, Reduce red,
                                                        int np_pq, int np_rz,
-                                                       double *__restrict__ alpha_out) {
+                                                       double *__restrict__ alpha_out,
+                                                       const int *__restrict__ cnt) {
   const int rr = blockIdx.y;
   if (!act[rr]) return;
   const double pq = sum_parts(red, S_PQ, rr, np_pq);
   const double rz = sum_parts(red, rzslot, rr, np_rz);
   const double alpha = pq != 0.0 ? rz / pq : 0.0;
-  if (alpha_out && blockIdx.x == 0 && threadIdx.x == 0) alpha_out[rr] = alpha;
+  // Lanczos record: row = iterations this right-hand side has completed
+  if (alpha_out && blockIdx.x == 0 && threadIdx.x == 0) alpha_out[(size_t)cnt[rr] * kMaxR + rr] = alpha;
   const T a = (T)alpha;
   const size_t off = (size_t)rr * 3 * V;
   const T *Ds = D ? D + (size_t)(rr / kRHS) * 3 * V : nullptr;
@@ -436,7 +438,8 @@ __global__ __launch_bounds__(kBlock) void k_p_update(int V, T *__restrict__ p,
                                                      const T *__restrict__ z, int rz_new,
                                                      int rz_old, int first,
                                                      const int *__restrict__ act, Reduce red,
-                                                     int np, double *__restrict__ beta_out) {
+                                                     int np, double *__restrict__ beta_out,
+                                                     const int *__restrict__ cnt) {
   const int rr = blockIdx.y;
   if (!act[rr]) return;
   double beta = 0;
@@ -445,7 +448,8 @@ __global__ __launch_bounds__(kBlock) void k_p_update(int V, T *__restrict__ p,
     const double num = sum_parts(red, rz_new, rr, np);
     beta = den != 0.0 ? num / den : 0.0;
   }
-  if (beta_out && blockIdx.x == 0 && threadIdx.x == 0) beta_out[rr] = beta;
+  // (after the convergence check counted this iteration: row cnt - 1)
+  if (beta_out && !first && blockIdx.x == 0 && threadIdx.x == 0) beta_out[(size_t)(cnt[rr] - 1) * kMaxR + rr] = beta;
   const T bt = (T)beta;
   const size_t off = (size_t)rr * 3 * V;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < 3 * V; t += gridDim.x * kBlock)
@@ -474,7 +478,8 @@ __global__ __launch_bounds__(kBlock) void k_permute_case(int n, double *__restri
 __global__ __launch_bounds__(kBlock) void k_beta(int rz_new, int rz_old, int first,
                                                  const int *__restrict__ act, Reduce red, int np,
                                                  double *__restrict__ beta,
-                                                 double *__restrict__ beta_out) {
+                                                 double *__restrict__ beta_out,
+                                                 const int *__restrict__ cnt) {
   const int rr = blockIdx.x;
   if (!act[rr]) return;
   double b = 0;
@@ -485,7 +490,35 @@ __global__ __launch_bounds__(kBlock) void k_beta(int rz_new, int rz_old, int fir
   }
   if (threadIdx.x == 0) {
     beta[rr] = b;
-    if (beta_out) beta_out[rr] = b;
+    if (beta_out && !first) beta_out[(size_t)(cnt[rr] - 1) * kMaxR + rr] = b;
+  }
+}
+
+// PCG convergence test on the device, one block per right-hand side: the
+// residual norm from the S_RR partials gives relres = sqrt(r.r / b.b); a
+// right-hand side that reaches tol (state 1) or turns non-finite (state 2)
+// leaves the activity array, so the rest of the iteration and later
+// iterations skip it.  cnt counts each right-hand side's iterations.
+__global__ __launch_bounds__(kBlock) void k_pcg_check(int *__restrict__ act, Reduce red, int np,
+                                                      const double *__restrict__ bb, double tol,
+                                                      int *__restrict__ cnt, double *__restrict__ relres,
+                                                      int *__restrict__ state) {
+  const int rr = blockIdx.x;
+  if (!act[rr]) return;
+  const double s = sum_parts(red, S_RR, rr, np);
+  if (threadIdx.x == 0) {
+    cnt[rr] += 1;
+    if (!(s <= 1.7976931348623157e308) || s != s) {
+      state[rr] = 2;
+      act[rr] = 0;
+      return;
+    }
+    const double rel = sqrt(s / bb[rr]);
+    relres[rr] = rel;
+    if (rel <= tol) {
+      state[rr] = 1;
+      act[rr] = 0;
+    }
   }
 }
 
@@ -914,24 +947,29 @@ void launch_cheb_first(int V, int R, const T *r, const T *D, T *d, T *x, double 
 template <typename T>
 void launch_pcg_update(int V, int R, T *x, T *r, const T *p, const T *q, const T *D, T *z,
                        int rzslot, int rz_out, const int *act, const Reduce &red, int np_pq,
-                       int np_rz, hipStream_t s, double *alpha_out) {
+                       int np_rz, hipStream_t s, double *alpha_out, const int *cnt) {
   hipLaunchKernelGGL(k_pcg_update<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, x, r, p, q, D, z,
-                     rzslot, rz_out, act, red, np_pq, np_rz, alpha_out);
+                     rzslot, rz_out, act, red, np_pq, np_rz, alpha_out, cnt);
 }
 template <typename T>
 void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int first,
-                     const int *act, const Reduce &red, int np_rz, hipStream_t s, double *beta_out) {
+                     const int *act, const Reduce &red, int np_rz, hipStream_t s, double *beta_out,
+                     const int *cnt) {
   hipLaunchKernelGGL(k_p_update<T>, grid2(3 * V, R), dim3(kBlock), 0, s, V, p, z, rz_new, rz_old,
-                     first, act, red, np_rz, beta_out);
+                     first, act, red, np_rz, beta_out, cnt);
+}
+void launch_pcg_check(int R, int *act, const Reduce &red, int np, const double *bb, double tol, int *cnt,
+                      double *relres, int *state, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_check, dim3(R), dim3(kBlock), 0, s, act, red, np, bb, tol, cnt, relres, state);
 }
 void launch_permute_case(int n, int nb, double *x, int src, int dst, int swap, unsigned mask, hipStream_t s) {
   hipLaunchKernelGGL(k_permute_case, dim3(grid_for(n * n * n), nb), dim3(kBlock), 0, s, n, x, src, dst, swap,
                      mask);
 }
 void launch_beta(int R, int rz_new, int rz_old, int first, const int *act, const Reduce &red, int np_rz,
-                 double *beta, hipStream_t s, double *beta_out) {
+                 double *beta, hipStream_t s, double *beta_out, const int *cnt) {
   hipLaunchKernelGGL(k_beta, dim3(R), dim3(kBlock), 0, s, rz_new, rz_old, first, act, red, np_rz, beta,
-                     beta_out);
+                     beta_out, cnt);
 }
 template <typename T>
 void launch_dot(int V, int R, const T *a, const T *b, int slot, const int *act, const Reduce &red,
@@ -1052,9 +1090,9 @@ void reduce_init(Reduce &red, hipStream_t s) {
                                      const int *, hipStream_t);                                   \
   template void launch_pcg_update<T>(int, int, T *, T *, const T *, const T *, const T *, T *,    \
                                      int, int, const int *, const Reduce &, int, int,             \
-                                     hipStream_t, double *);                                      \
+                                     hipStream_t, double *, const int *);                         \
   template void launch_p_update<T>(int, int, T *, const T *, int, int, int, const int *,          \
-                                   const Reduce &, int, hipStream_t, double *);                   \
+                                   const Reduce &, int, hipStream_t, double *, const int *);      \
   template void launch_dot<T>(int, int, const T *, const T *, int, const int *, const Reduce &,   \
                               hipStream_t);                                                       \
   template void launch_remove_mean<T>(int, int, T *, const int *, const Reduce &, hipStream_t);   \
@@ -1069,5 +1107,6 @@ FEMH_INST(float)
 template void launch_convert<double, float>(size_t, const double *, float *, hipStream_t);
 template void launch_convert<float, double>(size_t, const float *, double *, hipStream_t);
 template void launch_convert<double, double>(size_t, const double *, double *, hipStream_t);
+template void launch_fill<int>(size_t, int *, int, hipStream_t);
 
 }  // namespace femh

@@ -125,20 +125,28 @@ void launch_cheb_first(int V, int R, const T *r, const T *D, T *d, T *x, double 
 // PCG: alpha = rz[rzslot]/pq ; x += alpha p ; r -= alpha q ; rr -> S_RR
 //      jacobi (D != null): z = r / D and rz -> slot rz_out (vec_parts(V) partials)
 // np_pq / np_rz: partial counts of the slots read (set by their producers)
-// alpha_out (may be null): alpha of every active RHS (the Lanczos record)
+// alpha_out (may be null): alpha of every active RHS (the Lanczos record,
+// [iteration][kMaxR], row cnt[r] = the iterations RHS r has completed)
 template <typename T>
 void launch_pcg_update(int V, int R, T *x, T *r, const T *p, const T *q, const T *D, T *z,
                        int rzslot, int rz_out, const int *act, const Reduce &red, int np_pq,
-                       int np_rz, hipStream_t s, double *alpha_out = nullptr);
-// p = z + beta p, beta = rz[new]/rz[old] (first: p = z); beta_out as alpha_out
+                       int np_rz, hipStream_t s, double *alpha_out = nullptr, const int *cnt = nullptr);
+// p = z + beta p, beta = rz[new]/rz[old] (first: p = z); beta_out as
+// alpha_out, row cnt[r] - 1 (after the iteration's convergence check)
 template <typename T>
 void launch_p_update(int V, int R, T *p, const T *z, int rz_new, int rz_old, int first,
                      const int *act, const Reduce &red, int np_rz, hipStream_t s,
-                     double *beta_out = nullptr);
+                     double *beta_out = nullptr, const int *cnt = nullptr);
 // beta[r] = rz[new]/rz[old] per active RHS (first: 0), for the fused
-// direction update of the fine operator (ST_PUPD); beta_out as alpha_out
+// direction update of the fine operator (ST_PUPD); beta_out as for p_update
 void launch_beta(int R, int rz_new, int rz_old, int first, const int *act, const Reduce &red, int np_rz,
-                 double *beta, hipStream_t s, double *beta_out = nullptr);
+                 double *beta, hipStream_t s, double *beta_out = nullptr, const int *cnt = nullptr);
+// PCG convergence check on the device (after the update of an iteration):
+// for every active RHS, cnt += 1 and relres = sqrt(r.r / bb) from the np
+// partials of S_RR; relres <= tol clears act (state 1), a non-finite r.r too
+// (state 2)
+void launch_pcg_check(int R, int *act, const Reduce &red, int np, const double *bb, double tol, int *cnt,
+                      double *relres, int *state, hipStream_t s);
 // cubic structure: load case dst from load case src by an axis swap of the
 // cube grid (swap 0: x <-> y, 1: x <-> z), points and displacement components
 // alike, for the samples s with bit s of mask set

@@ -99,6 +99,7 @@ extern "C" int femh_homogenize_device_noalloc(const femh_opts *o, const double *
     femh_opts oo = *o;
     oo.batch = 1;
     oo.streams = 1;
+    oo.graph = 0;  // recording a launch sequence allocates
     femh::Context c(oo);
     c.load(d_rho, 1, true, false);
     double C[36];

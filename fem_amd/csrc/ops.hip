@@ -116,28 +116,52 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, 
   }
 }
 
-static inline int tiled_chunk(const Dims &g, int R, int want) {
-  const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
+#ifndef FEMH_WAVES64
+#define FEMH_WAVES64 1
+#endif
+constexpr int kWaves64 = FEMH_WAVES64;  // FP64 fine operator: waves per SIMD the registers allow
+
+// Tile shapes (TJ x TK owned node columns per workgroup, TJ*TK threads).
+// Launch shape encoding: bits 0-7 node planes per workgroup (0 = built-in),
+// bits 8-15 tile variant (0 = built-in, i.e. 16 x 16).
+struct TileShape {
+  int tj, tk;
+};
+constexpr TileShape kTileShapes[kNumTileShapes] = {{16, 16}, {16, 16}, {8, 32}, {16, 32}};
+
+static inline TileShape tile_of(int shape) {
+  int v = (shape >> 8) & 0xff;
+  if (v < 0 || v >= kNumTileShapes) v = 0;
+  return kTileShapes[v];
+}
+
+static inline int tiled_tiles(const Dims &g, const TileShape &ts) {
+  return ((g.ny + ts.tj - 1) / ts.tj) * ((g.nz + ts.tk - 1) / ts.tk);
+}
+
+static inline int tiled_chunk(const Dims &g, int R, int shape) {
+  const TileShape ts = tile_of(shape);
+  const int tiles = tiled_tiles(g, ts);
+  const int want = shape & 0xff;
   // planes per chunk: `want` when given (an autotuned shape), else about 16
-  // (halo slab overhead 1/16) but enough chunks to put ~2 workgroups per CU
-  // across the right-hand sides; at most kMaxPartials workgroups per
-  // right-hand side either way
+  // (halo slab overhead 1/16) but enough chunks to put ~2 workgroups of 256
+  // threads per CU across the right-hand sides; at most kMaxPartials
+  // workgroups per right-hand side either way
   int TI = 16;
   if (want > 0)
     TI = want;
   else
-    while (TI > 2 && (long)tiles * ((g.nx + TI - 1) / TI) * R < 512) TI /= 2;
+    while (TI > 2 && (long)tiles * ((g.nx + TI - 1) / TI) * R * ts.tj * ts.tk < 512L * 256) TI /= 2;
   if (TI > g.nx) TI = g.nx;
   while ((long)tiles * ((g.nx + TI - 1) / TI) > kMaxPartials) TI++;
   return TI;
 }
 
-int tiled_slabs(const Dims &g, int R, int want) { return tiled_chunk(g, R, want); }
+int tiled_slabs(const Dims &g, int R, int shape) { return tiled_chunk(g, R, shape); }
 
-int tiled_parts(const Dims &g, int R, int want) {
-  const int tiles = ((g.ny + kTile - 1) / kTile) * ((g.nz + kTile - 1) / kTile);
-  const int TI = tiled_chunk(g, R, want);
-  return tiles * ((g.nx + TI - 1) / TI);
+int tiled_parts(const Dims &g, int R, int shape) {
+  const int TI = tiled_chunk(g, R, shape);
+  return tiled_tiles(g, tile_of(shape)) * ((g.nx + TI - 1) / TI);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b to XCD
@@ -150,47 +174,78 @@ __device__ __forceinline__ int xcd_order(int b, int n) {
   return x * per + (x < rem ? x : rem) + slot;
 }
 
+// raw buffer access: base pointer and byte range in a resource, one 32-bit
+// lane offset plus a scalar offset per access (no 64-bit address arithmetic)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void *p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float *) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double *) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 // grid: one dimension of tiles * chunks * R workgroups; logical order (cell,
 // chunk, tile, load case), load case fastest, so that the six load cases of a
 // tile (same moduli and diagonal) and then the neighbouring tiles follow each
-// other on one XCD
-template <typename T, int ST, int EP>
-__global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g, const T *__restrict__ E, ModeC<T> mc,
-                                                  OpArgs<T> a, int TI, int tiles_k, int tiles,
-                                                  int chunks) {
+// other on one XCD.
+//
+// LDS: staged node planes in a two-buffer ring, node-interleaved
+// (up[buf][3 * slot + c]), so that the 12 values of a voxel face and the
+// owned node's value are immediate offsets from one address; eo holds each
+// voxel's 12 bottom-face forces (odd stride: no bank conflicts).  A buffer is
+// overwritten two slabs after it was filled, behind the barrier that closes
+// the slab in between (every read of it -- the FP64 bottom face, the owned
+// node's staged value -- happens before that barrier).
+template <typename T, int ST, int EP, int TJ, int TK>
+__global__ __launch_bounds__(TJ * TK, sizeof(T) == 4 ? 4 : kWaves64) void k_tiled(Dims g, const T *__restrict__ E,
+                                                                           ModeC<T> mc, OpArgs<T> a, int TI,
+                                                                           int tiles_k, int tiles, int chunks) {
+  constexpr int NT = TJ * TK;                           // threads = owned nodes per plane
+  constexpr int NJ = TJ + 2, NK = TK + 2, NS = NJ * NK;  // staged nodes (one-node halo)
+  constexpr int EJ = TJ + 1, EK = TK + 1, NE = EJ * EK;  // voxels touching the tile
+  constexpr int QS = (NS + NT - 1) / NT, QE = (NE + NT - 1) / NT;
+  static_assert(NT % 64 == 0 && QS <= 2 && QE <= 2, "tile shape");
+  constexpr unsigned SZ = sizeof(T);
   const int L = xcd_order(blockIdx.x, gridDim.x);
   const int lc = L % kRHS, tile = (L / kRHS) % tiles, chunk = (L / (kRHS * tiles)) % chunks;
   const int smp = L / (kRHS * tiles * chunks);
   const int r = smp * kRHS + lc;
   if (!a.act[r]) return;
-  E += (size_t)smp * g.V;
-  const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * g.V : nullptr;
-  const T ti = a.theta_inv[smp], c1 = a.c1[smp], c2 = a.c2[smp];
+  const int V = g.V;
+  const unsigned VB = (unsigned)V * SZ;  // bytes per component plane
+  const __amdgpu_buffer_rsrc_t rE = brsrc(E + (size_t)smp * V, VB);
+  const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * V : nullptr;
+  const size_t off = (size_t)r * 3 * V;
+  const __amdgpu_buffer_rsrc_t rU = brsrc(a.u + off, 3 * VB);
+  const __amdgpu_buffer_rsrc_t rU2 = brsrc(ST == ST_PUPD ? a.u2 + off : a.u + off, 3 * VB);
+  const __amdgpu_buffer_rsrc_t rD = brsrc(Ds ? Ds : a.u + off, 3 * VB);
+  const T ti = a.cheb ? (T)a.cheb[smp] : (T)0, c1 = a.cheb ? (T)a.cheb[kMaxB + smp] : (T)0,
+          c2 = a.cheb ? (T)a.cheb[2 * kMaxB + smp] : (T)0;
   const T bt = ST == ST_PUPD ? (T)a.beta[r] : (T)0;
-  __shared__ T up[3][3][kTN * kTN];
-  __shared__ T eo[kTE * kTE * kEOH];
+  __shared__ T up[2][3 * NS];
+  __shared__ T eo[NE * kEOH];
   const int tid = threadIdx.x;
-  // staging slots and voxels (324 and 289 of them) go to threads in an order
-  // rotated by a wave per workgroup, so that the second round of the extra
-  // ones does not fall on the same wave -- hence the same SIMD -- in every
-  // workgroup; owned nodes (the epilogue) keep the plain thread order
-  const int vt = (tid + 64 * (L & 3)) & (kBlock - 1);
-  const int j0 = (tile / tiles_k) * kTile, k0 = (tile % tiles_k) * kTile;
+  // staging slots and voxels go to threads in an order rotated by a wave per
+  // workgroup, so that the second round of the extra ones does not fall on
+  // the same wave -- hence the same SIMD -- in every workgroup; owned nodes
+  // (the epilogue) keep the plain thread order
+  const int vt = (tid + 64 * (L & (NT / 64 - 1))) & (NT - 1);
+  const int j0 = (tile / tiles_k) * TJ, k0 = (tile % tiles_k) * TK;
   const int i0 = chunk * TI;
   const int iend = min(i0 + TI, g.nx);
-  const int V = g.V;
-  const size_t off = (size_t)r * 3 * V;
   const int plane = g.ny * g.nz;
 
   // per-thread staging slots (fixed across planes): node offsets in a plane
-  int soff[2], sjc[2][2], skc[2][2];
-  float sjw[2], skw[2];
+  int soff[QS], sjc[QS][2], skc[QS][2];
+  float sjw[QS], skw[QS];
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
-    const int t = vt + q * kBlock;
+  for (int q = 0; q < QS; q++) {
+    const int t = vt + q * NT;
     soff[q] = -1;
-    if (t < kTN * kTN) {
-      const int nj = t / kTN, nk = t - nj * kTN;
+    if (t < NS) {
+      const int nj = t / NK, nk = t - nj * NK;
       // slots of a tile wider than the grid wrap more than once
       const int jj = ((j0 - 1 + nj) % g.ny + g.ny) % g.ny, kk = ((k0 - 1 + nk) % g.nz + g.nz) % g.nz;
       soff[q] = jj * g.nz + kk;
@@ -205,47 +260,47 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
     }
   }
   // per-thread voxels (fixed across slabs): modulus offsets in a plane
-  int eoff[2];
+  int eoff[QE];
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
-    const int e = vt + q * kBlock;
+  for (int q = 0; q < QE; q++) {
+    const int e = vt + q * NT;
     eoff[q] = -1;
-    if (e < kTE * kTE) {
-      const int ej = e / kTE, ek = e - ej * kTE;
+    if (e < NE) {
+      const int ej = e / EK, ek = e - ej * EK;
       const int jj = ((j0 - 1 + ej) % g.ny + g.ny) % g.ny, kk = ((k0 - 1 + ek) % g.nz + g.nz) % g.nz;
       eoff[q] = jj * g.nz + kk;
     }
   }
 
-  // staging of node plane ip into LDS buffer buf.  PLAIN / DSCALE are
-  // software-pipelined: fetch() issues the global loads of a plane into
-  // registers one slab ahead, put() scales and stores them.  PROLONG (one
-  // sample only) stages directly.
-  T pf[2][3], pd[2][3];
+  // staging of node plane ip into LDS buffer buf.  PLAIN / DSCALE / PUPD are
+  // software-pipelined: fetch() issues the loads of a plane into registers
+  // one slab ahead, put() scales and stores them.  PROLONG (one sample only)
+  // stages directly.
+  T pf[QS][3], pd[QS][3];
   auto fetch = [&](int ip) {
     const int iw = wrap1(ip, g.nx);
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
+    for (int q = 0; q < QS; q++) {
       if (soff[q] < 0) continue;
-      const int n = iw * plane + soff[q];
+      const unsigned vo = (unsigned)(iw * plane + soff[q]) * SZ;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        pf[q][c] = a.u[off + (size_t)c * V + n];
-        if (ST == ST_DSCALE) pd[q][c] = Ds[(size_t)c * V + n];
-        if (ST == ST_PUPD) pd[q][c] = a.u2[off + (size_t)c * V + n];
+        pf[q][c] = bload(rU, vo, c * VB, (T *)nullptr);
+        if (ST == ST_DSCALE) pd[q][c] = bload(rD, vo, c * VB, (T *)nullptr);
+        if (ST == ST_PUPD) pd[q][c] = bload(rU2, vo, c * VB, (T *)nullptr);
       }
     }
   };
   auto put = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
+    for (int q = 0; q < QS; q++) {
       if (soff[q] < 0) continue;
-      const int t = vt + q * kBlock;
+      T *dst = &up[buf][3 * (vt + q * NT)];
 #pragma unroll
       for (int c = 0; c < 3; c++)
-        up[buf][c][t] = (ST == ST_DSCALE) ? pf[q][c] * (ti * pd[q][c])
-                        : (ST == ST_PUPD) ? pf[q][c] + bt * pd[q][c]
-                                          : pf[q][c];
+        dst[c] = (ST == ST_DSCALE) ? pf[q][c] * (ti * pd[q][c])
+                 : (ST == ST_PUPD) ? pf[q][c] + bt * pd[q][c]
+                                   : pf[q][c];
     }
   };
   auto stage = [&](int buf, int ip) {
@@ -256,9 +311,9 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
     }
     const int iw = wrap1(ip, g.nx);
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
+    for (int q = 0; q < QS; q++) {
       if (soff[q] < 0) continue;
-      const int t = vt + q * kBlock;
+      const int t = vt + q * NT;
       const int n = iw * plane + soff[q];
       T v[3];
 #pragma unroll
@@ -277,14 +332,14 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
             for (int c = 0; c < 3; c++) sum[c] += a.xc[offc + (size_t)c * a.gc.V + N];
           }
 #pragma unroll
-      for (int c = 0; c < 3; c++) up[buf][c][t] = v[c] + w * sum[c];
+      for (int c = 0; c < 3; c++) up[buf][3 * t + c] = v[c] + w * sum[c];
     }
   };
 
-  const int nj = tid / kTile, nk = tid % kTile;  // owned node, output tile coords
+  const int nj = tid / TK, nk = tid % TK;  // owned node, output tile coords
   const int gj = j0 + nj, gk = k0 + nk;
   const bool owns = (gj < g.ny) && (gk < g.nz);
-  const int ot = (nj + 1) * kTN + nk + 1;  // owned node in the staged plane
+  const int ot = 3 * ((nj + 1) * NK + nk + 1);  // owned node in the staged plane
   DotAcc dv;
 
   // Per voxel (three components, corners v = 4 bi + 2 bj + bk) the cube-mode
@@ -297,14 +352,13 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
   // same node plane s, so they are summed before one inverse face transform
   // -- plane s is then complete after a single LDS exchange.
   constexpr bool kReuse = sizeof(T) == 4;
-  T ftop[2][3][4], tmod[2][3][4];
+  T ftop[QE][3][4], tmod[QE][3][4];
   auto face_load = [&](int buf, int ej, int ek, T (&f)[3][4]) {
+    const T *src = &up[buf][3 * (ej * NK + ek)];
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int t = (ej + (v >> 1)) * kTN + ek + (v & 1);
+    for (int v = 0; v < 4; v++)
 #pragma unroll
-      for (int c = 0; c < 3; c++) f[c][v] = up[buf][c][t];
-    }
+      for (int c = 0; c < 3; c++) f[c][v] = src[3 * ((v >> 1) * NK + (v & 1)) + c];
   };
   // forward (j,k) transform per axis: (a, b) -> (a + b, b - a)
   auto face_fwd = [](T (&f)[3][4]) {
@@ -338,13 +392,14 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
   stage(0, i0 - 1);
   if (ST != ST_PROLONG) fetch(i0);
   for (int sl = i0 - 1, p = 0; sl < iend; sl++, p++) {
-    const int cur = p % 3, nxt = (p + 1) % 3;
+    const int cur = p & 1, nxt = cur ^ 1;
     const int sw = wrap1(sl, g.nx);
     // loads in the order they are consumed: moduli of this slab, epilogue
     // inputs of plane sl, then plane sl + 2 for the next slab
-    T Ev[2];
+    T Ev[QE];
 #pragma unroll
-    for (int q = 0; q < 2; q++) Ev[q] = eoff[q] >= 0 ? E[sw * plane + eoff[q]] : (T)0;
+    for (int q = 0; q < QE; q++)
+      Ev[q] = eoff[q] >= 0 ? bload(rE, (unsigned)(sw * plane + eoff[q]) * SZ, 0u, (T *)nullptr) : (T)0;
     const size_t nown = (size_t)sl * plane + gj * g.nz + gk;
     EpiIn<T> pin;
     if ((EP != EP_AX || a.ydinv) && sl >= i0 && owns) epi_load<T, EP>(a, Ds, off, V, (int)nown, pin);
@@ -354,14 +409,19 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
     } else {
       stage(nxt, sl + 1);
     }
-    // (the previous slab's gather of eo and reads of this ring slot are
-    // behind the barrier of the previous slab's face exchange)
+    // (this buffer was last read before the barrier that closed the previous
+    // slab; the previous slab's gather of eo is behind that barrier too)
     __syncthreads();
+    T uo[3];
+    if (p > 0 && sl >= i0 && owns) {
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
+      for (int c = 0; c < 3; c++) uo[c] = up[cur][ot + c];
+    }
+#pragma unroll
+    for (int q = 0; q < QE; q++) {
       if (eoff[q] < 0) continue;
-      const int e = vt + q * kBlock;
-      const int ej = e / kTE, ek = e - ej * kTE;
+      const int e = vt + q * NT;
+      const int ej = e / EK, ek = e - ej * EK;
       T fb[3][4], ft[3][4];
       face_load(nxt, ej, ek, ft);
       face_fwd(ft);
@@ -389,19 +449,25 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
           w[c][v] = fb[c][v] + ft[c][v];
           w[c][4 + v] = ft[c][v] - fb[c][v];
         }
-      T z[3][8];
-      mode_couple<T>(w, Ev[q], mc, z);
-      // inverse i stage: bottom-face (plane sl) and top-face (plane sl+1)
-      // modal forces
+      // coupling and inverse i stage: bottom-face (plane sl) and top-face
+      // (plane sl+1) modal forces.  FP64 folds the two per component (fewer
+      // live registers); FP32 keeps them apart, which schedules without
+      // spills at 4 waves per SIMD
       T bm[3][4];
+      if constexpr (sizeof(T) == 8) {
+        couple_split<T>(w, Ev[q], mc, bm, tmod[q], p > 0);
+      } else {
+        T z[3][8];
+        mode_couple<T>(w, Ev[q], mc, z);
 #pragma unroll
-      for (int c = 0; c < 3; c++)
+        for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-          bm[c][v] = z[c][v] - z[c][4 + v];
-          if (p > 0) bm[c][v] += tmod[q][c][v];
-          tmod[q][c][v] = z[c][v] + z[c][4 + v];
-        }
+          for (int v = 0; v < 4; v++) {
+            bm[c][v] = z[c][v] - z[c][4 + v];
+            if (p > 0) bm[c][v] += tmod[q][c][v];
+            tmod[q][c][v] = z[c][v] + z[c][4 + v];
+          }
+      }
       if (p > 0) {
         face_inv(bm);
         T *dst = eo + e * kEOH;
@@ -411,49 +477,64 @@ __global__ __launch_bounds__(kBlock, sizeof(T) == 4 ? 4 : 1) void k_tiled(Dims g
           for (int c = 0; c < 3; c++) dst[3 * v + c] = bm[c][v];
       }
     }
-    if (p == 0) continue;  // plane i0 - 1 is not owned (uniform branch)
+    // plane i0 - 1 is not owned (uniform branch); its buffer is read above
+    // and refilled by the next slab's put, so the slab still closes with a
+    // barrier
     __syncthreads();
+    if (p == 0) continue;
     // plane sl: voxel columns (nj+dj, nk+dk) hold this node as face corner
     // (1-dj, 1-dk)
     if (sl >= i0 && owns) {
-      T Au[3] = {0, 0, 0}, uo[3];
+      T Au[3] = {0, 0, 0};
 #pragma unroll
       for (int dj = 0; dj < 2; dj++)
 #pragma unroll
         for (int dk = 0; dk < 2; dk++) {
-          const T *src = eo + ((nj + dj) * kTE + nk + dk) * kEOH + 3 * (2 * (1 - dj) + (1 - dk));
+          const T *src = eo + ((nj + dj) * EK + nk + dk) * kEOH + 3 * (2 * (1 - dj) + (1 - dk));
 #pragma unroll
           for (int c = 0; c < 3; c++) Au[c] += src[c];
         }
-#pragma unroll
-      for (int c = 0; c < 3; c++) uo[c] = up[cur][c][ot];
       epilogue<T, EP>(a, pin, ti, c1, c2, off, V, (int)nown, Au, uo, dv);
     }
   }
   if ((EP == EP_AX || EP == EP_STEP || EP == EP_RES || EP == EP_CHEB0) && a.dot) {
     const int pidx = chunk * tiles + tile;
-    block_partial(a.red, a.slot, r, pidx, block_sum(dv.a));
+    block_partial(a.red, a.slot, r, pidx, block_sum_n<NT>(dv.a));
     if (EP == EP_RES) {
-      block_partial(a.red, a.slot2, r, pidx, block_sum(dv.b));
-      block_partial(a.red, a.slot3, r, pidx, block_sum(dv.c));
+      block_partial(a.red, a.slot2, r, pidx, block_sum_n<NT>(dv.b));
+      block_partial(a.red, a.slot3, r, pidx, block_sum_n<NT>(dv.c));
     }
   }
 }
 
-template <typename T, int ST, int EP>
+template <typename T, int ST, int EP, int TJ, int TK>
 static void tiled_go(const Dims &g, int R, const T *E, const ModeC<T> &mc, const OpArgs<T> &a,
-                     hipStream_t s, int want) {
-  const int tiles_k = (g.nz + kTile - 1) / kTile, tiles_j = (g.ny + kTile - 1) / kTile;
-  const int TI = tiled_chunk(g, R, want);
+                     hipStream_t s, int shape) {
+  const int tiles_k = (g.nz + TK - 1) / TK, tiles_j = (g.ny + TJ - 1) / TJ;
+  const int TI = tiled_chunk(g, R, shape);
   const int tiles = tiles_j * tiles_k, chunks = (g.nx + TI - 1) / TI;
   if (R % kRHS) throw std::runtime_error("right-hand sides must come in groups of six");
-  hipLaunchKernelGGL((k_tiled<T, ST, EP>), dim3(tiles * chunks * R), dim3(kBlock), 0, s, g, E, mc, a,
+  // buffer resources address a right-hand side's vector with 32-bit offsets
+  if ((size_t)3 * g.V * sizeof(T) >= ((size_t)1 << 31)) throw std::runtime_error("grid too large for the tiled operator");
+  hipLaunchKernelGGL((k_tiled<T, ST, EP, TJ, TK>), dim3(tiles * chunks * R), dim3(TJ * TK), 0, s, g, E, mc, a,
                      TI, tiles_k, tiles, chunks);
+}
+
+template <typename T, int ST, int EP>
+static void tiled_shape(const Dims &g, int R, const T *E, const ModeC<T> &mc, const OpArgs<T> &a, hipStream_t s,
+                        int shape) {
+  const TileShape ts = tile_of(shape);
+  if (ts.tj == 8 && ts.tk == 32)
+    tiled_go<T, ST, EP, 8, 32>(g, R, E, mc, a, s, shape);
+  else if (ts.tj == 16 && ts.tk == 32)
+    tiled_go<T, ST, EP, 16, 32>(g, R, E, mc, a, s, shape);
+  else
+    tiled_go<T, ST, EP, 16, 16>(g, R, E, mc, a, s, shape);
 }
 
 template <typename T>
 void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCoef &m,
-                  const OpArgs<T> &a, hipStream_t s, int slabs) {
+                  const OpArgs<T> &a, hipStream_t s, int shape) {
   // the 9 distinct coefficients, checked against every entry of the table
   ModeC<T> mc;
   double dv[5] = {0, 0, 0, 0, 0}, xv[4] = {0, 0, 0, 0};
@@ -475,10 +556,10 @@ void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCo
     }
   for (int q = 0; q < 5; q++) mc.d[q] = (T)dv[q];
   for (int q = 0; q < 4; q++) mc.x[q] = (T)xv[q];
-#define GO(S_, E_)                                 \
-  if (st == S_ && ep == E_) {                      \
-    tiled_go<T, S_, E_>(g, R, E, mc, a, s, slabs); \
-    return;                                        \
+#define GO(S_, E_)                                     \
+  if (st == S_ && ep == E_) {                          \
+    tiled_shape<T, S_, E_>(g, R, E, mc, a, s, shape);  \
+    return;                                            \
   }
   GO(ST_PLAIN, EP_AX)
   GO(ST_PUPD, EP_AX)
@@ -521,7 +602,8 @@ __device__ __forceinline__ void stencil_node(const Dims &g, const T *__restrict_
   const int V = g.V;
   const T *__restrict__ Ss = S + (size_t)smp * 243 * V;
   const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * V : nullptr;
-  const T ti = a.theta_inv[smp];
+  const T ti = a.cheb ? (T)a.cheb[smp] : (T)0;
+  const T c1 = a.cheb ? (T)a.cheb[kMaxB + smp] : (T)0, c2 = a.cheb ? (T)a.cheb[2 * kMaxB + smp] : (T)0;
   const size_t off0 = (size_t)(smp * kRHS + lc0) * 3 * V;
   T acc[NR][3];
 #pragma unroll
@@ -560,7 +642,7 @@ __device__ __forceinline__ void stencil_node(const Dims &g, const T *__restrict_
     DotAcc dv;
     EpiIn<T> pin;
     epi_load<T, EP>(a, Ds, off, V, n, pin);
-    epilogue<T, EP>(a, pin, ti, a.c1[smp], a.c2[smp], off, V, n, acc[t], uo, dv);
+    epilogue<T, EP>(a, pin, ti, c1, c2, off, V, n, acc[t], uo, dv);
   }
 }
 
@@ -598,15 +680,10 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Dims g, const T *__restrict_
 // to that plane.  Against a three-plane ring this needs 2/3 of the LDS, so
 // more workgroups share a CU and hide the coefficient loads' latency.
 
-// Stencil coefficients through a raw buffer resource: a scalar plane offset
-// plus one 32-bit lane offset per load (no vector address arithmetic), and a
-// load past the sample's stencil returns 0 instead of faulting.
-__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float *) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double *) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
+// Stencil coefficients through a raw buffer resource (bload): a scalar plane
+// offset plus one 32-bit lane offset per load (no vector address
+// arithmetic), and a load past the sample's stencil returns 0 instead of
+// faulting.
 
 constexpr int kStBatch = 3;  // stencil blocks whose coefficient loads are issued together
 
@@ -630,7 +707,8 @@ __global__ __launch_bounds__(kBlock) void k_stencil_plane(Dims g, const T *__res
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
       (void *)(S + (size_t)smp * 243 * V), (short)0, (int)(243u * vb), 0x00020000);
   const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * V : nullptr;
-  const T ti = a.theta_inv[smp];
+  const T ti = a.cheb ? (T)a.cheb[smp] : (T)0;
+  const T c1 = a.cheb ? (T)a.cheb[kMaxB + smp] : (T)0, c2 = a.cheb ? (T)a.cheb[2 * kMaxB + smp] : (T)0;
   const size_t off0 = (size_t)(smp * kRHS + lc0) * 3 * V;
   int soff[2];
 #pragma unroll
@@ -756,7 +834,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil_plane(Dims g, const T *__res
       DotAcc dv;
       EpiIn<T> pin;
       epi_load<T, EP>(a, Ds, off, V, n, pin);
-      epilogue<T, EP>(a, pin, ti, a.c1[smp], a.c2[smp], off, V, n, acc[t], uo[t], dv);
+      epilogue<T, EP>(a, pin, ti, c1, c2, off, V, n, acc[t], uo[t], dv);
     }
   }
 }

@@ -59,19 +59,26 @@ struct OpArgs {
   int slot = 0;
   int dot = 0;               // EP_RES: r.r -> slot, b.x -> slot2, x.r -> slot3
   int slot2 = 0, slot3 = 0;
-  T theta_inv[kMaxB] = {};   // per sample
-  T c1[kMaxB] = {}, c2[kMaxB] = {};
+  // Chebyshev coefficients of this launch, device memory [3][kMaxB]: 1/theta,
+  // c1, c2 per sample (null where the variant uses none).  In device memory,
+  // not in the arguments, so that a recorded launch sequence stays valid when
+  // a new solve brings new spectra.
+  const double *cheb = nullptr;
 };
 
-// fine level (tiled cube-mode voxel kernel).  slabs: node planes per
-// workgroup (0: the built-in choice; an autotuned value otherwise).
+// fine level (tiled cube-mode voxel kernel).  shape: launch shape, bits 0-7
+// node planes per workgroup and bits 8-15 the tile variant (TJ x TK owned
+// node columns per workgroup: 1 = 16 x 16, 2 = 8 x 32, 3 = 16 x 32); 0 in a
+// field is the built-in choice (16 x 16, planes from the grid).
 // tiled_parts: partial count of a dot launch with that shape; tiled_slabs:
 // the planes per workgroup actually used
+constexpr int kNumTileShapes = 4;
+inline int tiled_shape_code(int tile, int planes) { return (tile << 8) | planes; }
 template <typename T>
 void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCoef &mc,
-                  const OpArgs<T> &a, hipStream_t s, int slabs = 0);
-int tiled_parts(const Dims &g, int R, int slabs = 0);
-int tiled_slabs(const Dims &g, int R, int slabs = 0);
+                  const OpArgs<T> &a, hipStream_t s, int shape = 0);
+int tiled_parts(const Dims &g, int R, int shape = 0);
+int tiled_slabs(const Dims &g, int R, int shape = 0);
 
 // coarse level (27-point block stencil, one RHS per thread)
 template <typename T>

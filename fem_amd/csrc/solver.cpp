@@ -142,7 +142,21 @@ Context::Context(const femh_opts &opts) : o(opts) {
   FEMH_HIP(hipStreamSynchronize(stream));
 }
 
+// pointers of every device buffer a recorded PCG iteration may touch
+template <typename T>
+long long Context::hier_hash(const Hierarchy<T> &h) const {
+  long long hsh = 1469598103934665603ll;
+  auto mix = [&](const void *p) { hsh = (hsh ^ (long long)(uintptr_t)p) * 1099511628211ll; };
+  for (const Level<T> &L : h.lv)
+    for (const DBuf *b : {&L.S, &L.D, &L.Dinv, &L.x, &L.b, &L.r, &L.d, &L.d2, &L.xn, &L.Ad}) mix(b->p);
+  for (const DBuf *b : {&h.E, &h.Ke, &h.Ainv, &h.cheb, &palt, &palt32, &beta_d, &lanczos, &pcg_state, &act, &red_part})
+    mix(b->p);
+  return hsh;
+}
+
 Context::~Context() {
+  for (auto &g : graphs_)
+    if (g.second) (void)hipGraphExecDestroy(g.second);
   for (auto &pe : pending) {
     (void)hipEventDestroy(pe.a);
     (void)hipEventDestroy(pe.b);
@@ -356,6 +370,8 @@ void Context::build_levels(Hierarchy<T> &h) {
   h.Ke.alloc(sizeof(T) * 576);
 }
 
+constexpr double kLmaxScale = 1.3;  // safety factor on the power-iteration lambda_max
+
 template <typename T>
 void Context::estimate_lmax(Hierarchy<T> &h) {
   // power iteration on D^-1 A, one vector per sample (RHS 6 s)
@@ -371,7 +387,13 @@ void Context::estimate_lmax(Hierarchy<T> &h) {
     launch_fill_hash<T>((size_t)R * 3 * V, (size_t)kRHS * 3 * V, xv, 0x51ED27ull + l, stream);
     // y_k = (D^-1 A)^k x_0 without normalising (ten steps grow it by at most
     // lambda^10 <= 1e10 or so), one fused launch per step; the estimate
-    // ||y_10|| / ||y_9|| is that of the normalised iteration
+    // ||y_10|| / ||y_9|| is that of the normalised iteration.  It approaches
+    // lambda_max from below, and a Chebyshev residual polynomial on
+    // [lmax/ratio, lmax] exceeds 1 in magnitude beyond lmax (1 + 1/ratio): an
+    // eigenvalue past that point makes the smoother, and so the V-cycle,
+    // indefinite (PCG then meets r.z < 0).  Ten steps have underestimated
+    // lambda_max by more than 15% on binary cells with a 1e-9 void phase and
+    // on SIMP cells with the degree-2 smoother; the estimate is scaled by 1.3.
     for (int it = 0; it < 10; it++) {
       OpArgs<T> a;
       a.u = xv;
@@ -389,7 +411,7 @@ void Context::estimate_lmax(Hierarchy<T> &h) {
     read_scalars(S_RR, n10.data(), R, vec_parts(V));
     for (int s = 0; s < nb; s++) {
       const double a9 = n9[kRHS * s], a10 = n10[kRHS * s];
-      L.lmax[s] = 1.1 * std::sqrt(a9 > 0 ? std::max(a10, 0.0) / a9 : 0.0);
+      L.lmax[s] = kLmaxScale * std::sqrt(a9 > 0 ? std::max(a10, 0.0) / a9 : 0.0);
       if (!(L.lmax[s] > 0) || !std::isfinite(L.lmax[s]))
         throw Error(FEMH_NONFINITE, "operator is not finite");
     }
@@ -550,10 +572,19 @@ void Context::autotune(Hierarchy<T> &h) {
   shape_fine = 0;
   shape_xfer.assign(L > 1 ? L - 1 : 0, 0);
   if (o.apply_kernel == 0) {
+    // tile variants x node planes per workgroup (a shape equivalent to an
+    // earlier candidate -- same tile, same planes -- is not timed twice)
     std::vector<int> cand{0};
-    const int base = tiled_slabs(g, R, 0);
-    for (int c : {2, 4, 8, 16, 32})
-      if (c <= g.nx && tiled_slabs(g, R, c) == c && c != base) cand.push_back(c);
+    std::vector<std::pair<int, int>> seen{{1, tiled_slabs(g, R, 0)}};
+    for (int tile = 1; tile < kNumTileShapes; tile++)
+      for (int c : {0, 2, 4, 8, 16, 32}) {
+        const int code = tiled_shape_code(tile, c);
+        if (c > g.nx || (c > 0 && tiled_slabs(g, R, code) != c)) continue;
+        const std::pair<int, int> key{tile, tiled_slabs(g, R, code)};
+        if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+        seen.push_back(key);
+        cand.push_back(code);
+      }
     std::vector<double> t;
     for (int c : cand) {
       shape_fine = c;
@@ -599,6 +630,7 @@ void Context::build_hierarchy() {
     h64_ops = true;
     if (o.precond == FEMH_PRECOND_MG) {
       estimate_lmax(h64);
+      upload_cheb(h64);
       // the coarsest level is inverted densely when it is small enough, else
       // solved approximately by a long Chebyshev polynomial (cheb_run)
       if (nlevels > 1) dense_coarse(h64);
@@ -627,6 +659,7 @@ void Context::build_hierarchy() {
         }
       }
       estimate_lmax(h32);
+      upload_cheb(h32);
       for (int l = 0; l < nlevels; l++) h64.lv[l].lmax = h32.lv[l].lmax;
     }
     FEMH_HIP(hipGetLastError());
@@ -640,6 +673,7 @@ void Context::ensure_h64_ops() {
   build_ops<double>(h64);
   // (spectrum bounds stay those of the FP32 operators)
   if (o.precond == FEMH_PRECOND_MG && nlevels > 1) dense_coarse(h64);
+  if (o.precond == FEMH_PRECOND_MG) upload_cheb(h64);
   h64_ops = true;
 }
 
@@ -662,18 +696,42 @@ static void cheb_step(double lmax, double ratio, int step, double *theta_inv, do
   }
 }
 
-// per-sample 1/theta and the coefficients of Chebyshev step `step` (1-based)
+// the device table of Chebyshev coefficients of h (after its spectrum
+// estimates): per level, step and interval, 1/theta and (from step 1 on) c1
+// and c2 of every sample
+template <typename T>
+void Context::upload_cheb(Hierarchy<T> &h) {
+  const int L = (int)h.lv.size();
+  const int steps = std::max(kMaxCheb, std::max(2, o.coarse_iters)) + 1;
+  const size_t per = (size_t)3 * kMaxB;
+  std::vector<double> tab((size_t)L * steps * 2 * per, 0.0);
+  for (int l = 0; l < L; l++)
+    for (int st = 0; st < steps; st++)
+      for (int kind = 0; kind < 2; kind++) {
+        double *t = tab.data() + (((size_t)l * steps + st) * 2 + kind) * per;
+        for (int s = 0; s < nb; s++) {
+          double ti, c1, c2;
+          cheb_step(h.lv[l].lmax[s], kind ? kCoarseChebRatio : o.cheby_ratio, st, &ti, &c1, &c2);
+          t[s] = ti;
+          t[kMaxB + s] = st >= 1 ? c1 : 0.0;
+          t[2 * kMaxB + s] = st >= 1 ? c2 : 0.0;
+        }
+      }
+  h.cheb.alloc(sizeof(double) * tab.size());
+  FEMH_HIP(hipMemcpyAsync(h.cheb.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+  h.cheb_steps = steps;
+}
+
+// per-sample 1/theta and the coefficients of Chebyshev step `step` (1-based):
+// the launch reads them from the hierarchy's device table
 template <typename T>
 void Context::set_cheb(const Hierarchy<T> &h, int l, OpArgs<T> &a, int step, double ratio) {
-  for (int s = 0; s < nb; s++) {
-    double ti, c1, c2;
-    cheb_step(h.lv[l].lmax[s], ratio, step, &ti, &c1, &c2);
-    a.theta_inv[s] = (T)ti;
-    if (step >= 1) {
-      a.c1[s] = (T)c1;
-      a.c2[s] = (T)c2;
-    }
-  }
+  const int kind = ratio == o.cheby_ratio ? 0 : 1;
+  if (kind == 1 && ratio != kCoarseChebRatio) throw Error(FEMH_DEVICE_ERROR, "no Chebyshev table for this interval");
+  if (!h.cheb.p || step < 0 || step >= h.cheb_steps || l >= (int)h.lv.size())
+    throw Error(FEMH_DEVICE_ERROR, "Chebyshev coefficients not prepared");
+  a.cheb = h.cheb.template as<double>() + (((size_t)l * h.cheb_steps + step) * 2 + kind) * 3 * kMaxB;
 }
 
 // k Chebyshev steps on level l from x = 0 (pre-smoothing, and the coarsest
@@ -896,14 +954,24 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
   } else {
     launch_p_update<T>(V, R, pv, zv, S_RZ0, S_RZ0, 1, ap, red, np_rz, stream);
   }
-  int cur = 0;
-  std::vector<double> rr(R);
-  for (int it = 1; it <= max_iter; it++) {
+  // Per-RHS iteration state on the device: iterations done, relative
+  // residual, state (1 converged, 2 non-finite), and |b|^2.  The convergence
+  // test runs on the device after every update, so an iteration is a fixed
+  // launch sequence that needs no host decision.
+  pcg_state.alloc(sizeof(double) * 4 * kMaxR);
+  int *cnt = pcg_state.as<int>(), *pst = cnt + kMaxR;
+  double *relres_d = pcg_state.as<double>() + kMaxR, *bb_d = relres_d + kMaxR;
+  launch_fill<int>(2 * kMaxR, cnt, 0, stream);
+  FEMH_HIP(hipMemcpyAsync(bb_d, bnorm2, sizeof(double) * R, hipMemcpyHostToDevice, stream));
+  int *apw = act.as<int>() + A_PCG * kMaxR;
+  const int np_rr = vec_parts(V);
+  // one PCG iteration; `cur` selects the r.z parity, `first` the plain p = z
+  auto iteration = [&](int cur, bool first) {
     const int rzc = cur ? S_RZ1 : S_RZ0, rzn = cur ? S_RZ0 : S_RZ1;
     if (fuse_p) {
       OpArgs<T> a;
       a.u = zv;
-      a.u2 = it == 1 ? zv : pcur;  // beta = 0 on the first step: p = z
+      a.u2 = first ? zv : pcur;  // beta = 0 on the first step: p = z
       a.beta = beta_d.as<double>();
       a.pout = pnxt;
       a.y = qv;
@@ -916,35 +984,91 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
     } else {
       apply<T>(h, 0, pv, qv, nullptr, 0, ap, &red, S_PQ);
     }
-    double *lza = lz_alpha ? lz_alpha + (size_t)(it - 1) * kMaxR : nullptr;
-    double *lzb = lz_beta ? lz_beta + (size_t)(it - 1) * kMaxR : nullptr;
-    launch_pcg_update<T>(V, R, xv, rv, pcur, qv, jac ? D0 : nullptr, zv, rzc, rzn, ap, red,
-                         apply_parts(), np_rz, stream, lza);
-    FEMH_HIP(hipGetLastError());
-    if (it % o.check_every == 0 || it == max_iter) {
-      read_scalars(S_RR, rr.data(), R, vec_parts(V));
-      bool changed = false;
-      for (int t = 0; t < R; t++) {
-        if (!active[t]) continue;
-        if (!std::isfinite(rr[t])) throw Error(FEMH_NONFINITE, "residual became non-finite");
-        const double rel = std::sqrt(rr[t] / bnorm2[t]);
-        iters[t] = it;
-        relres[t] = rel;
-        if (rel <= tol) {
-          active[t] = 0;
-          changed = true;
-          nact--;
-        }
-      }
-      if (!nact) break;
-      if (changed) set_act(A_PCG, active);
-    }
+    launch_pcg_update<T>(V, R, xv, rv, pcur, qv, jac ? D0 : nullptr, zv, rzc, rzn, ap, red, apply_parts(), np_rz,
+                         stream, lz_alpha, cnt);
+    launch_pcg_check(R, apw, red, np_rr, bb_d, tol, cnt, relres_d, pst, stream);
     if (!jac) precond<T>(h, rv, zv, ap, rzn);
     if (fuse_p)
-      launch_beta(R, rzn, rzc, 0, ap, red, np_rz, beta_d.as<double>(), stream, lzb);
+      launch_beta(R, rzn, rzc, 0, ap, red, np_rz, beta_d.as<double>(), stream, lz_beta, cnt);
     else
-      launch_p_update<T>(V, R, pv, zv, rzn, rzc, 0, ap, red, np_rz, stream, lzb);
-    cur ^= 1;
+      launch_p_update<T>(V, R, pv, zv, rzn, rzc, 0, ap, red, np_rz, stream, lz_beta, cnt);
+  };
+  std::vector<int> hact(kMaxR), hst(kMaxR);
+  // host look at the device state: false when no right-hand side is left
+  auto poll = [&]() {
+    FEMH_HIP(hipMemcpyAsync(hact.data(), apw, sizeof(int) * R, hipMemcpyDeviceToHost, stream));
+    FEMH_HIP(hipMemcpyAsync(hst.data(), pst, sizeof(int) * R, hipMemcpyDeviceToHost, stream));
+    FEMH_HIP(hipStreamSynchronize(stream));
+    bool any = false;
+    for (int t = 0; t < R; t++) {
+      if (hst[t] == 2) throw Error(FEMH_NONFINITE, "residual became non-finite");
+      any = any || hact[t] != 0;
+    }
+    return any;
+  };
+  // Iterations 2k and 2k+1 use the same buffers and slots (ping-pong of the
+  // direction and of the r.z parity), so with graphs on, the pair is
+  // recorded once into a hipGraph and replayed; the host looks at the device
+  // state after every pair (or every check_every iterations without graphs).
+  FEMH_HIP(hipGetLastError());
+  int it = 1;
+  iteration(0, true);
+  bool live = it < max_iter ? poll() : false;
+  const bool graphs = o.graph && !prof;
+  hipGraphExec_t exec = nullptr;
+  if (graphs && live && max_iter - it >= 2) {
+    GraphKey key{};
+    key.v = {(long long)sizeof(T), (long long)R, (long long)(uintptr_t)xv, (long long)(uintptr_t)rv,
+             (long long)(uintptr_t)zv, (long long)(uintptr_t)pv, (long long)(uintptr_t)qv,
+             (long long)(uintptr_t)lz_alpha, (long long)(uintptr_t)h.cheb.p, (long long)h.lv.size(),
+             (long long)h.ndense, (long long)shape_fine, (long long)xfer_hash(), (long long)z_ready,
+             hier_hash(h)};
+    std::memcpy(&key.tol, &tol, sizeof(double));
+    exec = find_graph(key);
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      FEMH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        iteration(1, false);
+        iteration(0, false);
+      } catch (...) {
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(stream, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        throw;
+      }
+      FEMH_HIP(hipStreamEndCapture(stream, &graph));
+      const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      FEMH_HIP(ie);
+      graphs_.push_back({key, exec});
+    }
+  }
+  while (live && it < max_iter) {
+    if (exec && max_iter - it >= 2) {
+      FEMH_HIP(hipGraphLaunch(exec, stream));
+      it += 2;
+      live = poll();
+      continue;
+    }
+    iteration(it & 1, false);
+    it++;
+    if (it % o.check_every == 0 || it == max_iter) live = poll();
+  }
+  if (live) poll();
+  // results of the run: iterations and residual per right-hand side, and the
+  // activity that remains (not converged within max_iter)
+  std::vector<int> hcnt(R);
+  std::vector<double> hrel(R);
+  FEMH_HIP(hipMemcpyAsync(hcnt.data(), cnt, sizeof(int) * R, hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipMemcpyAsync(hrel.data(), relres_d, sizeof(double) * R, hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipMemcpyAsync(hact.data(), apw, sizeof(int) * R, hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+  for (int t = 0; t < R; t++) {
+    if (!active[t]) continue;
+    iters[t] = hcnt[t];
+    relres[t] = hrel[t];
+    active[t] = hact[t] != 0;
   }
   FEMH_HIP(hipGetLastError());
 }
@@ -1113,7 +1237,7 @@ void Context::homogenize(double *C, femh_stats *st) {
   if (timed) FEMH_HIP(hipEventRecord(ev[1], stream));
 
   std::vector<int> iters(R, 0), outer(R, 0);
-  std::vector<double> relres(R, 0.0), ch_est(nb, 0.0);
+  std::vector<double> relres(R, 0.0), ch_est(nb, 0.0), ritz(nb, 0.0);
   fallback_.assign(nb, 0);
   int status = FEMH_OK;
   if (warm) {
@@ -1243,6 +1367,22 @@ void Context::homogenize(double *C, femh_stats *st) {
         if (!nact) break;
       }
       std::vector<int> act_in = active;
+      // Per-cell sweep tolerance: with a finite estimate above ch_tol, the
+      // estimate scales with r.M^-1 r, i.e. with the square of the residual,
+      // so a residual reduction of rho = 0.5 sqrt(ch_tol / est) leaves it a
+      // factor 4 below the target; the sweep stops there (never looser than
+      // 0.1, never tighter than inner_tol).  The device test compares
+      // |r|^2 / bnorm2 with inner^2, so scaling bnorm2 by (rho / inner)^2
+      // gives that right-hand side its own tolerance.
+      const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-4;
+      std::vector<double> rr_in = rr;
+      if (o.ch_tol > 0 && o.adapt_inner)
+        for (int t = 0; t < R; t++) {
+          const double est = ch_est[t / kRHS];
+          if (!active[t] || !(est > o.ch_tol) || !std::isfinite(est)) continue;
+          const double rho = std::min(0.1, std::max(inner, 0.5 * std::sqrt(o.ch_tol / est)));
+          rr_in[t] = rr[t] * (rho / inner) * (rho / inner);
+        }
       {
         // the record covers this FP32 sweep only (at most kInnerMaxIter
         // iterations), never the FP64 fallback
@@ -1257,8 +1397,8 @@ void Context::homogenize(double *C, femh_stats *st) {
           ~Record() { c.lz_alpha = c.lz_beta = nullptr; }
         } rec(*this, bound);
         pcg<float>(h32, r32.as<float>(), x32.as<float>(), r32.as<float>(), z32.as<float>(),
-                   p32.as<float>(), q32.as<float>(), act_in, o.inner_tol > 0 ? o.inner_tol : 1e-4,
-                   std::min(o.max_iter, kInnerMaxIter), rr.data(), it_in.data(), rel_in.data(), true);
+                   p32.as<float>(), q32.as<float>(), act_in, inner, std::min(o.max_iter, kInnerMaxIter),
+                   rr_in.data(), it_in.data(), rel_in.data(), true);
       }
       if (bound) {
         int kmax = 0;
@@ -1348,6 +1488,7 @@ void Context::homogenize(double *C, femh_stats *st) {
     }
     if (nact) status = FEMH_NOT_CONVERGED;
     theta_prev = theta;
+    for (int s_ = 0; s_ < nb; s_++) ritz[s_] = std::isfinite(theta[s_]) ? theta[s_] : 0.0;
     // every right-hand side vanished before a sweep: u = 0
     if (!x_set) launch_fill<double>(field_bytes(g, nb, sizeof(double)) / sizeof(double), x.as<double>(), 0.0, stream);
   }
@@ -1405,6 +1546,7 @@ void Context::homogenize(double *C, femh_stats *st) {
       }
       o_.ch_err_est = fallback_[s] ? 0.0 : ch_est[s];
       o_.fp64_fallback = fallback_[s];
+      o_.ritz_min = ritz[s];
       o_.ms_setup = ms1;
       o_.ms_solve = ms2;
       o_.ms_total = ms_total;
@@ -1487,6 +1629,46 @@ void Context::thermal(const double *alpha, double *beta, double *alpha_eff) {
       for (int k = i + 1; k < 6; k++) v -= L[k][i] * alpha_eff[kRHS * smp + k];
       alpha_eff[kRHS * smp + i] = v / L[i][i];
     }
+  }
+}
+
+// The reference's force-driven path (SamplesGenerator/NumericalCoarsening.cpp:
+// 1148-1306 with femlib/src/MeshUtilities.cpp:960-1060) applies six face
+// tractions under periodic constraints, then reduces C from the tensor shear
+// strains at every Gauss point and at the cell-sized hex,
+//   C = (1/nelem) G^-T (sum_e sum_q w_q Gv^T C_e Gv) G^-1,
+// with C_e the engineering-shear stiffness.  On a cube cell the loads carry no
+// net moment, so (without the reference's 1e-5 diagonal regularisation) each
+// constrained solution is a macro strain plus the strain-driven fluctuation
+// of that strain, G cancels, and C = T^-1 M T^-1 with
+// M = (1/V) sum_e E_e chi_a^T KT chi_b over the strain-driven total fields
+// chi and KT the element matrix of T C0 T, T = diag(1,1,1,1/2,1/2,1/2).
+void Context::coarse_strain(double *C) {
+  if (!have_fields) throw Error(FEMH_NO_FIELDS, "no solved fields: call femh_homogenize first");
+  if (!(g.nx == g.ny && g.ny == g.nz))
+    throw Error(FEMH_BAD_ARG, "the coarse-strain tensor needs a cube cell (on other cells the reference's "
+                              "shear loads carry a net moment)");
+  ModeCoef mt;
+  std::memcpy(mt.diag, tab.LT.diag, sizeof(mt.diag));
+  std::memcpy(mt.cross, tab.LT.cross, sizeof(mt.cross));
+  const int nblk = 1024;
+  launch_ch_energy(g, nb, E64.as<double>(), mt, uaff.as<double>(), x.as<double>(), ch_part.as<double>(), nblk,
+                   ch_out.as<double>(), stream);
+  FEMH_HIP(hipGetLastError());
+  std::vector<double> s(21 * (size_t)nb);
+  FEMH_HIP(hipMemcpyAsync(s.data(), ch_out.p, sizeof(double) * s.size(), hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+  static const double Tv[6] = {1, 1, 1, 0.5, 0.5, 0.5};
+  for (int smp = 0; smp < nb; smp++) {
+    int p = 0;
+    double *Cs = C + 36 * smp;
+    for (int a = 0; a < 6; a++)
+      for (int b2 = a; b2 < 6; b2++) {
+        Cs[6 * a + b2] = Cs[6 * b2 + a] = s[21 * smp + p] / ((double)g.V * Tv[a] * Tv[b2]);
+        p++;
+      }
+    for (int q = 0; q < 36; q++)
+      if (!std::isfinite(Cs[q])) throw Error(FEMH_NONFINITE, "coarse-strain tensor is not finite");
   }
 }
 

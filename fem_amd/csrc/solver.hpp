@@ -6,6 +6,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -95,6 +96,11 @@ struct Hierarchy {
   DBuf Ke;    // unit stiffness in T
   DBuf Ainv;  // per sample: dense (regularised) inverse of the coarsest operator
   DBuf gt1, gt2;  // Galerkin intermediates (one- and two-axis coarsened stencils)
+  // Chebyshev coefficients of every level, step (0..cheb_steps-1) and
+  // interval (0: smoothing ratio, 1: the coarsest solve's) for every sample:
+  // [level][step][interval][3][kMaxB] doubles (1/theta, c1, c2)
+  DBuf cheb;
+  int cheb_steps = 0;
   int ndense = 0;
   bool ready = false;
 };
@@ -146,6 +152,30 @@ struct Context {
   DBuf lanczos;
   double *lz_alpha = nullptr, *lz_beta = nullptr;
   std::vector<double> theta_prev;  // per slot: smallest Ritz value of the last call
+  // PCG iteration state on the device: iterations, state (int [kMaxR] each),
+  // relative residual and |b|^2 (double [kMaxR] each)
+  DBuf pcg_state;
+  // recorded PCG iteration pairs (hipGraph), keyed by everything their
+  // launches bake in: precision, right-hand sides, vector and hierarchy
+  // buffers, launch shapes, tolerance
+  struct GraphKey {
+    std::array<long long, 15> v{};
+    long long tol = 0;
+    bool operator==(const GraphKey &o) const { return v == o.v && tol == o.tol; }
+  };
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs_;
+  hipGraphExec_t find_graph(const GraphKey &k) const {
+    for (const auto &g : graphs_)
+      if (g.first == k) return g.second;
+    return nullptr;
+  }
+  long long xfer_hash() const {
+    long long hsh = 1469598103934665603ll;
+    for (int v : shape_xfer) hsh = (hsh ^ v) * 1099511628211ll;
+    return hsh;
+  }
+  template <typename T>
+  long long hier_hash(const Hierarchy<T> &h) const;
   bool modulus_input = false;  // last load gave moduli (dE/drho = 1)
   int nlevels = 1;
   // launch shapes (0: the built-in choice): node planes per workgroup of the
@@ -193,6 +223,10 @@ struct Context {
   // effective thermal stress beta[6 nb] and expansion C_H^-1 beta (either
   // may be null) for per-voxel expansion coefficients alpha[nb V] (host)
   void thermal(const double *alpha, double *beta, double *alpha_eff);
+  // coarse-strain (force-driven) form of C_H of the cells of the last call,
+  // C[36 nb]: C = T^-1 [(1/V) sum_e E_e chi^T KT chi] T^-1 with KT the
+  // element matrix of T C0 T (cube cells only)
+  void coarse_strain(double *C);
   void apply_level(int level, const double *u, double *y);
 
  private:
@@ -225,6 +259,8 @@ struct Context {
            double *relres, bool z_ready = false);
   template <typename T>
   void set_cheb(const Hierarchy<T> &h, int l, OpArgs<T> &a, int step, double ratio);
+  template <typename T>
+  void upload_cheb(Hierarchy<T> &h);
   template <typename T>
   void cheb_run(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_, int k, double ratio);
   int apply_parts() const;

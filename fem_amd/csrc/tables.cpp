@@ -26,13 +26,14 @@ void macro_strain(int lc, double eps[9]) {
 
 static inline int bit(int s, int ax) { return (s >> (2 - ax)) & 1; }
 
-ElementTables build_tables(double nu) {
-  ElementTables t;
-  t.nu = nu;
-  lame(1.0, nu, &t.lambda, &t.mu);
-  const double lam = t.lambda, mu = t.mu;
+// Same-component block: c11 int d_c N d_c N + c44 sum_{i != c} int d_i N d_i N;
+// cross block (c, d): c12 int d_c N_a d_d N_b + c44 int d_d N_a d_c N_b.  For
+// the isotropic material (c11, c12, c44) = (lambda + 2 mu, lambda, mu) the
+// diagonal reads (lambda + mu) b_c W(c) + mu sum_i b_i W(i).
+ModeCoupling cubic_coupling(double c11, double c12, double c44) {
+  ModeCoupling L;
   const double w[2] = {0.25, 1.0 / 12.0};
-  std::memset(&t.L, 0, sizeof(t.L));
+  std::memset(&L, 0, sizeof(L));
   for (int s = 0; s < 8; s++) {
     for (int c = 0; c < 3; c++) {
       auto W = [&](int i) {  // product of w over the two axes other than i
@@ -41,19 +42,56 @@ ElementTables build_tables(double nu) {
           if (k != i) p *= w[bit(s, k)];
         return p;
       };
-      double D = (lam + mu) * bit(s, c) * W(c);
-      for (int i = 0; i < 3; i++) D += mu * bit(s, i) * W(i);
-      t.L.diag[c][s] = D;
+      double D = c11 * bit(s, c) * W(c);
+      for (int i = 0; i < 3; i++)
+        if (i != c) D += c44 * bit(s, i) * W(i);
+      L.diag[c][s] = D;
       for (int d = 0; d < 3; d++) {
         if (d == c) continue;
         int k = 3 - c - d;
         double v = 0;
-        if (bit(s, c) == 1 && bit(s, d) == 0) v = lam * 0.25 * w[bit(s, k)];
-        if (bit(s, c) == 0 && bit(s, d) == 1) v = mu * 0.25 * w[bit(s, k)];
-        t.L.cross[c][d][s] = v;
+        if (bit(s, c) == 1 && bit(s, d) == 0) v = c12 * 0.25 * w[bit(s, k)];
+        if (bit(s, c) == 0 && bit(s, d) == 1) v = c44 * 0.25 * w[bit(s, k)];
+        L.cross[c][d][s] = v;
       }
     }
   }
+  return L;
+}
+
+ElementTables build_tables(double nu) {
+  ElementTables t;
+  t.nu = nu;
+  lame(1.0, nu, &t.lambda, &t.mu);
+  const double lam = t.lambda, mu = t.mu;
+  // (lambda + mu) b_c W(c) + mu sum_i b_i W(i), as the isotropic case of the
+  // cubic coupling but summed in the order used since the first tables
+  {
+    const double w[2] = {0.25, 1.0 / 12.0};
+    std::memset(&t.L, 0, sizeof(t.L));
+    for (int s = 0; s < 8; s++) {
+      for (int c = 0; c < 3; c++) {
+        auto W = [&](int i) {
+          double p = 1;
+          for (int k = 0; k < 3; k++)
+            if (k != i) p *= w[bit(s, k)];
+          return p;
+        };
+        double D = (lam + mu) * bit(s, c) * W(c);
+        for (int i = 0; i < 3; i++) D += mu * bit(s, i) * W(i);
+        t.L.diag[c][s] = D;
+        for (int d = 0; d < 3; d++) {
+          if (d == c) continue;
+          int k = 3 - c - d;
+          double v = 0;
+          if (bit(s, c) == 1 && bit(s, d) == 0) v = lam * 0.25 * w[bit(s, k)];
+          if (bit(s, c) == 0 && bit(s, d) == 1) v = mu * 0.25 * w[bit(s, k)];
+          t.L.cross[c][d][s] = v;
+        }
+      }
+    }
+  }
+  t.LT = cubic_coupling(lam + 2.0 * mu, lam, 0.25 * mu);
   // K0 = H^T Lambda H
   double H[8][8];
   for (int s = 0; s < 8; s++)

@@ -33,7 +33,17 @@ struct ElementTables {
   std::array<double, 6 * 24> g{};      // K0 * u_affine(load case), per case
   std::array<double, 6 * 24> u_aff{};  // affine local field per load case
   std::vector<double> Mchild;          // 8 x 576: P_f^T K0 P_f (level-1 Galerkin)
+  // Coupling of the coarse-strain (force-driven) form of C_H: the same
+  // element integral with the engineering stiffness applied to tensor shear
+  // strains on both sides, i.e. T C0 T with T = diag(1,1,1,1/2,1/2,1/2) --
+  // the stiffness of a cubic material c11 = lambda + 2 mu, c12 = lambda,
+  // c44 = mu / 4 (unit modulus)
+  ModeCoupling LT;
 };
+
+// Hadamard-space coupling of a cubic material (c11, c12, c44) on a unit
+// voxel; the isotropic one is c11 = lambda + 2 mu, c12 = lambda, c44 = mu
+ModeCoupling cubic_coupling(double c11, double c12, double c44);
 
 void lame(double E, double nu, double *lambda, double *mu);
 void macro_strain(int lc, double eps[9]);

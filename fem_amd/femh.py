@@ -27,7 +27,8 @@ EXPORTS = ("femh_abi_version", "femh_default_opts", "femh_create", "femh_destroy
            "femh_homogenize", "femh_homogenize_modulus", "femh_homogenize_device",
            "femh_homogenize_many", "femh_homogenize_many_modulus", "femh_homogenize_many_device",
            "femh_gradient", "femh_energy_terms", "femh_get_fields", "femh_thermal_expansion",
-           "femh_launch_shapes", "femh_homogenize_batch",
+           "femh_coarse_strain_tensor",
+           "femh_launch_shapes", "femh_set_launch_shapes", "femh_homogenize_batch",
            "femh_prepare", "femh_apply_level", "femh_level_dims", "femh_set_profiling",
            "femh_kernel_stats", "femh_element_stiffness", "femh_last_error",
            "femh_homogenize_device_noalloc")
@@ -53,7 +54,8 @@ class Opts(ctypes.Structure):
                 ("apply_kernel", ctypes.c_int), ("batch", ctypes.c_int),
                 ("coarse_nu", ctypes.c_int), ("warm_start", ctypes.c_int),
                 ("structure", ctypes.c_int), ("autotune", ctypes.c_int),
-                ("streams", ctypes.c_int), ("batch_threads", ctypes.c_int)]
+                ("streams", ctypes.c_int), ("batch_threads", ctypes.c_int),
+                ("graph", ctypes.c_int), ("adapt_inner", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -61,12 +63,13 @@ class Stats(ctypes.Structure):
                 ("relres", ctypes.c_double * 6), ("ch_err_est", ctypes.c_double),
                 ("ms_setup", ctypes.c_double), ("ms_solve", ctypes.c_double),
                 ("ms_total", ctypes.c_double), ("levels", ctypes.c_int),
-                ("fp64_fallback", ctypes.c_int)]
+                ("fp64_fallback", ctypes.c_int), ("ritz_min", ctypes.c_double)]
 
     def as_dict(self):
         return dict(iters=list(self.iters), outer=list(self.outer), relres=list(self.relres),
                     ch_err_est=self.ch_err_est, ms_setup=self.ms_setup, ms_solve=self.ms_solve,
-                    ms_total=self.ms_total, levels=self.levels, fp64_fallback=self.fp64_fallback)
+                    ms_total=self.ms_total, levels=self.levels, fp64_fallback=self.fp64_fallback,
+                    ritz_min=self.ritz_min)
 
 
 def lib():
@@ -97,7 +100,9 @@ def lib():
         L.femh_energy_terms.argtypes = [vp, vp]
         L.femh_get_fields.argtypes = [vp, vp]
         L.femh_thermal_expansion.argtypes = [vp, vp, vp, vp]
+        L.femh_coarse_strain_tensor.argtypes = [vp, vp]
         L.femh_launch_shapes.argtypes = [vp, vp, ctypes.c_int]
+        L.femh_set_launch_shapes.argtypes = [vp, vp, ctypes.c_int]
         L.femh_homogenize_batch.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.femh_prepare.argtypes = [vp, vp]
         L.femh_apply_level.argtypes = [vp, ctypes.c_int, vp, vp]
@@ -120,7 +125,8 @@ def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond
               levels=0, nu1=2, nu2=2, cheby_ratio=20.0, coarse_iters=64, tol=1e-10,
               max_iter=5000, precision="fp64", ch_tol=0.0, check_every=1, device=0,
               inner_tol=1e-4, max_outer=12, apply_kernel="tiled", batch=1, warm_start=False,
-              coarse_nu=3, structure="general", autotune=False, streams=1, batch_threads=0):
+              coarse_nu=3, structure="general", autotune=False, streams=1, batch_threads=0,
+              graph=True, adapt_inner=True):
     o = Opts()
     lib().femh_default_opts(ctypes.byref(o))
     o.nx, o.ny, o.nz = (int(s) for s in shape)
@@ -141,6 +147,8 @@ def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond
     o.autotune = int(bool(autotune))
     o.streams = int(streams)
     o.batch_threads = int(batch_threads)
+    o.graph = int(bool(graph))
+    o.adapt_inner = int(bool(adapt_inner))
     return o
 
 
@@ -260,6 +268,13 @@ class Homogenizer:
             return beta, aeff
         return beta.reshape(nb, 6), aeff.reshape(nb, 6)
 
+    def coarse_strain_tensor(self, nb=1):
+        """The force-driven (coarse-strain) form of C_H of the last batch of
+        nb cells: [6, 6] (nb = 1) or [nb, 6, 6]."""
+        C = np.zeros(36 * nb)
+        self._check(lib().femh_coarse_strain_tensor(self._h, _ptr(C)))
+        return C.reshape(6, 6) if nb == 1 else C.reshape(nb, 6, 6)
+
     def launch_shapes(self):
         """Launch shapes in use: [fine-operator planes per workgroup, transfer
         planes per thread of each level pair]; 0 = the built-in choice."""
@@ -268,6 +283,11 @@ class Homogenizer:
         if n < 0:
             self._check(-n)
         return [buf[i] for i in range(min(n, 16))]
+
+    def set_launch_shapes(self, shapes):
+        """Pin the launch shapes (femh_launch_shapes encoding; 0 = built-in)."""
+        a = np.ascontiguousarray(shapes, dtype=np.int32)
+        self._check(lib().femh_set_launch_shapes(self._h, _ptr(a), int(a.size)))
 
     def set_profiling(self, on=True):
         self._check(lib().femh_set_profiling(self._h, int(bool(on))))

@@ -497,11 +497,21 @@ void NumericalCoarsening::ensure(int N[3], int rep, int sub, const std::vector<B
   if (!same) init(N, rep, sub, mats);
 }
 
+void NumericalCoarsening::toCoarseStrain(std::vector<double> &tv) {
+  if (!m_coarseStrain || m_n[0] != m_n[1] || m_n[1] != m_n[2]) return;
+  check(m_ctx, femh_coarse_strain_tensor(m_ctx, m_C.a.data()), "femh_coarse_strain_tensor");
+  if (tv.size() < 21) return;
+  size_t at = tv.size() - 21;
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j <= i; j++) tv[at++] = m_referenceTriangle && j < i ? 0.0 : m_C(i, j);
+}
+
 Mat6 NumericalCoarsening::computeCoarsenedElasticityTensor(const std::vector<int> &iMaterials, int N[3], int rep,
                                                           int sub, const std::vector<BaseMaterial> &mats) {
   ensure(N, rep, sub, mats);
   std::vector<double> tv, pv;
   computeCoarsenedElasticityTensorAndParameters3D(iMaterials, N, {0.0, 1.0}, General, tv, pv);
+  toCoarseStrain(tv);
   return m_C;
 }
 
@@ -511,6 +521,7 @@ void NumericalCoarsening::computeCoarsenedElasticityTensorAndParameters(
   ensure(N, rep, sub, mats);
   std::vector<double> pv;
   computeCoarsenedElasticityTensorAndParameters3D(iMaterials, N, {0.0, 1.0}, General, ioTensorValues, pv);
+  toCoarseStrain(ioTensorValues);
   // compliance columns: strains under a unit uniaxial stress along x, y, z
   const Mat6 Sfull = inverse6(m_C);
   double S[3][3];  // S[a][b]: strain b under a unit stress along a

@@ -216,10 +216,16 @@ class NumericalCoarseningUsingMultigrid {
   void moduli(const std::vector<int> &iMaterials, int N[3], double *oE) const;
 };
 
-// Same entry point as the reference's PARDISO-based coarsening; it runs the
-// engine's periodic strain-driven solve (no face tractions, no Lagrange rows).
+// Same entry point as the reference's PARDISO-based coarsening.  It runs the
+// engine's periodic strain-driven solve; on a cube cell the tensor it returns
+// is the force-driven path's coarse-strain form (femh_coarse_strain_tensor:
+// what the reference's face-traction KKT solve and
+// MeshUtilities::computeCoarsenedElasticityTensor give, without the 1e-5
+// diagonal regularisation); on other cells, and with
+// setCoarseStrainTensor(false), the energy form of the multigrid path.
 class NumericalCoarsening : public NumericalCoarseningUsingMultigrid {
  public:
+  void setCoarseStrainTensor(bool v) { m_coarseStrain = v; }
   void init(int N[3], int iNbBlockRep, int iNbSubdiv, const std::vector<BaseMaterial> &iBaseMaterials) {
     NumericalCoarseningUsingMultigrid::init(N, iNbBlockRep, iNbSubdiv, iBaseMaterials, General);
     remember(N, iNbBlockRep, iNbSubdiv, iBaseMaterials);
@@ -244,8 +250,12 @@ class NumericalCoarsening : public NumericalCoarseningUsingMultigrid {
  private:
   int m_key[5] = {0, 0, 0, 0, 0};
   std::vector<BaseMaterial> m_keyMats;
+  bool m_coarseStrain = true;
   void remember(int N[3], int rep, int sub, const std::vector<BaseMaterial> &mats);
   void ensure(int N[3], int rep, int sub, const std::vector<BaseMaterial> &mats);
+  // replace the energy tensor of the last solve (m_C and the 21 values last
+  // appended to tv) by the coarse-strain form where it applies
+  void toCoarseStrain(std::vector<double> &tv);
 };
 
 // replicate a coarse assignment rep times per axis with each cell split into

@@ -29,6 +29,10 @@
  *   femh_launch_shapes
  *       reports the launch shapes in use (femh_opts.autotune); no counterpart
  *       in the reference.
+ *   femh_coarse_strain_tensor
+ *       replaces the force-driven NumericalCoarsening solve and its
+ *       coarse-strain reduction (SamplesGenerator/NumericalCoarsening.cpp:
+ *       1148-1306, femlib/src/MeshUtilities.cpp:960-1060).
  *   femh_homogenize_batch
  *       replaces the OpenMP sample loop of SamplesGeneratorImpl::
  *       computeParametersAndTensorValues (SamplesGenerator/SamplesGeneratorImpl
@@ -118,8 +122,9 @@ typedef struct femh_opts {
                            same two solves */
   int autotune;         /* 1: before the first solve of each (grid, batch, precision)
                            shape in the process, time the fine operator and the grid
-                           transfers for a few launch shapes (node planes per
-                           workgroup, planes per thread) and keep the fastest; the
+                           transfers for a few launch shapes (tile of node columns
+                           and node planes per workgroup, planes per thread) and
+                           keep the fastest; the
                            choice is cached for the process.  The fine-operator
                            shape changes the order of the dot-product partial sums,
                            so results move at rounding level.  0: built-in shapes */
@@ -135,6 +140,16 @@ typedef struct femh_opts {
   int batch_threads;    /* femh_homogenize_batch: host threads that keep batches in flight,
                            each on its own sub-context and stream (0 = 8).  1 runs the
                            batches one after another on one thread */
+  int graph;            /* 1 (default): PCG iterations run as a recorded launch sequence
+                           (hipGraph) of two iterations, replayed until every load case
+                           converged; the convergence test runs on the device and the
+                           host looks at it once per replay.  0: the same iterations
+                           launched one kernel at a time.  Results are the same */
+  int adapt_inner;      /* mixed with ch_tol > 0: 1 (default): a correction sweep after the
+                           first stops at the residual reduction its cell's estimate
+                           needs, 0.5 sqrt(ch_tol / estimate) clamped to [inner_tol, 0.1],
+                           rather than at inner_tol; the stop test stays the estimate.
+                           0: every sweep runs to inner_tol */
 } femh_opts;
 
 typedef struct femh_stats {
@@ -149,6 +164,9 @@ typedef struct femh_stats {
   int fp64_fallback;    /* mixed precision: 1 when an FP32 correction sweep failed to
                            converge for this sample and FP64 MG-PCG finished it (the
                            error is then that of the FP64 tolerance; ch_err_est is 0) */
+  double ritz_min;      /* mixed precision with ch_tol > 0: the smallest Ritz value of
+                           the preconditioned operator M^-1 K seen in the correction
+                           sweeps (it divides ch_err_est); 0 when none was formed */
 } femh_stats;
 
 void femh_default_opts(femh_opts *o);
@@ -191,6 +209,24 @@ int femh_gradient(struct femh_ctx *ctx, const double *dJdC, double *dJdrho);
  * (the unscaled adjoint building block). */
 int femh_energy_terms(struct femh_ctx *ctx, double *W);
 
+/* The force-driven (coarse-strain) form of C_H of the cells of the last
+ * homogenize call (36 doubles per cell), from the same fields: what the
+ * reference's NumericalCoarsening path computes -- six face-traction loads
+ * under periodic, translation and rotation constraints solved on the
+ * (n+1)^3-vertex mesh (SamplesGenerator/NumericalCoarsening.cpp:1148-1306),
+ * then MeshUtilities::computeCoarsenedElasticityTensor
+ * (femlib/src/MeshUtilities.cpp:960-1060), which applies the
+ * engineering-shear stiffness to tensor shear strains -- without the
+ * reference's 1e-5 diagonal regularisation (that changes C by ~1e-6
+ * relative).  It differs from the energy form of femh_homogenize by the
+ * convention (a few percent on heterogeneous cells, nothing on homogeneous
+ * ones).  Unlike the energy form it is first order in the error of the
+ * fields, so it wants fields solved to the residual tolerance (in mixed
+ * precision, ch_tol = 0).  Cube cells only (BAD_ARG otherwise: on other cells
+ * the reference's shear loads carry a net moment).  Returns FEMH_NO_FIELDS
+ * before a solve. */
+int femh_coarse_strain_tensor(struct femh_ctx *ctx, double *C);
+
 /* Copies the 6 solved periodic fluctuation fields (6*3*V doubles per sample
  * of the last batch). */
 int femh_get_fields(struct femh_ctx *ctx, double *u);
@@ -209,11 +245,20 @@ int femh_get_fields(struct femh_ctx *ctx, double *u);
 int femh_thermal_expansion(struct femh_ctx *ctx, const double *alpha, double *beta, double *alpha_eff);
 
 /* Launch shapes in use (after a solve; 0 = the built-in choice):
- * shapes[0] = node planes per workgroup of the fine operator, shapes[1 + l] =
+ * shapes[0] = the fine operator's shape (bits 0-7 node planes per workgroup,
+ * bits 8-15 the tile of owned node columns per workgroup: 1 = 16 x 16,
+ * 2 = 8 x 32, 3 = 16 x 32), shapes[1 + l] =
  * coarse planes per thread of the transfers between levels l and l + 1.
  * Writes at most n values; returns their count (levels) or a negative status.
  * No counterpart in the reference, whose CPU loops have no launch shapes. */
 int femh_launch_shapes(struct femh_ctx *ctx, int *shapes, int n);
+
+/* Sets the launch shapes of later calls on this context (the encoding of
+ * femh_launch_shapes; 0 = built-in), e.g. shapes an earlier autotuned run
+ * reported.  With femh_opts.autotune set, the tuner's choice replaces them.
+ * Results move at rounding level with the fine-operator shape (the order of
+ * its dot-product partial sums).  No counterpart in the reference. */
+int femh_set_launch_shapes(struct femh_ctx *ctx, const int *shapes, int n);
 
 /* Batch: n samples, sample s has dims[3s..3s+2] and modulus grid E[s] (host).
  * C receives 36*n doubles.  Material nu/Emin from the context options.

@@ -875,3 +875,423 @@ int or_homogenize(int nx, int ny, int nz, const double *E, double nu, const or_s
                   double *C, double *u, int *iters, double *relres) {
   return or_homogenize_cases(nx, ny, nz, E, nu, o, 63, C, u, iters, relres, NULL);
 }
+
+/* ======================================================================
+ * Force-driven periodic homogenization (the reference's NumericalCoarsening
+ * path) on the (n+1)^3-vertex mesh, restated for small grids:
+ *   stiffness   element K = E_e Ke(h = 1/max n) assembled on the vertices
+ *               (SamplesGenerator/NumericalCoarsening.cpp:1238-1306); with
+ *               `regularize` every diagonal entry is scaled by 1 + 1e-5
+ *               (:1277-1279)
+ *   constraints translation (3 rows), rotation (3 rows) and periodicity
+ *               u(v+) - u(v-) = u(c_a) - u(c_0) per face-vertex pair and
+ *               axis, the pairs already tied by a lower axis left out
+ *               (femlib/src/EMeshStiffness.cpp:53-274)
+ *   loads       the 6 face-traction cases of computeHarmonicDisplacements
+ *               (NumericalCoarsening.cpp:1152-1160, getExternalForces
+ *               :941-989): per element face vertex a force 1/(n_b n_c)/4
+ *               times the magnitude, - on the low and + on the high face;
+ *               order xx, yy, zz, xy, yz, xz
+ *   solve       the symmetric KKT system, dense, by LU with partial pivoting
+ *               (the reference factorizes it with PARDISO)
+ *   tensor      the coarse-strain form of femlib/src/MeshUtilities.cpp:960-
+ *               1060: G = tensor strains of the six solutions at the centre
+ *               of the cell-sized hex, Gv = tensor strains at the 8 Gauss
+ *               points of every element, C = (1/nelem) G^-T (sum_e sum_q w_q
+ *               Gv^T C_e Gv) G^-1 with C_e the engineering-shear stiffness
+ *               (femlib/src/StrainLin.cpp:39-58), Voigt (xx,yy,zz,yz,xz,xy)
+ * ====================================================================== */
+
+static int vidx(int ny, int nz, int i, int j, int k) { return (i * (ny + 1) + j) * (nz + 1) + k; }
+
+/* 6x6 engineering-shear isotropic stiffness of modulus E */
+static void emat(double E, double nu, double C[36]) {
+  double lam, mu;
+  or_lame(E, nu, &lam, &mu);
+  memset(C, 0, 36 * sizeof(double));
+  for (int a = 0; a < 3; a++) {
+    for (int b = 0; b < 3; b++) C[6 * a + b] = lam;
+    C[6 * a + a] = lam + 2 * mu;
+    C[6 * (a + 3) + a + 3] = mu;
+  }
+}
+
+/* in-place Gauss-Jordan inverse of an n x n matrix; returns 0 on success */
+static int invert_small(int n, double *A) {
+  double *M = (double *)malloc(sizeof(double) * n * 2 * n);
+  for (int r = 0; r < n; r++)
+    for (int c = 0; c < 2 * n; c++) M[r * 2 * n + c] = c < n ? A[r * n + c] : (c - n == r ? 1.0 : 0.0);
+  for (int c = 0; c < n; c++) {
+    int p = c;
+    for (int r = c + 1; r < n; r++)
+      if (fabs(M[r * 2 * n + c]) > fabs(M[p * 2 * n + c])) p = r;
+    if (M[p * 2 * n + c] == 0.0) {
+      free(M);
+      return 1;
+    }
+    if (p != c)
+      for (int t = 0; t < 2 * n; t++) {
+        double s = M[c * 2 * n + t];
+        M[c * 2 * n + t] = M[p * 2 * n + t];
+        M[p * 2 * n + t] = s;
+      }
+    double d = M[c * 2 * n + c];
+    for (int t = 0; t < 2 * n; t++) M[c * 2 * n + t] /= d;
+    for (int r = 0; r < n; r++) {
+      if (r == c) continue;
+      double f = M[r * 2 * n + c];
+      if (f != 0.0)
+        for (int t = 0; t < 2 * n; t++) M[r * 2 * n + t] -= f * M[c * 2 * n + t];
+    }
+  }
+  for (int r = 0; r < n; r++)
+    for (int c = 0; c < n; c++) A[r * n + c] = M[r * 2 * n + n + c];
+  free(M);
+  return 0;
+}
+
+/* dense LU with partial pivoting of the n x n matrix A (in place), then the
+ * solution of A x = b for nrhs right-hand sides (columns of b, n x nrhs
+ * row-major, overwritten by x).  Returns 0 on success. */
+static int lu_solve(int n, double *A, int nrhs, double *b) {
+  int *piv = (int *)malloc(sizeof(int) * n);
+  for (int c = 0; c < n; c++) {
+    int p = c;
+    double best = fabs(A[(size_t)c * n + c]);
+    for (int r = c + 1; r < n; r++)
+      if (fabs(A[(size_t)r * n + c]) > best) {
+        best = fabs(A[(size_t)r * n + c]);
+        p = r;
+      }
+    piv[c] = p;
+    if (best == 0.0) {
+      free(piv);
+      return 1;
+    }
+    if (p != c) {
+      for (int t = 0; t < n; t++) {
+        double s = A[(size_t)c * n + t];
+        A[(size_t)c * n + t] = A[(size_t)p * n + t];
+        A[(size_t)p * n + t] = s;
+      }
+      for (int t = 0; t < nrhs; t++) {
+        double s = b[(size_t)c * nrhs + t];
+        b[(size_t)c * nrhs + t] = b[(size_t)p * nrhs + t];
+        b[(size_t)p * nrhs + t] = s;
+      }
+    }
+    const double d = A[(size_t)c * n + c];
+#pragma omp parallel for schedule(static)
+    for (int r = c + 1; r < n; r++) {
+      const double f = A[(size_t)r * n + c] / d;
+      A[(size_t)r * n + c] = f;
+      if (f == 0.0) continue;
+      for (int t = c + 1; t < n; t++) A[(size_t)r * n + t] -= f * A[(size_t)c * n + t];
+      for (int t = 0; t < nrhs; t++) b[(size_t)r * nrhs + t] -= f * b[(size_t)c * nrhs + t];
+    }
+  }
+  for (int r = n - 1; r >= 0; r--)
+    for (int t = 0; t < nrhs; t++) {
+      double s = b[(size_t)r * nrhs + t];
+      for (int c = r + 1; c < n; c++) s -= A[(size_t)r * n + c] * b[(size_t)c * nrhs + t];
+      b[(size_t)r * nrhs + t] = s / A[(size_t)r * n + r];
+    }
+  free(piv);
+  return 0;
+}
+
+/* vertex spacing 1/max n; femlib computes it in float (ElementRegGrid.cpp:116) */
+static double grid_h(int nx, int ny, int nz, int faithful) {
+  const int m = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
+  return faithful ? (double)(1.0f / (float)m) : 1.0 / (double)m;
+}
+
+int or_kkt_constraints(int nx, int ny, int nz, int faithful, double *rows, int max_rows) {
+  const int NV = (nx + 1) * (ny + 1) * (nz + 1), ND = 3 * NV;
+  const double h = grid_h(nx, ny, nz, faithful);
+  int nr = 0;
+#define ROW(r) (rows + (size_t)(r) * ND)
+  int need = 6 + 3 * ((ny + 1) * (nz + 1) - 1 + nx * (nz + 1) - 1 + nx * ny - 1);
+  if (!rows) return need;
+  if (max_rows < need) return -need;
+  memset(rows, 0, sizeof(double) * (size_t)need * ND);
+  /* translation: -sum_v u_v = 0 */
+  for (int kk = 0; kk < 3; kk++, nr++)
+    for (int v = 0; v < NV; v++) ROW(nr)[3 * v + kk] = -1.0;
+  /* rotation: sum_v c_v u_v = 0, c_v = -skew(X_v - centre) */
+  double cen[3] = {0, 0, 0};
+  for (int i = 0; i <= nx; i++)
+    for (int j = 0; j <= ny; j++)
+      for (int k = 0; k <= nz; k++) {
+        cen[0] += i * h;
+        cen[1] += j * h;
+        cen[2] += k * h;
+      }
+  for (int a = 0; a < 3; a++) cen[a] /= NV;
+  for (int i = 0; i <= nx; i++)
+    for (int j = 0; j <= ny; j++)
+      for (int k = 0; k <= nz; k++) {
+        const int v = vidx(ny, nz, i, j, k);
+        const double r0 = i * h - cen[0], r1 = j * h - cen[1], r2 = k * h - cen[2];
+        const double c[3][3] = {{0, r2, -r1}, {-r2, 0, r0}, {r1, -r0, 0}};
+        for (int kk = 0; kk < 3; kk++)
+          for (int ll = 0; ll < 3; ll++) ROW(nr + kk)[3 * v + ll] = c[kk][ll];
+      }
+  nr += 3;
+  /* periodicity: -u(c0) + u(c_a) + u(v-) - u(v+) = 0 per component */
+  const int c0 = vidx(ny, nz, 0, 0, 0);
+  const int ca[3] = {vidx(ny, nz, nx, 0, 0), vidx(ny, nz, 0, ny, 0), vidx(ny, nz, 0, 0, nz)};
+  for (int ax = 0; ax < 3; ax++) {
+    const int na = ax == 0 ? ny + 1 : nx, nb = ax == 2 ? ny : nz + 1;
+    for (int p = 0; p < na; p++)
+      for (int q = 0; q < nb; q++) {
+        if (p == 0 && q == 0) continue;
+        int vm, vp;
+        if (ax == 0) {
+          vm = vidx(ny, nz, 0, p, q);
+          vp = vidx(ny, nz, nx, p, q);
+        } else if (ax == 1) {
+          vm = vidx(ny, nz, p, 0, q);
+          vp = vidx(ny, nz, p, ny, q);
+        } else {
+          vm = vidx(ny, nz, p, q, 0);
+          vp = vidx(ny, nz, p, q, nz);
+        }
+        for (int kk = 0; kk < 3; kk++, nr++) {
+          ROW(nr)[3 * c0 + kk] += -1.0;
+          ROW(nr)[3 * ca[ax] + kk] += 1.0;
+          ROW(nr)[3 * vm + kk] += 1.0;
+          ROW(nr)[3 * vp + kk] += -1.0;
+        }
+      }
+  }
+#undef ROW
+  return nr;
+}
+
+/* coarse-strain tensor from the six vertex displacement fields u[6][3 NV]
+ * (MeshUtilities.cpp:960-1060) */
+void or_coarse_strain_tensor(int nx, int ny, int nz, const double *E, double nu, int faithful,
+                             const double *u, double *C) {
+  const int NV = (nx + 1) * (ny + 1) * (nz + 1);
+  const double h = grid_h(nx, ny, nz, faithful);
+  const int float_gauss = faithful;
+  const double L[3] = {nx * h, ny * h, nz * h};
+  /* G: cell-sized hex at its centre; columns = load cases */
+  double G[36] = {0};
+  for (int d = 0; d < 6; d++) {
+    double gu[3][3] = {{0}};
+    for (int v = 0; v < 8; v++) {
+      const int b[3] = {(v >> 2) & 1, (v >> 1) & 1, v & 1};
+      const int vx = vidx(ny, nz, b[0] * nx, b[1] * ny, b[2] * nz);
+      double gN[3];
+      for (int a = 0; a < 3; a++) gN[a] = (b[a] ? 1.0 : -1.0) / (4.0 * L[a]);
+      for (int c = 0; c < 3; c++)
+        for (int a = 0; a < 3; a++) gu[c][a] += u[(size_t)d * 3 * NV + 3 * vx + c] * gN[a];
+    }
+    G[0 * 6 + d] = gu[0][0];
+    G[1 * 6 + d] = gu[1][1];
+    G[2 * 6 + d] = gu[2][2];
+    G[3 * 6 + d] = 0.5 * (gu[1][2] + gu[2][1]);
+    G[4 * 6 + d] = 0.5 * (gu[0][2] + gu[2][0]);
+    G[5 * 6 + d] = 0.5 * (gu[0][1] + gu[1][0]);
+  }
+  const double g = float_gauss ? (double)(1.0f / sqrtf(3.0f)) : 1.0 / sqrt(3.0);
+  double S[36] = {0};
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        double Ce[36];
+        emat(E[(i * ny + j) * nz + k], nu, Ce);
+        for (int q = 0; q < 8; q++) {
+          const double xi[3] = {(q & 4) ? g : -g, (q & 2) ? g : -g, (q & 1) ? g : -g};
+          double gN[8][3];
+          for (int v = 0; v < 8; v++) {
+            const int sgn[3] = {((v >> 2) & 1) ? 1 : -1, ((v >> 1) & 1) ? 1 : -1, (v & 1) ? 1 : -1};
+            gN[v][0] = sgn[0] / (4.0 * h) * (1 + sgn[1] * xi[1]) * (1 + sgn[2] * xi[2]);
+            gN[v][1] = sgn[1] / (4.0 * h) * (1 + sgn[0] * xi[0]) * (1 + sgn[2] * xi[2]);
+            gN[v][2] = sgn[2] / (4.0 * h) * (1 + sgn[0] * xi[0]) * (1 + sgn[1] * xi[1]);
+          }
+          double Gv[36];
+          for (int d = 0; d < 6; d++) {
+            double gu[3][3] = {{0}};
+            for (int v = 0; v < 8; v++) {
+              const int vx = vidx(ny, nz, i + ((v >> 2) & 1), j + ((v >> 1) & 1), k + (v & 1));
+              for (int c = 0; c < 3; c++)
+                for (int a = 0; a < 3; a++) gu[c][a] += u[(size_t)d * 3 * NV + 3 * vx + c] * gN[v][a];
+            }
+            Gv[0 * 6 + d] = gu[0][0];
+            Gv[1 * 6 + d] = gu[1][1];
+            Gv[2 * 6 + d] = gu[2][2];
+            Gv[3 * 6 + d] = 0.5 * (gu[1][2] + gu[2][1]);
+            Gv[4 * 6 + d] = 0.5 * (gu[0][2] + gu[2][0]);
+            Gv[5 * 6 + d] = 0.5 * (gu[0][1] + gu[1][0]);
+          }
+          for (int a = 0; a < 6; a++)
+            for (int b = 0; b < 6; b++) {
+              double s = 0;
+              for (int r = 0; r < 6; r++)
+                for (int t = 0; t < 6; t++) s += Gv[r * 6 + a] * Ce[r * 6 + t] * Gv[t * 6 + b];
+              S[a * 6 + b] += 0.125 * s;
+            }
+        }
+      }
+  double Gi[36];
+  memcpy(Gi, G, sizeof(Gi));
+  if (invert_small(6, Gi)) {
+    for (int t = 0; t < 36; t++) C[t] = NAN;
+    return;
+  }
+  /* femlib scales by (1.f / nelem) in float (MeshUtilities.cpp:1059) */
+  const double inel = faithful ? (double)(1.0f / (float)(nx * ny * nz)) : 1.0 / ((double)nx * ny * nz);
+  for (int a = 0; a < 6; a++)
+    for (int b = 0; b < 6; b++) {
+      double s = 0;
+      for (int r = 0; r < 6; r++)
+        for (int t = 0; t < 6; t++) s += Gi[r * 6 + a] * S[r * 6 + t] * Gi[t * 6 + b];
+      C[a * 6 + b] = s * inel;
+    }
+}
+
+int or_force_driven(int nx, int ny, int nz, const double *E, double nu, int regularize, int faithful,
+                    double *C, double *u) {
+  const int NV = (nx + 1) * (ny + 1) * (nz + 1), ND = 3 * NV;
+  const double h = grid_h(nx, ny, nz, faithful);
+  const int ncon = or_kkt_constraints(nx, ny, nz, faithful, NULL, 0);
+  const int n = ND + ncon;
+  double *A = (double *)calloc((size_t)n * n, sizeof(double));
+  double *rows = (double *)malloc(sizeof(double) * (size_t)ncon * ND);
+  double *b = (double *)calloc((size_t)n * 6, sizeof(double));
+  if (!A || !rows || !b) {
+    free(A);
+    free(rows);
+    free(b);
+    return 2;
+  }
+  double Ke[576];
+  or_ke_hex(1.0, nu, h, faithful, Ke);
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        const double Ee = E[(i * ny + j) * nz + k];
+        int vv[8];
+        for (int v = 0; v < 8; v++) vv[v] = vidx(ny, nz, i + ((v >> 2) & 1), j + ((v >> 1) & 1), k + (v & 1));
+        for (int a = 0; a < 8; a++)
+          for (int ca = 0; ca < 3; ca++)
+            for (int bb = 0; bb < 8; bb++)
+              for (int cb = 0; cb < 3; cb++)
+                A[(size_t)(3 * vv[a] + ca) * n + 3 * vv[bb] + cb] += Ee * Ke[(3 * a + ca) * 24 + 3 * bb + cb];
+      }
+  if (regularize)
+    for (int t = 0; t < ND; t++) A[(size_t)t * n + t] *= 1.0 + 1e-5;
+  or_kkt_constraints(nx, ny, nz, faithful, rows, ncon);
+  for (int r = 0; r < ncon; r++)
+    for (int c = 0; c < ND; c++) {
+      const double v = rows[(size_t)r * ND + c];
+      A[(size_t)(ND + r) * n + c] = v;
+      A[(size_t)c * n + ND + r] = v;
+    }
+  /* loads: (fx, fy, fz) per case, reference order xx yy zz xy yz xz; a
+   * force along direction `dir` on the faces normal to `ax` */
+  static const int pairs[6][2][2] = {{{0, 0}, {-1, -1}}, {{1, 1}, {-1, -1}}, {{2, 2}, {-1, -1}},
+                                     {{1, 0}, {0, 1}},   {{2, 1}, {1, 2}},   {{2, 0}, {0, 2}}};
+  const int nn[3] = {nx, ny, nz};
+  for (int d = 0; d < 6; d++)
+    for (int t = 0; t < 2; t++) {
+      const int ax = pairs[d][t][0], dir = pairs[d][t][1];
+      if (ax < 0) continue;
+      const double mag = (d < 3) ? 1.0 : 0.5;
+      const double coeff = mag / ((double)nn[(ax + 1) % 3] * nn[(ax + 2) % 3]) / 4.0;
+      for (int side = 0; side < 2; side++) {
+        const double sign = side ? 1.0 : -1.0;
+        for (int i = 0; i < nx; i++)
+          for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) {
+              const int ijk[3] = {i, j, k};
+              if (ijk[ax] != (side ? nn[ax] - 1 : 0)) continue;
+              for (int v = 0; v < 8; v++) {
+                const int bv[3] = {(v >> 2) & 1, (v >> 1) & 1, v & 1};
+                if (bv[ax] != side) continue;
+                const int vx = vidx(ny, nz, i + bv[0], j + bv[1], k + bv[2]);
+                b[(size_t)(3 * vx + dir) * 6 + d] += sign * coeff;
+              }
+            }
+      }
+    }
+  const int st = lu_solve(n, A, 6, b);
+  if (!st) {
+    for (int d = 0; d < 6; d++)
+      for (int t = 0; t < ND; t++) u[(size_t)d * ND + t] = b[(size_t)t * 6 + d];
+    or_coarse_strain_tensor(nx, ny, nz, E, nu, faithful, u, C);
+  }
+  free(A);
+  free(rows);
+  free(b);
+  return st;
+}
+
+/* The same coarse-strain tensor from the strain-driven periodic fields
+ * u[6][3][V] (zero-mean fluctuations of the unit engineering strains, as
+ * or_homogenize returns them): with a = the engineering strains of the six
+ * total fields at a Gauss point, T = diag(1,1,1,1/2,1/2,1/2) (engineering to
+ * tensor shear) and C0 the unit-modulus engineering stiffness,
+ * C = T^-1 [ (1/V) sum_e E_e sum_q w_q a^T T C0 T a ] T^-1.  This is what the
+ * force-driven tensor reduces to when the constrained solutions are affine
+ * plus periodic (no diagonal regularisation). */
+void or_coarse_strain_from_fields(int nx, int ny, int nz, const double *E, double nu, int float_gauss,
+                                  const double *u, double *C) {
+  const size_t V = (size_t)nx * ny * nz;
+  double C0[36], D[36];
+  emat(1.0, nu, C0);
+  const double Tv[6] = {1, 1, 1, 0.5, 0.5, 0.5};
+  for (int a = 0; a < 6; a++)
+    for (int b = 0; b < 6; b++) D[a * 6 + b] = Tv[a] * C0[a * 6 + b] * Tv[b];
+  const double g = float_gauss ? (double)(1.0f / sqrtf(3.0f)) : 1.0 / sqrt(3.0);
+  double M[36] = {0};
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        const double Ee = E[(i * ny + j) * nz + k];
+        double chi[6][24];
+        for (int lc = 0; lc < 6; lc++) {
+          double ua[24];
+          affine_local(lc, ua);
+          for (int v = 0; v < 8; v++) {
+            const int n = (wrap(i + ((v >> 2) & 1), nx) * ny + wrap(j + ((v >> 1) & 1), ny)) * nz + wrap(k + (v & 1), nz);
+            for (int c = 0; c < 3; c++) chi[lc][3 * v + c] = ua[3 * v + c] + u[((size_t)lc * 3 + c) * V + n];
+          }
+        }
+        for (int q = 0; q < 8; q++) {
+          const double xi[3] = {(q & 4) ? g : -g, (q & 2) ? g : -g, (q & 1) ? g : -g};
+          double gN[8][3];
+          for (int v = 0; v < 8; v++) {
+            const int sgn[3] = {((v >> 2) & 1) ? 1 : -1, ((v >> 1) & 1) ? 1 : -1, (v & 1) ? 1 : -1};
+            gN[v][0] = sgn[0] / 4.0 * (1 + sgn[1] * xi[1]) * (1 + sgn[2] * xi[2]);
+            gN[v][1] = sgn[1] / 4.0 * (1 + sgn[0] * xi[0]) * (1 + sgn[2] * xi[2]);
+            gN[v][2] = sgn[2] / 4.0 * (1 + sgn[0] * xi[0]) * (1 + sgn[1] * xi[1]);
+          }
+          double a[6][6];
+          for (int lc = 0; lc < 6; lc++) {
+            double gu[3][3] = {{0}};
+            for (int v = 0; v < 8; v++)
+              for (int c = 0; c < 3; c++)
+                for (int d = 0; d < 3; d++) gu[c][d] += chi[lc][3 * v + c] * gN[v][d];
+            a[0][lc] = gu[0][0];
+            a[1][lc] = gu[1][1];
+            a[2][lc] = gu[2][2];
+            a[3][lc] = gu[1][2] + gu[2][1];
+            a[4][lc] = gu[0][2] + gu[2][0];
+            a[5][lc] = gu[0][1] + gu[1][0];
+          }
+          for (int p = 0; p < 6; p++)
+            for (int s = 0; s < 6; s++) {
+              double acc = 0;
+              for (int r = 0; r < 6; r++)
+                for (int t = 0; t < 6; t++) acc += a[r][p] * D[r * 6 + t] * a[t][s];
+              M[p * 6 + s] += Ee * 0.125 * acc;
+            }
+        }
+      }
+  for (int p = 0; p < 6; p++)
+    for (int s = 0; s < 6; s++) C[p * 6 + s] = M[p * 6 + s] / ((double)V * Tv[p] * Tv[s]);
+}

@@ -107,6 +107,26 @@ void or_mg_apply(void *mg, int l, const double *u, double *y);
 void or_mg_restrict(void *mg, int l, const double *rf, double *rc);
 void or_mg_prolong_add(void *mg, int l, const double *ec, double *ef);
 
+/* Force-driven periodic homogenization of the reference's NumericalCoarsening
+ * path on the (n+1)^3-vertex mesh (small grids: dense KKT solve).  C[36]
+ * receives the coarse-strain tensor (Voigt, engineering shear), u[6][3 NV]
+ * the vertex displacements of the six face-traction load cases (reference
+ * order xx, yy, zz, xy, yz, xz; vertex (i*(ny+1)+j)*(nz+1)+k, dof 3 v + c).
+ * regularize: diagonal x (1 + 1e-5) as the reference assembles it;
+ * faithful: the reference's float roundings (Gauss abscissa, vertex spacing
+ * 1/max n, the 1/nelem factor).  Returns 0 on success. */
+int or_force_driven(int nx, int ny, int nz, const double *E, double nu, int regularize,
+                    int faithful, double *C, double *u);
+/* the constraint rows of that system, dense [nrows][3 NV]; with rows == NULL
+ * returns the row count */
+int or_kkt_constraints(int nx, int ny, int nz, int faithful, double *rows, int max_rows);
+/* the coarse-strain tensor of given vertex displacement fields */
+void or_coarse_strain_tensor(int nx, int ny, int nz, const double *E, double nu, int faithful,
+                             const double *u, double *C);
+/* the same tensor from strain-driven periodic fields u[6][3][V] */
+void or_coarse_strain_from_fields(int nx, int ny, int nz, const double *E, double nu, int float_gauss,
+                                  const double *u, double *C);
+
 double or_wall(void);
 void or_set_threads(int n);
 

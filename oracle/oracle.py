@@ -208,3 +208,47 @@ def set_threads(n):
 
 def wall():
     return lib().or_wall()
+
+
+def force_driven(E, nu, regularize=False, faithful=False):
+    """The reference's force-driven path (KKT on the (n+1)^3 mesh, face
+    tractions, coarse-strain tensor), dense; small grids only.  faithful: the
+    reference's float roundings.  Returns dict(C=6x6, u=[6, 3 NV] vertex
+    displacements, status)."""
+    E = _E(E)
+    nx, ny, nz = E.shape
+    NV = (nx + 1) * (ny + 1) * (nz + 1)
+    C = np.zeros(36)
+    u = np.zeros((6, 3 * NV))
+    st = lib().or_force_driven(ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_int(nz), _dp(E),
+                               ctypes.c_double(nu), ctypes.c_int(int(regularize)),
+                               ctypes.c_int(int(faithful)), _dp(C), _dp(u))
+    return dict(C=C.reshape(6, 6), u=u, status=st)
+
+
+def kkt_constraints(shape, faithful=False):
+    nx, ny, nz = shape
+    NV = (nx + 1) * (ny + 1) * (nz + 1)
+    n = lib().or_kkt_constraints(nx, ny, nz, int(faithful), None, 0)
+    rows = np.zeros((n, 3 * NV))
+    got = lib().or_kkt_constraints(nx, ny, nz, int(faithful), _dp(rows), ctypes.c_int(n))
+    assert got == n
+    return rows
+
+
+def coarse_strain_tensor(E, nu, u, faithful=False):
+    E = _E(E)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    C = np.zeros(36)
+    lib().or_coarse_strain_tensor(*[ctypes.c_int(n) for n in E.shape], _dp(E), ctypes.c_double(nu),
+                                  ctypes.c_int(int(faithful)), _dp(u), _dp(C))
+    return C.reshape(6, 6)
+
+
+def coarse_strain_from_fields(E, nu, u, float_gauss=False):
+    E = _E(E)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    C = np.zeros(36)
+    lib().or_coarse_strain_from_fields(*[ctypes.c_int(n) for n in E.shape], _dp(E), ctypes.c_double(nu),
+                                       ctypes.c_int(int(float_gauss)), _dp(u), _dp(C))
+    return C.reshape(6, 6)

@@ -25,13 +25,21 @@ def test_autotuned_fp64_solve_matches_builtin_shapes():
         assert rel(Ct[s], Cb[s]) <= 1e-10
     shapes = ht.launch_shapes()
     assert len(shapes) == st[0]["levels"] >= 3
-    assert shapes[0] in (0, 2, 4, 8, 16, 32) and all(c in (0, 1, 2, 4, 8) for c in shapes[1:])
+    # fine operator: bits 0-7 node planes per workgroup, bits 8-15 tile of
+    # owned node columns (0 = built-in, 1 = 16 x 16, 2 = 8 x 32, 3 = 16 x 32)
+    assert (shapes[0] & 0xFF) in (0, 2, 4, 8, 16, 32) and (shapes[0] >> 8) in (0, 1, 2, 3)
+    assert all(c in (0, 1, 2, 4, 8) for c in shapes[1:])
     assert hb.launch_shapes() == [0] * len(shapes)
     # a second context of the same shape takes the cached choice
     h2 = Homogenizer((n, n, n), nu=0.3, tol=1e-12, batch=2, autotune=True)
     h2.homogenize_many(Es, modulus=True)
     assert h2.launch_shapes() == shapes
-    for h in (ht, hb, h2):
+    # a context pinned to the tuned shapes gives the tuned result exactly
+    hp = Homogenizer((n, n, n), nu=0.3, tol=1e-12, batch=2)
+    hp.set_launch_shapes(shapes)
+    Cp, _ = hp.homogenize_many(Es, modulus=True)
+    assert np.array_equal(Cp, Ct)
+    for h in (ht, hb, h2, hp):
         h.close()
 
 

@@ -78,16 +78,16 @@ def test_batch_mixes_trivial_and_hard_samples():
 
 def test_mixed_batch_meets_1e6_and_estimates():
     rho = simp_set(5, 32, seed0=60, corr=4.0)
-    hm = Homogenizer((32, 32, 32), nu=NU, precision="mixed", ch_tol=1e-8, batch=4)
+    hm = Homogenizer((32, 32, 32), nu=NU, precision="mixed", ch_tol=1e-6, batch=4)
     Cm, stm = hm.homogenize_many(rho)
     h64 = Homogenizer((32, 32, 32), nu=NU, tol=1e-12, batch=4)
     C64, _ = h64.homogenize_many(rho)
     for s in range(5):
         err = rel(Cm[s], C64[s])
         assert err <= 1e-6, (s, err)
-        assert 0 < stm[s]["ch_err_est"] <= 1e-8
-        # the estimate bounds the error it reports (allowing a 10x slack)
-        assert err <= 10 * stm[s]["ch_err_est"] + 1e-12
+        assert 0 < stm[s]["ch_err_est"] <= 1e-6
+        # the estimate bounds the error it reports
+        assert err <= stm[s]["ch_err_est"], (s, err, stm[s])
 
 
 def test_degree1_smoothing_batch_path():
@@ -168,7 +168,7 @@ def test_uneven_binary_batch_mixed_meets_1e6():
     precision per cell within 1e-6 of FP64, as in the 512-cell workload."""
     grids = W.batch_uneven(24, 3)
     kw = dict(nu=NU, tol=1e-10, max_iter=20000, batch=8)
-    Cm, _ = homogenize_batch(grids, precision="mixed", ch_tol=1e-8, **kw)
+    Cm, _ = homogenize_batch(grids, precision="mixed", ch_tol=1e-6, inner_tol=3e-4, **kw)
     C64, _ = homogenize_batch(grids, **kw)
     errs = [rel(a, b) for a, b in zip(Cm, C64)]
     assert max(errs) <= 1e-6, errs
