@@ -1314,10 +1314,12 @@ void Context::homogenize(double *C, femh_stats *st) {
     const int max_outer = o.max_outer > 0 ? o.max_outer : 12;
     // Energy-norm bound: r.K^-1 r <= r.M^-1 r / lambda_min(M^-1 K).  The
     // correction sweeps' PCG coefficients give Lanczos matrices whose smallest
-    // Ritz value estimates lambda_min (from above, converging as the sweeps
-    // iterate); the smallest over all sweeps and load cases of a cell divides
-    // that cell's estimate.  A warm start begins with the value of the slot's
-    // previous call.
+    // Ritz value estimates lambda_min, and so does the Rayleigh quotient of
+    // each preconditioned residual at a stop test; the smallest of them over
+    // all sweeps and load cases of a cell divides that cell's estimate.  All
+    // of them lie at or above lambda_min, so the estimate is a bound once
+    // they have come down to it (they converge as the sweeps iterate).  A
+    // warm start begins with the value of the slot's previous call.
     const bool bound = o.ch_tol > 0;
     std::vector<double> theta(nb, INFINITY);
     if (warm && (int)theta_prev.size() >= nb)
@@ -1335,10 +1337,20 @@ void Context::homogenize(double *C, femh_stats *st) {
       precond<float>(h32, r32.as<float>(), z32.as<float>(), ao, S_RZ0);
       FEMH_HIP(hipGetLastError());
       if (o.ch_tol > 0 && (sweep > 0 || warm)) {
+        // z.Kz of the preconditioned residual: its Rayleigh quotient
+        // mu = z.Kz / r.z (M z = r) lies in the spectrum of M^-1 K, so it is
+        // another upper estimate of lambda_min.  A sweep leaves its residual
+        // in the modes it has not resolved, so mu catches a small isolated
+        // eigenvalue that the sweep's short Lanczos matrix missed (a stiff
+        // island in a soft phase: Ritz 0.26 after the first sweep, 0.0054
+        // after the second).  One FP32 fine-operator launch.
+        apply<float>(h32, 0, z32.as<float>(), q32.as<float>(), nullptr, 0, ao, &red, S_PQ);
         read_scalars(S_RZ0, rz.data(), R, rz_parts(h32));
+        std::vector<double> zkz(R);
+        read_scalars(S_PQ, zkz.data(), R, apply_parts());
         for (int s = 0; s < nb; s++) {
           double trC = 0, srz = 0;
-          bool any = false;
+          bool any = false, definite = true;
           for (int k = 0; k < kRHS; k++) {
             const int t = kRHS * s + k;
             // (cubic: trace C = 3 (C_xx,xx + C_yz,yz) from the two solved cases)
@@ -1347,14 +1359,19 @@ void Context::homogenize(double *C, femh_stats *st) {
             else
               trC += (aii[k] * mst[3 * s] - bx[t] - xr[t]) / g.V;
             if (active[t]) {
-              srz += std::max(rz[t], 0.0);
               any = true;
+              if (!(rz[t] > 0) || !(zkz[t] > 0)) {
+                definite = false;  // the preconditioner is not positive definite here
+              } else {
+                srz += rz[t];
+                theta[s] = std::min(theta[s], zkz[t] / rz[t]);
+              }
             }
           }
           if (!any) continue;
-          // no Ritz value yet (a warm start's first sweep with no earlier
-          // measurement): no bound, so the sweep runs
-          ch_est[s] = (trC > 0 && std::isfinite(theta[s]) && theta[s] > 0)
+          // no bound without a positive definite measurement (or, on a warm
+          // start's first sweep, without any): the sweep runs
+          ch_est[s] = (definite && trC > 0 && std::isfinite(theta[s]) && theta[s] > 0)
                           ? std::sqrt(6.0) * srz / (g.V * trC) / theta[s]
                           : INFINITY;
           if (ch_est[s] <= o.ch_tol)
