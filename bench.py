@@ -89,15 +89,16 @@ def parse():
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tiled_fine_operator_64cube_mixed_batch16_hbm_traffic.txt")
+TRAFFIC_FILE_PLAIN = os.path.join(ROOT, "profiles", "k_tiled_fp64_fine_operator_64cube_plain_one_cell.txt")
 
 
-def pmc_traffic():
+def pmc_traffic(path=TRAFFIC_FILE):
     """HBM bytes per fine-operator launch from the committed FETCH_SIZE /
     WRITE_SIZE counter passes (copy-calibrated), with the algorithmic bytes
     per launch of the same run; None when the summary is absent."""
     try:
         vals = {}
-        for line in open(TRAFFIC_FILE):
+        for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
                 vals["traffic"] = float(line.split(":")[1])
             if line.startswith("algorithmic_bytes_per_launch"):
@@ -430,7 +431,10 @@ def run_opt(a):
                                "isotropic targets E~U[0.05,0.15] nu 0.25, 30 steps", "batch": a.batch,
                    "ch_tol": a.ch_tol},
         "solves": opt.solves, "objective_ratio_median": float(np.median(Jn / J0)),
-        "ch_err_est_max": float(max(opt.est)) if opt.est else None,
+        # (solves without a bound -- no positive definite Lanczos measurement
+        # yet -- ran to the residual tolerance; counted, not in the max)
+        "ch_err_est_max": max((float(e) for e in opt.est if np.isfinite(e)), default=None),
+        "ch_est_unbounded_solves": int(sum(1 for e in opt.est if not np.isfinite(e))),
         "ch_rel_err_vs_fp64_max_final": max(errs) if errs else None}))
     opt.close()
 
@@ -479,6 +483,7 @@ def run_plain(a):
         err0 = float(np.linalg.norm(C[0] - g) / np.linalg.norm(g))
     ach = k64["bytes"] / (k64["ms"] * 1e-3) / 1e9 if k64["ms"] > 0 else 0.0
     cpu = None if a.no_cpu else cpu_baseline(n, a.cpu_runs)
+    tr = pmc_traffic(TRAFFIC_FILE_PLAIN) if n == 64 else None
     print(json.dumps({
         "metric": "64^3 plain FP64 homogenizations/sec (one cell per call, 3-level V(2,2)-PCG to "
                   "1e-10); % HBM roofline of the FP64 fine operator",
@@ -492,7 +497,9 @@ def run_plain(a):
                      "kernel": "k_tiled<double> (fine-level operator, all stage/epilogue variants)",
                      "launches": k64["launches"],
                      "avg_us": 1e3 * k64["ms"] / max(k64["launches"], 1),
-                     "bytes_per_launch": k64["bytes"] / max(k64["launches"], 1)},
+                     "bytes_per_launch": k64["bytes"] / max(k64["launches"], 1),
+                     **({"traffic": tr[0], "traffic_source": os.path.relpath(TRAFFIC_FILE_PLAIN, ROOT),
+                         "traffic_over_algorithmic": tr[0] / tr[1]} if tr else {})},
         "cpu_baseline": cpu,
         "ch_rel_err_vs_golden_seed0": err0,
         "iters_max": max(max(s["iters"]) for s in stats)}))

@@ -2,6 +2,7 @@
 // contract).  Grids carry the right-hand side q = 6 s + lc; per-sample data
 // (moduli, stencils, inverse diagonals, Chebyshev coefficients) is selected
 // by s = q / 6.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -15,23 +16,24 @@ namespace femh {
 // ---------------------------------------------------------------------------
 // fine level: tiled cube-mode voxel kernel
 //
-// K0 = H^T Lambda H per voxel (tables.hpp).  A workgroup owns a 16x16 tile of
-// node columns (j,k) and marches over a chunk of TI planes in i.  Per voxel
-// slab (planes s, s+1):
-//   1. node plane s+1 of the tile plus a one-node halo on the low j/k sides
-//      (18x18 nodes, periodic wrap) is staged into LDS (a three-plane ring,
-//      so the owned node's staged value stays readable for the epilogue);
-//   2. each of the 17x17 voxels touching the tile transforms its top face
-//      ((j,k) butterflies; the bottom face is the top face of the slab
-//      below, carried in registers), forms the 8 cube modes per component
-//      with the i stage, applies the 45-term mode coupling scaled by its
+// K0 = H^T Lambda H per voxel (tables.hpp).  A workgroup owns a TJ x TK tile
+// of node columns (j,k) (16 x 16, 8 x 32 or 16 x 32, one thread per column)
+// and marches over a chunk of TI planes in i.  Per voxel slab (planes s, s+1):
+//   1. node plane s+1 of the tile plus a one-node halo on each j/k side
+//      ((TJ+2) x (TK+2) nodes, periodic wrap) is staged into LDS (a
+//      two-buffer ring, see below);
+//   2. each of the (TJ+1) x (TK+1) voxels touching the tile transforms its
+//      top face ((j,k) butterflies; the bottom face is the top face of the
+//      slab below, carried in registers), forms the 8 cube modes per
+//      component with the i stage, applies the mode coupling scaled by its
 //      modulus and splits the result back into bottom- and top-face modes;
 //   3. the bottom-face modes plus the top-face modes of the voxel below (the
 //      same node plane s) go through one inverse face transform into LDS,
 //      and the 4 tile nodes sharing each corner gather them: plane s is
 //      complete and passes through the epilogue.
-// The j/k halo voxels are recomputed by neighbouring tiles (17^2/16^2 of the
-// voxel work) and the i halo slab once per chunk ((TI+1)/TI).
+// The j/k halo voxels are recomputed by neighbouring tiles ((TJ+1)(TK+1) /
+// (TJ TK) of the voxel work: 1.13 for 16 x 16, 1.10 for 16 x 32) and the i
+// halo slab once per chunk ((TI+1)/TI).
 constexpr int kTile = 16;
 constexpr int kTN = kTile + 2;   // staged nodes per row
 constexpr int kTE = kTile + 1;   // voxels per row
@@ -509,32 +511,39 @@ __global__ __launch_bounds__(TJ * TK, sizeof(T) == 4 ? 4 : kWaves64) void k_tile
 
 template <typename T, int ST, int EP, int TJ, int TK>
 static void tiled_go(const Dims &g, int R, const T *E, const ModeC<T> &mc, const OpArgs<T> &a,
-                     hipStream_t s, int shape) {
+                     hipStream_t s, int shape, hipEvent_t ev0, hipEvent_t ev1) {
   const int tiles_k = (g.nz + TK - 1) / TK, tiles_j = (g.ny + TJ - 1) / TJ;
   const int TI = tiled_chunk(g, R, shape);
   const int tiles = tiles_j * tiles_k, chunks = (g.nx + TI - 1) / TI;
   if (R % kRHS) throw std::runtime_error("right-hand sides must come in groups of six");
   // buffer resources address a right-hand side's vector with 32-bit offsets
   if ((size_t)3 * g.V * sizeof(T) >= ((size_t)1 << 31)) throw std::runtime_error("grid too large for the tiled operator");
+  if (ev0) {
+    // timed launch: the runtime records the events at the kernel's own start
+    // and end, so no queue gap is counted
+    hipExtLaunchKernelGGL((k_tiled<T, ST, EP, TJ, TK>), dim3(tiles * chunks * R), dim3(TJ * TK), 0, s, ev0, ev1,
+                          0, g, E, mc, a, TI, tiles_k, tiles, chunks);
+    return;
+  }
   hipLaunchKernelGGL((k_tiled<T, ST, EP, TJ, TK>), dim3(tiles * chunks * R), dim3(TJ * TK), 0, s, g, E, mc, a,
                      TI, tiles_k, tiles, chunks);
 }
 
 template <typename T, int ST, int EP>
 static void tiled_shape(const Dims &g, int R, const T *E, const ModeC<T> &mc, const OpArgs<T> &a, hipStream_t s,
-                        int shape) {
+                        int shape, hipEvent_t ev0, hipEvent_t ev1) {
   const TileShape ts = tile_of(shape);
   if (ts.tj == 8 && ts.tk == 32)
-    tiled_go<T, ST, EP, 8, 32>(g, R, E, mc, a, s, shape);
+    tiled_go<T, ST, EP, 8, 32>(g, R, E, mc, a, s, shape, ev0, ev1);
   else if (ts.tj == 16 && ts.tk == 32)
-    tiled_go<T, ST, EP, 16, 32>(g, R, E, mc, a, s, shape);
+    tiled_go<T, ST, EP, 16, 32>(g, R, E, mc, a, s, shape, ev0, ev1);
   else
-    tiled_go<T, ST, EP, 16, 16>(g, R, E, mc, a, s, shape);
+    tiled_go<T, ST, EP, 16, 16>(g, R, E, mc, a, s, shape, ev0, ev1);
 }
 
 template <typename T>
 void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCoef &m,
-                  const OpArgs<T> &a, hipStream_t s, int shape) {
+                  const OpArgs<T> &a, hipStream_t s, int shape, hipEvent_t ev0, hipEvent_t ev1) {
   // the 9 distinct coefficients, checked against every entry of the table
   ModeC<T> mc;
   double dv[5] = {0, 0, 0, 0, 0}, xv[4] = {0, 0, 0, 0};
@@ -558,7 +567,7 @@ void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCo
   for (int q = 0; q < 4; q++) mc.x[q] = (T)xv[q];
 #define GO(S_, E_)                                     \
   if (st == S_ && ep == E_) {                          \
-    tiled_shape<T, S_, E_>(g, R, E, mc, a, s, shape);  \
+    tiled_shape<T, S_, E_>(g, R, E, mc, a, s, shape, ev0, ev1);  \
     return;                                            \
   }
   GO(ST_PLAIN, EP_AX)
@@ -901,7 +910,7 @@ void launch_add2(int V, int R, const T *xn, const T *d, T *x, const int *act, hi
 
 #define FEMH_OPS_INST(T)                                                                        \
   template void launch_tiled<T>(int, int, const Dims &, int, const T *, const ModeCoef &,       \
-                                const OpArgs<T> &, hipStream_t, int);                           \
+                                const OpArgs<T> &, hipStream_t, int, hipEvent_t, hipEvent_t);   \
   template void launch_stencil<T>(int, int, const Dims &, int, const T *, const OpArgs<T> &,    \
                                   hipStream_t);                                                 \
   template void launch_add2<T>(int, int, const T *, const T *, T *, const int *, hipStream_t);

@@ -247,14 +247,14 @@ void Context::op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a) {
   if (l == 0) {
     const Dims &g0 = L.g;
     const bool dense = o.apply_kernel == 1 && st == ST_PLAIN && (ep == EP_AX || ep == EP_RES);
-    auto launch = [&] {
+    auto launch = [&](hipEvent_t e0, hipEvent_t e1) {
       if (dense) {
         FineOp<T> f{g0, h.E.template as<T>(), h.Ke.template as<T>()};
         launch_dense_apply<T>(f, R, a.u, a.y, a.b, ep == EP_RES ? 1 : 0, a.act,
                               a.dot ? &a.red : nullptr, a.slot, stream);
         if (a.ydinv) launch_scale_by_dinv_norm<T>(g0.V, R, a.y, a.Dinv, a.act, red, stream);
       } else {
-        launch_tiled<T>(st, ep, g0, R, h.E.template as<T>(), modecoef, a, stream, shape_fine);
+        launch_tiled<T>(st, ep, g0, R, h.E.template as<T>(), modecoef, a, stream, shape_fine, e0, e1);
       }
     };
     if (prof) {
@@ -277,12 +277,16 @@ void Context::op(Hierarchy<T> &h, int l, int st, int ep, OpArgs<T> a) {
       const double Rn = na, V = (double)g0.V, sz = sizeof(T);
       pe.bytes = tiled_alg_bytes(st, ep, a, V, sz, Rn, ns);
       pe.flops = V * Rn * 1152.0;
-      FEMH_HIP(hipEventRecord(pe.a, stream));
-      launch();
-      FEMH_HIP(hipEventRecord(pe.b, stream));
+      if (dense) {
+        FEMH_HIP(hipEventRecord(pe.a, stream));
+        launch(nullptr, nullptr);
+        FEMH_HIP(hipEventRecord(pe.b, stream));
+      } else {
+        launch(pe.a, pe.b);  // events at the kernel's own start and end
+      }
       pending.push_back(pe);
     } else {
-      launch();
+      launch(nullptr, nullptr);
     }
   } else {
     launch_stencil<T>(st, ep, L.g, R, L.S.template as<T>(), a, stream);
