@@ -52,9 +52,10 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solves")
     ap.add_argument("--nu1", type=int, default=1)
     ap.add_argument("--nu2", type=int, default=1)
-    ap.add_argument("--coarse-nu", type=int, default=3,
-                    help="Chebyshev degree on the coarse levels (0: --nu1/--nu2)")
-    ap.add_argument("--cheby-ratio", type=float, default=14.0,
+    ap.add_argument("--coarse-nu", type=int, default=None,
+                    help="Chebyshev degree on the coarse levels (0: --nu1/--nu2; default 3 for simp64, "
+                         "2 for uneven512)")
+    ap.add_argument("--cheby-ratio", type=float, default=18.0,
                     help="Chebyshev smoothing interval [lmax/ratio, lmax]")
     ap.add_argument("--workload", default="simp64",
                     choices=["simp64", "uneven512", "opt96", "plain64"],
@@ -85,6 +86,8 @@ def parse():
     a = ap.parse_args()
     if a.inner_tol is None:
         a.inner_tol = 4e-4 if a.workload == "simp64" else 3e-4
+    if a.coarse_nu is None:
+        a.coarse_nu = 2 if a.workload == "uneven512" else 3
     return a
 
 
@@ -345,6 +348,7 @@ def run_uneven(a):
     count = a.cells or 512
     grids = W.batch_uneven(count, 3)
     kw = dict(nu=0.3, precision="mixed", tol=1e-10, ch_tol=a.ch_tol, inner_tol=a.inner_tol,
+              coarse_nu=a.coarse_nu,
               max_iter=20000, batch=a.batch, autotune=bool(a.autotune), check_every=a.check_every,
               batch_threads=a.batch_threads, graph=bool(a.graph), adapt_inner=bool(a.adapt_inner))
     # warm-up: one untimed pass over the same cells builds the resident
@@ -378,7 +382,8 @@ def run_uneven(a):
         "vs_baseline": None, "dtype": "fp32+fp64", "data": "synthetic",
         "config": {"workload": f"{count} binary cells, edge U{{16..48}}, fill U[0.2,0.8], "
                                f"E_void 1e-9, nu 0.3", "batch": a.batch, "ch_tol": a.ch_tol,
-                   "inner_tol": a.inner_tol, "adapt_inner": a.adapt_inner, "voxels": vox,
+                   "inner_tol": a.inner_tol, "coarse_nu": a.coarse_nu, "adapt_inner": a.adapt_inner,
+                   "voxels": vox,
                    "autotune": a.autotune, "batch_threads": a.batch_threads or 8, "graph": a.graph},
         "ch_rel_err_vs_fp64_max": max(errs) if errs else None,
         "ch_err_ok": (max(errs) <= 1e-6) if errs else None,

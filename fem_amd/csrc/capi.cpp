@@ -138,7 +138,7 @@ void femh_default_opts(femh_opts *o) {
   o->nu1 = 2;
   o->nu2 = 2;
   o->cheby_ratio = 20.0;
-  o->coarse_iters = 64;
+  o->coarse_iters = 32;
   o->tol = 1e-10;
   o->max_iter = 5000;
   o->precision = FEMH_FP64;

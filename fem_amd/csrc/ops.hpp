@@ -69,12 +69,13 @@ struct OpArgs {
 // fine level (tiled cube-mode voxel kernel).  shape: launch shape, bits 0-7
 // node planes per workgroup and bits 8-15 the tile variant (TJ x TK owned
 // node columns per workgroup: 1 = 16 x 16, 2 = 8 x 32, 3 = 16 x 32); 0 in a
-// field is the built-in choice (16 x 16, planes from the grid).
+// field is the built-in choice (tiled_default_tile, planes from the grid).
 // tiled_parts: partial count of a dot launch with that shape; tiled_slabs:
 // the planes per workgroup actually used.  ev0/ev1 (optional): events the
 // runtime records at the kernel's own start and end (kernel timing).
 constexpr int kNumTileShapes = 4;
 inline int tiled_shape_code(int tile, int planes) { return (tile << 8) | planes; }
+int tiled_default_tile(const Dims &g, int R);  // the tile variant of the built-in shape
 template <typename T>
 void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCoef &mc,
                   const OpArgs<T> &a, hipStream_t s, int shape = 0, hipEvent_t ev0 = nullptr,

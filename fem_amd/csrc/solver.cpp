@@ -579,7 +579,7 @@ void Context::autotune(Hierarchy<T> &h) {
     // tile variants x node planes per workgroup (a shape equivalent to an
     // earlier candidate -- same tile, same planes -- is not timed twice)
     std::vector<int> cand{0};
-    std::vector<std::pair<int, int>> seen{{1, tiled_slabs(g, R, 0)}};
+    std::vector<std::pair<int, int>> seen{{tiled_default_tile(g, R), tiled_slabs(g, R, 0)}};
     for (int tile = 1; tile < kNumTileShapes; tile++)
       for (int c : {0, 2, 4, 8, 16, 32}) {
         const int code = tiled_shape_code(tile, c);

@@ -122,7 +122,7 @@ def _ptr(a):
 
 
 def make_opts(shape, nu=0.3, E0=1.0, Emin=0.0, penal=3.0, interp="simp", precond="mg",
-              levels=0, nu1=2, nu2=2, cheby_ratio=20.0, coarse_iters=64, tol=1e-10,
+              levels=0, nu1=2, nu2=2, cheby_ratio=20.0, coarse_iters=32, tol=1e-10,
               max_iter=5000, precision="fp64", ch_tol=0.0, check_every=1, device=0,
               inner_tol=1e-4, max_outer=12, apply_kernel="tiled", batch=1, warm_start=False,
               coarse_nu=3, structure="general", autotune=False, streams=1, batch_threads=0,

@@ -93,7 +93,7 @@ typedef struct femh_opts {
   int nu1, nu2;         /* Chebyshev smoothing degree before / after coarse correction */
   double cheby_ratio;   /* smoothing interval [lmax/ratio, lmax] of D^-1 A */
   int coarse_iters;     /* Chebyshev steps of the coarsest solve when that level is too
-                           large for its dense inverse (> 648 dofs) */
+                           large for its dense inverse (> 648 dofs); default 32 */
   double tol;           /* ||r||/||b|| target (FP64) / outer refinement target (mixed) */
   int max_iter;         /* outer iteration cap per load case */
   int precision;        /* femh_precision */
