@@ -37,15 +37,26 @@ class BatchedDescent:
         self.wG = np.ones((self.n, 6, 6)) if wG is None else np.broadcast_to(wG, (self.n, 6, 6))
         self.m0, self.mw, self.lb, self.ub = m0, mw, lb, ub
         self.batch = batch
-        # one context per group of `batch` cells keeps each group's fields
-        # resident for the warm start of its next solve
         self.groups = [list(range(g, min(g + batch, self.n))) for g in range(0, self.n, batch)]
-        self.ctx = [Homogenizer(self.shape, nu=nu, batch=len(g), interp=interp, penal=penal,
-                                warm_start=True, **solver) for g in self.groups]
+        self.ctx = self._open(nu, interp, penal, solver)
         self.solves = 0
         self.est = []
         self.last_C = None  # C_H at the start point of the last step (its forward solve)
         self._pool = ThreadPoolExecutor(max_workers=min(len(self.groups), 4))
+
+    def _open(self, nu, interp, penal, solver):
+        # one context per group of `batch` cells keeps each group's fields
+        # resident for the warm start of its next solve
+        return [Homogenizer(self.shape, nu=nu, batch=len(g), interp=interp, penal=penal,
+                            warm_start=True, **solver) for g in self.groups]
+
+    def _group_solve(self, gi, xg):
+        """C_H [nb,6,6] and per-cell stats of group gi at densities xg [nb,V]."""
+        return self.ctx[gi].homogenize_many(xg.reshape(len(xg), *self.shape))
+
+    def _group_gradient(self, gi, dJdC):
+        """sum_ij dJdC_ij dC_ij/drho [nb,V] at the group's last solve."""
+        return self.ctx[gi].gradient(dJdC, nb=len(dJdC)).reshape(len(dJdC), -1)
 
     def evaluate(self, x, cells=None, grad=False):
         """C (and dJ/drho when grad) of the given cells at densities x[cells]."""
@@ -58,13 +69,12 @@ class BatchedDescent:
             # the warm start follows each cell); unselected cells ride along
             # at their current point
             grp = self.groups[gi]
-            Cg, st = self.ctx[gi].homogenize_many(x[grp].reshape(len(grp), *self.shape))
+            Cg, st = self._group_solve(gi, x[grp])
             g = None
             if grad:
-                nb = len(grp)
                 rho_mean = x[grp].mean(axis=1)
                 dJdC = self.wG[grp] * (Cg - self.C0[grp])
-                g = self.ctx[gi].gradient(dJdC, nb=nb).reshape(nb, -1)
+                g = self._group_gradient(gi, dJdC)
                 g += (self.mw * (rho_mean - self.m0) / x.shape[1])[:, None]
             return Cg, st, g
 

@@ -38,3 +38,24 @@ def test_batched_descent_groups_and_steps():
     x1, J1, acc, _ = bd.step(x, max_step=0.2)
     assert acc.all() and (J1 < J0).all()
     bd.close()
+
+
+def test_engine_driven_loop_matches_the_oracle_replay():
+    """The batched loop driven by the engine (FP64, warm-started batch of 3)
+    follows the oracle-driven one, which tests/test_optimize_replay.py pins to
+    a literal restatement of fast_fem/run3d.cpp:238-289: same accepted steps,
+    same points to solver tolerance."""
+    from test_optimize_replay import EMIN, OracleDescent, problem
+    shape, x0, C0 = problem()
+    kw = dict(m0=0.4, mw=0.5, batch=len(x0), Emin=EMIN)
+    gd = BatchedDescent(shape, C0, tol=1e-12, **kw)
+    od = OracleDescent(shape, C0, **kw)
+    xg, xo = x0.copy(), x0.copy()
+    for _ in range(4):
+        xg, Jg, ag, _ = gd.step(xg, 1.0)
+        xo, Jo, ao, _ = od.step(xo, 1.0)
+        assert np.array_equal(ag, ao)
+        assert np.abs(xg - xo).max() <= 1e-9
+        assert np.abs(Jg - Jo).max() <= 1e-9 * np.abs(Jo).max()
+    gd.close()
+    od.close()
