@@ -667,28 +667,88 @@ __device__ __forceinline__ void stencil_node(const Dims &g, const T *__restrict_
   }
 }
 
-// NR = 1: one right-hand side per thread (blockIdx.y = RHS), for small levels
-// where parallelism matters more than reuse; NR = 6: all load cases of a
-// sample per thread (blockIdx.y = sample), reading the stencil once.
-template <typename T, int ST, int EP, int NR>
+// All load cases of a sample per thread (blockIdx.y = sample), reading the
+// stencil once; the small levels run k_stencil_split below.
+template <typename T, int ST, int EP>
 __global__ __launch_bounds__(kBlock) void k_stencil(Dims g, const T *__restrict__ S, OpArgs<T> a,
                                                     int sym) {
-  int smp, lc0;
+  const int smp = blockIdx.y;
   unsigned mask = 0;
-  if (NR == 1) {
-    const int r = blockIdx.y;
-    smp = r / kRHS;
-    lc0 = r % kRHS;
-    mask = a.act[r] ? 1u : 0u;
-  } else {
-    smp = blockIdx.y;
-    lc0 = 0;
 #pragma unroll
-    for (int t = 0; t < NR; t++) mask |= a.act[smp * kRHS + t] ? (1u << t) : 0u;
-  }
+  for (int t = 0; t < kRHS; t++) mask |= a.act[smp * kRHS + t] ? (1u << t) : 0u;
   if (!mask) return;
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock)
-    stencil_node<T, ST, EP, NR>(g, S, a, n, smp, lc0, mask, sym != 0);
+    stencil_node<T, ST, EP, kRHS>(g, S, a, n, smp, 0, mask, sym != 0);
+}
+
+// Small levels (a few thousand nodes: too few threads to cover the latency of
+// a node's 27 dependent block loads): one right-hand side per workgroup row
+// (blockIdx.y) and 64 consecutive nodes per workgroup, whose stencil rows are
+// split over nine waves -- wave w = 3 di + dj applies the three blocks
+// (di, dj, 0..2) -- so a thread's load chain is three blocks long.  The nine
+// partial sums meet in LDS and wave 0 finishes the node.  Same block order
+// and symmetric reads as stencil_node; only the summation order differs.
+constexpr int kSplitNodes = 64, kSplitWaves = 9;
+
+template <typename T, int ST, int EP>
+__global__ __launch_bounds__(kSplitNodes *kSplitWaves) void k_stencil_split(Dims g, const T *__restrict__ S,
+                                                                             OpArgs<T> a, int sym) {
+  const int r = blockIdx.y;
+  if (!a.act[r]) return;
+  const int smp = r / kRHS, lane = threadIdx.x % kSplitNodes, w = threadIdx.x / kSplitNodes;
+  const int n = blockIdx.x * kSplitNodes + lane;
+  const bool live = n < g.V;
+  const int V = g.V;
+  const T *__restrict__ Ss = S + (size_t)smp * 243 * V;
+  const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * V : nullptr;
+  const T ti = a.cheb ? (T)a.cheb[smp] : (T)0;
+  const size_t off = (size_t)r * 3 * V;
+  __shared__ T part[kSplitWaves][3][kSplitNodes];
+  T acc[3] = {0, 0, 0};
+  if (live) {
+    int i, j, k;
+    node_ijk(g, n, i, j, k);
+    const int di = w / 3, dj = w % 3;
+    const int row = (wrap1(i - 1 + di, g.nx) * g.ny + wrap1(j - 1 + dj, g.ny)) * g.nz;
+    const int K[3] = {wrap1(k - 1, g.nz), k, wrap1(k + 1, g.nz)};
+#pragma unroll
+    for (int dk = 0; dk < 3; dk++) {
+      const int o = 3 * w + dk, nb = row + K[dk];
+      T sv[9];
+      if (sym && o < 13) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+          for (int d = 0; d < 3; d++) sv[3 * c + d] = Ss[(size_t)((26 - o) * 9 + 3 * d + c) * V + nb];
+      } else {
+#pragma unroll
+        for (int q = 0; q < 9; q++) sv[q] = Ss[(size_t)(o * 9 + q) * V + n];
+      }
+      const T u0 = stage_at<T, ST>(a, Ds, ti, off, V, 0, nb), u1 = stage_at<T, ST>(a, Ds, ti, off, V, 1, nb),
+              u2 = stage_at<T, ST>(a, Ds, ti, off, V, 2, nb);
+      acc[0] += sv[0] * u0 + sv[1] * u1 + sv[2] * u2;
+      acc[1] += sv[3] * u0 + sv[4] * u1 + sv[5] * u2;
+      acc[2] += sv[6] * u0 + sv[7] * u1 + sv[8] * u2;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) part[w][c][lane] = acc[c];
+  __syncthreads();
+  if (w != 0 || !live) return;
+  T Au[3], uo[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T s = part[0][c][lane];
+#pragma unroll
+    for (int q = 1; q < kSplitWaves; q++) s += part[q][c][lane];
+    Au[c] = s;
+    uo[c] = stage_at<T, ST>(a, Ds, ti, off, V, c, n);
+  }
+  const T c1 = a.cheb ? (T)a.cheb[kMaxB + smp] : (T)0, c2 = a.cheb ? (T)a.cheb[2 * kMaxB + smp] : (T)0;
+  DotAcc dv;
+  EpiIn<T> pin;
+  epi_load<T, EP>(a, Ds, off, V, n, pin);
+  epilogue<T, EP>(a, pin, ti, c1, c2, off, V, n, Au, uo, dv);
 }
 
 // One-plane form of the tiled stencil kernel, for levels with planes of at
@@ -878,6 +938,7 @@ void launch_stencil(int st, int ep, const Dims &g, int R, const T *S, const OpAr
                      (size_t)243 * g.V * sizeof(T) < ((size_t)1 << 31);
   // all load cases per thread once that still leaves >= 2 workgroups per CU
   const bool multi = (long)blocks * nsmp >= 512 && R % kRHS == 0;
+  const int split_blocks = (g.V + kSplitNodes - 1) / kSplitNodes;
 #define GO(S_, E_)                                                                               \
   if (st == S_ && ep == E_) {                                                                    \
     if (plane && sym)                                                                            \
@@ -887,11 +948,11 @@ void launch_stencil(int st, int ep, const Dims &g, int R, const T *S, const OpAr
       hipLaunchKernelGGL((k_stencil_plane<T, S_, E_, NG, 0>), dim3((unsigned)pl_blocks), dim3(kBlock), \
                          0, s, g, S, a, tiles_k, tiles_j * tiles_k);                             \
     else if (multi)                                                                              \
-      hipLaunchKernelGGL((k_stencil<T, S_, E_, kRHS>), dim3(blocks, nsmp), dim3(kBlock), 0, s, g, S, \
+      hipLaunchKernelGGL((k_stencil<T, S_, E_>), dim3(blocks, nsmp), dim3(kBlock), 0, s, g, S,       \
                          a, sym);                                                                \
     else                                                                                         \
-      hipLaunchKernelGGL((k_stencil<T, S_, E_, 1>), dim3(blocks, R), dim3(kBlock), 0, s, g, S, a,  \
-                         sym);                                                                   \
+      hipLaunchKernelGGL((k_stencil_split<T, S_, E_>), dim3(split_blocks, R),                       \
+                         dim3(kSplitNodes * kSplitWaves), 0, s, g, S, a, sym);                   \
     return;                                                                                      \
   }
   GO(ST_PLAIN, EP_AX)
