@@ -71,8 +71,19 @@ struct DotAcc {
   double a = 0, b = 0, c = 0;
 };
 
+// output store of an epilogue; NT: a streaming (non-temporal) store, for the
+// fine level, whose output vectors are far larger than the caches and are
+// read back only by a later launch
+template <bool NT, typename T>
+__device__ __forceinline__ void ep_store(T *p, T v) {
+  if (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 // epilogue at one owned node (all three components)
-template <typename T, int EP>
+template <typename T, int EP, bool NT = false>
 __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, T ti, T c1, T c2,
                                          size_t off, int V, int n, const T Au[3], const T uo[3],
                                          DotAcc &dv) {
@@ -80,12 +91,12 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, 
   for (int c = 0; c < 3; c++) {
     const size_t q = off + (size_t)c * V + n;
     if (EP == EP_AX) {
-      a.y[q] = a.ydinv ? Au[c] * p.e1[c] : Au[c];
-      if (a.pout) a.pout[q] = uo[c];
+      ep_store<NT>(&a.y[q], a.ydinv ? Au[c] * p.e1[c] : Au[c]);
+      if (a.pout) ep_store<NT>(&a.pout[q], uo[c]);
       if (a.dot) dv.a += (double)uo[c] * (double)Au[c];
     } else if (EP == EP_RES) {
       const T rv = p.e0[c] - Au[c];
-      a.y[q] = rv;
+      ep_store<NT>(&a.y[q], rv);
       if (a.dot) {
         dv.a += (double)rv * (double)rv;
         dv.b += (double)p.e0[c] * (double)uo[c];
@@ -95,12 +106,12 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, 
       const T rv = p.e0[c] - Au[c];
       if (a.fin1) {
         const T xv = uo[c] + rv * ti * p.e1[c];
-        a.x[q] = xv;
+        ep_store<NT>(&a.x[q], xv);
         if (a.dot) dv.a += (double)p.e0[c] * (double)xv;
       } else {
-        a.r[q] = rv;
-        if (!a.last) a.dout[q] = rv * ti * p.e1[c];  // `last`: no smoothing step follows
-        a.x[q] = uo[c];
+        ep_store<NT>(&a.r[q], rv);
+        if (!a.last) ep_store<NT>(&a.dout[q], rv * ti * p.e1[c]);  // `last`: no smoothing step follows
+        ep_store<NT>(&a.x[q], uo[c]);
       }
     } else {  // EP_STEP
       const T rv = p.e0[c] - Au[c];
@@ -109,10 +120,10 @@ __device__ __forceinline__ void epilogue(const OpArgs<T> &a, const EpiIn<T> &p, 
       if (a.last) {
         xv += dn;  // the last step's r and d are not read again
       } else {
-        a.r[q] = rv;
-        a.dout[q] = dn;
+        ep_store<NT>(&a.r[q], rv);
+        ep_store<NT>(&a.dout[q], dn);
       }
-      a.x[q] = xv;
+      ep_store<NT>(&a.x[q], xv);
       if (a.dot) dv.a += (double)p.e3[c] * (double)xv;
     }
   }
@@ -508,7 +519,7 @@ __global__ __launch_bounds__(TJ * TK, sizeof(T) == 4 ? 4 : kWaves64) void k_tile
 #pragma unroll
           for (int c = 0; c < 3; c++) Au[c] += src[c];
         }
-      epilogue<T, EP>(a, pin, ti, c1, c2, off, V, (int)nown, Au, uo, dv);
+      epilogue<T, EP, true>(a, pin, ti, c1, c2, off, V, (int)nown, Au, uo, dv);
     }
   }
   if ((EP == EP_AX || EP == EP_STEP || EP == EP_RES || EP == EP_CHEB0) && a.dot) {
