@@ -998,15 +998,16 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
       launch_p_update<T>(V, R, pv, zv, rzn, rzc, 0, ap, red, np_rz, stream, lz_beta, cnt);
   };
   std::vector<int> hact(kMaxR), hst(kMaxR);
-  // host look at the device state: false when no right-hand side is left
+  // host look at the device state: false when no right-hand side is left.
+  // Only k_pcg_check clears a right-hand side's activity, and it sets the
+  // state (1 or 2) when it does, so the state alone tells which are left.
   auto poll = [&]() {
-    FEMH_HIP(hipMemcpyAsync(hact.data(), apw, sizeof(int) * R, hipMemcpyDeviceToHost, stream));
     FEMH_HIP(hipMemcpyAsync(hst.data(), pst, sizeof(int) * R, hipMemcpyDeviceToHost, stream));
     FEMH_HIP(hipStreamSynchronize(stream));
     bool any = false;
     for (int t = 0; t < R; t++) {
       if (hst[t] == 2) throw Error(FEMH_NONFINITE, "residual became non-finite");
-      any = any || hact[t] != 0;
+      any = any || (active[t] != 0 && hst[t] == 0);
     }
     return any;
   };
