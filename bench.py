@@ -398,7 +398,7 @@ def run_opt(a):
     import numpy as np
     from fem_amd import Homogenizer
     from fem_amd import workloads as W
-    from fem_amd.optimize import BatchedDescent
+    from fem_amd.optimize import DeviceDescent
     from oracle import oracle as O
     n, count, steps = 96, a.cells or 64, 30
     x = np.stack([W.simp_density(n, 100 + s, 6.0).reshape(-1) for s in range(count)])
@@ -410,19 +410,21 @@ def run_opt(a):
         C0[s, range(3), range(3)] += 2 * mu
         C0[s, 3, 3] = C0[s, 4, 4] = C0[s, 5, 5] = mu
     # SIMP of MaterialOpt/src/FEM3DFun.cpp:244-257: E = 1e-3 + (1 - 1e-3) rho^3
-    opt = BatchedDescent((n, n, n), C0, m0=0.0, mw=0.0, batch=a.batch, precision="mixed",
-                         tol=1e-10, ch_tol=a.ch_tol, inner_tol=a.inner_tol, Emin=1e-3)
-    opt.evaluate(x)  # warm-up (hierarchies, first fields)
+    # densities, gradients and trial points stay in device memory
+    opt = DeviceDescent((n, n, n), C0, m0=0.0, mw=0.0, batch=a.batch, precision="mixed",
+                        tol=1e-10, ch_tol=a.ch_tol, inner_tol=a.inner_tol, Emin=1e-3)
+    xd = opt.to_device(x)
+    opt.evaluate(xd)  # warm-up (hierarchies, first fields)
     t0 = time.perf_counter()
-    xf, hist = opt.run(x, steps, log=lambda h: print(f"step {h['step']}: accepted "
-                                                      f"{h['accepted']}, {h['seconds']:.2f} s",
-                                                      file=sys.stderr, flush=True))
+    xf, hist = opt.run(xd, steps, log=lambda h: print(f"step {h['step']}: accepted "
+                                                       f"{h['accepted']}, {h['seconds']:.2f} s",
+                                                       file=sys.stderr, flush=True))
     elapsed = time.perf_counter() - t0
     Cm, _ = opt.evaluate(xf)
     errs = []
     if not a.no_fp64:
         h = Homogenizer((n, n, n), nu=0.3, tol=1e-10, batch=8, Emin=1e-3)
-        C64, _ = h.homogenize_many(xf[:8].reshape(8, n, n, n))
+        C64, _ = h.homogenize_many(xf[:8].cpu().numpy().reshape(8, n, n, n))
         errs = [float(np.linalg.norm(Cm[s] - C64[s]) / np.linalg.norm(C64[s])) for s in range(8)]
     J0 = hist[0]["J0"]
     Jn = hist[-1]["J"]

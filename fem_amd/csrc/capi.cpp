@@ -228,6 +228,26 @@ int femh_gradient(femh_ctx *ctx, const double *dJdC, double *dJdrho) {
   return guarded(ctx, [&] { ctx->c->gradient(dJdC, dJdrho, false); });
 }
 
+int femh_gradient_device(femh_ctx *ctx, const double *dJdC, const double *shift, double *d_dJdrho,
+                         double *gmax, void *stream) {
+  if (int s = need_ctx(ctx)) return s;
+  if (!dJdC || !d_dJdrho) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] {
+    FEMH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    ctx->c->gradient_device(dJdC, shift, d_dJdrho, gmax);
+  });
+}
+
+int femh_descent_trial(femh_ctx *ctx, int n, const double *d_x, const double *d_g, const double *h,
+                       double lb, double ub, double *d_out, double *sums, void *stream) {
+  if (int s = need_ctx(ctx)) return s;
+  if (n < 1 || n > 65535 || !d_x || !d_g || !h || !d_out || !(lb <= ub)) return FEMH_BAD_ARG;
+  return guarded(ctx, [&] {
+    FEMH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    ctx->c->descent_trial(n, d_x, d_g, h, lb, ub, d_out, sums);
+  });
+}
+
 int femh_energy_terms(femh_ctx *ctx, double *W) {
   if (int s = need_ctx(ctx)) return s;
   if (!W) return FEMH_BAD_ARG;

@@ -857,6 +857,74 @@ __global__ __launch_bounds__(kBlock, 2) void k_gradient(Dims g, const double *__
   }
 }
 
+// ------------------------------------------------------------------------
+// projected-gradient line search with the densities on the device
+// (fast_fem/run3d.cpp:238-265: x' = clamp(x - h g, lb, ub) per cell)
+
+// per cell s (blockIdx.y): h[s] > 0: out = clamp(x - h[s] g, lb, ub); h[s] < 0:
+// out = x; h[s] == 0: out unchanged.  partials[s][b] = block b's sum of the
+// resulting out (the cell's density sum).  The product and the difference
+// are rounded separately, as the host loop computes them, so both give the
+// same trial point: contraction into a fused multiply-add is switched off
+// for the kernel (the _rn intrinsics alone do not prevent it).
+__global__ __launch_bounds__(kBlock) void k_descent_trial(long long V, const double *__restrict__ x,
+                                                          const double *__restrict__ g,
+                                                          const double *__restrict__ h, double lb, double ub,
+                                                          double *__restrict__ out, double *__restrict__ partials) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.y;
+  const double hs = h[s];
+  const size_t o = (size_t)s * (size_t)V;
+  double acc = 0;
+  for (long long e = blockIdx.x * (long long)kBlock + threadIdx.x; e < V; e += (long long)gridDim.x * kBlock) {
+    double v;
+    if (hs > 0) {
+      v = x[o + e] - hs * g[o + e];
+      v = v < lb ? lb : (v > ub ? ub : v);
+      out[o + e] = v;
+    } else if (hs < 0) {
+      v = x[o + e];
+      out[o + e] = v;
+    } else {
+      v = out[o + e];
+    }
+    acc += v;
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) partials[(size_t)s * gridDim.x + blockIdx.x] = acc;
+}
+
+// max that keeps a NaN (as a host max over an array with a NaN does)
+__device__ __forceinline__ double max_nan(double a, double b) { return a != a ? a : (b != b ? b : fmax(a, b)); }
+
+// g[s] += shift[s] (when shift is given) and partials[s][b] = block b's
+// max |g[s]|
+__global__ __launch_bounds__(kBlock) void k_shift_absmax(long long V, double *__restrict__ g,
+                                                         const double *__restrict__ shift,
+                                                         double *__restrict__ partials) {
+  const int s = blockIdx.y;
+  const size_t o = (size_t)s * (size_t)V;
+  const double sh = shift ? shift[s] : 0.0;
+  double m = 0;
+  for (long long e = blockIdx.x * (long long)kBlock + threadIdx.x; e < V; e += (long long)gridDim.x * kBlock) {
+    double v = g[o + e];
+    if (shift) {
+      v += sh;
+      g[o + e] = v;
+    }
+    m = max_nan(m, fabs(v));
+  }
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) m = max_nan(m, __shfl_xor(m, w, 64));
+  __shared__ double sm[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; w++) m = max_nan(m, sm[w]);
+    partials[(size_t)s * gridDim.x + blockIdx.x] = m;
+  }
+}
+
 // scalars[slot][idx] = sum of the np partials, for idx < nidx (host reads)
 __global__ __launch_bounds__(kBlock) void k_finalize(Reduce red, int slot, int np) {
   const double s = sum_parts(red, slot, blockIdx.x, np);
@@ -1059,6 +1127,14 @@ void launch_gradient(Dims g, int B, const double *rho, const ModeCoef &m, const 
                      double p, int energy_only, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_gradient, grid2(g.V, B), dim3(kBlock), 0, s, g, rho, mode_table(m), uaff, u,
                      Gs, interp, E0, Emin, p, energy_only, out);
+}
+void launch_descent_trial(int n, long long V, const double *x, const double *g, const double *h, double lb,
+                          double ub, double *out, double *partials, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL(k_descent_trial, dim3(nblk, n), dim3(kBlock), 0, s, V, x, g, h, lb, ub, out, partials);
+}
+void launch_shift_absmax(int n, long long V, double *g, const double *shift, double *partials, int nblk,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_shift_absmax, dim3(nblk, n), dim3(kBlock), 0, s, V, g, shift, partials);
 }
 int vec_parts(int V) { return grid_for(3 * V); }
 

@@ -200,6 +200,16 @@ void launch_gradient(Dims g, int B, const double *rho, const ModeCoef &m, const 
                      const double *u, const double *Gs /*device [B][21]*/, int interp, double E0,
                      double Emin, double p, int energy_only, double *out, hipStream_t s);
 
+// Projected-gradient trial point of n cells of V voxels: per cell, h > 0:
+// out = clamp(x - h g, lb, ub); h < 0: out = x; h == 0: out unchanged
+// (h: device [n]); partials [n][nblk] receive the block sums of out.
+void launch_descent_trial(int n, long long V, const double *x, const double *g, const double *h, double lb,
+                          double ub, double *out, double *partials, int nblk, hipStream_t s);
+// g[s] += shift[s] (shift: device [n] or null); partials [n][nblk] receive the
+// block maxima of |g[s]|
+void launch_shift_absmax(int n, long long V, double *g, const double *shift, double *partials, int nblk,
+                         hipStream_t s);
+
 void reduce_init(Reduce &red, hipStream_t s);
 
 }  // namespace femh

@@ -1581,27 +1581,80 @@ void Context::homogenize(double *C, femh_stats *st) {
   if (status != FEMH_OK) throw Error(status, "PCG did not reach the tolerance within max_iter");
 }
 
-void Context::gradient(const double *dJdC, double *out_host, bool energy_only) {
-  if (!have_fields) throw Error(FEMH_NO_FIELDS, "no solved fields: call femh_homogenize first");
-  std::vector<double> Gsym(21 * nb);
-  for (int s = 0; s < nb; s++) {
+// dJ/drho (or, energy_only, the 21 energy terms per voxel) of the last call's
+// cells into device memory, queued on the context's stream
+static void gradient_launch(Context &c, const double *dJdC, double *d_out, bool energy_only) {
+  if (!c.have_fields) throw Error(FEMH_NO_FIELDS, "no solved fields: call femh_homogenize first");
+  std::vector<double> Gsym(21 * c.nb);
+  for (int s = 0; s < c.nb; s++) {
     const double *G = dJdC + 36 * s;
     int p = 0;
     for (int a = 0; a < 6; a++)
       for (int bb = a; bb < 6; bb++)
         Gsym[21 * s + p++] = energy_only ? 0.0 : (a == bb ? G[6 * a + a] : G[6 * a + bb] + G[6 * bb + a]);
   }
-  FEMH_HIP(hipMemcpyAsync(Gs.p, Gsym.data(), sizeof(double) * Gsym.size(), hipMemcpyHostToDevice, stream));
+  // (pageable source: the copy is complete when the call returns)
+  FEMH_HIP(hipMemcpyAsync(c.Gs.p, Gsym.data(), sizeof(double) * Gsym.size(), hipMemcpyHostToDevice, c.stream));
+  launch_gradient(c.g, c.nb, c.rho.as<double>(), c.modecoef, c.uaff.as<double>(), c.x.as<double>(),
+                  c.Gs.as<double>(), c.modulus_input ? FEMH_INTERP_MODULUS : c.o.interp, c.o.E0, c.o.Emin,
+                  c.o.penal, energy_only ? 1 : 0, d_out, c.stream);
+  FEMH_HIP(hipGetLastError());
+}
+
+void Context::gradient(const double *dJdC, double *out_host, bool energy_only) {
   const size_t n = (energy_only ? (size_t)21 : 1) * g.V * nb;
   DBuf out;
   out.alloc(sizeof(double) * n);
-  launch_gradient(g, nb, rho.as<double>(), modecoef, uaff.as<double>(), x.as<double>(),
-                  Gs.as<double>(), modulus_input ? FEMH_INTERP_MODULUS : o.interp, o.E0, o.Emin,
-                  o.penal, energy_only ? 1 : 0,
-                  out.as<double>(), stream);
-  FEMH_HIP(hipGetLastError());
+  gradient_launch(*this, dJdC, out.as<double>(), energy_only);
   FEMH_HIP(hipMemcpyAsync(out_host, out.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
   FEMH_HIP(hipStreamSynchronize(stream));
+}
+
+void Context::gradient_device(const double *dJdC, const double *shift, double *d_out, double *gmax) {
+  gradient_launch(*this, dJdC, d_out, false);
+  if (!shift && !gmax) {
+    FEMH_HIP(hipStreamSynchronize(stream));
+    return;
+  }
+  const int nblk = kRedBlocks;
+  ls_part.alloc(sizeof(double) * (size_t)nb * nblk);
+  if (shift) {
+    ls_h.alloc(sizeof(double) * (size_t)nb);
+    FEMH_HIP(hipMemcpyAsync(ls_h.p, shift, sizeof(double) * nb, hipMemcpyHostToDevice, stream));
+  }
+  launch_shift_absmax(nb, g.V, d_out, shift ? ls_h.as<double>() : nullptr, ls_part.as<double>(), nblk, stream);
+  FEMH_HIP(hipGetLastError());
+  std::vector<double> part((size_t)nb * nblk);
+  FEMH_HIP(hipMemcpyAsync(part.data(), ls_part.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+  if (gmax)
+    for (int s = 0; s < nb; s++) {
+      double m = 0;
+      for (int b = 0; b < nblk; b++) {
+        const double v = part[(size_t)s * nblk + b];
+        m = (m != m) ? m : (v != v ? v : std::max(m, v));
+      }
+      gmax[s] = m;
+    }
+}
+
+void Context::descent_trial(int n, const double *xd, const double *gd, const double *h, double lb, double ub,
+                            double *out, double *sums) {
+  const int nblk = kRedBlocks;
+  ls_part.alloc(sizeof(double) * (size_t)n * nblk);
+  ls_h.alloc(sizeof(double) * (size_t)n);
+  FEMH_HIP(hipMemcpyAsync(ls_h.p, h, sizeof(double) * n, hipMemcpyHostToDevice, stream));
+  launch_descent_trial(n, g.V, xd, gd, ls_h.as<double>(), lb, ub, out, ls_part.as<double>(), nblk, stream);
+  FEMH_HIP(hipGetLastError());
+  std::vector<double> part((size_t)n * nblk);
+  FEMH_HIP(hipMemcpyAsync(part.data(), ls_part.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, stream));
+  FEMH_HIP(hipStreamSynchronize(stream));
+  if (sums)
+    for (int s = 0; s < n; s++) {
+      double t = 0;
+      for (int b = 0; b < nblk; b++) t += part[(size_t)s * nblk + b];
+      sums[s] = t;
+    }
 }
 
 void Context::thermal(const double *alpha, double *beta, double *alpha_eff) {

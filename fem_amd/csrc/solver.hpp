@@ -155,6 +155,7 @@ struct Context {
   // PCG iteration state on the device: iterations, state (int [kMaxR] each),
   // relative residual and |b|^2 (double [kMaxR] each)
   DBuf pcg_state;
+  DBuf ls_part, ls_h;  // device line search: per-cell block partials [n][kRedBlocks], steps [n]
   // recorded PCG iteration pairs (hipGraph), keyed by everything their
   // launches bake in: precision, right-hand sides, vector and hierarchy
   // buffers, launch shapes, tolerance
@@ -219,6 +220,15 @@ struct Context {
   bool h64_ops = false;
   void homogenize(double *C, femh_stats *st);  // C[36 nb], st[nb] (may be null)
   void gradient(const double *dJdC, double *out_host, bool energy_only);
+  // dJ/drho of the last call's nb cells into device memory d_out[nb V], plus
+  // shift[s] (host, may be null) per cell; gmax[s] (host, may be null)
+  // receives max |d_out[s]|
+  void gradient_device(const double *dJdC, const double *shift, double *d_out, double *gmax);
+  // projected-gradient trial point of n cells of this grid (device x, g,
+  // out [n V]; host h[n]): h > 0: out = clamp(x - h g, lb, ub), h < 0:
+  // out = x, h == 0: out unchanged; sums[s] (host, may be null) = sum of out[s]
+  void descent_trial(int n, const double *x, const double *g, const double *h, double lb, double ub,
+                     double *out, double *sums);
   void get_fields(double *u);
   // effective thermal stress beta[6 nb] and expansion C_H^-1 beta (either
   // may be null) for per-voxel expansion coefficients alpha[nb V] (host)

@@ -26,7 +26,8 @@ PRECISION = {"fp64": 0, "mixed": 1}
 EXPORTS = ("femh_abi_version", "femh_default_opts", "femh_create", "femh_destroy",
            "femh_homogenize", "femh_homogenize_modulus", "femh_homogenize_device",
            "femh_homogenize_many", "femh_homogenize_many_modulus", "femh_homogenize_many_device",
-           "femh_gradient", "femh_energy_terms", "femh_get_fields", "femh_thermal_expansion",
+           "femh_gradient", "femh_gradient_device", "femh_descent_trial",
+           "femh_energy_terms", "femh_get_fields", "femh_thermal_expansion",
            "femh_coarse_strain_tensor",
            "femh_launch_shapes", "femh_set_launch_shapes", "femh_homogenize_batch",
            "femh_prepare", "femh_apply_level", "femh_level_dims", "femh_set_profiling",
@@ -97,6 +98,9 @@ def lib():
             getattr(L, name).argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.femh_homogenize_many_device.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.femh_gradient.argtypes = [vp, vp, vp]
+        L.femh_gradient_device.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.femh_descent_trial.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.c_double, ctypes.c_double,
+                                         vp, vp, vp]
         L.femh_energy_terms.argtypes = [vp, vp]
         L.femh_get_fields.argtypes = [vp, vp]
         L.femh_thermal_expansion.argtypes = [vp, vp, vp, vp]
@@ -244,6 +248,29 @@ class Homogenizer:
         out = np.zeros(self.V * nb)
         self._check(lib().femh_gradient(self._h, _ptr(G), _ptr(out)))
         return out.reshape(self.shape) if nb == 1 else out.reshape((nb,) + self.shape)
+
+    def gradient_device(self, dJdC, d_out_ptr, nb=1, shift=None, stream_ptr=None):
+        """dJ/drho of the last batch of nb samples into device memory at
+        d_out_ptr (nb*V doubles), plus shift[s] per sample when given.
+        Returns max |dJ/drho| per sample."""
+        G = np.ascontiguousarray(dJdC, dtype=np.float64).reshape(36 * nb)
+        sh = None if shift is None else np.ascontiguousarray(shift, dtype=np.float64).reshape(nb)
+        gmax = np.zeros(nb)
+        self._check(lib().femh_gradient_device(self._h, _ptr(G), None if sh is None else _ptr(sh),
+                                               ctypes.c_void_p(d_out_ptr), _ptr(gmax),
+                                               ctypes.c_void_p(stream_ptr or 0)))
+        return gmax
+
+    def descent_trial(self, n, d_x_ptr, d_g_ptr, h, lb, ub, d_out_ptr, stream_ptr=None):
+        """Projected-gradient trial points of n cells of this grid in device
+        memory: per cell, h > 0: out = clamp(x - h g, lb, ub); h < 0: out = x;
+        h == 0: out unchanged.  Returns each cell's sum of out."""
+        hh = np.ascontiguousarray(h, dtype=np.float64).reshape(n)
+        sums = np.zeros(n)
+        self._check(lib().femh_descent_trial(self._h, n, ctypes.c_void_p(d_x_ptr), ctypes.c_void_p(d_g_ptr),
+                                             _ptr(hh), float(lb), float(ub), ctypes.c_void_p(d_out_ptr),
+                                             _ptr(sums), ctypes.c_void_p(stream_ptr or 0)))
+        return sums
 
     def energy_terms(self, nb=1):
         W = np.zeros((nb * self.V, 21))

@@ -205,6 +205,27 @@ int femh_homogenize_many_device(struct femh_ctx *ctx, int n, const double *d_rho
  * Returns FEMH_NO_FIELDS when no solve has been done. */
 int femh_gradient(struct femh_ctx *ctx, const double *dJdC, double *dJdrho);
 
+/* Device-memory form of femh_gradient for a loop that keeps its densities on
+ * the device: d_dJdrho (device, nb*V) receives dJ/drho of the last call's nb
+ * cells plus shift[s] per cell (host, nb values, may be NULL: the volume
+ * term mw (mean(rho) - m0) / V of fast_fem/FEM3DFun.cpp:222-268), and gmax
+ * (host, nb, may be NULL) max |dJ/drho| per cell, the norm that
+ * gradientDescent divides the step by (fast_fem/run3d.cpp:268-289).  Work on
+ * `stream` is waited for first; the call returns when the result is in
+ * place.  Returns FEMH_NO_FIELDS when no solve has been done. */
+int femh_gradient_device(struct femh_ctx *ctx, const double *dJdC, const double *shift, double *d_dJdrho,
+                         double *gmax, void *stream);
+
+/* Trial points of a projected-gradient line search over n cells of the
+ * context's grid (fast_fem/run3d.cpp:238-265, lineSearch: x' = clamp(x - h g,
+ * lb, ub)), in device memory: d_x, d_g, d_out hold n*V doubles; h (host, n
+ * values) selects per cell: h > 0 writes clamp(x - h g, lb, ub) (product and
+ * difference rounded separately), h < 0 copies x, h == 0 leaves the cell's
+ * d_out as it is.  sums (host, n, may be NULL) receives each cell's sum of
+ * d_out after the call (for the volume term).  Waits for `stream` first. */
+int femh_descent_trial(struct femh_ctx *ctx, int n, const double *d_x, const double *d_g, const double *h,
+                       double lb, double ub, double *d_out, double *sums, void *stream);
+
 /* Per-voxel energy cross terms W[e*21 + p] = chi_i^T K0 chi_j / V, p over i<=j
  * (the unscaled adjoint building block). */
 int femh_energy_terms(struct femh_ctx *ctx, double *W);
