@@ -5,6 +5,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <climits>
 #include <cmath>
 #include <stdexcept>
 
@@ -142,16 +144,21 @@ struct TileShape {
 };
 constexpr TileShape kTileShapes[kNumTileShapes] = {{16, 16}, {16, 16}, {8, 32}, {16, 32}};
 
-// The built-in tile is 16 x 32 for batched launches (24 or more right-hand
-// sides) on grids whose z extent is a multiple of 32 (and y of 16) and at
-// least 64: a tile row is then a whole 128-byte line in FP32, and the j/k
-// halo recompute drops from 1.13 to 1.10 (measured at 64^3 with 16 cells: 15%
-// less time than 16 x 16 over the four variants of a PCG iteration).  One
-// cell (6 right-hand sides, too few workgroups for the larger tile) and
-// smaller or odd grids (the 24^3 and 48^3 cells of an uneven batch, which
+// The built-in tile on grids whose z extent is a multiple of 32 (and y of
+// 16) and at least 64:
+//  * batched launches (24 or more right-hand sides): 16 x 32 -- a tile row is
+//    a whole 128-byte line in FP32 and the j/k halo recompute drops from 1.13
+//    to 1.10 (64^3, 16 cells: 15% less time than 16 x 16 over the four
+//    variants of a PCG iteration);
+//  * one cell (6 right-hand sides, too few workgroups for the larger tile):
+//    8 x 32 -- as many tiles as 16 x 16, so as many workgroups, with
+//    whole-line rows (64^3: 53.3 -> 50.8 us per FP64 launch, 31.2 -> 28.8 us
+//    per FP32 launch).
+// Smaller or odd grids (the 24^3 and 48^3 cells of an uneven batch, which
 // waste less on partial tiles) keep 16 x 16.
 int tiled_default_tile(const Dims &g, int R) {
-  return (R >= 24 && g.nz >= 64 && g.nz % 32 == 0 && g.ny % 16 == 0) ? 3 : 1;
+  if (g.nz >= 64 && g.nz % 32 == 0 && g.ny % 16 == 0) return R >= 24 ? 3 : (R <= kRHS ? 2 : 1);
+  return 1;
 }
 
 static inline TileShape tile_of(const Dims &g, int R, int shape) {
@@ -164,28 +171,48 @@ static inline int tiled_tiles(const Dims &g, const TileShape &ts) {
   return ((g.ny + ts.tj - 1) / ts.tj) * ((g.nz + ts.tk - 1) / ts.tk);
 }
 
-static inline int tiled_chunk(const Dims &g, int R, int shape) {
+static int tiled_resident(int sz, const TileShape &ts);
+
+static inline int tiled_chunk(const Dims &g, int R, int shape, int sz) {
   const TileShape ts = tile_of(g, R, shape);
   const int tiles = tiled_tiles(g, ts);
   const int want = shape & 0xff;
-  // planes per chunk: `want` when given (an autotuned shape), else about 32
-  // (halo slab overhead 1/32) but enough chunks to put ~2 workgroups of 256
-  // threads per CU across the right-hand sides; at most kMaxPartials
-  // workgroups per right-hand side either way
+  // planes per chunk: `want` when given (an autotuned shape).  One cell (six
+  // right-hand sides) gives too few workgroups to fill the device more than
+  // about once, so the rounds of resident workgroups set the time: take the
+  // chunk length that minimises rounds x slabs per workgroup (64^3 in FP64:
+  // 13 planes, 480 workgroups in one round, against 8 planes and 768
+  // workgroups in two).  Batches: about 32 planes (halo slab overhead 1/32)
+  // but enough chunks to put ~2 workgroups of 256 threads per CU across the
+  // right-hand sides.  At most kMaxPartials workgroups per right-hand side
+  // either way.
   int TI = 32;
-  if (want > 0)
+  if (want > 0) {
     TI = want;
-  else
+  } else if (R <= kRHS) {
+    const long res = tiled_resident(sz, ts);
+    long best = LONG_MAX;
+    for (int c = 1; c <= g.nx; c++) {
+      const int ti = (g.nx + c - 1) / c, ch = (g.nx + ti - 1) / ti;
+      if ((long)tiles * ch > kMaxPartials) continue;
+      const long rounds = ((long)tiles * ch * R + res - 1) / res, cost = rounds * (ti + 1);
+      if (cost < best) {
+        best = cost;
+        TI = ti;
+      }
+    }
+  } else {
     while (TI > 2 && (long)tiles * ((g.nx + TI - 1) / TI) * R * ts.tj * ts.tk < 512L * 256) TI /= 2;
+  }
   if (TI > g.nx) TI = g.nx;
   while ((long)tiles * ((g.nx + TI - 1) / TI) > kMaxPartials) TI++;
   return TI;
 }
 
-int tiled_slabs(const Dims &g, int R, int shape) { return tiled_chunk(g, R, shape); }
+int tiled_slabs(const Dims &g, int R, int shape, int sz) { return tiled_chunk(g, R, shape, sz); }
 
-int tiled_parts(const Dims &g, int R, int shape) {
-  const int TI = tiled_chunk(g, R, shape);
+int tiled_parts(const Dims &g, int R, int shape, int sz) {
+  const int TI = tiled_chunk(g, R, shape, sz);
   return tiled_tiles(g, tile_of(g, R, shape)) * ((g.nx + TI - 1) / TI);
 }
 
@@ -532,11 +559,41 @@ __global__ __launch_bounds__(TJ * TK, sizeof(T) == 4 ? 4 : kWaves64) void k_tile
   }
 }
 
+// Workgroups of the fine operator resident on the device at once for an
+// element size and tile: the occupancy of a representative variant (a
+// Chebyshev step) times the CU count, looked up once per process.
+template <typename T, int TJ, int TK>
+static int tiled_resident_of() {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tiled<T, ST_PLAIN, EP_STEP, TJ, TK>, TJ * TK, 0) !=
+          hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu < 1 ||
+      cus < 1)
+    throw std::runtime_error("occupancy query of the fine operator failed");
+  return per_cu * cus;
+}
+
+static int tiled_resident(int sz, const TileShape &ts) {
+  static std::atomic<int> cache[2][kNumTileShapes] = {};
+  const int v = (ts.tj == 8 && ts.tk == 32) ? 2 : (ts.tj == 16 && ts.tk == 32) ? 3 : 1, d = sz == 8 ? 1 : 0;
+  int r = cache[d][v].load(std::memory_order_relaxed);
+  if (r > 0) return r;
+  if (d == 0)
+    r = v == 2 ? tiled_resident_of<float, 8, 32>() : v == 3 ? tiled_resident_of<float, 16, 32>()
+                                                           : tiled_resident_of<float, 16, 16>();
+  else
+    r = v == 2 ? tiled_resident_of<double, 8, 32>() : v == 3 ? tiled_resident_of<double, 16, 32>()
+                                                            : tiled_resident_of<double, 16, 16>();
+  cache[d][v].store(r, std::memory_order_relaxed);
+  return r;
+}
+
 template <typename T, int ST, int EP, int TJ, int TK>
 static void tiled_go(const Dims &g, int R, const T *E, const ModeC<T> &mc, const OpArgs<T> &a,
                      hipStream_t s, int shape, hipEvent_t ev0, hipEvent_t ev1) {
   const int tiles_k = (g.nz + TK - 1) / TK, tiles_j = (g.ny + TJ - 1) / TJ;
-  const int TI = tiled_chunk(g, R, shape);
+  const int TI = tiled_chunk(g, R, shape, (int)sizeof(T));
   const int tiles = tiles_j * tiles_k, chunks = (g.nx + TI - 1) / TI;
   if (R % kRHS) throw std::runtime_error("right-hand sides must come in groups of six");
   // buffer resources address a right-hand side's vector with 32-bit offsets

@@ -71,7 +71,8 @@ struct OpArgs {
 // node columns per workgroup: 1 = 16 x 16, 2 = 8 x 32, 3 = 16 x 32); 0 in a
 // field is the built-in choice (tiled_default_tile, planes from the grid).
 // tiled_parts: partial count of a dot launch with that shape; tiled_slabs:
-// the planes per workgroup actually used.  ev0/ev1 (optional): events the
+// the planes per workgroup actually used (both depend on the element size
+// sz, 4 or 8, for a one-cell launch).  ev0/ev1 (optional): events the
 // runtime records at the kernel's own start and end (kernel timing).
 constexpr int kNumTileShapes = 4;
 inline int tiled_shape_code(int tile, int planes) { return (tile << 8) | planes; }
@@ -80,8 +81,8 @@ template <typename T>
 void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCoef &mc,
                   const OpArgs<T> &a, hipStream_t s, int shape = 0, hipEvent_t ev0 = nullptr,
                   hipEvent_t ev1 = nullptr);
-int tiled_parts(const Dims &g, int R, int shape = 0);
-int tiled_slabs(const Dims &g, int R, int shape = 0);
+int tiled_parts(const Dims &g, int R, int shape, int sz);
+int tiled_slabs(const Dims &g, int R, int shape, int sz);
 
 // coarse level (27-point block stencil, one RHS per thread)
 template <typename T>

@@ -309,7 +309,7 @@ int Context::residual_dots(const int *act_) {
     a.slot3 = S_XR;
     a.dot = 1;
     op<double>(h64, 0, ST_PLAIN, EP_RES, a);
-    return tiled_parts(g, R, shape_fine);
+    return tiled_parts(g, R, shape_fine, (int)sizeof(double));
   }
   apply<double>(h64, 0, x.as<double>(), r.as<double>(), b.as<double>(), 1, act_, nullptr, 0);
   launch_dot<double>(g.V, R, r.as<double>(), r.as<double>(), S_RR, act_, red, stream);
@@ -318,8 +318,8 @@ int Context::residual_dots(const int *act_) {
   return vec_parts(g.V);
 }
 
-int Context::apply_parts() const {
-  return o.apply_kernel == 1 ? dense_parts(g) : tiled_parts(g, R, shape_fine);
+int Context::apply_parts(int sz) const {
+  return o.apply_kernel == 1 ? dense_parts(g) : tiled_parts(g, R, shape_fine, sz);
 }
 
 template <typename T>
@@ -579,12 +579,13 @@ void Context::autotune(Hierarchy<T> &h) {
     // tile variants x node planes per workgroup (a shape equivalent to an
     // earlier candidate -- same tile, same planes -- is not timed twice)
     std::vector<int> cand{0};
-    std::vector<std::pair<int, int>> seen{{tiled_default_tile(g, R), tiled_slabs(g, R, 0)}};
+    const int sz = (int)sizeof(T);
+    std::vector<std::pair<int, int>> seen{{tiled_default_tile(g, R), tiled_slabs(g, R, 0, sz)}};
     for (int tile = 1; tile < kNumTileShapes; tile++)
       for (int c : {0, 2, 4, 8, 16, 32}) {
         const int code = tiled_shape_code(tile, c);
-        if (c > g.nx || (c > 0 && tiled_slabs(g, R, code) != c)) continue;
-        const std::pair<int, int> key{tile, tiled_slabs(g, R, code)};
+        if (c > g.nx || (c > 0 && tiled_slabs(g, R, code, sz) != c)) continue;
+        const std::pair<int, int> key{tile, tiled_slabs(g, R, code, sz)};
         if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
         seen.push_back(key);
         cand.push_back(code);
@@ -915,7 +916,7 @@ bool Context::fuses_rz(const Hierarchy<T> &h) const {
 
 template <typename T>
 int Context::rz_parts(const Hierarchy<T> &h) const {
-  return fuses_rz(h) ? tiled_parts(g, R, shape_fine) : vec_parts(g.V);
+  return fuses_rz(h) ? tiled_parts(g, R, shape_fine, (int)sizeof(T)) : vec_parts(g.V);
 }
 
 // PCG over all right-hand sides flagged in `active` (updated in place as
@@ -988,7 +989,7 @@ void Context::pcg(Hierarchy<T> &h, const T *bvec, T *xv, T *rv, T *zv, T *pv, T 
     } else {
       apply<T>(h, 0, pv, qv, nullptr, 0, ap, &red, S_PQ);
     }
-    launch_pcg_update<T>(V, R, xv, rv, pcur, qv, jac ? D0 : nullptr, zv, rzc, rzn, ap, red, apply_parts(), np_rz,
+    launch_pcg_update<T>(V, R, xv, rv, pcur, qv, jac ? D0 : nullptr, zv, rzc, rzn, ap, red, apply_parts((int)sizeof(T)), np_rz,
                          stream, lz_alpha, cnt);
     launch_pcg_check(R, apw, red, np_rr, bb_d, tol, cnt, relres_d, pst, stream);
     if (!jac) precond<T>(h, rv, zv, ap, rzn);
@@ -1352,7 +1353,7 @@ void Context::homogenize(double *C, femh_stats *st) {
         apply<float>(h32, 0, z32.as<float>(), q32.as<float>(), nullptr, 0, ao, &red, S_PQ);
         read_scalars(S_RZ0, rz.data(), R, rz_parts(h32));
         std::vector<double> zkz(R);
-        read_scalars(S_PQ, zkz.data(), R, apply_parts());
+        read_scalars(S_PQ, zkz.data(), R, apply_parts((int)sizeof(float)));
         for (int s = 0; s < nb; s++) {
           double trC = 0, srz = 0;
           bool any = false, definite = true;

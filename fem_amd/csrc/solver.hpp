@@ -273,7 +273,7 @@ struct Context {
   void upload_cheb(Hierarchy<T> &h);
   template <typename T>
   void cheb_run(Hierarchy<T> &h, int l, const T *b, T *x, const int *act_, int k, double ratio);
-  int apply_parts() const;
+  int apply_parts(int sz) const;  // partials of a fine-operator dot, element size sz
   // Chebyshev degrees of pre- and post-smoothing on level l
   int nu_pre(int l) const { return (l > 0 && o.coarse_nu > 0) ? o.coarse_nu : o.nu1; }
   int nu_post(int l) const { return (l > 0 && o.coarse_nu > 0) ? o.coarse_nu : o.nu2; }
