@@ -26,7 +26,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
 
 
 def parse():
@@ -274,9 +273,7 @@ def main():
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "kernel": dom_name, "launches": dom["launches"], "avg_us": avg_ms * 1e3,
-            "bytes_per_launch": dom["bytes"] / max(dom["launches"], 1),
-            "flops_frac_fp64_peak": (dom["flops"] / (dom["ms"] * 1e-3) / 1e12) / FP64_PEAK_TFLOPS
-            if dom["ms"] > 0 else 0.0}
+            "bytes_per_launch": dom["bytes"] / max(dom["launches"], 1)}
     # the configuration the counters were read on
     tr = pmc_traffic() if (n == 64 and B == 16 and a.structure == "general") else None
     if tr:

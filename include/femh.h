@@ -298,9 +298,11 @@ int femh_level_dims(struct femh_ctx *ctx, int level, int *dims);
  * enabled, every fine-grid operator launch is bracketed by HIP events on the
  * context's stream; femh_kernel_stats returns, for which = 0 (FP64 apply) or
  * 1 (FP32 apply), the launch count, summed device milliseconds and the
- * algorithmic bytes / flops of those launches (bytes: u read, y written, b
- * read for residual launches, E read once; flops: 1152 per voxel per RHS).
- * Enabling resets the counters. */
+ * algorithmic bytes / flops of those launches (bytes: the vectors each
+ * stage/epilogue variant reads and writes, E once; flops: 1152 per voxel per
+ * RHS, the dense 24x24 element product -- a dense-equivalent figure, not the
+ * instructions the mode-form kernel executes).  Enabling resets the
+ * counters. */
 int femh_set_profiling(struct femh_ctx *ctx, int enable);
 int femh_kernel_stats(struct femh_ctx *ctx, int which, long long *launches, double *ms,
                       double *bytes, double *flops);
