@@ -173,6 +173,26 @@ static inline int tiled_tiles(const Dims &g, const TileShape &ts) {
 
 static int tiled_resident(int sz, const TileShape &ts);
 
+// Chunk length for a launch whose active workgroups (Ract active right-hand
+// sides) fill the device about once or less: the rounds of resident
+// workgroups set the time, so take the length that minimises rounds x slabs
+// per workgroup (at most 255 planes, the width of the launch-shape field).
+static int round_chunk(const Dims &g, const TileShape &ts, int tiles, int Ract, int sz) {
+  const long res = tiled_resident(sz, ts);
+  long best = LONG_MAX;
+  int TI = g.nx;
+  for (int c = (g.nx + 254) / 255; c <= g.nx; c++) {
+    const int ti = (g.nx + c - 1) / c, ch = (g.nx + ti - 1) / ti;
+    if ((long)tiles * ch > kMaxPartials) continue;
+    const long rounds = ((long)tiles * ch * Ract + res - 1) / res, cost = rounds * (ti + 1);
+    if (cost < best) {
+      best = cost;
+      TI = ti;
+    }
+  }
+  return TI;
+}
+
 static inline int tiled_chunk(const Dims &g, int R, int shape, int sz) {
   const TileShape ts = tile_of(g, R, shape);
   const int tiles = tiled_tiles(g, ts);
@@ -190,17 +210,7 @@ static inline int tiled_chunk(const Dims &g, int R, int shape, int sz) {
   if (want > 0) {
     TI = want;
   } else if (R <= kRHS) {
-    const long res = tiled_resident(sz, ts);
-    long best = LONG_MAX;
-    for (int c = 1; c <= g.nx; c++) {
-      const int ti = (g.nx + c - 1) / c, ch = (g.nx + ti - 1) / ti;
-      if ((long)tiles * ch > kMaxPartials) continue;
-      const long rounds = ((long)tiles * ch * R + res - 1) / res, cost = rounds * (ti + 1);
-      if (cost < best) {
-        best = cost;
-        TI = ti;
-      }
-    }
+    TI = round_chunk(g, ts, tiles, R, sz);
   } else {
     while (TI > 2 && (long)tiles * ((g.nx + TI - 1) / TI) * R * ts.tj * ts.tk < 512L * 256) TI /= 2;
   }
@@ -210,6 +220,14 @@ static inline int tiled_chunk(const Dims &g, int R, int shape, int sz) {
 }
 
 int tiled_slabs(const Dims &g, int R, int shape, int sz) { return tiled_chunk(g, R, shape, sz); }
+
+int tiled_sparse_shape(const Dims &g, int R, int Ract, int shape, int sz) {
+  if (shape & 0xff) return shape;  // planes pinned (autotuned or set)
+  int v = (shape >> 8) & 0xff;
+  if (v <= 0 || v >= kNumTileShapes) v = tiled_default_tile(g, R);
+  const TileShape ts = kTileShapes[v];
+  return tiled_shape_code(v, round_chunk(g, ts, tiled_tiles(g, ts), Ract, sz));
+}
 
 int tiled_parts(const Dims &g, int R, int shape, int sz) {
   const int TI = tiled_chunk(g, R, shape, sz);

@@ -83,6 +83,11 @@ void launch_tiled(int st, int ep, const Dims &g, int R, const T *E, const ModeCo
                   hipEvent_t ev1 = nullptr);
 int tiled_parts(const Dims &g, int R, int shape, int sz);
 int tiled_slabs(const Dims &g, int R, int shape, int sz);
+// launch shape for a launch with only Ract of its R right-hand sides active
+// (the power iteration: one per cell) and no fused dot: shape's tile, the
+// chunk length from the rounds of the active workgroups (shape returned as
+// it is when its planes are pinned)
+int tiled_sparse_shape(const Dims &g, int R, int Ract, int shape, int sz);
 
 // coarse level (27-point block stencil, one RHS per thread)
 template <typename T>

@@ -398,6 +398,14 @@ void Context::estimate_lmax(Hierarchy<T> &h) {
     // indefinite (PCG then meets r.z < 0).  Ten steps have underestimated
     // lambda_max by more than 15% on binary cells with a 1e-9 void phase and
     // on SIMP cells with the degree-2 smoother; the estimate is scaled by 1.3.
+    // one right-hand side per cell is active: the fine-level launches take
+    // a chunk length sized for those workgroups alone (no dot is fused in
+    // them, so no partial count depends on it)
+    struct Restore {
+      int &ref, val;
+      ~Restore() { ref = val; }
+    } restore{shape_fine, shape_fine};
+    if (l == 0 && o.apply_kernel == 0) shape_fine = tiled_sparse_shape(g, R, nb, shape_fine, (int)sizeof(T));
     for (int it = 0; it < 10; it++) {
       OpArgs<T> a;
       a.u = xv;
