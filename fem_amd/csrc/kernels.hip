@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_build_level1(Dims fg, Dims cg, const
       }
     }
 #pragma unroll
-    for (int q = 0; q < 9; q++) Ss[(size_t)(o * 9 + q) * cg.V + N] = (T)blk[q];
+    for (int q = 0; q < 9; q++) Ss[stencil_index(o, q, cg.V, N)] = (T)blk[q];
   }
 }
 
@@ -216,13 +216,13 @@ __global__ __launch_bounds__(kBlock) void k_galerkin_axis(Dims gi, Dims go, cons
       for (int d = -1; d <= 1; d++) {
         const double w = pair_weight(o, d, D);
         if (w == 0.0) continue;
-        const T *src = Sis + (size_t)((oc + (d - D) * st) * 9) * gi.V + n;
+        const int os = oc + (d - D) * st;
 #pragma unroll
-        for (int q = 0; q < 9; q++) blk[q] += w * (double)src[(size_t)q * gi.V];
+        for (int q = 0; q < 9; q++) blk[q] += w * (double)Sis[stencil_index(os, q, gi.V, n)];
       }
     }
 #pragma unroll
-    for (int q = 0; q < 9; q++) Sos[(size_t)(oc * 9 + q) * go.V + N] = (T)blk[q];
+    for (int q = 0; q < 9; q++) Sos[stencil_index(oc, q, go.V, N)] = (T)blk[q];
   }
 }
 
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil_diag(StencilOp<T> op, T *__r
   for (int n = blockIdx.x * kBlock + threadIdx.x; n < g.V; n += gridDim.x * kBlock) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const T v = S[(size_t)(13 * 9 + 4 * c) * g.V + n];
+      const T v = S[stencil_index(13, 4 * c, g.V, n)];
       ds[(size_t)c * g.V + n] = v > (T)0 ? v : (T)1;
     }
   }

@@ -11,9 +11,10 @@
 //   node/voxel index  (i*ny + j)*nz + k, z fastest
 //   vector fields     [q][component][V]: RHS stride 3*V, component stride V
 //   fine moduli       E[s][V] (unit-modulus stiffness K0 per voxel)
-//   coarse operators  27-point block stencils, S[s][(off*9 + 3*c + d)*V + node],
-//                     off = (di+1)*9 + (dj+1)*3 + (dk+1) for the neighbour
-//                     (i+di, j+dj, k+dk) with periodic wrap
+//   coarse operators  27-point block stencils, 243*V values per sample: block
+//                     off = (di+1)*9 + (dj+1)*3 + (dk+1) couples a node to its
+//                     neighbour (i+di, j+dj, k+dk) (periodic wrap); its entry
+//                     q = 3*c + d of node n sits at stencil_index(off, q, V, n)
 //   diagonals         D[s][c][V] and its inverse Dinv (what the kernels use)
 // Reductions are FP64 and deterministic: each block writes a partial sum and
 // the consuming kernel (or launch_finalize, for host reads) sums the partials
@@ -24,6 +25,17 @@
 #include <cstddef>
 
 namespace femh {
+
+// Offset of entry q (row-major in the 3x3 block) of stencil block `off` at
+// node n, in one sample's stencil of V nodes.  Each block is 9*V values:
+// entries 0-3 of every node as [V][4], entries 4-7 as [V][4], entry 8 as
+// [V], so a node's block is two 16-byte loads and one scalar load.
+__host__ __device__ inline size_t stencil_index(int off, int q, int V, int n) {
+  const size_t b = (size_t)off * 9 * (size_t)V;
+  if (q < 4) return b + (size_t)n * 4 + (size_t)q;
+  if (q < 8) return b + 4 * (size_t)V + (size_t)n * 4 + (size_t)(q - 4);
+  return b + 8 * (size_t)V + (size_t)n;
+}
 
 constexpr int kRHS = 6;             // load cases per sample
 constexpr int kMaxB = 16;           // samples per context

@@ -691,6 +691,20 @@ __device__ __forceinline__ T stage_at(const OpArgs<T> &a, const T *__restrict__ 
   return v;
 }
 
+// Stencil block `off` of node n (stencil_index layout); `tr`: as the
+// transposed block seen from the neighbour, S(m, o) = S(m + o, 26 - o)^T
+// (the Galerkin operator is symmetric)
+template <typename T>
+__device__ __forceinline__ void load_block(const T *__restrict__ Ss, int off, int V, int n, bool tr, T (&sv)[9]) {
+  T w[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) w[q] = Ss[stencil_index(off, q, V, n)];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int d = 0; d < 3; d++) sv[3 * c + d] = tr ? w[3 * d + c] : w[3 * c + d];
+}
+
 // Coarse-level node update for NR right-hand sides of one sample (load cases
 // lc0 .. lc0+NR-1, those set in `mask`): the 27 3x3 blocks of the node's
 // stencil row are read once and applied to every right-hand side.  With
@@ -719,15 +733,10 @@ __device__ __forceinline__ void stencil_node(const Dims &g, const T *__restrict_
   for (int o = 0; o < 27; o++) {
     const int nb = (I[o / 9] * g.ny + J[(o / 3) % 3]) * g.nz + K[o % 3];
     T sv[9];
-    if (sym && o < 13) {
-#pragma unroll
-      for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) sv[3 * c + d] = Ss[(size_t)((26 - o) * 9 + 3 * d + c) * V + nb];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 9; q++) sv[q] = Ss[(size_t)(o * 9 + q) * V + n];
-    }
+    if (sym && o < 13)
+      load_block(Ss, 26 - o, V, nb, true, sv);
+    else
+      load_block(Ss, o, V, n, false, sv);
 #pragma unroll
     for (int t = 0; t < NR; t++) {
       if (!((mask >> t) & 1u)) continue;
@@ -801,15 +810,10 @@ __global__ __launch_bounds__(kSplitNodes *kSplitWaves) void k_stencil_split(Dims
     for (int dk = 0; dk < 3; dk++) {
       const int o = 3 * w + dk, nb = row + K[dk];
       T sv[9];
-      if (sym && o < 13) {
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-#pragma unroll
-          for (int d = 0; d < 3; d++) sv[3 * c + d] = Ss[(size_t)((26 - o) * 9 + 3 * d + c) * V + nb];
-      } else {
-#pragma unroll
-        for (int q = 0; q < 9; q++) sv[q] = Ss[(size_t)(o * 9 + q) * V + n];
-      }
+      if (sym && o < 13)
+        load_block(Ss, 26 - o, V, nb, true, sv);
+      else
+        load_block(Ss, o, V, n, false, sv);
       const T u0 = stage_at<T, ST>(a, Ds, ti, off, V, 0, nb), u1 = stage_at<T, ST>(a, Ds, ti, off, V, 1, nb),
               u2 = stage_at<T, ST>(a, Ds, ti, off, V, 2, nb);
       acc[0] += sv[0] * u0 + sv[1] * u1 + sv[2] * u2;
@@ -847,10 +851,11 @@ __global__ __launch_bounds__(kSplitNodes *kSplitWaves) void k_stencil_split(Dims
 // to that plane.  Against a three-plane ring this needs 2/3 of the LDS, so
 // more workgroups share a CU and hide the coefficient loads' latency.
 
-// Stencil coefficients through a raw buffer resource (bload): a scalar plane
+// Stencil coefficients through a raw buffer resource (bload): a scalar block
 // offset plus one 32-bit lane offset per load (no vector address
-// arithmetic), and a load past the sample's stencil returns 0 instead of
-// faulting.
+// arithmetic), a node's block as two 16-byte loads and one scalar load
+// (stencil_index layout), and a load past the sample's stencil returns 0
+// instead of faulting.
 
 constexpr int kStBatch = 3;  // stencil blocks whose coefficient loads are issued together
 
@@ -870,7 +875,8 @@ __global__ __launch_bounds__(kBlock) void k_stencil_plane(Dims g, const T *__res
   const int tid = threadIdx.x;
   const int j0 = (tile / tiles_k) * kTile, k0 = (tile % tiles_k) * kTile;
   const int V = g.V, plane = g.ny * g.nz;
-  const unsigned vb = (unsigned)V * (unsigned)sizeof(T);  // bytes per coefficient plane
+  constexpr unsigned SZ = sizeof(T);
+  const unsigned vb = (unsigned)V * SZ;  // bytes per V stencil values
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
       (void *)(S + (size_t)smp * 243 * V), (short)0, (int)(243u * vb), 0x00020000);
   const T *__restrict__ Ds = a.Dinv ? a.Dinv + (size_t)smp * 3 * V : nullptr;
@@ -938,18 +944,25 @@ __global__ __launch_bounds__(kBlock) void k_stencil_plane(Dims g, const T *__res
       const int o = di * 9 + pos;
       const int slot = (nj + dj) * kTN + nk + dk;
       T sv[9];
-      if (SYM && o < 13) {
-        // S(n, o) = S(n + o, 26 - o)^T
-        const unsigned nb = (unsigned)(pw * plane + Jn[dj] * g.nz + Kn[dk]) * (unsigned)sizeof(T);
+      {
+        // S(n, o) = S(n + o, 26 - o)^T for the lower offsets
+        const bool tr = SYM && o < 13;
+        const int ob = tr ? 26 - o : o;
+        const unsigned node = tr ? (unsigned)(pw * plane + Jn[dj] * g.nz + Kn[dk]) : (unsigned)n;
+        const unsigned base = (unsigned)(ob * 9) * vb;  // block ob (stencil_index layout)
+        // (the four consecutive loads of each [V][4] group are issued by
+        // the compiler as one 16-byte load)
+        T w[9];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          w[q] = bload(rs, (node * 4u + q) * SZ, base, (T *)nullptr);
+          w[4 + q] = bload(rs, (node * 4u + q) * SZ, base + 4u * vb, (T *)nullptr);
+        }
+        w[8] = bload(rs, node * SZ, base + 8u * vb, (T *)nullptr);
 #pragma unroll
         for (int c = 0; c < 3; c++)
 #pragma unroll
-          for (int d = 0; d < 3; d++)
-            sv[3 * c + d] = bload(rs, nb, (unsigned)((26 - o) * 9 + 3 * d + c) * vb, (T *)nullptr);
-      } else {
-        const unsigned no = (unsigned)n * (unsigned)sizeof(T);
-#pragma unroll
-        for (int q = 0; q < 9; q++) sv[q] = bload(rs, no, (unsigned)(o * 9 + q) * vb, (T *)nullptr);
+          for (int d = 0; d < 3; d++) sv[3 * c + d] = tr ? w[3 * d + c] : w[3 * c + d];
       }
 #pragma unroll
       for (int t = 0; t < NG; t++) {

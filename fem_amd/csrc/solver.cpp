@@ -455,7 +455,7 @@ void Context::dense_coarse(Hierarchy<double> &h) {
         const int M = (ni * d.ny + nj) * d.nz + nk;
         for (int c = 0; c < 3; c++)
           for (int dd = 0; dd < 3; dd++)
-            A[(size_t)(c * V + N) * n + dd * V + M] += S[(size_t)(off * 9 + 3 * c + dd) * V + N];
+            A[(size_t)(c * V + N) * n + dd * V + M] += S[stencil_index(off, 3 * c + dd, V, N)];
       }
     }
     double dmean = 0;
