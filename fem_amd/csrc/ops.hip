@@ -73,9 +73,9 @@ struct DotAcc {
   double a = 0, b = 0, c = 0;
 };
 
-// output store of an epilogue; NT: a streaming (non-temporal) store, for the
-// fine level, whose output vectors are far larger than the caches and are
-// read back only by a later launch
+// output store of an epilogue; NT: a streaming (non-temporal) store, for
+// output vectors far larger than the caches (read back only by a later
+// launch, which would find them evicted anyway)
 template <bool NT, typename T>
 __device__ __forceinline__ void ep_store(T *p, T v) {
   if (NT)
@@ -268,8 +268,17 @@ __device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned voff,
 // overwritten two slabs after it was filled, behind the barrier that closes
 // the slab in between (every read of it -- the FP64 bottom face, the owned
 // node's staged value -- happens before that barrier).
+// Waves per SIMD the FP32 variants are compiled for: 4 (at most 128 VGPRs),
+// except the D^-1-scaled Chebyshev step (two smoothing steps in one launch),
+// whose staged diagonal and four epilogue inputs do not fit: held to 128 it
+// spilled 16 VGPRs to scratch.
+template <typename T, int ST, int EP>
+constexpr int tiled_waves() {
+  return sizeof(T) == 4 ? ((ST == ST_DSCALE && EP == EP_STEP) ? 3 : 4) : kWaves64;
+}
+
 template <typename T, int ST, int EP, int TJ, int TK>
-__global__ __launch_bounds__(TJ * TK, sizeof(T) == 4 ? 4 : kWaves64) void k_tiled(Dims g, const T *__restrict__ E,
+__global__ __launch_bounds__(TJ * TK, (tiled_waves<T, ST, EP>())) void k_tiled(Dims g, const T *__restrict__ E,
                                                                            ModeC<T> mc, OpArgs<T> a, int TI,
                                                                            int tiles_k, int tiles, int chunks) {
   constexpr int NT = TJ * TK;                           // threads = owned nodes per plane
@@ -564,7 +573,10 @@ __global__ __launch_bounds__(TJ * TK, sizeof(T) == 4 ? 4 : kWaves64) void k_tile
 #pragma unroll
           for (int c = 0; c < 3; c++) Au[c] += src[c];
         }
-      epilogue<T, EP, true>(a, pin, ti, c1, c2, off, V, (int)nown, Au, uo, dv);
+      if (a.nt)  // (one uniform branch per node, not per store)
+        epilogue<T, EP, true>(a, pin, ti, c1, c2, off, V, (int)nown, Au, uo, dv);
+      else
+        epilogue<T, EP, false>(a, pin, ti, c1, c2, off, V, (int)nown, Au, uo, dv);
     }
   }
   if ((EP == EP_AX || EP == EP_STEP || EP == EP_RES || EP == EP_CHEB0) && a.dot) {
@@ -613,6 +625,12 @@ static void tiled_go(const Dims &g, int R, const T *E, const ModeC<T> &mc, const
   const int tiles_k = (g.nz + TK - 1) / TK, tiles_j = (g.ny + TJ - 1) / TJ;
   const int TI = tiled_chunk(g, R, shape, (int)sizeof(T));
   const int tiles = tiles_j * tiles_k, chunks = (g.nx + TI - 1) / TI;
+  // non-temporal output stores once one output vector (all right-hand
+  // sides) passes 128 MiB: 64^3 x 16 cells (302 MB in FP32) gains 1-2% from
+  // them, 48^3 x 16 cells (127 MB) and one cell lose, since the next launch
+  // finds such a vector still in the 256 MB last-level cache
+  OpArgs<T> an = a;
+  an.nt = (size_t)R * 3 * (size_t)g.V * sizeof(T) >= ((size_t)128 << 20) ? 1 : 0;
   if (R % kRHS) throw std::runtime_error("right-hand sides must come in groups of six");
   // buffer resources address a right-hand side's vector with 32-bit offsets
   if ((size_t)3 * g.V * sizeof(T) >= ((size_t)1 << 31)) throw std::runtime_error("grid too large for the tiled operator");
@@ -620,10 +638,10 @@ static void tiled_go(const Dims &g, int R, const T *E, const ModeC<T> &mc, const
     // timed launch: the runtime records the events at the kernel's own start
     // and end, so no queue gap is counted
     hipExtLaunchKernelGGL((k_tiled<T, ST, EP, TJ, TK>), dim3(tiles * chunks * R), dim3(TJ * TK), 0, s, ev0, ev1,
-                          0, g, E, mc, a, TI, tiles_k, tiles, chunks);
+                          0, g, E, mc, an, TI, tiles_k, tiles, chunks);
     return;
   }
-  hipLaunchKernelGGL((k_tiled<T, ST, EP, TJ, TK>), dim3(tiles * chunks * R), dim3(TJ * TK), 0, s, g, E, mc, a,
+  hipLaunchKernelGGL((k_tiled<T, ST, EP, TJ, TK>), dim3(tiles * chunks * R), dim3(TJ * TK), 0, s, g, E, mc, an,
                      TI, tiles_k, tiles, chunks);
 }
 

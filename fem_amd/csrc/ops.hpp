@@ -64,6 +64,9 @@ struct OpArgs {
   // not in the arguments, so that a recorded launch sequence stays valid when
   // a new solve brings new spectra.
   const double *cheb = nullptr;
+  // outputs stored non-temporally (set by the fine-level launcher for
+  // vectors too large to stay in the caches until the next launch reads them)
+  int nt = 0;
 };
 
 // fine level (tiled cube-mode voxel kernel).  shape: launch shape, bits 0-7
