@@ -627,8 +627,9 @@ static void tiled_go(const Dims &g, int R, const T *E, const ModeC<T> &mc, const
   const int tiles = tiles_j * tiles_k, chunks = (g.nx + TI - 1) / TI;
   // non-temporal output stores once one output vector (all right-hand
   // sides) passes 128 MiB: 64^3 x 16 cells (302 MB in FP32) gains 1-2% from
-  // them, 48^3 x 16 cells (127 MB) and one cell lose, since the next launch
-  // finds such a vector still in the 256 MB last-level cache
+  // them; 48^3 x 16 cells (127 MB) loses, since the next launch finds such a
+  // vector still in the 256 MB last-level cache (one 64^3 cell in FP64, 38
+  // MB, solves as fast either way)
   OpArgs<T> an = a;
   an.nt = (size_t)R * 3 * (size_t)g.V * sizeof(T) >= ((size_t)128 << 20) ? 1 : 0;
   if (R % kRHS) throw std::runtime_error("right-hand sides must come in groups of six");
