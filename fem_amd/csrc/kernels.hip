@@ -6,6 +6,8 @@
 // wave64s).  Reductions: see device_common.hpp.
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -358,21 +360,35 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Dims fg, Dims cg, const 
   }
 }
 
+// x[r] = Ainv[s] b[r] for the active right-hand sides r of sample s
+// (blockIdx.y): one wave per row i, the row read across the wave's lanes
+// (coalesced) and applied to the sample's six right-hand sides at once, FP64
+// sums reduced over the wave.  Inactive right-hand sides keep their x.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dense_solve(int n, const T *__restrict__ Ainv,
                                                         const T *__restrict__ b, T *__restrict__ x,
                                                         const int *__restrict__ act) {
-  const int r = blockIdx.y;
-  if (!act[r]) return;
-  const T *A = Ainv + (size_t)(r / kRHS) * n * n;
-  const T *br = b + (size_t)r * n;
-  T *xr = x + (size_t)r * n;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    double acc = 0;
-    const T *row = A + (size_t)i * n;
-    for (int j = 0; j < n; j++) acc += (double)row[j] * (double)br[j];
-    xr[i] = (T)acc;
+  const int s = blockIdx.y, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  unsigned mask = 0;
+#pragma unroll
+  for (int t = 0; t < kRHS; t++) mask |= act[s * kRHS + t] ? (1u << t) : 0u;
+  if (!mask || i >= n) return;  // (per wave: the kernel has no barrier)
+  const T *__restrict__ row = Ainv + (size_t)s * n * n + (size_t)i * n;
+  const T *__restrict__ bs = b + (size_t)s * kRHS * n;
+  double acc[kRHS] = {0, 0, 0, 0, 0, 0};
+  for (int j = lane; j < n; j += 64) {
+    const double a = (double)row[j];
+#pragma unroll
+    for (int t = 0; t < kRHS; t++)
+      if ((mask >> t) & 1u) acc[t] += a * (double)bs[(size_t)t * n + j];
   }
+#pragma unroll
+  for (int t = 0; t < kRHS; t++) acc[t] = wave_sum(acc[t]);
+  if (lane == 0)
+#pragma unroll
+    for (int t = 0; t < kRHS; t++)
+      if ((mask >> t) & 1u) x[(size_t)(s * kRHS + t) * n + i] = (T)acc[t];
 }
 
 // ------------------------------------------------------------------------
@@ -1004,7 +1020,9 @@ void launch_prolong_add(Dims fine, Dims coarse, int R, const T *ec, T *ef, const
 template <typename T>
 void launch_dense_solve(int n, int R, const T *Ainv, const T *b, T *x, const int *act,
                         hipStream_t s) {
-  hipLaunchKernelGGL(k_dense_solve<T>, grid2(n, R), dim3(kBlock), 0, s, n, Ainv, b, x, act);
+  if (R % kRHS) throw std::runtime_error("right-hand sides must come in groups of six");
+  hipLaunchKernelGGL(k_dense_solve<T>, dim3((n + kBlock / 64 - 1) / (kBlock / 64), R / kRHS), dim3(kBlock), 0, s,
+                     n, Ainv, b, x, act);
 }
 template <typename T>
 void launch_cheb_first(int V, int R, const T *r, const T *D, T *d, T *x, double inv_theta,
