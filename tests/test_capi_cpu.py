@@ -111,3 +111,15 @@ def test_noalloc_entry_point_rejects_bad_arguments_before_device_work():
     assert st == femh.BAD_ARG and b"Jacobi" in buf.value
     st = L.femh_homogenize_device_noalloc(None, fake, fake, None, None, buf, 256)
     assert st == femh.BAD_ARG
+
+
+def test_device_loop_entry_points_reject_bad_arguments_on_the_host():
+    """femh_gradient_device and femh_descent_trial check their arguments on
+    the host: without a context (or with a null pointer) they return BAD_ARG
+    and launch nothing."""
+    L = fem_amd.lib()
+    fake = ctypes.c_void_p(16)
+    h = np.ones(4)
+    G = np.zeros(36)
+    assert L.femh_gradient_device(None, G.ctypes.data, None, fake, None, None) == femh.BAD_ARG
+    assert L.femh_descent_trial(None, 4, fake, fake, h.ctypes.data, 1e-3, 1.0, fake, None, None) == femh.BAD_ARG
